@@ -1,0 +1,249 @@
+"""Pins the CPU oracle (CPU only).
+
+Every oracle function is checked against something it does not itself
+compute: glibc, LAPACK fixtures frozen in tests/golden, closed forms that
+Householder QR satisfies, or brute force on tiny inputs.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+from scipy.linalg import lapack as lp
+
+from oracle import candmc_oracle as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+EPS = np.finfo(np.float64).eps
+
+
+def gold(name):
+    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+
+
+def test_drand48_matches_glibc():
+    libc = ctypes.CDLL("libc.so.6")
+    libc.drand48.restype = ctypes.c_double
+    seeds = [0, 1, 99900, 99900 + 61 * 12345, 2**31 - 1, 2**32 + 7, 99900 + 61 * (8191 + 63 * 8192)]
+    want = []
+    for s in seeds:
+        libc.srand48(ctypes.c_long(s))
+        want.append(libc.drand48())
+    assert np.array_equal(O.drand48_elem(np.array(seeds)), np.array(want))
+
+
+def test_hh_recon_test_matrix_is_fixture_input():
+    g = gold("hh_recon_136x17.npz")
+    assert np.array_equal(O.hh_recon_test_matrix(136, 17), g["A"])
+    g = gold("qr2d_96x48.npz")
+    assert np.array_equal(O.qr_2d_test_matrix(96, 48), g["A"])
+
+
+def test_pack_unpack_roundtrip():
+    rng = np.random.default_rng(0)
+    for b in (1, 5, 16):
+        R = np.triu(rng.standard_normal((b, b)))
+        p = O.pack_upper(R, b)
+        assert p.size == O.psz_upr(b)
+        assert np.array_equal(O.unpack_upper(p, b), R)
+        L = np.tril(R.T)
+        assert np.array_equal(O.unpack_lower(O.pack_lower(L, b, 1), b, 1), L)
+
+
+def test_dlarft_equals_reflector_product_and_dgeqrt():
+    A = O.gaussian(40, 9, 3)
+    qr, tau = O.local_tsqr(A)
+    V = O.unit_lower(qr, 9)
+    T = O.dlarft(V, tau)
+    H = np.eye(40)
+    for i in range(9):            # brute force: H_1 H_2 ... H_k
+        v = V[:, i]
+        H = H @ (np.eye(40) - tau[i] * np.outer(v, v))
+    assert np.abs(np.eye(40) - V @ T @ V.T - H).max() < 1e-14
+    _, t, _ = lp.dgeqrt(9, np.asfortranarray(A))
+    assert np.abs(np.triu(t) - T).max() < 1e-14
+
+
+def test_tau_recon_recovers_lapack_tau():
+    A = O.gaussian(64, 12, 5)
+    qr, tau = O.local_tsqr(A)
+    assert np.abs(O.tau_recon("L", qr, 12, 64) - tau).max() < 1e-15
+
+
+def test_tree_tsqr_closed_form_and_fixture():
+    g = gold("tpqrt_b20.npz")
+    b = 20
+    R, V2, T = O.tree_tsqr(g["R1"], g["R2"], b)
+    assert np.abs(R - g["R"]).max() < 1e-14 and np.abs(V2 - g["V2"]).max() < 1e-14
+    # Closed form: Q = I − [I;V2]·T_full·[I;V2]ᵀ is orthogonal and Qᵀ[R1;R2] = [R;0]
+    Vf = np.vstack([np.eye(b), V2])
+    tau = np.array([T[i % 16, i] for i in range(b)])
+    Tf = O.dlarft(Vf, tau)
+    Q = np.eye(2 * b) - Vf @ Tf @ Vf.T
+    assert np.abs(Q.T @ Q - np.eye(2 * b)).max() < 1e-14
+    QtA = Q.T @ np.vstack([g["R1"], g["R2"]])
+    assert np.abs(QtA[:b] - R).max() < 1e-13 and np.abs(QtA[b:]).max() < 1e-13
+    # Same R as dgeqrf of the stacked matrix (same dlarfg sign rule)
+    qr, _ = O.local_tsqr(np.vstack([g["R1"], g["R2"]]))
+    assert np.abs(np.triu(qr[:b]) - R).max() < 1e-13
+
+
+@pytest.mark.parametrize("m,b,P", [(136, 17, 4), (8192, 64, 4), (1000, 50, 3), (512, 32, 1), (640, 64, 5)])
+def test_bitree_R_and_construct_Q1(m, b, P):
+    A = O.hh_recon_test_matrix(m, b)
+    ranges = O.split_rows(m, b, P)
+    blocks = [A[s:e].copy(order="F") for s, e in ranges]
+    Ys, R, taus, td = O.bitree_tsqr(blocks, b)
+    qr, _ = O.local_tsqr(A)
+    Rl = np.triu(qr[:b])
+    sg = np.sign(np.diag(R) / np.diag(Rl))
+    assert np.abs(R - sg[:, None] * Rl).max() / np.abs(Rl).max() < 1e-13
+    Q1 = np.vstack(O.construct_Q1(Ys, taus, td, b))
+    assert np.abs(Q1.T @ Q1 - np.eye(b)).max() < 1e-13
+    assert np.abs(Q1 @ R - A).max() / np.abs(A).max() < 1e-13
+    Ql = O.explicit_q(qr)[:, :b]
+    assert np.abs(Q1 - Ql * sg[None, :]).max() < 1e-12
+
+
+def _brute_signed_lu(M):
+    b = M.shape[0]
+    A = [[float(M[i, j]) for j in range(b)] for i in range(b)]
+    s = [0] * b
+    for i in range(b):
+        s[i] = -1 if A[i][i] > 0 else 1
+        A[i][i] -= s[i]
+        for r in range(i + 1, b):
+            A[r][i] /= A[i][i]
+        for r in range(i + 1, b):
+            for c in range(i + 1, b):
+                A[r][c] -= A[r][i] * A[i][c]
+    return np.array(A), np.array(s)
+
+
+def test_signed_nlu_brute_force():
+    Q, _ = np.linalg.qr(O.gaussian(30, 6, 11))
+    A = np.asfortranarray(Q[:6, :6].copy())
+    R = np.triu(O.gaussian(6, 6, 12))
+    R0 = R.copy()
+    signs = np.zeros(6, dtype=np.int64)
+    O.signed_NLU(A, R, signs)
+    Ab, sb = _brute_signed_lu(Q[:6, :6])
+    assert np.array_equal(signs, sb)
+    assert np.abs(A - Ab).max() < 1e-14
+    L = np.tril(A, -1) + np.eye(6)
+    assert np.abs(L @ np.triu(A) - (Q[:6, :6] - np.diag(signs))).max() < 1e-14
+    assert np.array_equal(R, np.where(signs == -1, -1.0, 1.0)[:, None] * R0)
+
+
+def test_recursive_nlu_matches_unblocked():
+    Q, _ = np.linalg.qr(O.gaussian(400, 128, 13))
+    Ar = np.asfortranarray(Q[:128, :128].copy())
+    Au = Ar.copy(order="F")
+    Rr = np.triu(O.gaussian(128, 128, 14))
+    Ru = Rr.copy(order="F")
+    sr = np.zeros(128, dtype=np.int64)
+    su = np.zeros(128, dtype=np.int64)
+    O.recursive_NLU(Ar, Rr, sr)
+    O.signed_NLU(Au, Ru, su)
+    assert np.array_equal(sr, su)
+    assert np.abs(Ar - Au).max() < 1e-12
+    assert np.array_equal(Rr, Ru)
+    L = np.tril(Ar, -1) + np.eye(128)
+    assert np.abs(L @ np.triu(Ar) - (Q[:128, :128] - np.diag(sr))).max() < 1e-13
+
+
+def test_hh_recon_matches_golden_dgeqrf():
+    g = gold("hh_recon_136x17.npz")
+    YR, W, _ = O.hh_recon_qr(g["A"], 17, P=4)
+    assert np.abs(YR - g["YR"]).max() / np.abs(g["YR"]).max() < 1e-13
+    Y = O.unit_lower(YR, 17)
+    T = np.triu(g["T"])
+    assert np.abs(W + T @ Y[:17, :17].T).max() < 1e-13          # W = −T·Y1ᵀ
+    assert O.column_relative_residual(g["A"], YR) < 1e-9           # test_hh_recon.cxx:263
+
+
+def test_hh_recon_c1_summary():
+    g = gold("c1_summary.npz")
+    A = O.hh_recon_test_matrix(8192, 64)
+    YR, W, _ = O.hh_recon_qr(A, 64, P=4)
+    R = np.triu(YR[:64, :64])
+    assert np.abs(np.diag(R) - g["Rdiag"]).max() / np.abs(g["Rdiag"]).max() < 1e-13
+    tau = O.tau_recon("L", YR, 64, 8192)
+    assert np.abs(tau - g["tau"]).max() < 1e-13
+    assert np.abs(np.linalg.norm(np.tril(YR, -1), axis=0) - g["Ycolnorm"]).max() < 1e-12
+
+
+def test_invT_two_closed_forms_agree():
+    A = O.gaussian(700, 40, 17)
+    YR, W, _ = O.hh_recon_qr(A, 40, P=3)
+    Y = O.unit_lower(YR, 40)
+    a = O.compute_invT_from_W(W, Y[:40, :40])
+    b = O.compute_invT_from_Y(Y)
+    assert np.abs(a - b).max() / np.abs(b).max() < 1e-13
+
+
+def test_upd_A_equals_reflector_product():
+    A = O.gaussian(90, 30, 19)
+    YR, W, _ = O.hh_recon_qr(A[:, :10], 10, P=2)
+    Y = O.unit_lower(YR, 10)
+    tau = O.tau_recon("L", YR, 10, 90)
+    C = A[:, 10:].copy()
+    out = O.upd_A(Y, C, O.compute_invT_from_W(W, Y[:10, :10]))
+    ref = C.copy()
+    for i in range(10):          # Qᵀ C = H_b ··· H_1 C, H_1 applied first
+        v = Y[:, i]
+        ref = ref - tau[i] * np.outer(v, v @ ref)
+    assert np.abs(out - ref).max() < 1e-13
+
+
+def leading_rows(YR, tol=1e-8):
+    """Rows of R above the first pivot below tol·max|diag| (rank-revealing cut)."""
+    d = np.abs(np.diag(YR))
+    small = np.nonzero(d < tol * d.max())[0]
+    return int(small[0]) if small.size else d.size
+
+
+@pytest.mark.parametrize("name,b", [("qr2d_8x4.npz", 2), ("qr2d_96x48.npz", 8)])
+def test_qr_2d_reference_inputs(name, b):
+    # test_qr_2d.cxx's generator gives numerically singular matrices: the
+    # reference itself only checks the column-relative residual (:292-297).
+    g = gold(name)
+    YR = O.QR_2D(g["A"], b)
+    assert O.column_relative_residual(g["A"], YR) < 1e-9
+    r = leading_rows(g["YR"])
+    assert r >= 2
+    assert np.abs(np.triu(YR[:r]) - np.triu(g["YR"][:r])).max() < 1e-13
+
+
+def test_qr_2d_matches_golden_gaussian():
+    g = gold("qr2d_gauss_160x96.npz")
+    for b in (8, 16, 32):
+        YR = O.QR_2D(g["A"], b)
+        assert np.abs(YR - g["YR"]).max() / np.abs(g["YR"]).max() < 1e-13
+        assert O.column_relative_residual(g["A"], YR) < 1e-9
+
+
+def test_qr_2d_square_last_sign_quirk():
+    A = O.gaussian(48, 48, 37)
+    YR = O.QR_2D(A, 8)
+    qr, _ = O.local_tsqr(A)
+    d = np.abs(YR - qr)
+    d[47, 47] = abs(YR[47, 47] + qr[47, 47])                        # test_qr_2d.cxx:135-137
+    assert d.max() / np.abs(qr).max() < 1e-12
+    assert abs(YR[47, 47] - qr[47, 47]) > 1e-3 * abs(qr[47, 47])
+
+
+def test_qr_2d_2d_equals_qr_2d():
+    A = O.gaussian(256, 96, 23)
+    a = O.QR_2D(A, 16)
+    b = O.QR_2D_2D(A, 32, 16)
+    assert np.abs(a - b).max() / np.abs(a).max() < 1e-13
+
+
+def test_lstsq_matches_scipy():
+    A = O.gaussian(300, 40, 29)
+    B = O.gaussian(300, 3, 31)
+    x = O.lstsq_qr(A, B, b=16)
+    xs = sl.lstsq(A, B)[0]
+    assert np.abs(x - xs).max() < 1e-12
