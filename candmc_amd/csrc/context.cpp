@@ -1,0 +1,171 @@
+// Context, errors, workspace arena and TSQR planning (host only).
+#include <cstdarg>
+#include <cstring>
+
+#include "cqr_internal.h"
+
+namespace cqr {
+
+int set_err(cqr_ctx* c, int code, const char* fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return code;
+}
+
+int check_launch(cqr_ctx* c, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err(c, CQR_ERR_DEVICE, "%s launch: %s", what, hipGetErrorString(e));
+  return CQR_OK;
+}
+
+int ensure_ws(cqr_ctx* c, size_t bytes) {
+  if (bytes <= c->ws_bytes) return CQR_OK;
+  // Growing replaces the arena: wait for work that may still use it.
+  if (c->ws) {
+    CQR_HIP(c, hipStreamSynchronize(c->stream));
+    CQR_HIP(c, hipFree(c->ws));
+    c->ws = nullptr;
+    c->ws_bytes = 0;
+  }
+  size_t want = bytes + bytes / 8;  // headroom against small regrowths
+  hipError_t e = hipMalloc(&c->ws, want);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    c->ws = nullptr;
+    return set_err(c, CQR_ERR_NOMEM, "workspace of %zu bytes: %s", want, hipGetErrorString(e));
+  }
+  c->ws_bytes = want;
+  return CQR_OK;
+}
+
+int make_plan(int64_t m, int64_t b, int64_t nleaves_hint, int64_t leaf_rows, TsqrPlan& p) {
+  if (m < b || b < 1) return CQR_ERR_INVALID;
+  if (b > CQR_MAX_B) return CQR_ERR_UNSUPPORTED;
+  const int64_t cap = leaf_capacity_rows(b);
+  int64_t rows = leaf_rows > 0 ? std::min(leaf_rows, cap) : cap;
+  rows = std::max(rows, b);
+  int64_t L = (m + rows - 1) / rows;                 // fewest leaves that fit
+  L = std::max<int64_t>(L, std::max<int64_t>(nleaves_hint, 1));
+  L = std::min<int64_t>(L, m / b);                   // every leaf keeps ≥ b rows
+  L = std::max<int64_t>(L, 1);
+  if ((m + L - 1) / L > cap) return CQR_ERR_UNSUPPORTED;
+  p.m = m;
+  p.b = b;
+  p.nleaves = L;
+  p.max_leaf_rows = (m + L - 1) / L;
+  p.leaf_off.resize(L + 1);
+  for (int64_t l = 0; l <= L; ++l) p.leaf_off[l] = (l * m) / L;
+  p.levels.clear();
+  p.level_node0.clear();
+  p.nnodes = 0;
+  for (int64_t n = L; n > 1; n = n / 2 + n % 2) {    // bitree_tsqr.cxx:172
+    std::vector<std::pair<int, int>> lv;
+    for (int64_t r = 0; r < n / 2; ++r) lv.push_back({(int)r, (int)(r + (n + 1) / 2)});
+    p.level_node0.push_back(p.nnodes);
+    p.nnodes += (int64_t)lv.size();
+    p.levels.push_back(std::move(lv));
+  }
+  return CQR_OK;
+}
+
+}  // namespace cqr
+
+using namespace cqr;
+
+extern "C" {
+
+int cqr_create(cqr_ctx** ctx, int device) {
+  if (!ctx) return CQR_ERR_INVALID;
+  *ctx = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+    (void)hipGetLastError();
+    return CQR_ERR_DEVICE;
+  }
+  if (hipSetDevice(device) != hipSuccess) return CQR_ERR_DEVICE;
+  cqr_ctx* c = new cqr_ctx();
+  c->device = device;
+  if (hipMalloc(&c->d_flag, sizeof(int)) != hipSuccess) {
+    delete c;
+    return CQR_ERR_NOMEM;
+  }
+  *ctx = c;
+  return CQR_OK;
+}
+
+int cqr_destroy(cqr_ctx* c) {
+  if (!c) return CQR_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  if (c->ws) (void)hipFree(c->ws);
+  if (c->d_flag) (void)hipFree(c->d_flag);
+  delete c;
+  return CQR_OK;
+}
+
+int cqr_set_stream(cqr_ctx* c, void* s) {
+  if (!c) return CQR_ERR_INVALID;
+  c->stream = static_cast<hipStream_t>(s);
+  return CQR_OK;
+}
+
+void* cqr_get_stream(const cqr_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int cqr_sync(cqr_ctx* c) {
+  if (!c) return CQR_ERR_INVALID;
+  CQR_HIP(c, hipStreamSynchronize(c->stream));
+  return CQR_OK;
+}
+
+int cqr_reserve(cqr_ctx* c, size_t bytes) {
+  if (!c) return CQR_ERR_INVALID;
+  (void)hipSetDevice(c->device);
+  return ensure_ws(c, bytes);
+}
+
+size_t cqr_workspace_bytes(const cqr_ctx* c) { return c ? c->ws_bytes : 0; }
+
+int cqr_set_check_finite(cqr_ctx* c, int on) {
+  if (!c) return CQR_ERR_INVALID;
+  c->check_finite = on ? 1 : 0;
+  return CQR_OK;
+}
+
+int cqr_set_leaf_rows(cqr_ctx* c, int64_t rows) {
+  if (!c || rows < 0) return CQR_ERR_INVALID;
+  c->leaf_rows = rows;
+  return CQR_OK;
+}
+
+const char* cqr_status_string(int s) {
+  switch (s) {
+    case CQR_OK: return "ok";
+    case CQR_ERR_INVALID: return "invalid argument";
+    case CQR_ERR_UNSUPPORTED: return "unsupported shape";
+    case CQR_ERR_NOMEM: return "out of device memory";
+    case CQR_ERR_DEVICE: return "HIP runtime error";
+    case CQR_ERR_NONFINITE: return "non-finite input";
+    default: return "unknown status";
+  }
+}
+
+const char* cqr_last_error(const cqr_ctx* c) { return c ? c->err.c_str() : ""; }
+
+int cqr_plan_tsqr(int64_t m, int64_t b, int64_t nleaves_hint, int64_t leaf_rows, int64_t* nleaves,
+                  int64_t* max_leaf_rows, int64_t* nlevels) {
+  TsqrPlan p;
+  const int s = make_plan(m, b, nleaves_hint, leaf_rows, p);
+  if (s != CQR_OK) return s;
+  if (nleaves) *nleaves = p.nleaves;
+  if (max_leaf_rows) *max_leaf_rows = p.max_leaf_rows;
+  if (nlevels) *nlevels = (int64_t)p.levels.size();
+  return CQR_OK;
+}
+
+}  // extern "C"

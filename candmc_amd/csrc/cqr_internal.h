@@ -1,0 +1,154 @@
+// Internal declarations shared by the host drivers and the HIP kernels of
+// the CA-QR library.  Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/candmc_qr.h"
+
+struct cqr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void* ws = nullptr;           // device workspace arena
+  size_t ws_bytes = 0;
+  int check_finite = 0;
+  int64_t leaf_rows = 0;        // 0 = automatic
+  int* d_flag = nullptr;        // device flag for non-finite scans
+  std::string err;
+};
+
+namespace cqr {
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+int set_err(cqr_ctx* c, int code, const char* fmt, ...);
+#define CQR_HIP(ctx, call)                                                    \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      return ::cqr::set_err(ctx, CQR_ERR_DEVICE, "%s: %s", #call,             \
+                            hipGetErrorString(e_));                           \
+  } while (0)
+#define CQR_TRY(expr)                                                         \
+  do {                                                                        \
+    int s_ = (expr);                                                          \
+    if (s_ != CQR_OK) return s_;                                              \
+  } while (0)
+int check_launch(cqr_ctx* c, const char* what);
+
+// ---------------------------------------------------------------------------
+// Workspace: a bump allocator over the context's arena.  Callers first
+// total their needs with a Sizer, grow the arena once, then carve.
+// ---------------------------------------------------------------------------
+struct Sizer {
+  size_t bytes = 0;
+  size_t add(size_t n) {            // returns offset
+    size_t off = (bytes + 255) & ~size_t(255);
+    bytes = off + n;
+    return off;
+  }
+};
+int ensure_ws(cqr_ctx* c, size_t bytes);
+template <class T>
+inline T* ws_at(cqr_ctx* c, size_t off) {
+  return reinterpret_cast<T*>(static_cast<char*>(c->ws) + off);
+}
+
+// ---------------------------------------------------------------------------
+// TSQR plan: leaves are contiguous row blocks; the tree pairs them with the
+// reference's schedule (alg/QR/tsqr/bitree_tsqr.cxx:172-208).
+// ---------------------------------------------------------------------------
+struct TsqrPlan {
+  int64_t m = 0, b = 0;
+  int64_t nleaves = 0;
+  int64_t max_leaf_rows = 0;
+  std::vector<int64_t> leaf_off;     // nleaves + 1 row offsets
+  // levels in forward order; each level is a list of (receiver, sender)
+  std::vector<std::vector<std::pair<int, int>>> levels;
+  int64_t nnodes = 0;                // total tree nodes
+  std::vector<int64_t> level_node0;  // first node index of each level
+};
+int64_t leaf_capacity_rows(int64_t b);          // largest leaf one CU holds
+int make_plan(int64_t m, int64_t b, int64_t nleaves_hint, int64_t leaf_rows,
+              TsqrPlan& p);
+
+// ---------------------------------------------------------------------------
+// Batched small GEMM addressing: batch index z selects each operand as
+//   BM_STRIDED  p + z·zs          (per-leaf / per-node b×b arrays)
+//   BM_LEAFROWS p + rowoff(z)     (leaf z's rows inside a column-major panel)
+//   BM_SHIFTED  p + (z+zoff)·zs   (the sender of tree pair z)
+// ---------------------------------------------------------------------------
+enum { BM_STRIDED = 0, BM_LEAFROWS = 1, BM_SHIFTED = 2 };
+struct BMat {
+  const double* p;
+  int64_t ld;
+  int64_t zs;
+  int mode;
+};
+struct BgemmArgs {
+  BMat A, B, C;
+  int m, n, k;            // m < 0: rows of leaf z
+  int ta, tb, smA, smB, smC;
+  double alpha, beta;
+  int64_t pm;             // panel rows for BM_LEAFROWS / m < 0
+  int pL;                 // leaves
+  int zoff;               // for BM_SHIFTED
+};
+inline BMat bmat(const double* p, int64_t ld, int64_t zs = 0, int mode = BM_STRIDED) {
+  BMat b; b.p = p; b.ld = ld; b.zs = zs; b.mode = mode; return b;
+}
+
+// Kernel launchers (tsqr_kernels.hip)
+int leaf_qr(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int b, double* R, double* T,
+            double* tau);
+int tree_level(cqr_ctx* c, double* Rleaf, int n, int node0, int b, double* V2, double* T,
+               double* tau);
+int bgemm(cqr_ctx* c, const BgemmArgs& g, int batch, int max_m);
+int hr_core(cqr_ctx* c, const double* Q1top, int64_t ldq, const double* Rroot, double* Atop,
+            int64_t lda, double* W, double* Uinv, double* Winv, double* LinvT, int64_t* signs, int b);
+int init_q1(cqr_ctx* c, double* Y, int64_t ldy, int64_t m, int L, int b, const double* C);
+int set_identity(cqr_ctx* c, double* C, int b);
+int launch_diag(cqr_ctx* c, const double* T, int64_t ldt, double* tau, int b);
+int launch_signed_nlu(cqr_ctx* c, double* A, int64_t lda, double* R, int64_t lda_R,
+                      int64_t* signs, int b);
+int launch_tri_inverse(cqr_ctx* c, const double* X, int64_t ldx, double* Xinv,
+                       int64_t ldi, int b, int upper, int unit);
+int launch_tau_recon(cqr_ctx* c, char s, int64_t ncol, int64_t nrow, int64_t lda,
+                     const double* Y, double* tau);
+int launch_pack(cqr_ctx* c, int op, const double* src, double* dst, int64_t b,
+                int64_t lda, int has_diag);
+int launch_copy_unit_lower(cqr_ctx* c, const double* YR, int64_t ldyr, double* Y,
+                           int64_t ldy, int64_t m, int64_t b);
+int launch_blocked_T(cqr_ctx* c, const double* Tfull, double* Tblk, int b, int nb);
+int launch_finite_scan(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n);
+
+// Structural masks for GEMM operands (stored-matrix coordinates)
+enum { SM_NONE = 0, SM_UNIT_LOWER = 1, SM_UPPER = 2, SM_LOWER = 3, SM_IDENTITY = 4, SM_UNIT_UPPER = 5 };
+
+// FP64 MFMA GEMMs (gemm_f64.hip)
+//   C := alpha·op(A)·op(B) + beta·C, column-major; A's stored top-left square
+//   may carry a structural mask (e.g. the implicit unit-lower Y of a panel).
+int gemm_f64(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha,
+             const double* A, int64_t lda, int smA, const double* B, int64_t ldb,
+             double beta, double* C, int64_t ldc);
+// Same with an explicit split-K scratch (nullptr: no split) and a mask on B.
+int gemm_f64_ws(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha,
+                const double* A, int64_t lda, int smA, const double* B, int64_t ldb, int smB,
+                double beta, double* C, int64_t ldc, double* scratch);
+// Trailing update of one panel: A (m×n) −= Y·Tᵀ·(Yᵀ·A), Y unit lower in its
+// top b×b (stored as [Y\R] in the panel), Tt = Tᵀ (b×b, ld b).
+int wy_update_f64(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit,
+                  const double* Tt, double* A, int64_t lda, int64_t m, int64_t n, int b);
+
+// Drivers (qr_driver.hip)
+int hh_recon_core(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, double* W,
+                  double* T, double* invT_T, double* Tt, int64_t nleaves);
+
+}  // namespace cqr

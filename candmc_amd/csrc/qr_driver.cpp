@@ -1,0 +1,568 @@
+// Host drivers of the CA-QR hot path: the C-ABI entry points declared in
+// include/candmc_qr.h.  Each follows the reference function it replaces;
+// MPI hand-offs become stream-ordered kernel launches over in-HBM buffers.
+#include <cstring>
+
+#include "cqr_internal.h"
+
+namespace cqr {
+
+size_t wy_update_ws_doubles(int64_t m, int64_t n, int b);
+int wy_update_f64_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const double* Tt,
+                     double* A, int64_t lda, int64_t m, int64_t n, int b, double* ws);
+size_t gemm_f64_scratch_doubles(int64_t m, int64_t n, int64_t k);
+int launch_tril_halve(cqr_ctx* c, const double* G, int b, double* T, int64_t ldt);
+int launch_copy2d(cqr_ctx* c, const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m,
+                  int64_t n);
+
+// ---------------------------------------------------------------------------
+// Panel workspace: every buffer TSQR + HR needs for one m×b panel.
+// ---------------------------------------------------------------------------
+struct PanelWS {
+  size_t leafR, leafT, leafTau, nodeV2, nodeT, nodeTau, C, X, Z, M, Y, Uinv, Winv, LinvT, signs,
+      Wtmp, Ttmp;
+};
+
+static PanelWS panel_layout(const TsqrPlan& p, Sizer& s) {
+  const size_t b = p.b, bb = b * b, L = p.nleaves, nn = std::max<int64_t>(p.nnodes, 1);
+  PanelWS w;
+  w.leafR = s.add(8 * L * bb);
+  w.leafT = s.add(8 * L * bb);
+  w.leafTau = s.add(8 * L * b);
+  w.nodeV2 = s.add(8 * nn * bb);
+  w.nodeT = s.add(8 * nn * bb);
+  w.nodeTau = s.add(8 * nn * b);
+  w.C = s.add(8 * L * bb);
+  w.X = s.add(8 * std::max<size_t>(L / 2, 1) * bb);
+  w.Z = s.add(8 * L * bb);
+  w.M = s.add(8 * L * bb);
+  w.Y = s.add(8 * (size_t)p.m * b);
+  w.Uinv = s.add(8 * bb);
+  w.Winv = s.add(8 * bb);
+  w.LinvT = s.add(8 * bb);
+  w.signs = s.add(8 * b);
+  w.Wtmp = s.add(8 * bb);
+  w.Ttmp = s.add(8 * bb);
+  return w;
+}
+
+static BgemmArgs bg(int m, int n, int k, double alpha, double beta) {
+  BgemmArgs g{};
+  g.m = m; g.n = n; g.k = k; g.alpha = alpha; g.beta = beta;
+  return g;
+}
+
+// Reverse tree (construct_tsqr_Q.cxx:147-217): C_0 = I at the root, then for
+// every node from the top level down, [C_r; C_s] = Q_node·[C_r; 0]:
+//   X = T·C_r,  C_s = triu(−V2·X),  C_r = C_r − X.
+static int reverse_tree(cqr_ctx* c, const TsqrPlan& p, const double* nodeV2, const double* nodeT,
+                        double* C, double* X) {
+  const int b = (int)p.b;
+  const int64_t bb = (int64_t)b * b;
+  CQR_TRY(set_identity(c, C, b));
+  for (int lv = (int)p.levels.size() - 1; lv >= 0; --lv) {
+    const int npairs = (int)p.levels[lv].size();
+    const int n = (int)(npairs == 0 ? 1 : p.levels[lv].back().second + 1);
+    const int64_t node0 = p.level_node0[lv];
+    BgemmArgs g = bg(b, b, b, 1.0, 0.0);
+    g.A = bmat(nodeT + node0 * bb, b, bb); g.smA = SM_UPPER;
+    g.B = bmat(C, b, bb);
+    g.C = bmat(X, b, bb);
+    CQR_TRY(bgemm(c, g, npairs, b));
+    g = bg(b, b, b, -1.0, 0.0);
+    g.A = bmat(nodeV2 + node0 * bb, b, bb); g.smA = SM_UPPER;
+    g.B = bmat(X, b, bb);
+    g.C = bmat(C, b, bb, BM_SHIFTED); g.zoff = (n + 1) / 2; g.smC = SM_UPPER;
+    CQR_TRY(bgemm(c, g, npairs, b));
+    g = bg(b, b, b, -1.0, 1.0);
+    g.A = bmat(nullptr, b); g.smA = SM_IDENTITY;
+    g.B = bmat(X, b, bb);
+    g.C = bmat(C, b, bb);
+    CQR_TRY(bgemm(c, g, npairs, b));
+  }
+  return CQR_OK;
+}
+
+// Per leaf: M = T_l·(V_topᵀ·C_l), then Q1_l = [C_l; 0] − V_l·M_l into Q (ld ldq).
+static int leaf_apply(cqr_ctx* c, const TsqrPlan& p, const double* A, int64_t lda, const double* leafT,
+                      const double* C, double* Z, double* M, double* Q, int64_t ldq) {
+  const int b = (int)p.b, L = (int)p.nleaves;
+  const int64_t bb = (int64_t)b * b;
+  BgemmArgs g = bg(b, b, b, 1.0, 0.0);
+  g.A = bmat(A, lda, 0, BM_LEAFROWS); g.ta = 1; g.smA = SM_UNIT_LOWER;
+  g.B = bmat(C, b, bb);
+  g.C = bmat(Z, b, bb);
+  g.pm = p.m; g.pL = L;
+  CQR_TRY(bgemm(c, g, L, b));
+  g = bg(b, b, b, 1.0, 0.0);
+  g.A = bmat(leafT, b, bb); g.smA = SM_UPPER;
+  g.B = bmat(Z, b, bb);
+  g.C = bmat(M, b, bb);
+  CQR_TRY(bgemm(c, g, L, b));
+  CQR_TRY(init_q1(c, Q, ldq, p.m, L, b, C));
+  g = bg(-1, b, b, -1.0, 1.0);
+  g.A = bmat(A, lda, 0, BM_LEAFROWS); g.smA = SM_UNIT_LOWER;
+  g.B = bmat(M, b, bb);
+  g.C = bmat(Q, ldq, 0, BM_LEAFROWS);
+  g.pm = p.m; g.pL = L;
+  return bgemm(c, g, L, (int)p.max_leaf_rows);
+}
+
+size_t panel_ws_bytes(int64_t m, int64_t b, int64_t nleaves, int64_t leaf_rows) {
+  TsqrPlan p;
+  if (make_plan(m, b, nleaves, leaf_rows, p) != CQR_OK) return 0;
+  Sizer s;
+  panel_layout(p, s);
+  return s.bytes;
+}
+
+// TSQR + Householder reconstruction of one panel (hh_recon.cxx:129-321).
+// Outputs: A := [Y\R], W (b×b), and optionally T, T⁻ᵀ and Tᵀ (all ld b).
+int hh_recon_panel(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, double* W, double* T,
+                   double* invT_T, double* Tt, int64_t nleaves, char* base, int64_t* signs_out) {
+  TsqrPlan p;
+  int s = make_plan(m, b, nleaves, c->leaf_rows, p);
+  if (s != CQR_OK) return set_err(c, s, "hh_recon_qr: cannot plan m=%lld b=%lld", (long long)m, (long long)b);
+  Sizer sz;
+  const PanelWS w = panel_layout(p, sz);
+  auto at = [&](size_t off) { return reinterpret_cast<double*>(base + off); };
+  const int bi = (int)b, L = (int)p.nleaves;
+  const int64_t bb = b * b;
+  double* leafR = at(w.leafR);
+  double* leafT = at(w.leafT);
+  double* nodeV2 = at(w.nodeV2);
+  double* nodeT = at(w.nodeT);
+  // K1: leaves (bitree_tsqr.cxx:156)
+  CQR_TRY(leaf_qr(c, A, lda, m, L, bi, leafR, leafT, at(w.leafTau)));
+  // K2: tree levels (bitree_tsqr.cxx:172-232)
+  for (size_t lv = 0; lv < p.levels.size(); ++lv) {
+    const int n = p.levels[lv].back().second + 1;
+    CQR_TRY(tree_level(c, leafR, n, (int)p.level_node0[lv], bi, nodeV2, nodeT, at(w.nodeTau)));
+  }
+  // K3: construct_Q1 (construct_tsqr_Q.cxx:34-242) into the Y scratch
+  CQR_TRY(reverse_tree(c, p, nodeV2, nodeT, at(w.C), at(w.X)));
+  double* Y = at(w.Y);
+  CQR_TRY(leaf_apply(c, p, A, lda, leafT, at(w.C), at(w.Z), at(w.M), Y, m));
+  // K4: LU(Q1_top − S), R := S·R, W = U·S (hh_recon.cxx:282-308)
+  double* Wd = W ? W : at(w.Wtmp);
+  int64_t* sg = signs_out ? signs_out : reinterpret_cast<int64_t*>(base + w.signs);
+  CQR_TRY(hr_core(c, Y, m, leafR, A, lda, Wd, at(w.Uinv), at(w.Winv), at(w.LinvT), sg, bi));
+  // Y = Q1·U⁻¹ below the top b rows (hh_recon.cxx:294-301)
+  if (m > b) {
+    BgemmArgs g = bg((int)(m - b), bi, bi, 1.0, 0.0);
+    g.A = bmat(Y + b, m);
+    g.B = bmat(at(w.Uinv), b); g.smB = SM_UPPER;
+    g.C = bmat(A + b, lda);
+    CQR_TRY(bgemm(c, g, 1, (int)(m - b)));
+  }
+  // T = −W·L⁻ᵀ, Tᵀ = −L⁻¹·Wᵀ, T⁻ᵀ = −W⁻ᵀ·L (W = −T·Y1ᵀ)
+  if (Tt) {
+    BgemmArgs g = bg(bi, bi, bi, -1.0, 0.0);
+    g.A = bmat(at(w.LinvT), b); g.ta = 1; g.smA = SM_UNIT_UPPER;
+    g.B = bmat(Wd, b); g.tb = 1; g.smB = SM_UPPER;
+    g.C = bmat(Tt, b);
+    CQR_TRY(bgemm(c, g, 1, bi));
+  }
+  if (T) {
+    BgemmArgs g = bg(bi, bi, bi, -1.0, 0.0);
+    g.A = bmat(Wd, b); g.smA = SM_UPPER;
+    g.B = bmat(at(w.LinvT), b); g.smB = SM_UNIT_UPPER;
+    g.C = bmat(T, b); g.smC = SM_UPPER;
+    CQR_TRY(bgemm(c, g, 1, bi));
+  }
+  if (invT_T) {
+    BgemmArgs g = bg(bi, bi, bi, -1.0, 0.0);
+    g.A = bmat(at(w.Winv), b); g.ta = 1; g.smA = SM_UPPER;
+    g.B = bmat(A, lda); g.smB = SM_UNIT_LOWER;
+    g.C = bmat(invT_T, b);
+    CQR_TRY(bgemm(c, g, 1, bi));
+  }
+  (void)bb;
+  return CQR_OK;
+}
+
+int finite_check(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n) {
+  if (!c->check_finite) return CQR_OK;
+  CQR_HIP(c, hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+  CQR_TRY(launch_finite_scan(c, A, lda, m, n));
+  int h = 0;
+  CQR_HIP(c, hipMemcpyAsync(&h, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  CQR_HIP(c, hipStreamSynchronize(c->stream));
+  if (h) return set_err(c, CQR_ERR_NONFINITE, "input holds NaN or Inf");
+  return CQR_OK;
+}
+
+// Tᵀ from T⁻ᵀ (lower b×b, ld ldi): lower-triangular inverse.
+static int Tt_from_invTT(cqr_ctx* c, const double* invTT, int64_t ldi, double* Tt, int b) {
+  return launch_tri_inverse(c, invTT, ldi, Tt, b, b, 0, 0);
+}
+
+// T⁻ᵀ = −W⁻ᵀ·Y1 (hh_recon.cxx:26-31); Y1 given with its mask.
+static int invTT_from_W(cqr_ctx* c, const double* W, int b, const double* Y1, int64_t ldy, int smY,
+                        double* Winv_scratch, double* out) {
+  CQR_TRY(launch_tri_inverse(c, W, b, Winv_scratch, b, b, 1, 0));
+  BgemmArgs g = bg(b, b, b, -1.0, 0.0);
+  g.A = bmat(Winv_scratch, b); g.ta = 1; g.smA = SM_UPPER;
+  g.B = bmat(Y1, ldy); g.smB = smY;
+  g.C = bmat(out, b);
+  return bgemm(c, g, 1, b);
+}
+
+}  // namespace cqr
+
+using namespace cqr;
+
+#define CQR_CHECK_CTX(c)                     \
+  do {                                       \
+    if (!(c)) return CQR_ERR_INVALID;        \
+    (void)hipSetDevice((c)->device);         \
+  } while (0)
+
+extern "C" {
+
+int64_t cqr_tree_data_doubles(int64_t m, int64_t b, int64_t nleaves) {
+  TsqrPlan p;
+  if (make_plan(m, b, nleaves, 0, p) != CQR_OK) return -1;
+  return p.nleaves * b * b + p.nnodes * (2 * b * b + b);
+}
+
+int cqr_hh_recon_qr_ex(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, double* W,
+                       double* T, double* invT_T, int64_t nleaves) {
+  CQR_CHECK_CTX(c);
+  if (!A || !W || b < 1 || m < b || lda < m || nleaves < 0)
+    return set_err(c, CQR_ERR_INVALID, "hh_recon_qr: bad arguments (m=%lld b=%lld lda=%lld)",
+                   (long long)m, (long long)b, (long long)lda);
+  if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "hh_recon_qr: b=%lld > %d", (long long)b, CQR_MAX_B);
+  CQR_TRY(finite_check(c, A, lda, m, b));
+  const size_t need = panel_ws_bytes(m, b, nleaves, c->leaf_rows);
+  if (!need) return set_err(c, CQR_ERR_UNSUPPORTED, "hh_recon_qr: no leaf plan");
+  CQR_TRY(ensure_ws(c, need));
+  return hh_recon_panel(c, A, lda, m, b, W, T, invT_T, nullptr, nleaves, (char*)c->ws, nullptr);
+}
+
+int cqr_hh_recon_qr(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, double* W, int64_t nleaves) {
+  return cqr_hh_recon_qr_ex(c, A, lda, m, b, W, nullptr, nullptr, nleaves);
+}
+
+int cqr_local_tsqr(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, double* tau, double* T) {
+  CQR_CHECK_CTX(c);
+  if (!A || !tau || b < 1 || m < b || lda < m) return set_err(c, CQR_ERR_INVALID, "local_tsqr: bad arguments");
+  if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "local_tsqr: b > %d", CQR_MAX_B);
+  CQR_TRY(finite_check(c, A, lda, m, b));
+  const int64_t bb = b * b;
+  if (m <= leaf_capacity_rows(b)) {
+    CQR_TRY(ensure_ws(c, 8 * 2 * bb + 256));
+    double* R = static_cast<double*>(c->ws);
+    double* Tw = T ? T : R + bb;
+    return leaf_qr(c, A, lda, m, 1, (int)b, R, Tw, tau);
+  }
+  // Larger blocks: TSQR + HR gives the same [V\R]; tau = diag(T).
+  Sizer s;
+  const size_t pw = s.add(panel_ws_bytes(m, b, 0, c->leaf_rows));
+  const size_t tw = s.add(8 * bb), ww = s.add(8 * bb);
+  CQR_TRY(ensure_ws(c, s.bytes));
+  char* base = (char*)c->ws;
+  double* Tw = T ? T : reinterpret_cast<double*>(base + tw);
+  CQR_TRY(hh_recon_panel(c, A, lda, m, b, reinterpret_cast<double*>(base + ww), Tw, nullptr, nullptr, 0,
+                         base + pw, nullptr));
+  return launch_diag(c, Tw, b, tau, (int)b);
+}
+
+int cqr_tree_tsqr(cqr_ctx* c, double* A2b, int64_t b, double* T) {
+  CQR_CHECK_CTX(c);
+  if (!A2b || b < 1) return set_err(c, CQR_ERR_INVALID, "tree_tsqr: bad arguments");
+  if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "tree_tsqr: b > %d", CQR_MAX_B);
+  const int64_t bb = b * b;
+  CQR_TRY(ensure_ws(c, 8 * (5 * bb + b) + 1024));
+  double* Rl = static_cast<double*>(c->ws);   // [R1 | R2], ld b
+  double* V2 = Rl + 2 * bb;
+  double* Tf = V2 + bb;
+  double* tau = Tf + bb;
+  double* Tb = tau + b;
+  CQR_TRY(launch_copy2d(c, A2b, 2 * b, Rl, b, b, b));
+  CQR_TRY(launch_copy2d(c, A2b + b, 2 * b, Rl + bb, b, b, b));
+  CQR_TRY(tree_level(c, Rl, 2, 0, (int)b, V2, Tf, tau));
+  CQR_TRY(launch_copy2d(c, Rl, b, A2b, 2 * b, b, b));
+  CQR_TRY(launch_copy2d(c, V2, b, A2b + b, 2 * b, b, b));
+  if (T) {
+    const int nb = (int)std::min<int64_t>(b, 16);
+    CQR_TRY(launch_blocked_T(c, Tf, Tb, (int)b, nb));
+    CQR_TRY(launch_copy2d(c, Tb, nb, T, nb, nb, b));
+  }
+  return CQR_OK;
+}
+
+int cqr_bitree_tsqr(cqr_ctx* c, double* A, int64_t lda, double* R, double* tau, int64_t m, int64_t b,
+                    int64_t nleaves, int output_Y, double* tree_data) {
+  CQR_CHECK_CTX(c);
+  (void)output_Y;
+  if (!A || !R || !tau || !tree_data || b < 1 || m < b || lda < m || nleaves < 0)
+    return set_err(c, CQR_ERR_INVALID, "bitree_tsqr: bad arguments");
+  if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "bitree_tsqr: b > %d", CQR_MAX_B);
+  TsqrPlan p;
+  int s = make_plan(m, b, nleaves, 0, p);
+  if (s != CQR_OK) return set_err(c, s, "bitree_tsqr: cannot plan");
+  CQR_TRY(finite_check(c, A, lda, m, b));
+  const int64_t bb = b * b, L = p.nleaves, nn = p.nnodes;
+  CQR_TRY(ensure_ws(c, 8 * (L * bb) + 256));
+  double* leafR = static_cast<double*>(c->ws);
+  double* leafT = tree_data;
+  double* nodeV2 = leafT + L * bb;
+  double* nodeT = nodeV2 + nn * bb;
+  double* nodeTau = nodeT + nn * bb;
+  CQR_TRY(leaf_qr(c, A, lda, m, (int)L, (int)b, leafR, leafT, tau));
+  for (size_t lv = 0; lv < p.levels.size(); ++lv) {
+    const int n = p.levels[lv].back().second + 1;
+    CQR_TRY(tree_level(c, leafR, n, (int)p.level_node0[lv], (int)b, nodeV2, nodeT, nodeTau));
+  }
+  return launch_copy2d(c, leafR, b, R, b, b, b);
+}
+
+int cqr_construct_Q1(cqr_ctx* c, const double* Y, int64_t lda_Y, const double* tau, double* Q1,
+                     int64_t lda_Q, int64_t m, int64_t b, int64_t k, int64_t nleaves,
+                     const double* tree_data) {
+  CQR_CHECK_CTX(c);
+  (void)tau;
+  if (!Y || !Q1 || !tree_data || b < 1 || m < b || lda_Y < m || lda_Q < m)
+    return set_err(c, CQR_ERR_INVALID, "construct_Q1: bad arguments");
+  if (k != b) return set_err(c, CQR_ERR_UNSUPPORTED, "construct_Q1: only k == b is supported");
+  TsqrPlan p;
+  int s = make_plan(m, b, nleaves, 0, p);
+  if (s != CQR_OK) return set_err(c, s, "construct_Q1: cannot plan");
+  const int64_t bb = b * b, L = p.nleaves, nn = p.nnodes;
+  Sizer sz;
+  const size_t oC = sz.add(8 * L * bb), oX = sz.add(8 * std::max<int64_t>(L / 2, 1) * bb),
+               oZ = sz.add(8 * L * bb), oM = sz.add(8 * L * bb);
+  CQR_TRY(ensure_ws(c, sz.bytes));
+  char* base = (char*)c->ws;
+  auto at = [&](size_t o) { return reinterpret_cast<double*>(base + o); };
+  const double* leafT = tree_data;
+  const double* nodeV2 = leafT + L * bb;
+  const double* nodeT = nodeV2 + nn * bb;
+  CQR_TRY(reverse_tree(c, p, nodeV2, nodeT, at(oC), at(oX)));
+  return leaf_apply(c, p, Y, lda_Y, leafT, at(oC), at(oZ), at(oM), Q1, lda_Q);
+}
+
+int cqr_signed_nlu(cqr_ctx* c, double* A, int64_t lda, double* R, int64_t lda_R, int64_t b, int64_t* signs) {
+  CQR_CHECK_CTX(c);
+  if (!A || b < 1 || lda < b || (R && lda_R < b)) return set_err(c, CQR_ERR_INVALID, "signed_NLU: bad arguments");
+  if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "signed_NLU: b > %d", CQR_MAX_B);
+  return launch_signed_nlu(c, A, lda, R, lda_R, signs, (int)b);
+}
+
+int cqr_recursive_nlu(cqr_ctx* c, double* A, int64_t lda, double* R, int64_t lda_R, int64_t b, int64_t* signs) {
+  // The recursion of hh_recon.cxx:80-111 is a cache blocking of the same
+  // factorization; one workgroup holds the whole block in LDS instead.
+  return cqr_signed_nlu(c, A, lda, R, lda_R, b, signs);
+}
+
+int cqr_compute_invT_from_W(cqr_ctx* c, const double* W, int64_t b, double* invT) {
+  CQR_CHECK_CTX(c);
+  if (!W || !invT || b < 1) return set_err(c, CQR_ERR_INVALID, "compute_invT_from_W: bad arguments");
+  if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "compute_invT_from_W: b > %d", CQR_MAX_B);
+  const int64_t bb = b * b;
+  CQR_TRY(ensure_ws(c, 8 * 2 * bb + 256));
+  double* Winv = static_cast<double*>(c->ws);
+  double* out = Winv + bb;
+  CQR_TRY(invTT_from_W(c, W, (int)b, invT, b, SM_NONE, Winv, out));
+  return launch_copy2d(c, out, b, invT, b, b, b);
+}
+
+int cqr_compute_invT_from_Y(cqr_ctx* c, const double* Y, int64_t lda_Y, double* T, int64_t lda_T, int64_t mb,
+                            int64_t b) {
+  CQR_CHECK_CTX(c);
+  if (!Y || !T || b < 1 || mb < 0 || lda_Y < mb || lda_T < b)
+    return set_err(c, CQR_ERR_INVALID, "compute_invT_from_Y: bad arguments");
+  const int64_t bb = b * b;
+  const size_t sd = gemm_f64_scratch_doubles(b, b, mb);
+  Sizer s;
+  const size_t og = s.add(8 * bb), op = s.add(8 * (sd + 1));
+  CQR_TRY(ensure_ws(c, s.bytes));
+  char* base = (char*)c->ws;
+  double* G = reinterpret_cast<double*>(base + og);
+  double* part = reinterpret_cast<double*>(base + op);
+  if (mb > 0) {
+    CQR_TRY(gemm_f64_ws(c, 1, 0, b, b, mb, 1.0, Y, lda_Y, SM_NONE, Y, lda_Y, SM_NONE, 0.0, G, b, part));
+  } else {
+    CQR_HIP(c, hipMemsetAsync(G, 0, 8 * bb, c->stream));
+  }
+  return launch_tril_halve(c, G, (int)b, T, lda_T);
+}
+
+static int upd_core(cqr_ctx* c, const double* Y, int64_t lda_Y, int implicitY, double* A, int64_t lda_A,
+                    int64_t mb, int64_t kb, int64_t b, const double* W, int W_is_T) {
+  const int64_t bb = b * b;
+  const size_t sd = std::max(gemm_f64_scratch_doubles(b, b, mb), wy_update_ws_doubles(mb, kb, (int)b));
+  Sizer s;
+  const size_t oG = s.add(8 * bb), oI = s.add(8 * bb), oT = s.add(8 * bb), oW = s.add(8 * bb),
+               oS = s.add(8 * (sd + 64));
+  CQR_TRY(ensure_ws(c, s.bytes));
+  char* base = (char*)c->ws;
+  auto at = [&](size_t o) { return reinterpret_cast<double*>(base + o); };
+  const int bi = (int)b;
+  const int smY = implicitY ? SM_UNIT_LOWER : SM_NONE;
+  const double* invTT;
+  if (!W) {                                            // compute_invT_from_Y (qr_2d.cxx:242-246)
+    CQR_TRY(gemm_f64_ws(c, 1, 0, b, b, mb, 1.0, Y, lda_Y, smY, Y, lda_Y, smY, 0.0, at(oG), b, at(oS)));
+    CQR_TRY(launch_tril_halve(c, at(oG), bi, at(oI), b));
+    invTT = at(oI);
+  } else if (W_is_T) {                                 // W already holds T⁻ᵀ (qr_2d.cxx:248)
+    invTT = W;
+  } else {                                             // comp_bcast_T_from_W (qr_2d.cxx:179-208)
+    CQR_TRY(invTT_from_W(c, W, bi, Y, lda_Y, smY, at(oW), at(oI)));
+    invTT = at(oI);
+  }
+  CQR_TRY(Tt_from_invTT(c, invTT, b, at(oT), bi));
+  return wy_update_f64_ws(c, Y, lda_Y, implicitY, at(oT), A, lda_A, mb, kb, bi, at(oS));
+}
+
+int cqr_upd_A(cqr_ctx* c, const double* Y, int64_t lda_Y, double* A, int64_t lda_A, int64_t mb, int64_t kb,
+              int64_t b, const double* W, int W_is_T) {
+  CQR_CHECK_CTX(c);
+  if (!Y || !A || b < 1 || mb < 0 || kb < 0 || lda_Y < mb || lda_A < mb || mb < b)
+    return set_err(c, CQR_ERR_INVALID, "upd_A: bad arguments");
+  if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "upd_A: b > %d", CQR_MAX_B);
+  if (kb == 0) return CQR_OK;
+  return upd_core(c, Y, lda_Y, 0, A, lda_A, mb, kb, b, W, W_is_T);
+}
+
+int cqr_update_A(cqr_ctx* c, const double* Y, int64_t lda_Y, double* A, int64_t lda_A, int64_t m, int64_t k,
+                 int64_t b, const double* W, double* aggreg_Y, int64_t lda_aY, int W_is_T) {
+  CQR_CHECK_CTX(c);
+  if (!Y || !A || b < 1 || m < b || k < 0 || lda_Y < m || lda_A < m || (aggreg_Y && lda_aY < m))
+    return set_err(c, CQR_ERR_INVALID, "update_A: bad arguments");
+  if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "update_A: b > %d", CQR_MAX_B);
+  if (k > 0) CQR_TRY(upd_core(c, Y, lda_Y, 1, A, lda_A, m, k, b, W, W_is_T));
+  if (aggreg_Y) CQR_TRY(launch_copy_unit_lower(c, Y, lda_Y, aggreg_Y, lda_aY, m, b));
+  return CQR_OK;
+}
+
+// Workspace of QR_2D for an m×k matrix with panel width b: the maximum over
+// all panels (the split-K scratch of the update is not monotone in j).
+static size_t max_update_ws(int64_t m, int64_t k, int64_t b) {
+  size_t mx = 0;
+  for (int64_t j = 0; j < k; j += b) {
+    const int64_t mj = m - j, kj = k - j, bj = std::min(kj, b);
+    if (kj - b > 0 && mj - b > 0) mx = std::max(mx, wy_update_ws_doubles(mj, kj - bj, (int)bj));
+  }
+  return mx;
+}
+
+static size_t max_panel_ws(int64_t m, int64_t k, int64_t b, int64_t leaf_rows) {
+  size_t mx = 0;
+  for (int64_t j = 0; j < k; j += b) mx = std::max(mx, panel_ws_bytes(m - j, std::min(k - j, b), 0, leaf_rows));
+  return mx;
+}
+
+static size_t qr2d_ws(int64_t m, int64_t k, int64_t b, int64_t leaf_rows, size_t* o_panel, size_t* o_upd,
+                      size_t* o_W, size_t* o_Tt) {
+  Sizer s;
+  const int64_t b0 = std::min(k, b);
+  *o_panel = s.add(max_panel_ws(m, k, b, leaf_rows));
+  *o_upd = s.add(8 * (max_update_ws(m, k, b) + 64));
+  *o_W = s.add(8 * b0 * b0);
+  *o_Tt = s.add(8 * b0 * b0);
+  return s.bytes;
+}
+
+static int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b, double* aggreg_Y,
+                    int64_t lda_aY, char* base, size_t o_panel, size_t o_upd, size_t o_W, size_t o_Tt) {
+  double* W = reinterpret_cast<double*>(base + o_W);
+  double* Tt = reinterpret_cast<double*>(base + o_Tt);
+  double* ws_upd = reinterpret_cast<double*>(base + o_upd);
+  for (int64_t j = 0;; j += b) {                         // qr_2d.cxx:299-363
+    const int64_t mj = m - j, kj = k - j, bj = std::min(kj, b);
+    double* P = A + j + j * lda;
+    CQR_TRY(hh_recon_panel(c, P, lda, mj, bj, W, nullptr, nullptr, Tt, 0, base + o_panel, nullptr));
+    if (aggreg_Y)
+      CQR_TRY(launch_copy_unit_lower(c, P, lda, aggreg_Y + j + j * lda_aY, lda_aY, mj, bj));
+    if (kj - b > 0 && mj - b > 0) {
+      CQR_TRY(wy_update_f64_ws(c, P, lda, 1, Tt, P + bj * lda, lda, mj, kj - bj, (int)bj, ws_upd));
+    } else {
+      break;
+    }
+  }
+  return CQR_OK;
+}
+
+int cqr_qr_2d(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b, double* aggreg_Y,
+              int64_t lda_aY) {
+  CQR_CHECK_CTX(c);
+  if (!A || m < 1 || k < 1 || b < 1 || lda < m || m < k || (aggreg_Y && lda_aY < m))
+    return set_err(c, CQR_ERR_INVALID, "QR_2D: bad arguments (needs m >= k, lda >= m)");
+  if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "QR_2D: b > %d", CQR_MAX_B);
+  CQR_TRY(finite_check(c, A, lda, m, k));
+  size_t op, ou, ow, ot;
+  const size_t need = qr2d_ws(m, k, b, c->leaf_rows, &op, &ou, &ow, &ot);
+  CQR_TRY(ensure_ws(c, need));
+  return qr2d_run(c, A, lda, m, k, b, aggreg_Y, lda_aY, (char*)c->ws, op, ou, ow, ot);
+}
+
+int cqr_qr_2d_2d(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b, int64_t b_sub,
+                 double* aggreg_Y, int64_t lda_aY) {
+  CQR_CHECK_CTX(c);
+  if (!A || m < 1 || k < 1 || b < 1 || b_sub < 1 || lda < m || m < k || (b % b_sub) != 0 ||
+      (aggreg_Y && lda_aY < m))
+    return set_err(c, CQR_ERR_INVALID, "QR_2D_2D: bad arguments (needs m >= k, b %% b_sub == 0)");
+  if (b_sub > CQR_MAX_B || b > CQR_MAX_B)
+    return set_err(c, CQR_ERR_UNSUPPORTED, "QR_2D_2D: b > %d (aggregated T is one b×b block)", CQR_MAX_B);
+  CQR_TRY(finite_check(c, A, lda, m, k));
+  size_t op, ou, ow, ot;
+  const size_t q = qr2d_ws(m, std::min(k, b), b_sub, c->leaf_rows, &op, &ou, &ow, &ot);
+  Sizer s;
+  s.bytes = q;
+  const int64_t bb = b * b;
+  const size_t oG = s.add(8 * bb), oI = s.add(8 * bb), oTt = s.add(8 * bb);
+  const size_t oS = s.add(8 * (std::max(gemm_f64_scratch_doubles(b, b, m), max_update_ws(m, k, b)) + 64));
+  CQR_TRY(ensure_ws(c, s.bytes));
+  char* base = (char*)c->ws;
+  auto at = [&](size_t o) { return reinterpret_cast<double*>(base + o); };
+  for (int64_t j = 0;; j += b) {                         // qr_2d.cxx:772-897
+    const int64_t mj = m - j, kj = k - j;
+    double* P = A + j + j * lda;
+    double* aY = aggreg_Y ? aggreg_Y + j + j * lda_aY : nullptr;
+    if (kj - b > 0 && mj - b > 0) {
+      CQR_TRY(qr2d_run(c, P, lda, mj, b, b_sub, aY, lda_aY, base, op, ou, ow, ot));
+      // aggregated T⁻ᵀ = tril(YᵀY), diagonal halved (compute_invT_from_Y)
+      CQR_TRY(gemm_f64_ws(c, 1, 0, b, b, mj, 1.0, P, lda, SM_UNIT_LOWER, P, lda, SM_UNIT_LOWER, 0.0, at(oG), b,
+                          at(oS)));
+      CQR_TRY(launch_tril_halve(c, at(oG), (int)b, at(oI), b));
+      CQR_TRY(Tt_from_invTT(c, at(oI), b, at(oTt), (int)b));
+      CQR_TRY(wy_update_f64_ws(c, P, lda, 1, at(oTt), P + b * lda, lda, mj, kj - b, (int)b, at(oS)));
+    } else {
+      return qr2d_run(c, P, lda, mj, kj, b_sub, aY, lda_aY, base, op, ou, ow, ot);
+    }
+  }
+}
+
+int cqr_tau_recon(cqr_ctx* c, char structure, int64_t ncol, int64_t nrow, int64_t lda, const double* Y,
+                  double* tau) {
+  CQR_CHECK_CTX(c);
+  if (!Y || !tau || ncol < 0 || nrow < 0 || lda < 1) return set_err(c, CQR_ERR_INVALID, "tau_recon: bad arguments");
+  if (structure != 'L' && structure != 'U' && structure != 'R')
+    return set_err(c, CQR_ERR_INVALID, "tau_recon: structure must be 'R', 'L' or 'U'");
+  return launch_tau_recon(c, structure, ncol, nrow, lda, Y, tau);
+}
+
+int cqr_pack_upper(cqr_ctx* c, const double* R, double* P, int64_t b, int64_t lda) {
+  CQR_CHECK_CTX(c);
+  if (!R || !P || b < 0 || lda < b) return CQR_ERR_INVALID;
+  return launch_pack(c, 0, R, P, b, lda, 1);
+}
+int cqr_unpack_upper(cqr_ctx* c, const double* P, double* R, int64_t b, int64_t lda) {
+  CQR_CHECK_CTX(c);
+  if (!R || !P || b < 0 || lda < b) return CQR_ERR_INVALID;
+  return launch_pack(c, 1, P, R, b, lda, 1);
+}
+int cqr_pack_lower(cqr_ctx* c, const double* T, double* P, int64_t b, int64_t lda, int has_diag) {
+  CQR_CHECK_CTX(c);
+  if (!T || !P || b < 0 || lda < b) return CQR_ERR_INVALID;
+  return launch_pack(c, 2, T, P, b, lda, has_diag ? 1 : 0);
+}
+int cqr_unpack_lower(cqr_ctx* c, const double* P, double* T, int64_t b, int64_t lda, int has_diag) {
+  CQR_CHECK_CTX(c);
+  if (!T || !P || b < 0 || lda < b) return CQR_ERR_INVALID;
+  return launch_pack(c, 3, P, T, b, lda, has_diag ? 1 : 0);
+}
+
+}  // extern "C"

@@ -1,0 +1,809 @@
+// Panel kernels of the CA-QR hot path for gfx950 (FP64):
+//   leaf_qr       K1  Householder QR of every leaf row block (dgeqrf semantics)
+//   tree_node     K2  QR of stacked triangles [R_top; R_bot] (dtpqrt l=b)
+//   bgemm         small batched GEMMs for the reverse tree, the per-leaf
+//                 block-reflector products and the Y formation
+//   hr_core       K4  LU without pivoting of Q1_top − S, W, U⁻¹, L⁻ᵀ
+// plus element-wise helpers.  Leaves are contiguous row blocks
+// [floor(l·m/L), floor((l+1)·m/L)); the tree pairs receiver r with sender
+// r + ceil(n/2) on a level of n live ranks (alg/QR/tsqr/bitree_tsqr.cxx:172-208),
+// so every address is computed on the device from scalar arguments.
+#include "cqr_device.h"
+#include "cqr_internal.h"
+
+namespace cqr {
+
+__device__ __forceinline__ int64_t leaf_off(int64_t l, int64_t m, int64_t L) { return (l * m) / L; }
+
+// ---------------------------------------------------------------------------
+// K1: leaf QR.  One workgroup per leaf; the leaf lives in registers: wave w
+// owns columns w, w+WAVES, ... (CPW of them), lane l owns rows l, l+64, ...
+// (RPL of them).  Per column step the owner wave forms the reflector
+// (FP64 wave reduction for the norm, dlarfg semantics), publishes v through
+// LDS, and one barrier later every wave applies it to its columns.  The dot
+// products against the already-finished columns are the Gram entries
+// V(:,j)ᵀv_i that give the compact-WY T at the end.
+// ---------------------------------------------------------------------------
+template <int WAVES, int CPW, int RPL>
+__global__ void __launch_bounds__(64 * WAVES)
+leaf_qr_kernel(double* __restrict__ A, int64_t lda, int64_t m, int L, int b,
+               double* __restrict__ Rout, double* __restrict__ Tout, double* __restrict__ tauout) {
+  constexpr int NT = 64 * WAVES;
+  constexpr int SH = xreduce_shift<CPW>();
+  const int leaf = blockIdx.x;
+  const int64_t r0 = leaf_off(leaf, m, L);
+  const int r = (int)(leaf_off(leaf + 1, m, L) - r0);
+  double* Al = A + r0;
+  double* Rl = Rout + (int64_t)leaf * b * b;
+  double* Tl = Tout + (int64_t)leaf * b * b;
+  double* taul = tauout + (int64_t)leaf * b;
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  extern __shared__ double smem[];
+  double* G = smem;                       // b×b Gram, then T
+  double* vbuf = G + b * b;               // 2 × 64·RPL
+  double* stau = vbuf + 2 * 64 * RPL;     // b
+
+  double a[CPW][RPL];
+#pragma unroll
+  for (int cl = 0; cl < CPW; ++cl) {
+    const int col = w + WAVES * cl;
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+      const int row = lane + 64 * s;
+      a[cl][s] = (col < b && row < r) ? Al[row + col * lda] : 0.0;
+    }
+  }
+  for (int e = tid; e < b * b; e += NT) G[e] = 0.0;
+  __syncthreads();
+
+  for (int i = 0; i < b; ++i) {
+    const int ow = i % WAVES, ocl = i / WAVES;
+    const int plane = i & 63, pslot = i >> 6;
+    double* vb = vbuf + (i & 1) * (64 * RPL);
+    if (w == ow) {
+#pragma unroll
+      for (int cl = 0; cl < CPW; ++cl) {
+        if (cl != ocl) continue;
+        double ss = 0.0, piv = 0.0;
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) {
+          const int row = lane + 64 * s;
+          const double x = a[cl][s];
+          if (row > i) ss += x * x;
+          if (s == pslot) piv = x;
+        }
+        ss = wave_sum(ss);
+        const double alpha = readlane_d(piv, plane);
+        double xnorm;
+        if (ss_needs_rescale(ss)) {
+          double mx = 0.0;
+#pragma unroll
+          for (int s = 0; s < RPL; ++s)
+            if (lane + 64 * s > i) mx = fmax(mx, fabs(a[cl][s]));
+          mx = wave_max(mx);
+          if (mx == 0.0) {
+            xnorm = 0.0;
+          } else {
+            double s2 = 0.0;
+#pragma unroll
+            for (int s = 0; s < RPL; ++s)
+              if (lane + 64 * s > i) { const double t = a[cl][s] / mx; s2 += t * t; }
+            xnorm = mx * sqrt(wave_sum(s2));
+          }
+        } else {
+          xnorm = sqrt(ss);
+        }
+        const Reflector h = dlarfg_scalar(alpha, xnorm);
+        if (h.tau != 0.0) {
+#pragma unroll
+          for (int s = 0; s < RPL; ++s)
+            if (lane + 64 * s > i) {
+              double x = a[cl][s];
+              for (int k = 0; k < h.knt; ++k) x *= 0x1p969;
+              a[cl][s] = x * h.scal;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) {
+          const int row = lane + 64 * s;
+          if (row == i) a[cl][s] = h.beta;
+          vb[row] = row > i ? a[cl][s] : (row == i ? 1.0 : 0.0);
+        }
+        if (lane == 0) stau[i] = h.tau;
+      }
+    }
+    __syncthreads();
+    const double tau = stau[i];
+    double v[RPL];
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) v[s] = vb[lane + 64 * s];
+    double part[CPW];
+#pragma unroll
+    for (int cl = 0; cl < CPW; ++cl) {
+      double acc = 0.0;
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) acc += v[s] * a[cl][s];
+      part[cl] = acc;
+    }
+    const double mine = xreduce<CPW>(part, lane);
+#pragma unroll
+    for (int cl = 0; cl < CPW; ++cl) {
+      const int col = w + WAVES * cl;
+      const double dcl = readlane_d(mine, cl << SH);
+      if (col > i && col < b) {
+        const double f = tau * dcl;
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) a[cl][s] -= f * v[s];
+      } else if (col < i) {
+        if (lane == 0) G[col + i * b] = dcl;
+      }
+    }
+  }
+  __syncthreads();
+  tri_inv_upper_lds<NT>(G, b, stau, b);
+
+#pragma unroll
+  for (int cl = 0; cl < CPW; ++cl) {
+    const int col = w + WAVES * cl;
+    if (col >= b) continue;
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+      const int row = lane + 64 * s;
+      if (row < r) Al[row + col * lda] = a[cl][s];
+      if (row < b) Rl[row + col * b] = row <= col ? a[cl][s] : 0.0;
+    }
+  }
+  for (int e = tid; e < b * b; e += NT) {
+    const int row = e % b, col = e / b;
+    Tl[e] = row <= col ? G[e] : 0.0;
+  }
+  for (int p = tid; p < b; p += NT) taul[p] = stau[p];
+}
+
+// ---------------------------------------------------------------------------
+// K2: one tree level.  Node z stacks R of receiver z on R of sender
+// z + ceil(n/2) and computes its QR like dtpqrt with l = b: the reflector
+// for column i is [e_i; x] with x = R_bot(0:i, i), so R_bot becomes V2
+// (upper triangular) and R_top becomes the new R of the receiver.
+// ---------------------------------------------------------------------------
+template <int WAVES, int CPW, int RPL>
+__global__ void __launch_bounds__(64 * WAVES)
+tree_node_kernel(double* __restrict__ Rleaf, int n, int node0, int b, double* __restrict__ V2out,
+                 double* __restrict__ Tout, double* __restrict__ tauout) {
+  constexpr int NT = 64 * WAVES;
+  constexpr int SH = xreduce_shift<CPW>();
+  const int z = blockIdx.x;
+  const int snd = z + (n + 1) / 2;
+  const int64_t bb = (int64_t)b * b;
+  double* Rt = Rleaf + z * bb;
+  const double* Rb = Rleaf + snd * bb;
+  double* V2 = V2out + (node0 + z) * bb;
+  double* Tn = Tout + (node0 + z) * bb;
+  double* taun = tauout + (int64_t)(node0 + z) * b;
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  extern __shared__ double smem[];
+  double* G = smem;
+  double* vbuf = G + b * b;
+  double* stau = vbuf + 2 * 64 * RPL;
+
+  double a1[CPW][RPL], a2[CPW][RPL];
+#pragma unroll
+  for (int cl = 0; cl < CPW; ++cl) {
+    const int col = w + WAVES * cl;
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+      const int row = lane + 64 * s;
+      const bool in = col < b && row <= col;
+      a1[cl][s] = in ? Rt[row + col * b] : 0.0;
+      a2[cl][s] = in ? Rb[row + col * b] : 0.0;
+    }
+  }
+  for (int e = tid; e < b * b; e += NT) G[e] = 0.0;
+  __syncthreads();
+
+  for (int i = 0; i < b; ++i) {
+    const int ow = i % WAVES, ocl = i / WAVES;
+    const int plane = i & 63, pslot = i >> 6;
+    double* vb = vbuf + (i & 1) * (64 * RPL);
+    if (w == ow) {
+#pragma unroll
+      for (int cl = 0; cl < CPW; ++cl) {
+        if (cl != ocl) continue;
+        double ss = 0.0, piv = 0.0;
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) {
+          const int row = lane + 64 * s;
+          if (row <= i) ss += a2[cl][s] * a2[cl][s];
+          if (s == pslot) piv = a1[cl][s];
+        }
+        ss = wave_sum(ss);
+        const double alpha = readlane_d(piv, plane);
+        double xnorm;
+        if (ss_needs_rescale(ss)) {
+          double mx = 0.0;
+#pragma unroll
+          for (int s = 0; s < RPL; ++s)
+            if (lane + 64 * s <= i) mx = fmax(mx, fabs(a2[cl][s]));
+          mx = wave_max(mx);
+          if (mx == 0.0) {
+            xnorm = 0.0;
+          } else {
+            double s2 = 0.0;
+#pragma unroll
+            for (int s = 0; s < RPL; ++s)
+              if (lane + 64 * s <= i) { const double t = a2[cl][s] / mx; s2 += t * t; }
+            xnorm = mx * sqrt(wave_sum(s2));
+          }
+        } else {
+          xnorm = sqrt(ss);
+        }
+        const Reflector h = dlarfg_scalar(alpha, xnorm);
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) {
+          const int row = lane + 64 * s;
+          if (h.tau != 0.0 && row <= i) {
+            double x = a2[cl][s];
+            for (int k = 0; k < h.knt; ++k) x *= 0x1p969;
+            a2[cl][s] = x * h.scal;
+          }
+          if (row == i) a1[cl][s] = h.beta;
+          vb[row] = row <= i ? a2[cl][s] : 0.0;
+        }
+        if (lane == 0) stau[i] = h.tau;
+      }
+    }
+    __syncthreads();
+    const double tau = stau[i];
+    double v[RPL];
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) v[s] = vb[lane + 64 * s];
+    double part[CPW];
+#pragma unroll
+    for (int cl = 0; cl < CPW; ++cl) {
+      double acc = 0.0, top = 0.0;
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) {
+        acc += v[s] * a2[cl][s];
+        if (s == pslot) top = a1[cl][s];
+      }
+      part[cl] = acc + (lane == plane ? top : 0.0);
+    }
+    const double mine = xreduce<CPW>(part, lane);
+#pragma unroll
+    for (int cl = 0; cl < CPW; ++cl) {
+      const int col = w + WAVES * cl;
+      const double dcl = readlane_d(mine, cl << SH);
+      if (col > i && col < b) {
+        const double f = tau * dcl;
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) {
+          a2[cl][s] -= f * v[s];
+          if (s == pslot && lane == plane) a1[cl][s] -= f;
+        }
+      } else if (col < i) {
+        if (lane == 0) G[col + i * b] = dcl;
+      }
+    }
+  }
+  __syncthreads();
+  tri_inv_upper_lds<NT>(G, b, stau, b);
+#pragma unroll
+  for (int cl = 0; cl < CPW; ++cl) {
+    const int col = w + WAVES * cl;
+    if (col >= b) continue;
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+      const int row = lane + 64 * s;
+      if (row < b) {
+        Rt[row + col * b] = row <= col ? a1[cl][s] : 0.0;
+        V2[row + col * b] = row <= col ? a2[cl][s] : 0.0;
+      }
+    }
+  }
+  for (int e = tid; e < b * b; e += NT) {
+    const int row = e % b, col = e / b;
+    Tn[e] = row <= col ? G[e] : 0.0;
+  }
+  for (int p = tid; p < b; p += NT) taun[p] = stau[p];
+}
+
+// ---------------------------------------------------------------------------
+// Batched small GEMM (FP64 FMA, 64×64 tiles, 256 threads, 4×4 per thread):
+//   C_z := alpha·op(A_z)·op(B_z) + beta·C_z
+// Operands are addressed per batch index z (see BMat), carry structural
+// masks in stored coordinates, and the number of rows may follow the leaf
+// partition of a panel (m < 0).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ const double* bmat_ptr(const BMat& M, int z, const BgemmArgs& g) {
+  switch (M.mode) {
+    case BM_STRIDED: return M.p + (int64_t)z * M.zs;
+    case BM_LEAFROWS: return M.p + leaf_off(z, g.pm, g.pL);
+    default: return M.p + (int64_t)(z + g.zoff) * M.zs;
+  }
+}
+
+__device__ __forceinline__ double apply_mask(double v, int64_t r, int64_t c, int sm) {
+  switch (sm) {
+    case SM_UNIT_LOWER: return r < c ? 0.0 : (r == c ? 1.0 : v);
+    case SM_UPPER: return r > c ? 0.0 : v;
+    case SM_LOWER: return r < c ? 0.0 : v;
+    case SM_IDENTITY: return r == c ? 1.0 : 0.0;
+    case SM_UNIT_UPPER: return r > c ? 0.0 : (r == c ? 1.0 : v);
+    default: return v;
+  }
+}
+
+__global__ void __launch_bounds__(256) bgemm_kernel(BgemmArgs g) {
+  const int z = blockIdx.z;
+  int m = g.m;
+  if (m < 0) m = (int)(leaf_off(z + 1, g.pm, g.pL) - leaf_off(z, g.pm, g.pL));
+  const int n = g.n, k = g.k;
+  const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+  if (m0 >= m || n0 >= n) return;
+  const double* A = bmat_ptr(g.A, z, g);
+  const double* B = bmat_ptr(g.B, z, g);
+  double* C = const_cast<double*>(bmat_ptr(g.C, z, g));
+  const int64_t lda = g.A.ld, ldb = g.B.ld, ldc = g.C.ld;
+
+  __shared__ double As[16][65];
+  __shared__ double Bs[16][65];
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  double acc[4][4] = {};
+  for (int k0 = 0; k0 < k; k0 += 16) {
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int e = tid + 256 * rr;
+      int mm, kk;
+      if (!g.ta) { mm = e & 63; kk = e >> 6; } else { kk = e & 15; mm = e >> 4; }
+      const int gm = m0 + mm, gk = k0 + kk;
+      double v = 0.0;
+      if (gm < m && gk < k) {
+        const int64_t sr = g.ta ? gk : gm, sc = g.ta ? gm : gk;
+        v = (g.smA == SM_IDENTITY) ? 0.0 : A[sr + sc * lda];
+        v = apply_mask(v, sr, sc, g.smA);
+      }
+      As[kk][mm] = v;
+      int nn;
+      if (!g.tb) { kk = e & 15; nn = e >> 4; } else { nn = e & 63; kk = e >> 6; }
+      const int gn = n0 + nn, gk2 = k0 + kk;
+      double u = 0.0;
+      if (gn < n && gk2 < k) {
+        const int64_t sr = g.tb ? gn : gk2, sc = g.tb ? gk2 : gn;
+        u = (g.smB == SM_IDENTITY) ? 0.0 : B[sr + sc * ldb];
+        u = apply_mask(u, sr, sc, g.smB);
+      }
+      Bs[kk][nn] = u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) av[i] = As[kk][ty + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gm = m0 + ty + 16 * i;
+    if (gm >= m) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gn = n0 + tx + 16 * j;
+      if (gn >= n) continue;
+      double* cp = C + gm + (int64_t)gn * ldc;
+      double v = g.alpha * acc[i][j];
+      if (g.beta != 0.0) v += g.beta * *cp;
+      if (g.smC == SM_UPPER && gm > gn) v = 0.0;
+      *cp = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Signed LU of a b×b block in LDS (alg/QR/hh_recon/hh_recon.cxx:43-70):
+// s_i = −1 if the pivot is positive else +1, pivot −= s_i, column scaled by
+// the pivot (division), rank-1 update of the trailing block.
+// ---------------------------------------------------------------------------
+template <int NT>
+__device__ void signed_lu_lds(double* M, int b, double* sgn) {
+  const int tid = threadIdx.x;
+  for (int i = 0; i < b; ++i) {
+    const double s = M[i + i * b] > 0.0 ? -1.0 : 1.0;
+    const double piv = M[i + i * b] - s;
+    __syncthreads();
+    if (tid == 0) {
+      M[i + i * b] = piv;
+      sgn[i] = s;
+    }
+    for (int rr = i + 1 + tid; rr < b; rr += NT) M[rr + i * b] = M[rr + i * b] / piv;
+    __syncthreads();
+    const int nt = b - i - 1;
+    for (int e = tid; e < nt * nt; e += NT) {
+      const int rr = i + 1 + e % nt, cc = i + 1 + e / nt;
+      M[rr + cc * b] -= M[rr + i * b] * M[i + cc * b];
+    }
+    __syncthreads();
+  }
+}
+
+// K4: Householder reconstruction core (one workgroup).
+__global__ void __launch_bounds__(512)
+hr_core_kernel(const double* __restrict__ Q1top, int64_t ldq, const double* __restrict__ Rroot,
+               double* __restrict__ Atop, int64_t lda, double* __restrict__ W,
+               double* __restrict__ Uinv, double* __restrict__ Winv, double* __restrict__ LinvT,
+               int64_t* __restrict__ signs, int b) {
+  constexpr int NT = 512;
+  extern __shared__ double smem[];
+  double* M = smem;            // b×b
+  double* sgn = M + b * b;     // b
+  double* d = sgn + b;         // b
+  const int tid = threadIdx.x;
+  for (int e = tid; e < b * b; e += NT) M[e] = Q1top[(e % b) + (int64_t)(e / b) * ldq];
+  __syncthreads();
+  signed_lu_lds<NT>(M, b, sgn);
+  // outputs: L below the diagonal of Atop, S·R on and above it, W = U·S
+  for (int e = tid; e < b * b; e += NT) {
+    const int rr = e % b, cc = e / b;
+    if (rr > cc) Atop[rr + cc * lda] = M[e];
+    else Atop[rr + cc * lda] = sgn[rr] * Rroot[e];
+    W[e] = rr <= cc ? M[e] * sgn[cc] : 0.0;
+  }
+  if (signs)
+    for (int p = tid; p < b; p += NT) signs[p] = (int64_t)sgn[p];
+  for (int p = tid; p < b; p += NT) d[p] = 1.0 / M[p + p * b];
+  __syncthreads();
+  tri_inv_upper_lds<NT>(M, b, d, b);
+  for (int e = tid; e < b * b; e += NT) {
+    const int rr = e % b, cc = e / b;
+    const double u = rr <= cc ? M[e] : 0.0;
+    Uinv[e] = u;
+    Winv[e] = sgn[rr] * u;        // W⁻¹ = S·U⁻¹
+  }
+  __syncthreads();
+  // L⁻ᵀ: load Lᵀ (unit upper) and invert
+  for (int e = tid; e < b * b; e += NT) {
+    const int rr = e % b, cc = e / b;
+    M[e] = rr < cc ? Atop[cc + rr * lda] : 0.0;
+  }
+  for (int p = tid; p < b; p += NT) d[p] = 1.0;
+  __syncthreads();
+  tri_inv_upper_lds<NT>(M, b, d, b);
+  for (int e = tid; e < b * b; e += NT) {
+    const int rr = e % b, cc = e / b;
+    LinvT[e] = rr <= cc ? M[e] : 0.0;
+  }
+}
+
+// a5 standalone: signed LU of a b×b block with R scaling.
+__global__ void __launch_bounds__(512)
+signed_nlu_kernel(double* __restrict__ A, int64_t lda, double* __restrict__ R, int64_t ldr,
+                  int64_t* __restrict__ signs, int b) {
+  constexpr int NT = 512;
+  extern __shared__ double smem[];
+  double* M = smem;
+  double* sgn = M + b * b;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < b * b; e += NT) M[e] = A[(e % b) + (int64_t)(e / b) * lda];
+  __syncthreads();
+  signed_lu_lds<NT>(M, b, sgn);
+  for (int e = tid; e < b * b; e += NT) {
+    const int rr = e % b, cc = e / b;
+    A[rr + cc * lda] = M[e];
+    if (R && rr <= cc && sgn[rr] < 0) R[rr + cc * ldr] = -R[rr + cc * ldr];
+  }
+  if (signs)
+    for (int p = tid; p < b; p += NT) signs[p] = (int64_t)sgn[p];
+}
+
+// Inverse of a triangular b×b matrix (one workgroup).  upper: invert the
+// upper triangle of X; else the lower triangle (through its transpose).
+__global__ void __launch_bounds__(512)
+tri_inverse_kernel(const double* __restrict__ X, int64_t ldx, double* __restrict__ Xi, int64_t ldi,
+                   int b, int upper, int unit) {
+  constexpr int NT = 512;
+  extern __shared__ double smem[];
+  double* M = smem;
+  double* d = M + b * b;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < b * b; e += NT) {
+    const int rr = e % b, cc = e / b;
+    M[e] = rr < cc ? (upper ? X[rr + cc * ldx] : X[cc + rr * ldx]) : 0.0;
+  }
+  for (int p = tid; p < b; p += NT) d[p] = unit ? 1.0 : 1.0 / X[p + p * ldx];
+  __syncthreads();
+  tri_inv_upper_lds<NT>(M, b, d, b);
+  for (int e = tid; e < b * b; e += NT) {
+    const int rr = e % b, cc = e / b;
+    if (upper) Xi[rr + cc * ldi] = rr <= cc ? M[e] : 0.0;
+    else Xi[cc + rr * ldi] = rr <= cc ? M[e] : 0.0;
+  }
+}
+
+// Yscr rows of leaf z := [C_z; 0]
+__global__ void init_q1_kernel(double* __restrict__ Y, int64_t ldy, int64_t m, int L, int b,
+                               const double* __restrict__ C) {
+  const int z = blockIdx.y;
+  const int64_t r0 = leaf_off(z, m, L), r1 = leaf_off(z + 1, m, L);
+  const int64_t rows = r1 - r0;
+  const double* Cz = C + (int64_t)z * b * b;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < rows * b;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t rr = e % rows, cc = e / rows;
+    Y[r0 + rr + cc * ldy] = rr < b ? Cz[rr + cc * b] : 0.0;
+  }
+}
+
+__global__ void set_identity_kernel(double* __restrict__ C, int b) {
+  for (int e = threadIdx.x + blockIdx.x * blockDim.x; e < b * b; e += blockDim.x * gridDim.x)
+    C[e] = (e % b) == (e / b) ? 1.0 : 0.0;
+}
+
+// tau_recon (alg/shared/util.cxx:437-465), one wave per column.
+__global__ void tau_recon_kernel(char s, int64_t ncol, int64_t nrow, int64_t lda,
+                                 const double* __restrict__ Y, double* __restrict__ tau) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = blockIdx.x * (int64_t)(blockDim.x / 64) + (threadIdx.x >> 6);
+  if (i >= ncol) return;
+  int64_t lo, hi;
+  if (s == 'L') { lo = i + 1; hi = nrow; }
+  else if (s == 'U') { lo = 0; hi = i + 1; }
+  else { lo = 0; hi = nrow; }
+  const double* col = Y + i * lda;
+  double mx = 0.0;
+  for (int64_t r = lo + lane; r < hi; r += 64) mx = fmax(mx, fabs(col[r]));
+  mx = wave_max(mx);
+  double nrm = 0.0;
+  if (mx > 0.0) {
+    double ss = 0.0;
+    for (int64_t r = lo + lane; r < hi; r += 64) { const double t = col[r] / mx; ss += t * t; }
+    nrm = mx * sqrt(wave_sum(ss));
+  }
+  if (lane == 0) {
+    const double t = 2.0 / (nrm * nrm + 1.0);
+    tau[i] = t == 2.0 ? 0.0 : t;
+  }
+}
+
+// Packed triangles (alg/shared/util.cxx:322-406).  op: 0 pack_upper,
+// 1 unpack_upper, 2 pack_lower, 3 unpack_lower.
+__global__ void pack_kernel(int op, const double* __restrict__ src, double* __restrict__ dst,
+                            int64_t b, int64_t lda, int has_diag) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= b * b) return;
+  const int64_t rr = e % b, cc = e / b;
+  if (op <= 1) {
+    if (rr > cc) return;
+    const int64_t p = cc * (cc + 1) / 2 + rr;
+    if (op == 0) dst[p] = src[rr + cc * lda];
+    else dst[rr + cc * lda] = src[p];
+  } else {
+    const int64_t first = cc + 1 - has_diag;
+    if (rr < first) return;
+    const int64_t len_before = cc * (b - 1 + has_diag) - cc * (cc - 1) / 2;
+    const int64_t p = len_before + (rr - first);
+    if (op == 2) dst[p] = src[rr + cc * lda];
+    else dst[rr + cc * lda] = src[p];
+  }
+}
+
+// Explicit unit-lower Y (ld ldy) from a panel [Y\R] (qr_2d.cxx:157-162).
+__global__ void copy_unit_lower_kernel(const double* __restrict__ YR, int64_t ldyr,
+                                       double* __restrict__ Y, int64_t ldy, int64_t m, int64_t b) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m * b;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t rr = e % m, cc = e / m;
+    Y[rr + cc * ldy] = rr < cc ? 0.0 : (rr == cc ? 1.0 : YR[rr + cc * ldyr]);
+  }
+}
+
+// Blocked nb×b T (LAPACK dtpqrt/dgeqrt layout) from the full b×b T.
+__global__ void blocked_T_kernel(const double* __restrict__ Tf, double* __restrict__ Tb, int b, int nb) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nb * b) return;
+  const int rr = e % nb, cc = e / nb;
+  const int s = (cc / nb) * nb;
+  Tb[e] = (s + rr <= cc) ? Tf[(s + rr) + cc * b] : 0.0;
+}
+
+__global__ void finite_scan_kernel(const double* __restrict__ A, int64_t lda, int64_t m, int64_t n,
+                                   int* __restrict__ flag) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m * n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const double v = A[(e % m) + (e / m) * lda];
+    if (!isfinite(v)) *flag = 1;
+  }
+}
+
+// Leaf diagonal of T: tau_l(i) = T_l(i,i) (dgeqrf tau of a leaf).
+__global__ void diag_kernel(const double* __restrict__ T, int64_t ldt, double* __restrict__ tau, int b) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < b) tau[p] = T[p + p * ldt];
+}
+
+// C -= X element-wise over z-batched b×b blocks is done with bgemm
+// (SM_IDENTITY); no extra kernel needed.
+
+// T := tril(G) with the diagonal halved (qr_2d.cxx:42-51)
+__global__ void tril_halve_kernel(const double* __restrict__ G, int b, double* __restrict__ T, int64_t ldt) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= b * b) return;
+  const int rr = e % b, cc = e / b;
+  T[rr + cc * ldt] = rr < cc ? 0.0 : (rr == cc ? 0.5 * G[e] : G[e]);
+}
+
+__global__ void copy2d_kernel(const double* __restrict__ S, int64_t lds, double* __restrict__ D, int64_t ldd,
+                              int64_t m, int64_t n) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m * n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t rr = e % m, cc = e / m;
+    D[rr + cc * ldd] = S[rr + cc * lds];
+  }
+}
+
+// ===========================================================================
+// Launchers
+// ===========================================================================
+template <int WAVES, int CPW, int RPL>
+static int leaf_launch(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int b,
+                       double* R, double* T, double* tau) {
+  const size_t lds = sizeof(double) * ((size_t)b * b + 2 * 64 * RPL + 128);
+  auto k = leaf_qr_kernel<WAVES, CPW, RPL>;
+  static bool attr = false;
+  if (!attr) {
+    CQR_HIP(c, hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, 163840));
+    attr = true;
+  }
+  hipLaunchKernelGGL(k, dim3(L), dim3(64 * WAVES), lds, c->stream, A, lda, m, L, b, R, T, tau);
+  return check_launch(c, "leaf_qr");
+}
+
+int leaf_qr(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int b, double* R, double* T,
+            double* tau) {
+  if (b <= 16) return leaf_launch<4, 4, 16>(c, A, lda, m, L, b, R, T, tau);
+  if (b <= 32) return leaf_launch<8, 4, 16>(c, A, lda, m, L, b, R, T, tau);
+  if (b <= 64) return leaf_launch<8, 8, 8>(c, A, lda, m, L, b, R, T, tau);
+  return leaf_launch<8, 16, 4>(c, A, lda, m, L, b, R, T, tau);
+}
+
+int64_t leaf_capacity_rows(int64_t b) {
+  if (b <= 32) return 1024;
+  if (b <= 64) return 512;
+  return 256;
+}
+
+template <int WAVES, int CPW, int RPL>
+static int node_launch(cqr_ctx* c, double* Rleaf, int n, int node0, int b, double* V2, double* T,
+                       double* tau) {
+  const int npairs = n / 2;
+  if (npairs == 0) return CQR_OK;
+  const size_t lds = sizeof(double) * ((size_t)b * b + 2 * 64 * RPL + 128);
+  auto k = tree_node_kernel<WAVES, CPW, RPL>;
+  static bool attr = false;
+  if (!attr) {
+    CQR_HIP(c, hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, 163840));
+    attr = true;
+  }
+  hipLaunchKernelGGL(k, dim3(npairs), dim3(64 * WAVES), lds, c->stream, Rleaf, n, node0, b, V2, T, tau);
+  return check_launch(c, "tree_node");
+}
+
+int tree_level(cqr_ctx* c, double* Rleaf, int n, int node0, int b, double* V2, double* T, double* tau) {
+  if (b <= 16) return node_launch<4, 4, 1>(c, Rleaf, n, node0, b, V2, T, tau);
+  if (b <= 32) return node_launch<8, 4, 1>(c, Rleaf, n, node0, b, V2, T, tau);
+  if (b <= 64) return node_launch<8, 8, 1>(c, Rleaf, n, node0, b, V2, T, tau);
+  return node_launch<8, 16, 2>(c, Rleaf, n, node0, b, V2, T, tau);
+}
+
+int bgemm(cqr_ctx* c, const BgemmArgs& g, int batch, int max_m) {
+  if (batch <= 0 || g.n <= 0 || max_m <= 0) return CQR_OK;
+  dim3 grid((max_m + 63) / 64, (g.n + 63) / 64, batch);
+  hipLaunchKernelGGL(bgemm_kernel, grid, dim3(256), 0, c->stream, g);
+  return check_launch(c, "bgemm");
+}
+
+static int big_lds_attr(cqr_ctx* c, const void* k) {
+  CQR_HIP(c, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 163840));
+  return CQR_OK;
+}
+
+int hr_core(cqr_ctx* c, const double* Q1top, int64_t ldq, const double* Rroot, double* Atop,
+            int64_t lda, double* W, double* Uinv, double* Winv, double* LinvT, int64_t* signs, int b) {
+  static bool attr = false;
+  if (!attr) { CQR_TRY(big_lds_attr(c, (const void*)hr_core_kernel)); attr = true; }
+  const size_t lds = sizeof(double) * ((size_t)b * b + 2 * b);
+  hipLaunchKernelGGL(hr_core_kernel, dim3(1), dim3(512), lds, c->stream, Q1top, ldq, Rroot, Atop,
+                     lda, W, Uinv, Winv, LinvT, signs, b);
+  return check_launch(c, "hr_core");
+}
+
+int launch_signed_nlu(cqr_ctx* c, double* A, int64_t lda, double* R, int64_t lda_R, int64_t* signs, int b) {
+  static bool attr = false;
+  if (!attr) { CQR_TRY(big_lds_attr(c, (const void*)signed_nlu_kernel)); attr = true; }
+  const size_t lds = sizeof(double) * ((size_t)b * b + b);
+  hipLaunchKernelGGL(signed_nlu_kernel, dim3(1), dim3(512), lds, c->stream, A, lda, R, lda_R, signs, b);
+  return check_launch(c, "signed_nlu");
+}
+
+int launch_tri_inverse(cqr_ctx* c, const double* X, int64_t ldx, double* Xi, int64_t ldi, int b,
+                       int upper, int unit) {
+  static bool attr = false;
+  if (!attr) { CQR_TRY(big_lds_attr(c, (const void*)tri_inverse_kernel)); attr = true; }
+  const size_t lds = sizeof(double) * ((size_t)b * b + b);
+  hipLaunchKernelGGL(tri_inverse_kernel, dim3(1), dim3(512), lds, c->stream, X, ldx, Xi, ldi, b, upper, unit);
+  return check_launch(c, "tri_inverse");
+}
+
+int init_q1(cqr_ctx* c, double* Y, int64_t ldy, int64_t m, int L, int b, const double* C) {
+  const int64_t rows = (m + L - 1) / L;
+  const int gx = (int)std::min<int64_t>(64, (rows * b + 255) / 256);
+  hipLaunchKernelGGL(init_q1_kernel, dim3(gx, L), dim3(256), 0, c->stream, Y, ldy, m, L, b, C);
+  return check_launch(c, "init_q1");
+}
+
+int set_identity(cqr_ctx* c, double* C, int b) {
+  hipLaunchKernelGGL(set_identity_kernel, dim3((b * b + 255) / 256), dim3(256), 0, c->stream, C, b);
+  return check_launch(c, "set_identity");
+}
+
+int launch_tau_recon(cqr_ctx* c, char s, int64_t ncol, int64_t nrow, int64_t lda, const double* Y,
+                     double* tau) {
+  if (ncol <= 0) return CQR_OK;
+  hipLaunchKernelGGL(tau_recon_kernel, dim3((ncol + 3) / 4), dim3(256), 0, c->stream, s, ncol, nrow,
+                     lda, Y, tau);
+  return check_launch(c, "tau_recon");
+}
+
+int launch_pack(cqr_ctx* c, int op, const double* src, double* dst, int64_t b, int64_t lda, int has_diag) {
+  if (b <= 0) return CQR_OK;
+  hipLaunchKernelGGL(pack_kernel, dim3((b * b + 255) / 256), dim3(256), 0, c->stream, op, src, dst, b,
+                     lda, has_diag);
+  return check_launch(c, "pack");
+}
+
+int launch_copy_unit_lower(cqr_ctx* c, const double* YR, int64_t ldyr, double* Y, int64_t ldy,
+                           int64_t m, int64_t b) {
+  if (m <= 0 || b <= 0) return CQR_OK;
+  const int gx = (int)std::min<int64_t>(4096, (m * b + 255) / 256);
+  hipLaunchKernelGGL(copy_unit_lower_kernel, dim3(gx), dim3(256), 0, c->stream, YR, ldyr, Y, ldy, m, b);
+  return check_launch(c, "copy_unit_lower");
+}
+
+int launch_blocked_T(cqr_ctx* c, const double* Tf, double* Tb, int b, int nb) {
+  hipLaunchKernelGGL(blocked_T_kernel, dim3((nb * b + 255) / 256), dim3(256), 0, c->stream, Tf, Tb, b, nb);
+  return check_launch(c, "blocked_T");
+}
+
+int launch_finite_scan(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n) {
+  const int gx = (int)std::min<int64_t>(4096, (m * n + 255) / 256);
+  if (gx <= 0) return CQR_OK;
+  hipLaunchKernelGGL(finite_scan_kernel, dim3(gx), dim3(256), 0, c->stream, A, lda, m, n, c->d_flag);
+  return check_launch(c, "finite_scan");
+}
+
+int launch_tril_halve(cqr_ctx* c, const double* G, int b, double* T, int64_t ldt) {
+  hipLaunchKernelGGL(tril_halve_kernel, dim3((b * b + 255) / 256), dim3(256), 0, c->stream, G, b, T, ldt);
+  return check_launch(c, "tril_halve");
+}
+
+int launch_copy2d(cqr_ctx* c, const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m, int64_t n) {
+  if (m <= 0 || n <= 0) return CQR_OK;
+  const int gx = (int)std::min<int64_t>(2048, (m * n + 255) / 256);
+  hipLaunchKernelGGL(copy2d_kernel, dim3(gx), dim3(256), 0, c->stream, src, lds, dst, ldd, m, n);
+  return check_launch(c, "copy2d");
+}
+
+int launch_diag(cqr_ctx* c, const double* T, int64_t ldt, double* tau, int b) {
+  hipLaunchKernelGGL(diag_kernel, dim3((b + 127) / 128), dim3(128), 0, c->stream, T, ldt, tau, b);
+  return check_launch(c, "diag");
+}
+
+}  // namespace cqr

@@ -1,0 +1,254 @@
+"""Python mirror of CANDMC's alg/QR interface over the HIP C-ABI.
+
+Function names, argument order and meaning follow the reference headers
+(alg/QR/tsqr/bitree_tsqr.h, alg/QR/hh_recon/hh_recon.h, alg/QR/qr_2d/qr_2d.h).
+Matrices are torch float64 tensors on the GPU holding column-major data
+with the stated leading dimension (flat or any shape; only the storage is
+used).  The MPI arguments of the reference (myRank, numPes, root, req_id,
+cdt, pv) are accepted; on one GPU the process column collapses onto
+`numPes` virtual leaves of the TSQR tree.  A non-zero status raises
+CQRError (the reference asserts or aborts instead).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import CQRError, load
+
+__all__ = [
+    "Context", "context", "to_gpu", "from_gpu",
+    "local_tsqr", "tree_tsqr", "bitree_tsqr", "construct_Q1", "signed_NLU", "recursive_NLU",
+    "hh_recon_qr", "compute_invT_from_W", "compute_invT_from_Y", "upd_A", "update_A",
+    "QR_2D", "QR_2D_2D", "tau_recon", "pack_upper", "unpack_upper", "pack_lower", "unpack_lower",
+    "tree_data_size", "plan_tsqr",
+]
+
+
+class Context:
+    """One cqr_ctx: device, stream (torch's current stream) and workspace."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        s = self.lib.cqr_create(ctypes.byref(h), device)
+        if s != 0:
+            raise CQRError(s, "cqr_create failed")
+        self.h = h
+        self.device = device
+        self.use_stream(torch.cuda.current_stream(device))
+
+    def use_stream(self, stream):
+        self.stream = stream
+        self.check(self.lib.cqr_set_stream(self.h, ctypes.c_void_p(stream.cuda_stream)))
+
+    def check(self, status):
+        if status != 0:
+            raise CQRError(status, self.lib.cqr_last_error(self.h).decode())
+
+    def sync(self):
+        self.check(self.lib.cqr_sync(self.h))
+
+    def set_leaf_rows(self, rows: int):
+        self.check(self.lib.cqr_set_leaf_rows(self.h, int(rows)))
+
+    def set_check_finite(self, on: bool):
+        self.check(self.lib.cqr_set_check_finite(self.h, 1 if on else 0))
+
+    def workspace_bytes(self) -> int:
+        return int(self.lib.cqr_workspace_bytes(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cqr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_CTX = {}
+
+
+def context(device: int = 0) -> Context:
+    ctx = _CTX.get(device)
+    if ctx is None:
+        ctx = _CTX[device] = Context(device)
+    cur = torch.cuda.current_stream(device)
+    if cur.cuda_stream != ctx.stream.cuda_stream:
+        ctx.use_stream(cur)
+    return ctx
+
+
+def _p(t):
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("expected a torch tensor")
+    if not t.is_cuda:
+        raise TypeError("expected a GPU tensor (there is no CPU path)")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _pd(t):
+    if t is not None and t.dtype != torch.float64:
+        raise TypeError("expected float64")
+    return _p(t)
+
+
+def to_gpu(a, device=0) -> torch.Tensor:
+    """Host array → flat column-major float64 GPU tensor."""
+    a = np.asarray(a, dtype=np.float64)
+    flat = np.ascontiguousarray(a.reshape(-1, order="F")) if a.ndim == 2 else np.ascontiguousarray(a)
+    return torch.from_numpy(flat).to(f"cuda:{device}")
+
+
+def from_gpu(t: torch.Tensor, m: int, n: int, ld: int | None = None) -> np.ndarray:
+    """Flat column-major GPU tensor → m×n host array (Fortran order)."""
+    ld = m if ld is None else ld
+    h = t.detach().reshape(-1)[: ld * n].cpu().numpy()
+    return np.asfortranarray(h.reshape(n, ld).T[:m, :])
+
+
+def plan_tsqr(m, b, nleaves=0, leaf_rows=0):
+    lib = load()
+    L, R, V = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    s = lib.cqr_plan_tsqr(m, b, nleaves, leaf_rows, ctypes.byref(L), ctypes.byref(R), ctypes.byref(V))
+    if s != 0:
+        raise CQRError(s, f"cannot plan m={m} b={b}")
+    return L.value, R.value, V.value
+
+
+def tree_data_size(m, b, numPes=0):
+    return int(load().cqr_tree_data_doubles(m, b, numPes))
+
+
+# --- alg/QR/tsqr ----------------------------------------------------------
+def local_tsqr(A, R, m, b, lda_A, tau=None):
+    """bitree_tsqr.cxx:48-71: dgeqrf of one m×b block.  R is the caller's
+    scratch in the reference; here it receives the compact-WY T (b×b) when
+    given.  Returns tau (b)."""
+    ctx = context()
+    if tau is None:
+        tau = torch.empty(b, dtype=torch.float64, device=A.device)
+    ctx.check(ctx.lib.cqr_local_tsqr(ctx.h, _pd(A), lda_A, m, b, _pd(tau), _pd(R)))
+    return tau
+
+
+def tree_tsqr(A, buf, b, tau=None):
+    """bitree_tsqr.cxx:20-34: dtpqrt of [R1; R2] in a 2b×b buffer; tau
+    receives the nb×b blocked T (nb = min(b, 16)).  buf is unused."""
+    ctx = context()
+    ctx.check(ctx.lib.cqr_tree_tsqr(ctx.h, _pd(A), b, _pd(tau)))
+
+
+def bitree_tsqr(A, lda_A, R, tau, m, b, myRank=0, numPes=1, root=0, req_id=0, cdt=None,
+                output_Y=0, tree_data=None):
+    """bitree_tsqr.cxx:91-248 over numPes virtual leaves (0 = automatic).
+    tau must hold nleaves×b doubles and tree_data tree_data_size(m,b,numPes)."""
+    ctx = context()
+    if tree_data is None:
+        raise CQRError(1, "tree_data is required (size tree_data_size(m, b, numPes))")
+    ctx.check(ctx.lib.cqr_bitree_tsqr(ctx.h, _pd(A), lda_A, _pd(R), _pd(tau), m, b, numPes,
+                                      int(output_Y), _pd(tree_data)))
+
+
+def construct_Q1(Y, lda_Y, tau, Q1, lda_Q, m, b, k, myRank=0, numPes=1, root=0, cdt=None,
+                 tree_data=None, ID=None):
+    """construct_tsqr_Q.cxx:34-242 (k = b, ID = NULL)."""
+    ctx = context()
+    if ID is not None:
+        raise CQRError(2, "ID != NULL is not supported")
+    ctx.check(ctx.lib.cqr_construct_Q1(ctx.h, _pd(Y), lda_Y, _pd(tau), _pd(Q1), lda_Q, m, b, k,
+                                       numPes, _pd(tree_data)))
+
+
+# --- alg/QR/hh_recon -------------------------------------------------------
+def signed_NLU(A, R, b, lda_A, lda_R, signs=None):
+    """hh_recon.cxx:43-70; signs is an int64 GPU tensor or None."""
+    ctx = context()
+    ctx.check(ctx.lib.cqr_signed_nlu(ctx.h, _pd(A), lda_A, _pd(R), lda_R, b, _p(signs)))
+
+
+def recursive_NLU(A, R, b, lda_A, lda_R, signs=None):
+    """hh_recon.cxx:80-111."""
+    ctx = context()
+    ctx.check(ctx.lib.cqr_recursive_nlu(ctx.h, _pd(A), lda_A, _pd(R), lda_R, b, _p(signs)))
+
+
+def hh_recon_qr(A, lda_A, m, b, W, myRank=0, numPes=1, root=0, req_id=0, cdt=None,
+                T=None, invT_T=None):
+    """hh_recon.cxx:129-321: A := [Y\\R], W := −T·Y1ᵀ; numPes virtual leaves."""
+    ctx = context()
+    ctx.check(ctx.lib.cqr_hh_recon_qr_ex(ctx.h, _pd(A), lda_A, m, b, _pd(W), _pd(T), _pd(invT_T),
+                                         numPes))
+
+
+def compute_invT_from_W(W, b, invT):
+    """hh_recon.cxx:26-31: invT holds Y1 on entry, T⁻ᵀ on exit."""
+    ctx = context()
+    ctx.check(ctx.lib.cqr_compute_invT_from_W(ctx.h, _pd(W), b, _pd(invT)))
+
+
+# --- alg/QR/qr_2d ------------------------------------------------------------
+def compute_invT_from_Y(Y, lda_Y, T, lda_T, mb, b, pv=None):
+    """qr_2d.cxx:22-60."""
+    ctx = context()
+    ctx.check(ctx.lib.cqr_compute_invT_from_Y(ctx.h, _pd(Y), lda_Y, _pd(T), lda_T, mb, b))
+
+
+def upd_A(Ybuf, lda_Y, A, lda_A, mb, kb, b, W, pv=None, W_is_T=False):
+    """qr_2d.cxx:224-282 (Ybuf explicit)."""
+    ctx = context()
+    ctx.check(ctx.lib.cqr_upd_A(ctx.h, _pd(Ybuf), lda_Y, _pd(A), lda_A, mb, kb, b, _pd(W),
+                                1 if W_is_T else 0))
+
+
+def update_A(Y, lda_Y, A, lda_A, m, k, b, W, pv=None, aggreg_Y=None, lda_aY=0, W_is_T=False):
+    """qr_2d.cxx:124-177 (Y is the panel [Y\\R])."""
+    ctx = context()
+    ctx.check(ctx.lib.cqr_update_A(ctx.h, _pd(Y), lda_Y, _pd(A), lda_A, m, k, b, _pd(W),
+                                   _pd(aggreg_Y), lda_aY, 1 if W_is_T else 0))
+
+
+def QR_2D(A, lda_A, m, k, b, pv=None, aggreg_Y=None, lda_aY=0):
+    """qr_2d.cxx:299-363 on one GPU."""
+    ctx = context()
+    ctx.check(ctx.lib.cqr_qr_2d(ctx.h, _pd(A), lda_A, m, k, b, _pd(aggreg_Y), lda_aY))
+
+
+def QR_2D_2D(A, lda_A, m, k, b, b_sub, pv=None, aggreg_Y=None, lda_aY=0):
+    """qr_2d.cxx:772-897 on one GPU."""
+    ctx = context()
+    ctx.check(ctx.lib.cqr_qr_2d_2d(ctx.h, _pd(A), lda_A, m, k, b, b_sub, _pd(aggreg_Y), lda_aY))
+
+
+# --- alg/shared/util ---------------------------------------------------------
+def tau_recon(structure, ncol, nrow, lda, Y, tau):
+    ctx = context()
+    ctx.check(ctx.lib.cqr_tau_recon(ctx.h, structure.encode()[:1], ncol, nrow, lda, _pd(Y), _pd(tau)))
+
+
+def pack_upper(R_in, R_out, b, lda_in):
+    ctx = context()
+    ctx.check(ctx.lib.cqr_pack_upper(ctx.h, _pd(R_in), _pd(R_out), b, lda_in))
+
+
+def unpack_upper(R_in, R_out, b, lda_out):
+    ctx = context()
+    ctx.check(ctx.lib.cqr_unpack_upper(ctx.h, _pd(R_in), _pd(R_out), b, lda_out))
+
+
+def pack_lower(R_in, R_out, b, lda_in, has_diag):
+    ctx = context()
+    ctx.check(ctx.lib.cqr_pack_lower(ctx.h, _pd(R_in), _pd(R_out), b, lda_in, int(has_diag)))
+
+
+def unpack_lower(R_in, R_out, b, lda_out, has_diag):
+    ctx = context()
+    ctx.check(ctx.lib.cqr_unpack_lower(ctx.h, _pd(R_in), _pd(R_out), b, lda_out, int(has_diag)))

@@ -1,0 +1,159 @@
+/*
+ * candmc_qr.h — C-ABI of the MI355X communication-avoiding QR hot path.
+ *
+ * Every compute entry point takes device pointers to column-major FP64
+ * matrices with an explicit leading dimension, is asynchronous on the
+ * context's HIP stream, and returns a status code (CQR_OK on success).
+ * The MPI process grid of the reference collapses onto one GPU: the
+ * binary reduction tree runs over "virtual ranks" (leaf row blocks) in
+ * HBM, and every communicator argument disappears.  Each declaration
+ * names the reference function it replaces (paths relative to the
+ * CANDMC source tree).
+ */
+#ifndef CANDMC_QR_H
+#define CANDMC_QR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cqr_ctx cqr_ctx;
+
+enum cqr_status {
+  CQR_OK = 0,
+  CQR_ERR_INVALID = 1,     /* bad shape, leading dimension or pointer       */
+  CQR_ERR_UNSUPPORTED = 2, /* outside the kernels' limits (e.g. b > 128)    */
+  CQR_ERR_NOMEM = 3,       /* workspace could not be allocated              */
+  CQR_ERR_DEVICE = 4,      /* a HIP runtime call or kernel launch failed    */
+  CQR_ERR_NONFINITE = 5    /* input holds NaN/Inf (only with check_finite)  */
+};
+
+/* Largest panel width the FP64 panel kernels accept. */
+#define CQR_MAX_B 128
+
+/* ---- context, stream and workspace (no reference counterpart: the
+ *      reference mallocs scratch inside every call, e.g.
+ *      alg/QR/hh_recon/hh_recon.cxx:147-151) ---------------------------- */
+int cqr_create(cqr_ctx** ctx, int device);
+int cqr_destroy(cqr_ctx* ctx);
+int cqr_set_stream(cqr_ctx* ctx, void* hip_stream);   /* NULL = default stream */
+void* cqr_get_stream(const cqr_ctx* ctx);
+int cqr_sync(cqr_ctx* ctx);
+int cqr_reserve(cqr_ctx* ctx, size_t bytes);          /* pre-grow the arena   */
+size_t cqr_workspace_bytes(const cqr_ctx* ctx);
+int cqr_set_check_finite(cqr_ctx* ctx, int on);       /* scan inputs first    */
+int cqr_set_leaf_rows(cqr_ctx* ctx, int64_t rows);    /* 0 = automatic        */
+const char* cqr_status_string(int status);
+const char* cqr_last_error(const cqr_ctx* ctx);
+
+/* ---- host-only planning (no device needed) ----------------------------- */
+/* Leaf layout the TSQR uses for an m×b panel split over at least
+ * `nleaves_hint` row blocks (0 = automatic): the reference's binary tree
+ * (alg/QR/tsqr/bitree_tsqr.cxx:172) over `*nleaves` virtual ranks.      */
+int cqr_plan_tsqr(int64_t m, int64_t b, int64_t nleaves_hint, int64_t leaf_rows,
+                  int64_t* nleaves, int64_t* max_leaf_rows, int64_t* nlevels);
+/* Doubles needed for `tree_data` of cqr_bitree_tsqr (replaces the
+ * 2b²(ln P + 2) sizing of alg/QR/hh_recon/hh_recon.cxx:151).            */
+int64_t cqr_tree_data_doubles(int64_t m, int64_t b, int64_t nleaves);
+
+/* ---- a1  local_tsqr  (alg/QR/tsqr/bitree_tsqr.cxx:48-71) ---------------
+ * dgeqrf of one m×b block: A := [V\R], tau[b].  T (b×b upper, the compact
+ * WY factor, nullable) is returned too.  Blocks larger than one CU's leaf
+ * capacity are factored by TSQR + Householder reconstruction, which
+ * yields the same [V\R] and tau.                                          */
+int cqr_local_tsqr(cqr_ctx* ctx, double* A, int64_t lda, int64_t m, int64_t b,
+                   double* tau, double* T);
+
+/* ---- a2  tree_tsqr  (alg/QR/tsqr/bitree_tsqr.cxx:20-34) ----------------
+ * QR of two stacked b×b upper triangles held in a 2b×b buffer (ld 2b), as
+ * dtpqrt(b,b,l=b,nb=min(b,16)): top := R, bottom := V2 (upper), T := the
+ * nb×b blocked triangular factor (ld nb, nullable).                       */
+int cqr_tree_tsqr(cqr_ctx* ctx, double* A2b, int64_t b, double* T);
+
+/* ---- a3  bitree_tsqr  (alg/QR/tsqr/bitree_tsqr.cxx:91-248) -------------
+ * TSQR of the m×b panel over `nleaves` virtual ranks (0 = automatic).
+ * R (b×b, zero below the diagonal), tau (nleaves×b: each leaf's dgeqrf
+ * tau), A := leaf [V\R] blocks when output_Y != 0, tree_data := opaque
+ * per-leaf and per-node factors (size from cqr_tree_data_doubles).        */
+int cqr_bitree_tsqr(cqr_ctx* ctx, double* A, int64_t lda, double* R, double* tau,
+                    int64_t m, int64_t b, int64_t nleaves, int output_Y,
+                    double* tree_data);
+
+/* ---- a4  construct_Q1  (alg/QR/tsqr/construct_tsqr_Q.cxx:34-242) -------
+ * Explicit first k = b columns of the TSQR Q from the outputs of
+ * cqr_bitree_tsqr (same m, b, nleaves).                                   */
+int cqr_construct_Q1(cqr_ctx* ctx, const double* Y, int64_t lda_Y, const double* tau,
+                     double* Q1, int64_t lda_Q, int64_t m, int64_t b, int64_t k,
+                     int64_t nleaves, const double* tree_data);
+
+/* ---- a5  signed_NLU / recursive_NLU  (alg/QR/hh_recon/hh_recon.cxx:43-111)
+ * LU without pivoting of the b×b block A − S, S = diag(s) chosen on the fly
+ * (s_i = −1 if the pivot is positive, else +1); R's rows are scaled by S.
+ * signs (device, int64) may be NULL.                                      */
+int cqr_signed_nlu(cqr_ctx* ctx, double* A, int64_t lda, double* R, int64_t lda_R,
+                   int64_t b, int64_t* signs);
+int cqr_recursive_nlu(cqr_ctx* ctx, double* A, int64_t lda, double* R, int64_t lda_R,
+                      int64_t b, int64_t* signs);
+
+/* ---- a6  hh_recon_qr  (alg/QR/hh_recon/hh_recon.cxx:129-321) -----------
+ * TSQR + Householder reconstruction of the m×b panel: A := [Y\R] (Y unit
+ * lower, implicit diagonal), W := U·S = −T·Y1ᵀ (b×b upper).  The _ex form
+ * also returns T (b×b upper, Q = I − Y·T·Yᵀ) and T⁻ᵀ (b×b lower, the form
+ * upd_A consumes); either may be NULL.                                    */
+int cqr_hh_recon_qr(cqr_ctx* ctx, double* A, int64_t lda, int64_t m, int64_t b,
+                    double* W, int64_t nleaves);
+int cqr_hh_recon_qr_ex(cqr_ctx* ctx, double* A, int64_t lda, int64_t m, int64_t b,
+                       double* W, double* T, double* invT_T, int64_t nleaves);
+
+/* ---- a7  compute_invT_from_W  (alg/QR/hh_recon/hh_recon.cxx:26-31) -----
+ * invT holds Y1 (b×b, ld b) on entry and T⁻ᵀ = −W⁻ᵀ·Y1 on exit.           */
+int cqr_compute_invT_from_W(cqr_ctx* ctx, const double* W, int64_t b, double* invT);
+
+/* ---- a8  compute_invT_from_Y  (alg/QR/qr_2d/qr_2d.cxx:22-60) ------------
+ * T (b×b, ld lda_T) := tril(YᵀY) with the diagonal halved, Y explicit mb×b. */
+int cqr_compute_invT_from_Y(cqr_ctx* ctx, const double* Y, int64_t lda_Y, double* T,
+                            int64_t lda_T, int64_t mb, int64_t b);
+
+/* ---- a9  upd_A  (alg/QR/qr_2d/qr_2d.cxx:224-282) ------------------------
+ * A (mb×kb) := A − Y·(T⁻ᵀ)⁻¹·(Yᵀ·A) with Y explicit (mb×b).  W selects T⁻ᵀ
+ * as in the reference: NULL → compute_invT_from_Y(Y); W_is_T → W is T⁻ᵀ;
+ * otherwise W is the hh_recon W and T⁻ᵀ = compute_invT_from_W(W, Y1).     */
+int cqr_upd_A(cqr_ctx* ctx, const double* Y, int64_t lda_Y, double* A, int64_t lda_A,
+              int64_t mb, int64_t kb, int64_t b, const double* W, int W_is_T);
+
+/* ---- a10 update_A  (alg/QR/qr_2d/qr_2d.cxx:124-177) --------------------
+ * Builds the unit-lower Y from the panel [Y\R] (m×b, ld lda_Y) and applies
+ * upd_A to the m×k block A; copies Y to aggreg_Y (ld lda_aY) if non-NULL.  */
+int cqr_update_A(cqr_ctx* ctx, const double* Y, int64_t lda_Y, double* A, int64_t lda_A,
+                 int64_t m, int64_t k, int64_t b, const double* W,
+                 double* aggreg_Y, int64_t lda_aY, int W_is_T);
+
+/* ---- a11 QR_2D  (alg/QR/qr_2d/qr_2d.cxx:299-363) ------------------------
+ * Right-looking blocked QR with TSQR+HR panels of width b: A (m×k, m ≥ k)
+ * := [Y\R] in dgeqrf layout (no tau; cqr_tau_recon rebuilds it).          */
+int cqr_qr_2d(cqr_ctx* ctx, double* A, int64_t lda, int64_t m, int64_t k, int64_t b,
+              double* aggreg_Y, int64_t lda_aY);
+
+/* ---- a12 QR_2D_2D  (alg/QR/qr_2d/qr_2d.cxx:772-897) ---------------------
+ * Two-level blocking: QR_2D with b_sub panels on b-wide blocks, then one
+ * aggregated rank-b update with T⁻ᵀ from compute_invT_from_Y.             */
+int cqr_qr_2d_2d(cqr_ctx* ctx, double* A, int64_t lda, int64_t m, int64_t k, int64_t b,
+                 int64_t b_sub, double* aggreg_Y, int64_t lda_aY);
+
+/* ---- a14 tau_recon and packed triangles (alg/shared/util.cxx:263-465) --- */
+int cqr_tau_recon(cqr_ctx* ctx, char structure, int64_t ncol, int64_t nrow, int64_t lda,
+                  const double* Y, double* tau);
+int cqr_pack_upper(cqr_ctx* ctx, const double* R, double* P, int64_t b, int64_t lda);
+int cqr_unpack_upper(cqr_ctx* ctx, const double* P, double* R, int64_t b, int64_t lda);
+int cqr_pack_lower(cqr_ctx* ctx, const double* T, double* P, int64_t b, int64_t lda,
+                   int has_diag);
+int cqr_unpack_lower(cqr_ctx* ctx, const double* P, double* T, int64_t b, int64_t lda,
+                     int has_diag);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CANDMC_QR_H */

@@ -213,9 +213,14 @@ def rank_rows(m, b, P, rank, root=0):
 
 
 def split_rows(m, b, P):
-    """Contiguous per-rank row ranges with root 0 (test_hh_recon layout)."""
+    """Contiguous per-rank row ranges with root 0 (test_hh_recon layout).
+
+    The reference distributes whole b-row blocks (bitree_tsqr.cxx:110-111),
+    so with P > 1 it silently drops m % b trailing rows; refuse that case.
+    """
     if P == 1:
         return [(0, m)]
+    assert m % b == 0, "the reference's rank layout needs m % b == 0 when P > 1"
     out, o = [], 0
     for r in range(P):
         mb = rank_rows(m, b, P, r)

@@ -1,0 +1,71 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol the
+header declares, and its host-only planning matches the reference's tree
+schedule (alg/QR/tsqr/bitree_tsqr.cxx:172-208).  No compute call is made."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from candmc_amd import _lib
+from oracle import candmc_oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    src = open(os.path.join(ROOT, "include", "candmc_qr.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(cqr_[A-Za-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    syms = header_symbols()
+    assert len(syms) >= 30
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(_lib.SIGNATURES)
+
+
+def test_status_strings():
+    lib = _lib.load()
+    assert lib.cqr_status_string(0) == b"ok"
+    assert lib.cqr_status_string(2) == b"unsupported shape"
+
+
+def _plan(m, b, hint=0, rows=0):
+    lib = _lib.load()
+    L, R, V = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    s = lib.cqr_plan_tsqr(m, b, hint, rows, ctypes.byref(L), ctypes.byref(R), ctypes.byref(V))
+    return s, L.value, R.value, V.value
+
+
+@pytest.mark.parametrize("m,b,hint", [(8192, 64, 4), (32768, 128, 0), (136, 17, 4), (1000, 50, 3),
+                                      (16384, 16, 0), (300, 128, 0), (128, 128, 0)])
+def test_plan_matches_reference_tree(m, b, hint):
+    s, L, rows, levels = _plan(m, b, hint)
+    assert s == 0
+    assert L >= max(hint, 1) and L <= m // b
+    assert rows == -(-m // L)
+    assert levels == len(O.bitree_levels(L))
+    # every leaf keeps at least b rows
+    assert m // L >= b
+
+
+def test_plan_rejects_bad_shapes():
+    assert _plan(10, 20)[0] == 1        # m < b
+    assert _plan(1000, 129)[0] == 2     # b > 128
+
+
+def test_tree_data_size():
+    lib = _lib.load()
+    s, L, _, _ = _plan(8192, 64, 4)
+    nodes = sum(len(O.bitree_pairs(n)) for n in O.bitree_levels(L))
+    assert lib.cqr_tree_data_doubles(8192, 64, 4) == L * 64 * 64 + nodes * (2 * 64 * 64 + 64)
+
+
+def test_compute_entry_points_reject_null_context():
+    lib = _lib.load()
+    assert lib.cqr_hh_recon_qr(None, None, 1, 1, 1, None, 0) == 1
+    assert lib.cqr_qr_2d(None, None, 1, 1, 1, 1, None, 0) == 1

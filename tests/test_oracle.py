@@ -175,7 +175,7 @@ def test_hh_recon_c1_summary():
 
 
 def test_invT_two_closed_forms_agree():
-    A = O.gaussian(700, 40, 17)
+    A = O.gaussian(720, 40, 17)
     YR, W, _ = O.hh_recon_qr(A, 40, P=3)
     Y = O.unit_lower(YR, 40)
     a = O.compute_invT_from_W(W, Y[:40, :40])
