@@ -29,6 +29,9 @@ SIGNATURES = {
     "cqr_set_check_finite": (_I, [_P, _I]),
     "cqr_set_leaf_rows": (_I, [_P, _I64]),
     "cqr_status_string": (_c.c_char_p, [_I]),
+    "cqr_profile_enable": (_I, [_P, _I]),
+    "cqr_profile_read": (_I, [_P, _I, _c.POINTER(_D), _c.POINTER(_D), _c.POINTER(_D), _c.POINTER(_I64)]),
+    "cqr_profile_reset": (_I, [_P]),
     "cqr_last_error": (_c.c_char_p, [_P]),
     "cqr_plan_tsqr": (_I, [_I64, _I64, _I64, _I64, _c.POINTER(_I64), _c.POINTER(_I64), _c.POINTER(_I64)]),
     "cqr_tree_data_doubles": (_I64, [_I64, _I64, _I64]),

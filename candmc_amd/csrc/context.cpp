@@ -24,6 +24,55 @@ int check_launch(cqr_ctx* c, const char* what) {
   return CQR_OK;
 }
 
+static hipEvent_t pool_get(cqr_ctx* c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return e;
+}
+
+hipEvent_t prof_begin(cqr_ctx* c) {
+  if (!c->prof) return nullptr;
+  hipEvent_t a = pool_get(c);
+  if (a) (void)hipEventRecord(a, c->stream);
+  return a;
+}
+
+void prof_end(cqr_ctx* c, hipEvent_t a, int kind, double flops, double bytes) {
+  if (!c->prof || !a) return;
+  hipEvent_t b = pool_get(c);
+  if (!b) return;
+  (void)hipEventRecord(b, c->stream);
+  c->recs.push_back({a, b, kind, flops, bytes});
+}
+
+static int prof_drain(cqr_ctx* c) {
+  if (c->recs.empty()) return CQR_OK;
+  CQR_HIP(c, hipStreamSynchronize(c->stream));
+  for (const ProfRec& r : c->recs) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      c->acc_ms[r.kind] += ms;
+      c->acc_flops[r.kind] += r.flops;
+      c->acc_bytes[r.kind] += r.bytes;
+      c->acc_n[r.kind] += 1;
+    } else {
+      (void)hipGetLastError();
+    }
+    c->pool.push_back(r.a);
+    c->pool.push_back(r.b);
+  }
+  c->recs.clear();
+  return CQR_OK;
+}
+
 int ensure_ws(cqr_ctx* c, size_t bytes) {
   if (bytes <= c->ws_bytes) return CQR_OK;
   // Growing replaces the arena: wait for work that may still use it.
@@ -103,6 +152,11 @@ int cqr_destroy(cqr_ctx* c) {
   if (!c) return CQR_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  for (const ProfRec& r : c->recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
   if (c->ws) (void)hipFree(c->ws);
   if (c->d_flag) (void)hipFree(c->d_flag);
   delete c;
@@ -140,6 +194,32 @@ int cqr_set_check_finite(cqr_ctx* c, int on) {
 int cqr_set_leaf_rows(cqr_ctx* c, int64_t rows) {
   if (!c || rows < 0) return CQR_ERR_INVALID;
   c->leaf_rows = rows;
+  return CQR_OK;
+}
+
+int cqr_profile_enable(cqr_ctx* c, int on) {
+  if (!c) return CQR_ERR_INVALID;
+  c->prof = on ? 1 : 0;
+  return CQR_OK;
+}
+
+int cqr_profile_read(cqr_ctx* c, int kind, double* ms, double* flops, double* bytes, int64_t* launches) {
+  if (!c || kind < 0 || kind >= PK_NKINDS) return CQR_ERR_INVALID;
+  CQR_TRY(prof_drain(c));
+  if (ms) *ms = c->acc_ms[kind];
+  if (flops) *flops = c->acc_flops[kind];
+  if (bytes) *bytes = c->acc_bytes[kind];
+  if (launches) *launches = c->acc_n[kind];
+  return CQR_OK;
+}
+
+int cqr_profile_reset(cqr_ctx* c) {
+  if (!c) return CQR_ERR_INVALID;
+  CQR_TRY(prof_drain(c));
+  for (int k = 0; k < PK_NKINDS; ++k) {
+    c->acc_ms[k] = c->acc_flops[k] = c->acc_bytes[k] = 0.0;
+    c->acc_n[k] = 0;
+  }
   return CQR_OK;
 }
 

@@ -8,17 +8,9 @@ namespace cqr {
 // ---------------------------------------------------------------------------
 // Wave-level primitives
 // ---------------------------------------------------------------------------
-__device__ inline double wave_sum(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
-}
-
-__device__ inline double wave_max(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
-  return x;
-}
+// Index of the calling wave within its workgroup, as a wave-uniform value
+// (lets the compiler keep per-wave conditions in scalar branches).
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
 // Value of x held by lane `l` (l wave-uniform), through scalar registers.
 __device__ inline double readlane_d(double x, int l) {
@@ -27,29 +19,69 @@ __device__ inline double readlane_d(double x, int l) {
   return __hiloint2double(hi, lo);
 }
 
+// One DPP move of a double (lanes whose source is outside the pattern or
+// whose row is masked off receive 0).
+template <int CTRL, int ROWMASK>
+__device__ inline double dpp_d(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROWMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROWMASK, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Wave-wide sum with DPP (quad_perm, row mirrors, row broadcasts): no LDS
+// traffic; the total lands in lane 63 and is returned to every lane.
+__device__ inline double wave_sum(double x) {
+  x += dpp_d<0xB1, 0xf>(x);   // quad_perm [1,0,3,2]
+  x += dpp_d<0x4E, 0xf>(x);   // quad_perm [2,3,0,1]
+  x += dpp_d<0x141, 0xf>(x);  // row_half_mirror
+  x += dpp_d<0x140, 0xf>(x);  // row_mirror
+  x += dpp_d<0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
+  x += dpp_d<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
+  return readlane_d(x, 63);
+}
+
+__device__ inline double wave_max(double x) {
+  x = fmax(x, dpp_d<0xB1, 0xf>(x));
+  x = fmax(x, dpp_d<0x4E, 0xf>(x));
+  x = fmax(x, dpp_d<0x141, 0xf>(x));
+  x = fmax(x, dpp_d<0x140, 0xf>(x));
+  x = fmax(x, dpp_d<0x142, 0xa>(x));
+  x = fmax(x, dpp_d<0x143, 0xc>(x));
+  return readlane_d(x, 63);
+}
+
 // Reduce N per-lane partial values (N a power of two ≤ 64) across the 64
 // lanes of a wave with log2(64) exchange steps in total: the first log2(N)
 // steps halve the number of values each lane carries (each lane sends the
 // half it does not keep), the rest are plain butterflies.  On return every
 // lane holds the full sum of value index ((lane >> log2(64/N)) & (N-1)).
+template <int C, int O, int N>
+__device__ inline void xreduce_stage(double (&v)[N], int lane) {
+  if constexpr (C > 1) {
+    const bool hi = (lane & O) != 0;
+#pragma unroll
+    for (int t = 0; t < C / 2; ++t) {
+      const double send = hi ? v[t] : v[t + C / 2];
+      const double keep = hi ? v[t + C / 2] : v[t];
+      v[t] = keep + __shfl_xor(send, O, 64);
+    }
+    xreduce_stage<C / 2, O / 2, N>(v, lane);
+  }
+}
+template <int O>
+__device__ inline double xreduce_tail(double s) {
+  if constexpr (O > 0) {
+    s += __shfl_xor(s, O, 64);
+    return xreduce_tail<O / 2>(s);
+  } else {
+    return s;
+  }
+}
 template <int N>
 __device__ inline double xreduce(double (&v)[N], int lane) {
   static_assert(N >= 1 && N <= 64 && (N & (N - 1)) == 0, "N power of two");
-  int o = 32;
-#pragma unroll
-  for (int c = N; c > 1; c >>= 1, o >>= 1) {
-    const bool hi = (lane & o) != 0;
-#pragma unroll
-    for (int t = 0; t < c / 2; ++t) {
-      double send = hi ? v[t] : v[t + c / 2];
-      double keep = hi ? v[t + c / 2] : v[t];
-      v[t] = keep + __shfl_xor(send, o, 64);
-    }
-  }
-  double s = v[0];
-#pragma unroll
-  for (; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  return s;
+  xreduce_stage<N, 32, N>(v, lane);
+  return xreduce_tail<32 / N>(v[0]);
 }
 template <int N>
 __host__ __device__ constexpr int xreduce_shift() {
@@ -124,7 +156,7 @@ __device__ inline bool ss_needs_rescale(double ss) {
 // ---------------------------------------------------------------------------
 template <int NT>
 __device__ void tri_inv_upper_lds(double* M, int ld, const double* d, int b) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
   constexpr int NW = NT / 64;
   const int nblk = (b + 15) / 16;
   for (int blk = wave; blk < nblk; blk += NW) {
@@ -142,12 +174,12 @@ __device__ void tri_inv_upper_lds(double* M, int ld, const double* d, int b) {
     }
   }
   __syncthreads();
-  for (int h = 16; h < b; h *= 2) {
+  for (int h = 16, lh = 4; h < b; h *= 2, ++lh) {
     // pairs of blocks [a, a+h) and [a+h, a+2h)
     // step 1: Xs = N12·X22 stored in the strictly lower mirror block
     for (int e = tid; e < (b / (2 * h) + 1) * h * h; e += NT) {
-      const int pair = e / (h * h), r = e % (h * h);
-      const int p = r % h, j = r / h;
+      const int pair = e >> (2 * lh), r = e & (h * h - 1);
+      const int p = r & (h - 1), j = r >> lh;
       const int a = pair * 2 * h;
       const int h2 = min(h, b - a - h);
       if (a + h >= b || j >= h2) continue;
@@ -158,8 +190,8 @@ __device__ void tri_inv_upper_lds(double* M, int ld, const double* d, int b) {
     __syncthreads();
     // step 2: X12 = −X11·Xs
     for (int e = tid; e < (b / (2 * h) + 1) * h * h; e += NT) {
-      const int pair = e / (h * h), r = e % (h * h);
-      const int p = r % h, j = r / h;
+      const int pair = e >> (2 * lh), r = e & (h * h - 1);
+      const int p = r & (h - 1), j = r >> lh;
       const int a = pair * 2 * h;
       const int h2 = min(h, b - a - h);
       if (a + h >= b || j >= h2) continue;

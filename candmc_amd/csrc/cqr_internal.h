@@ -12,7 +12,21 @@
 
 #include "../../include/candmc_qr.h"
 
+enum { PK_GEMM_YTA = 0, PK_GEMM_UPD = 1, PK_GEMM_SMALL = 2, PK_REDUCE = 3, PK_LEAF = 4,
+       PK_TREE = 5, PK_HR = 6, PK_BGEMM = 7, PK_NKINDS = 8 };
+
+struct ProfRec {
+  hipEvent_t a, b;
+  int kind;
+  double flops, bytes;
+};
+
 struct cqr_ctx {
+  int prof = 0;
+  std::vector<ProfRec> recs;      // pending event pairs
+  std::vector<hipEvent_t> pool;   // free events
+  double acc_ms[PK_NKINDS] = {}, acc_flops[PK_NKINDS] = {}, acc_bytes[PK_NKINDS] = {};
+  int64_t acc_n[PK_NKINDS] = {};
   int device = 0;
   hipStream_t stream = nullptr;
   void* ws = nullptr;           // device workspace arena
@@ -42,6 +56,10 @@ int set_err(cqr_ctx* c, int code, const char* fmt, ...);
     if (s_ != CQR_OK) return s_;                                              \
   } while (0)
 int check_launch(cqr_ctx* c, const char* what);
+
+// Event bracketing of a launch when profiling is on (no-ops otherwise).
+hipEvent_t prof_begin(cqr_ctx* c);
+void prof_end(cqr_ctx* c, hipEvent_t a, int kind, double flops, double bytes);
 
 // ---------------------------------------------------------------------------
 // Workspace: a bump allocator over the context's arena.  Callers first

@@ -39,6 +39,18 @@ __device__ __forceinline__ double gmask(double v, int64_t r, int64_t c, int sm) 
   }
 }
 
+// Does a structural mask change any element of the stored block
+// rows [r_lo, r_hi] × cols [c_lo, c_hi]?
+__device__ __forceinline__ bool mask_active(int sm, int64_t r_lo, int64_t r_hi, int64_t c_lo, int64_t c_hi) {
+  switch (sm) {
+    case SM_UNIT_LOWER: return r_lo <= c_hi;
+    case SM_LOWER: return r_lo < c_hi;
+    case SM_UPPER: return r_hi > c_lo;
+    case SM_UNIT_UPPER: return r_hi >= c_lo;
+    default: return false;
+  }
+}
+
 template <bool TA, bool TB>
 __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
   constexpr int BM = 128, BN = 128, BK = 16, PAD = 4;
@@ -53,6 +65,21 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
   const int64_t kend = min(g.k, kbeg + g.kchunk);
   const double* __restrict__ A = g.A;
   const double* __restrict__ B = g.B;
+  const bool full_mn = (m0 + BM <= g.m) && (n0 + BN <= g.n);
+
+  // Each thread loads 8 elements of the A slab and 8 of the B slab:
+  //   K-contiguous operand: k = k0 + (tid & 15), index = base + (tid >> 4) + 16r
+  //   index-contiguous operand: index = base + (tid & 127), k = k0 + (tid >> 7) + 2r
+  const int ak = TA ? (tid & 15) : (tid >> 7);
+  const int am = TA ? (tid >> 4) : (tid & 127);
+  const int bk = TB ? (tid >> 7) : (tid & 15);
+  const int bn = TB ? (tid & 127) : (tid >> 4);
+  const int64_t a_rstep = TA ? 16 * g.lda : 2 * g.lda;   // between the 8 elements
+  const int64_t b_rstep = TB ? 2 * g.ldb : 16 * g.ldb;
+  const double* pa = TA ? A + (m0 + am) * g.lda + ak : A + (m0 + am) + (int64_t)ak * g.lda;
+  const double* pb = TB ? B + (n0 + bn) + (int64_t)bk * g.ldb : B + bk + (n0 + bn) * g.ldb;
+  const int64_t a_kstep = TA ? 1 : g.lda;                 // per unit of k
+  const int64_t b_kstep = TB ? g.ldb : 1;
 
   d4 acc[4][4];
 #pragma unroll
@@ -62,25 +89,35 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
 
   double ra[8], rb[8];
   auto load = [&](int64_t k0) {
+    const bool kfull = k0 + BK <= kend;
+    const bool mA = TA ? mask_active(g.smA, k0, k0 + BK - 1, m0, m0 + BM - 1)
+                       : mask_active(g.smA, m0, m0 + BM - 1, k0, k0 + BK - 1);
+    const bool mB = TB ? mask_active(g.smB, n0, n0 + BN - 1, k0, k0 + BK - 1)
+                       : mask_active(g.smB, k0, k0 + BK - 1, n0, n0 + BN - 1);
+    const double* qa = pa + k0 * a_kstep;
+    const double* qb = pb + k0 * b_kstep;
+    if (full_mn && kfull && !mA && !mB) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        ra[r] = qa[r * a_rstep];
+        rb[r] = qb[r * b_rstep];
+      }
+      return;
+    }
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-      const int e = tid + 256 * r;
-      int mm, kk;
-      if (!TA) { mm = e % BM; kk = e / BM; } else { kk = e % BK; mm = e / BK; }
-      const int64_t gm = m0 + mm, gk = k0 + kk;
+      const int64_t gm = m0 + am + (TA ? 16 * r : 0), gk = k0 + ak + (TA ? 0 : 2 * r);
       double v = 0.0;
       if (gm < g.m && gk < kend) {
-        const int64_t sr = TA ? gk : gm, sc = TA ? gm : gk;
-        v = gmask(A[sr + sc * g.lda], sr, sc, g.smA);
+        v = qa[r * a_rstep];
+        if (mA) v = gmask(v, TA ? gk : gm, TA ? gm : gk, g.smA);
       }
       ra[r] = v;
-      int nn;
-      if (!TB) { kk = e % BK; nn = e / BK; } else { nn = e % BN; kk = e / BN; }
-      const int64_t gn = n0 + nn, gk2 = k0 + kk;
+      const int64_t gn = n0 + bn + (TB ? 0 : 16 * r), gk2 = k0 + bk + (TB ? 2 * r : 0);
       double u = 0.0;
       if (gn < g.n && gk2 < kend) {
-        const int64_t sr = TB ? gn : gk2, sc = TB ? gk2 : gn;
-        u = gmask(B[sr + sc * g.ldb], sr, sc, g.smB);
+        u = qb[r * b_rstep];
+        if (mB) u = gmask(u, TB ? gn : gk2, TB ? gk2 : gn, g.smB);
       }
       rb[r] = u;
     }
@@ -88,13 +125,8 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
   auto store = [&](int buf) {
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-      const int e = tid + 256 * r;
-      int mm, kk;
-      if (!TA) { mm = e % BM; kk = e / BM; } else { kk = e % BK; mm = e / BK; }
-      As[buf][kk][mm] = ra[r];
-      int nn;
-      if (!TB) { kk = e % BK; nn = e / BK; } else { nn = e % BN; kk = e / BN; }
-      Bs[buf][kk][nn] = rb[r];
+      As[buf][ak + (TA ? 0 : 2 * r)][am + (TA ? 16 * r : 0)] = ra[r];
+      Bs[buf][bk + (TB ? 2 * r : 0)][bn + (TB ? 0 : 16 * r)] = rb[r];
     }
   };
 
@@ -129,22 +161,21 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
   double* C = g.C;
   const bool split = g.split_stride > 0;
   if (split) C += (int64_t)blockIdx.z * g.split_stride;
+  const bool has_beta = !split && g.beta != 0.0;
+  const double alpha = split ? 1.0 : g.alpha;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int64_t gm = m0 + wm0 + 16 * i + fc;
-    if (gm >= g.m) continue;
+    if (!full_mn && gm >= g.m) continue;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int64_t gn = n0 + wn0 + 16 * j + fr + 4 * v;
-        if (gn >= g.n) continue;
+        if (!full_mn && gn >= g.n) continue;
         double* cp = C + gm + gn * g.ldc;
-        double val = acc[i][j][v];
-        if (!split) {
-          val *= g.alpha;
-          if (g.beta != 0.0) val += g.beta * *cp;
-        }
+        double val = alpha * acc[i][j][v];
+        if (has_beta) val += g.beta * *cp;
         *cp = val;
       }
     }
@@ -167,12 +198,17 @@ __global__ void splitk_reduce_kernel(const double* __restrict__ P, int64_t strid
   }
 }
 
-static int launch_mfma(cqr_ctx* c, int ta, int tb, const GemmArgs& g, int nsplit) {
+static int launch_mfma(cqr_ctx* c, int ta, int tb, const GemmArgs& g, int nsplit, int kind) {
+  hipEvent_t ev = prof_begin(c);
   dim3 grid((unsigned)((g.m + 127) / 128), (unsigned)((g.n + 127) / 128), (unsigned)nsplit);
   if (!ta && !tb) hipLaunchKernelGGL((mfma_gemm_f64<false, false>), grid, dim3(256), 0, c->stream, g);
   else if (ta && !tb) hipLaunchKernelGGL((mfma_gemm_f64<true, false>), grid, dim3(256), 0, c->stream, g);
   else if (!ta && tb) hipLaunchKernelGGL((mfma_gemm_f64<false, true>), grid, dim3(256), 0, c->stream, g);
   else hipLaunchKernelGGL((mfma_gemm_f64<true, true>), grid, dim3(256), 0, c->stream, g);
+  const double out = (double)g.m * g.n;
+  const double bytes = 8.0 * ((double)g.m * g.k + (double)g.k * g.n +
+                              (nsplit > 1 ? nsplit * out : (g.beta != 0.0 ? 2.0 : 1.0) * out));
+  prof_end(c, ev, kind, 2.0 * g.m * g.n * g.k, bytes);
   return check_launch(c, "mfma_gemm_f64");
 }
 
@@ -197,7 +233,7 @@ static int choose_splits(int64_t m, int64_t n, int64_t k, int64_t* kchunk) {
 // doubles when splitting (callers size it with gemm_f64_scratch).
 static int gemm_f64_impl(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha,
                          const double* A, int64_t lda, int smA, const double* B, int64_t ldb, int smB,
-                         double beta, double* C, int64_t ldc, double* scratch) {
+                         double beta, double* C, int64_t ldc, double* scratch, int kind = PK_GEMM_SMALL) {
   if (m <= 0 || n <= 0) return CQR_OK;
   GemmArgs g{};
   g.m = m; g.n = n; g.k = k; g.alpha = alpha; g.beta = beta;
@@ -206,13 +242,15 @@ static int gemm_f64_impl(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64
   const int ns = scratch ? choose_splits(m, n, k, &kc) : 1;
   if (ns <= 1) {
     g.C = C; g.ldc = ldc; g.kchunk = std::max<int64_t>(k, 1); g.split_stride = 0;
-    return launch_mfma(c, ta, tb, g, 1);
+    return launch_mfma(c, ta, tb, g, 1, kind);
   }
   g.C = scratch; g.ldc = m; g.kchunk = kc; g.split_stride = m * n;
-  CQR_TRY(launch_mfma(c, ta, tb, g, ns));
+  CQR_TRY(launch_mfma(c, ta, tb, g, ns, kind));
   const int gx = (int)std::min<int64_t>(2048, (m * n + 255) / 256);
+  hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(splitk_reduce_kernel, dim3(gx), dim3(256), 0, c->stream, scratch, m * n, ns, m, n,
                      C, ldc, alpha, beta);
+  prof_end(c, ev, PK_REDUCE, (double)ns * m * n, 8.0 * (ns + (beta != 0.0 ? 2.0 : 1.0)) * m * n);
   return check_launch(c, "splitk_reduce");
 }
 
@@ -255,11 +293,12 @@ int wy_update_f64_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, c
   double* Zp = Z + ((size_t)b * n + 31) / 32 * 32;
   // Z = Yᵀ·A  (b×n)
   CQR_TRY(gemm_f64_impl(c, 1, 0, b, n, m, 1.0, Y, ldy, smY, A, lda, SM_NONE, 0.0, Z, b,
-                        np ? part : nullptr));
+                        np ? part : nullptr, PK_GEMM_YTA));
   // Z' = Tᵀ·Z
   CQR_TRY(gemm_f64_impl(c, 0, 0, b, n, b, 1.0, Tt, b, SM_LOWER, Z, b, SM_NONE, 0.0, Zp, b, nullptr));
   // A −= Y·Z'
-  return gemm_f64_impl(c, 0, 0, m, n, b, -1.0, Y, ldy, smY, Zp, b, SM_NONE, 1.0, A, lda, nullptr);
+  return gemm_f64_impl(c, 0, 0, m, n, b, -1.0, Y, ldy, smY, Zp, b, SM_NONE, 1.0, A, lda, nullptr,
+                       PK_GEMM_UPD);
 }
 
 int wy_update_f64(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const double* Tt,

@@ -15,14 +15,20 @@ namespace cqr {
 
 __device__ __forceinline__ int64_t leaf_off(int64_t l, int64_t m, int64_t L) { return (l * m) / L; }
 
+// Loop over the entries (rr, cc) of a b×b matrix: waves over columns, lanes
+// over rows (no integer division; coalesced along columns).
+#define FOR_RC(rr, cc, b, NT)                                                     \
+  for (int cc = wave_id(); cc < (b); cc += (NT) / 64)                             \
+    for (int rr = (int)(threadIdx.x & 63); rr < (b); rr += 64)
+
 // ---------------------------------------------------------------------------
 // K1: leaf QR.  One workgroup per leaf; the leaf lives in registers: wave w
 // owns columns w, w+WAVES, ... (CPW of them), lane l owns rows l, l+64, ...
 // (RPL of them).  Per column step the owner wave forms the reflector
-// (FP64 wave reduction for the norm, dlarfg semantics), publishes v through
-// LDS, and one barrier later every wave applies it to its columns.  The dot
-// products against the already-finished columns are the Gram entries
-// V(:,j)ᵀv_i that give the compact-WY T at the end.
+// (FP64 DPP wave reduction for the norm, dlarfg semantics), publishes v
+// through LDS, and one barrier later every wave applies it to its columns.
+// The dot products against the already-finished columns are the Gram
+// entries V(:,j)ᵀv_i that give the compact-WY T at the end.
 // ---------------------------------------------------------------------------
 template <int WAVES, int CPW, int RPL>
 __global__ void __launch_bounds__(64 * WAVES)
@@ -38,7 +44,7 @@ leaf_qr_kernel(double* __restrict__ A, int64_t lda, int64_t m, int L, int b,
   double* Tl = Tout + (int64_t)leaf * b * b;
   double* taul = tauout + (int64_t)leaf * b;
 
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
   extern __shared__ double smem[];
   double* G = smem;                       // b×b Gram, then T
   double* vbuf = G + b * b;               // 2 × 64·RPL
@@ -54,7 +60,8 @@ leaf_qr_kernel(double* __restrict__ A, int64_t lda, int64_t m, int L, int b,
       a[cl][s] = (col < b && row < r) ? Al[row + col * lda] : 0.0;
     }
   }
-  for (int e = tid; e < b * b; e += NT) G[e] = 0.0;
+  for (int cc = w; cc < b; cc += WAVES)
+    for (int rr = lane; rr < b; rr += 64) G[rr + cc * b] = 0.0;
   __syncthreads();
 
   for (int i = 0; i < b; ++i) {
@@ -127,16 +134,18 @@ leaf_qr_kernel(double* __restrict__ A, int64_t lda, int64_t m, int L, int b,
       part[cl] = acc;
     }
     const double mine = xreduce<CPW>(part, lane);
+    {
+      // Gram entries of finished columns: one lane per reduced value
+      const int col = w + WAVES * (lane >> SH);
+      if ((lane & ((1 << SH) - 1)) == 0 && col < i) G[col + i * b] = mine;
+    }
 #pragma unroll
     for (int cl = 0; cl < CPW; ++cl) {
       const int col = w + WAVES * cl;
-      const double dcl = readlane_d(mine, cl << SH);
       if (col > i && col < b) {
-        const double f = tau * dcl;
+        const double f = tau * readlane_d(mine, cl << SH);
 #pragma unroll
         for (int s = 0; s < RPL; ++s) a[cl][s] -= f * v[s];
-      } else if (col < i) {
-        if (lane == 0) G[col + i * b] = dcl;
       }
     }
   }
@@ -154,18 +163,23 @@ leaf_qr_kernel(double* __restrict__ A, int64_t lda, int64_t m, int L, int b,
       if (row < b) Rl[row + col * b] = row <= col ? a[cl][s] : 0.0;
     }
   }
-  for (int e = tid; e < b * b; e += NT) {
-    const int row = e % b, col = e / b;
-    Tl[e] = row <= col ? G[e] : 0.0;
-  }
+  for (int cc = w; cc < b; cc += WAVES)
+    for (int rr = lane; rr < b; rr += 64) Tl[rr + cc * b] = rr <= cc ? G[rr + cc * b] : 0.0;
   for (int p = tid; p < b; p += NT) taul[p] = stau[p];
 }
+
+// Packed column-major triangles in LDS: upper (r ≤ c) and strictly upper.
+__device__ __forceinline__ int pu(int r, int c) { return (c * (c + 1) >> 1) + r; }
+__device__ __forceinline__ int psu(int r, int c) { return (c * (c - 1) >> 1) + r; }
 
 // ---------------------------------------------------------------------------
 // K2: one tree level.  Node z stacks R of receiver z on R of sender
 // z + ceil(n/2) and computes its QR like dtpqrt with l = b: the reflector
 // for column i is [e_i; x] with x = R_bot(0:i, i), so R_bot becomes V2
-// (upper triangular) and R_top becomes the new R of the receiver.
+// (upper triangular, in registers as in K1) and R_top (packed in LDS)
+// becomes the new R of the receiver.  Row i of R_top couples into column
+// j's dot product only through its entry (i, j): the lane that ends up
+// holding column j's reduced sum adds it and applies the update.
 // ---------------------------------------------------------------------------
 template <int WAVES, int CPW, int RPL>
 __global__ void __launch_bounds__(64 * WAVES)
@@ -182,44 +196,43 @@ tree_node_kernel(double* __restrict__ Rleaf, int n, int node0, int b, double* __
   double* Tn = Tout + (node0 + z) * bb;
   double* taun = tauout + (int64_t)(node0 + z) * b;
 
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
   extern __shared__ double smem[];
-  double* G = smem;
-  double* vbuf = G + b * b;
-  double* stau = vbuf + 2 * 64 * RPL;
+  const int nu = b * (b + 1) / 2, nsu = b * (b - 1) / 2;
+  double* R1 = smem;                      // packed upper b×b
+  double* Gp = R1 + nu;                   // packed strictly-upper Gram
+  double* vbuf = Gp + nsu;                // 2 × 64·RPL
+  double* stau = vbuf + 2 * 64 * RPL;     // b
 
-  double a1[CPW][RPL], a2[CPW][RPL];
+  double a2[CPW][RPL];
 #pragma unroll
   for (int cl = 0; cl < CPW; ++cl) {
     const int col = w + WAVES * cl;
 #pragma unroll
     for (int s = 0; s < RPL; ++s) {
       const int row = lane + 64 * s;
-      const bool in = col < b && row <= col;
-      a1[cl][s] = in ? Rt[row + col * b] : 0.0;
-      a2[cl][s] = in ? Rb[row + col * b] : 0.0;
+      a2[cl][s] = (col < b && row <= col) ? Rb[row + col * b] : 0.0;
     }
   }
-  for (int e = tid; e < b * b; e += NT) G[e] = 0.0;
+  for (int cc = w; cc < b; cc += WAVES) {
+    for (int rr = lane; rr <= cc; rr += 64) R1[pu(rr, cc)] = Rt[rr + cc * b];
+    for (int rr = lane; rr < cc; rr += 64) Gp[psu(rr, cc)] = 0.0;
+  }
   __syncthreads();
 
   for (int i = 0; i < b; ++i) {
     const int ow = i % WAVES, ocl = i / WAVES;
-    const int plane = i & 63, pslot = i >> 6;
     double* vb = vbuf + (i & 1) * (64 * RPL);
     if (w == ow) {
 #pragma unroll
       for (int cl = 0; cl < CPW; ++cl) {
         if (cl != ocl) continue;
-        double ss = 0.0, piv = 0.0;
+        double ss = 0.0;
 #pragma unroll
-        for (int s = 0; s < RPL; ++s) {
-          const int row = lane + 64 * s;
-          if (row <= i) ss += a2[cl][s] * a2[cl][s];
-          if (s == pslot) piv = a1[cl][s];
-        }
+        for (int s = 0; s < RPL; ++s)
+          if (lane + 64 * s <= i) ss += a2[cl][s] * a2[cl][s];
         ss = wave_sum(ss);
-        const double alpha = readlane_d(piv, plane);
+        const double alpha = R1[pu(i, i)];
         double xnorm;
         if (ss_needs_rescale(ss)) {
           double mx = 0.0;
@@ -248,10 +261,12 @@ tree_node_kernel(double* __restrict__ Rleaf, int n, int node0, int b, double* __
             for (int k = 0; k < h.knt; ++k) x *= 0x1p969;
             a2[cl][s] = x * h.scal;
           }
-          if (row == i) a1[cl][s] = h.beta;
           vb[row] = row <= i ? a2[cl][s] : 0.0;
         }
-        if (lane == 0) stau[i] = h.tau;
+        if (lane == 0) {
+          R1[pu(i, i)] = h.beta;
+          stau[i] = h.tau;
+        }
       }
     }
     __syncthreads();
@@ -262,33 +277,38 @@ tree_node_kernel(double* __restrict__ Rleaf, int n, int node0, int b, double* __
     double part[CPW];
 #pragma unroll
     for (int cl = 0; cl < CPW; ++cl) {
-      double acc = 0.0, top = 0.0;
+      double acc = 0.0;
 #pragma unroll
-      for (int s = 0; s < RPL; ++s) {
-        acc += v[s] * a2[cl][s];
-        if (s == pslot) top = a1[cl][s];
-      }
-      part[cl] = acc + (lane == plane ? top : 0.0);
+      for (int s = 0; s < RPL; ++s) acc += v[s] * a2[cl][s];
+      part[cl] = acc;
     }
-    const double mine = xreduce<CPW>(part, lane);
+    double d = xreduce<CPW>(part, lane);
+    {
+      const int col = w + WAVES * (lane >> SH);
+      if ((lane & ((1 << SH) - 1)) == 0 && col < b) {
+        if (col > i) {
+          const double r1 = R1[pu(i, col)];
+          d += r1;
+          R1[pu(i, col)] = r1 - tau * d;
+        } else if (col < i) {
+          Gp[psu(col, i)] = d;
+        }
+      }
+    }
 #pragma unroll
     for (int cl = 0; cl < CPW; ++cl) {
       const int col = w + WAVES * cl;
-      const double dcl = readlane_d(mine, cl << SH);
       if (col > i && col < b) {
-        const double f = tau * dcl;
+        const double f = tau * readlane_d(d, cl << SH);
 #pragma unroll
-        for (int s = 0; s < RPL; ++s) {
-          a2[cl][s] -= f * v[s];
-          if (s == pslot && lane == plane) a1[cl][s] -= f;
-        }
-      } else if (col < i) {
-        if (lane == 0) G[col + i * b] = dcl;
+        for (int s = 0; s < RPL; ++s) a2[cl][s] -= f * v[s];
       }
     }
   }
   __syncthreads();
-  tri_inv_upper_lds<NT>(G, b, stau, b);
+  // outputs: new R of the receiver, V2; then T from the Gram in dense LDS
+  for (int cc = w; cc < b; cc += WAVES)
+    for (int rr = lane; rr < b; rr += 64) Rt[rr + cc * b] = rr <= cc ? R1[pu(rr, cc)] : 0.0;
 #pragma unroll
   for (int cl = 0; cl < CPW; ++cl) {
     const int col = w + WAVES * cl;
@@ -296,16 +316,38 @@ tree_node_kernel(double* __restrict__ Rleaf, int n, int node0, int b, double* __
 #pragma unroll
     for (int s = 0; s < RPL; ++s) {
       const int row = lane + 64 * s;
-      if (row < b) {
-        Rt[row + col * b] = row <= col ? a1[cl][s] : 0.0;
-        V2[row + col * b] = row <= col ? a2[cl][s] : 0.0;
-      }
+      if (row < b) V2[row + col * b] = row <= col ? a2[cl][s] : 0.0;
     }
   }
-  for (int e = tid; e < b * b; e += NT) {
-    const int row = e % b, col = e / b;
-    Tn[e] = row <= col ? G[e] : 0.0;
+  // Gram (packed, after R1) → dense b×b at the start of LDS, via registers
+  constexpr int GPT = (CQR_MAX_B * (CQR_MAX_B - 1) / 2 + NT - 1) / NT;
+  double g[GPT];
+#pragma unroll
+  for (int t = 0; t < GPT; ++t) {
+    const int e = tid + t * NT;
+    g[t] = e < nsu ? Gp[e] : 0.0;
   }
+  __syncthreads();
+  double* M = smem;
+  for (int cc = w; cc < b; cc += WAVES)
+    for (int rr = lane; rr < b; rr += 64) M[rr + cc * b] = 0.0;
+  __syncthreads();
+  // packed strictly-upper index e → (r, c): column c holds c entries
+#pragma unroll
+  for (int t = 0; t < GPT; ++t) {
+    const int e = tid + t * NT;
+    if (e < nsu) {
+      int c = (int)((1.0 + sqrt(1.0 + 8.0 * e)) * 0.5);
+      while (c * (c - 1) / 2 > e) --c;
+      while ((c + 1) * c / 2 <= e) ++c;
+      const int rr = e - c * (c - 1) / 2;
+      M[rr + c * b] = g[t];
+    }
+  }
+  __syncthreads();
+  tri_inv_upper_lds<NT>(M, b, stau, b);
+  for (int cc = w; cc < b; cc += WAVES)
+    for (int rr = lane; rr < b; rr += 64) Tn[rr + cc * b] = rr <= cc ? M[rr + cc * b] : 0.0;
   for (int p = tid; p < b; p += NT) taun[p] = stau[p];
 }
 
@@ -415,21 +457,22 @@ __global__ void __launch_bounds__(256) bgemm_kernel(BgemmArgs g) {
 // ---------------------------------------------------------------------------
 template <int NT>
 __device__ void signed_lu_lds(double* M, int b, double* sgn) {
-  const int tid = threadIdx.x;
+  const int lane = threadIdx.x & 63, w = wave_id();
+  constexpr int NW = NT / 64;
   for (int i = 0; i < b; ++i) {
     const double s = M[i + i * b] > 0.0 ? -1.0 : 1.0;
     const double piv = M[i + i * b] - s;
     __syncthreads();
-    if (tid == 0) {
+    if (threadIdx.x == 0) {
       M[i + i * b] = piv;
       sgn[i] = s;
     }
-    for (int rr = i + 1 + tid; rr < b; rr += NT) M[rr + i * b] = M[rr + i * b] / piv;
+    if (w == 0)
+      for (int rr = i + 1 + lane; rr < b; rr += 64) M[rr + i * b] = M[rr + i * b] / piv;
     __syncthreads();
-    const int nt = b - i - 1;
-    for (int e = tid; e < nt * nt; e += NT) {
-      const int rr = i + 1 + e % nt, cc = i + 1 + e / nt;
-      M[rr + cc * b] -= M[rr + i * b] * M[i + cc * b];
+    for (int cc = i + 1 + w; cc < b; cc += NW) {
+      const double u = M[i + cc * b];
+      for (int rr = i + 1 + lane; rr < b; rr += 64) M[rr + cc * b] -= M[rr + i * b] * u;
     }
     __syncthreads();
   }
@@ -447,12 +490,12 @@ hr_core_kernel(const double* __restrict__ Q1top, int64_t ldq, const double* __re
   double* sgn = M + b * b;     // b
   double* d = sgn + b;         // b
   const int tid = threadIdx.x;
-  for (int e = tid; e < b * b; e += NT) M[e] = Q1top[(e % b) + (int64_t)(e / b) * ldq];
+  FOR_RC(rr, cc, b, NT) M[rr + cc * b] = Q1top[rr + (int64_t)cc * ldq];
   __syncthreads();
   signed_lu_lds<NT>(M, b, sgn);
   // outputs: L below the diagonal of Atop, S·R on and above it, W = U·S
-  for (int e = tid; e < b * b; e += NT) {
-    const int rr = e % b, cc = e / b;
+  FOR_RC(rr, cc, b, NT) {
+    const int e = rr + cc * b;
     if (rr > cc) Atop[rr + cc * lda] = M[e];
     else Atop[rr + cc * lda] = sgn[rr] * Rroot[e];
     W[e] = rr <= cc ? M[e] * sgn[cc] : 0.0;
@@ -462,25 +505,19 @@ hr_core_kernel(const double* __restrict__ Q1top, int64_t ldq, const double* __re
   for (int p = tid; p < b; p += NT) d[p] = 1.0 / M[p + p * b];
   __syncthreads();
   tri_inv_upper_lds<NT>(M, b, d, b);
-  for (int e = tid; e < b * b; e += NT) {
-    const int rr = e % b, cc = e / b;
+  FOR_RC(rr, cc, b, NT) {
+    const int e = rr + cc * b;
     const double u = rr <= cc ? M[e] : 0.0;
     Uinv[e] = u;
     Winv[e] = sgn[rr] * u;        // W⁻¹ = S·U⁻¹
   }
   __syncthreads();
   // L⁻ᵀ: load Lᵀ (unit upper) and invert
-  for (int e = tid; e < b * b; e += NT) {
-    const int rr = e % b, cc = e / b;
-    M[e] = rr < cc ? Atop[cc + rr * lda] : 0.0;
-  }
+  FOR_RC(rr, cc, b, NT) M[rr + cc * b] = rr < cc ? Atop[cc + rr * lda] : 0.0;
   for (int p = tid; p < b; p += NT) d[p] = 1.0;
   __syncthreads();
   tri_inv_upper_lds<NT>(M, b, d, b);
-  for (int e = tid; e < b * b; e += NT) {
-    const int rr = e % b, cc = e / b;
-    LinvT[e] = rr <= cc ? M[e] : 0.0;
-  }
+  FOR_RC(rr, cc, b, NT) LinvT[rr + cc * b] = rr <= cc ? M[rr + cc * b] : 0.0;
 }
 
 // a5 standalone: signed LU of a b×b block with R scaling.
@@ -492,12 +529,11 @@ signed_nlu_kernel(double* __restrict__ A, int64_t lda, double* __restrict__ R, i
   double* M = smem;
   double* sgn = M + b * b;
   const int tid = threadIdx.x;
-  for (int e = tid; e < b * b; e += NT) M[e] = A[(e % b) + (int64_t)(e / b) * lda];
+  FOR_RC(rr, cc, b, NT) M[rr + cc * b] = A[rr + (int64_t)cc * lda];
   __syncthreads();
   signed_lu_lds<NT>(M, b, sgn);
-  for (int e = tid; e < b * b; e += NT) {
-    const int rr = e % b, cc = e / b;
-    A[rr + cc * lda] = M[e];
+  FOR_RC(rr, cc, b, NT) {
+    A[rr + cc * lda] = M[rr + cc * b];
     if (R && rr <= cc && sgn[rr] < 0) R[rr + cc * ldr] = -R[rr + cc * ldr];
   }
   if (signs)
@@ -514,17 +550,14 @@ tri_inverse_kernel(const double* __restrict__ X, int64_t ldx, double* __restrict
   double* M = smem;
   double* d = M + b * b;
   const int tid = threadIdx.x;
-  for (int e = tid; e < b * b; e += NT) {
-    const int rr = e % b, cc = e / b;
-    M[e] = rr < cc ? (upper ? X[rr + cc * ldx] : X[cc + rr * ldx]) : 0.0;
-  }
+  FOR_RC(rr, cc, b, NT) M[rr + cc * b] = rr < cc ? (upper ? X[rr + cc * ldx] : X[cc + rr * ldx]) : 0.0;
   for (int p = tid; p < b; p += NT) d[p] = unit ? 1.0 : 1.0 / X[p + p * ldx];
   __syncthreads();
   tri_inv_upper_lds<NT>(M, b, d, b);
-  for (int e = tid; e < b * b; e += NT) {
-    const int rr = e % b, cc = e / b;
-    if (upper) Xi[rr + cc * ldi] = rr <= cc ? M[e] : 0.0;
-    else Xi[cc + rr * ldi] = rr <= cc ? M[e] : 0.0;
+  FOR_RC(rr, cc, b, NT) {
+    const double v = rr <= cc ? M[rr + cc * b] : 0.0;
+    if (upper) Xi[rr + cc * ldi] = v;
+    else Xi[cc + rr * ldi] = v;
   }
 }
 
@@ -662,7 +695,9 @@ static int leaf_launch(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int
     CQR_HIP(c, hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, 163840));
     attr = true;
   }
+  hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(k, dim3(L), dim3(64 * WAVES), lds, c->stream, A, lda, m, L, b, R, T, tau);
+  prof_end(c, ev, PK_LEAF, 2.0 * m * b * b - (2.0 / 3.0) * L * (double)b * b * b, 16.0 * m * b);
   return check_launch(c, "leaf_qr");
 }
 
@@ -692,7 +727,9 @@ static int node_launch(cqr_ctx* c, double* Rleaf, int n, int node0, int b, doubl
     CQR_HIP(c, hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, 163840));
     attr = true;
   }
+  hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(k, dim3(npairs), dim3(64 * WAVES), lds, c->stream, Rleaf, n, node0, b, V2, T, tau);
+  prof_end(c, ev, PK_TREE, npairs * (2.0 / 3.0) * b * (double)b * b, npairs * 40.0 * b * b);
   return check_launch(c, "tree_node");
 }
 
@@ -706,7 +743,10 @@ int tree_level(cqr_ctx* c, double* Rleaf, int n, int node0, int b, double* V2, d
 int bgemm(cqr_ctx* c, const BgemmArgs& g, int batch, int max_m) {
   if (batch <= 0 || g.n <= 0 || max_m <= 0) return CQR_OK;
   dim3 grid((max_m + 63) / 64, (g.n + 63) / 64, batch);
+  hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(bgemm_kernel, grid, dim3(256), 0, c->stream, g);
+  prof_end(c, ev, PK_BGEMM, 2.0 * batch * (double)max_m * g.n * g.k,
+           8.0 * batch * ((double)max_m * g.k + (double)g.k * g.n + 2.0 * max_m * g.n));
   return check_launch(c, "bgemm");
 }
 
@@ -720,8 +760,10 @@ int hr_core(cqr_ctx* c, const double* Q1top, int64_t ldq, const double* Rroot, d
   static bool attr = false;
   if (!attr) { CQR_TRY(big_lds_attr(c, (const void*)hr_core_kernel)); attr = true; }
   const size_t lds = sizeof(double) * ((size_t)b * b + 2 * b);
+  hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(hr_core_kernel, dim3(1), dim3(512), lds, c->stream, Q1top, ldq, Rroot, Atop,
                      lda, W, Uinv, Winv, LinvT, signs, b);
+  prof_end(c, ev, PK_HR, (2.0 / 3.0) * b * (double)b * b, 64.0 * b * b);
   return check_launch(c, "hr_core");
 }
 

@@ -57,6 +57,25 @@ class Context:
     def set_check_finite(self, on: bool):
         self.check(self.lib.cqr_set_check_finite(self.h, 1 if on else 0))
 
+    PROFILE_KINDS = ("gemm_yta", "gemm_update", "gemm_small", "splitk_reduce", "leaf_qr", "tree_level",
+                     "hr_core", "bgemm")
+
+    def profile(self, on: bool):
+        self.check(self.lib.cqr_profile_enable(self.h, 1 if on else 0))
+
+    def profile_reset(self):
+        self.check(self.lib.cqr_profile_reset(self.h))
+
+    def profile_read(self):
+        """{kind: (ms, flops, bytes, launches)} summed since the last reset."""
+        out = {}
+        for k, name in enumerate(self.PROFILE_KINDS):
+            ms, fl, by, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+            self.check(self.lib.cqr_profile_read(self.h, k, ctypes.byref(ms), ctypes.byref(fl),
+                                                 ctypes.byref(by), ctypes.byref(n)))
+            out[name] = (ms.value, fl.value, by.value, n.value)
+        return out
+
     def workspace_bytes(self) -> int:
         return int(self.lib.cqr_workspace_bytes(self.h))
 

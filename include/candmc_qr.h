@@ -46,6 +46,17 @@ int cqr_reserve(cqr_ctx* ctx, size_t bytes);          /* pre-grow the arena   */
 size_t cqr_workspace_bytes(const cqr_ctx* ctx);
 int cqr_set_check_finite(cqr_ctx* ctx, int on);       /* scan inputs first    */
 int cqr_set_leaf_rows(cqr_ctx* ctx, int64_t rows);    /* 0 = automatic        */
+/* Kernel timing: while enabled, the library records a HIP event pair on
+ * its stream around every kernel it launches; cqr_profile_read waits for
+ * the stream and returns, for one kernel class, the summed event time,
+ * the algorithmic flops and HBM bytes of those launches and their count.
+ * Classes: 0 Yᵀ·A split-K GEMM, 1 A −= Y·Z' update GEMM, 2 small b×b
+ * GEMMs on MFMA, 3 split-K reduction, 4 leaf QR, 5 tree level, 6 HR core,
+ * 7 batched small GEMMs of the panel.                                    */
+int cqr_profile_enable(cqr_ctx* ctx, int on);
+int cqr_profile_read(cqr_ctx* ctx, int kind, double* ms, double* flops, double* bytes,
+                     int64_t* launches);
+int cqr_profile_reset(cqr_ctx* ctx);
 const char* cqr_status_string(int status);
 const char* cqr_last_error(const cqr_ctx* ctx);
 

@@ -416,25 +416,30 @@ def upd_A(Y, A, invT_T):
     return A - Y @ YT                                                # :275
 
 
-def QR_2D(A, b, P_panel=1):
+def QR_2D(A, b, P_panel=1, max_panels=None, copy=True):
     """qr_2d.cxx:299-363 on a 1×1 process grid.
 
     Right-looking: per panel hh_recon_qr on min(k,b) columns (P_panel
     virtual ranks), then update_A with T⁻ᵀ from W (:179-208).  Returns the
     m×k [Y\\R] buffer, no τ (the reference reconstructs τ with tau_recon).
+    max_panels stops after that many panels (a bounded timing sample).
     """
-    A = np.asfortranarray(np.array(A, dtype=np.float64))
+    A = np.asfortranarray(np.array(A, dtype=np.float64)) if copy else A
     m, k = A.shape
     j = 0
+    npan = 0
     while True:
         mj, kj = m - j, k - j
         bj = min(kj, b)
         YR, W, _ = hh_recon_qr(A[j:, j:j + bj], bj, P_panel)
         A[j:, j:j + bj] = YR
-        if kj - b > 0 and mj - b > 0:                               # :318
+        npan += 1
+        if kj - b > 0 and mj - b > 0 and (max_panels is None or npan < max_panels):   # :318
             Y = unit_lower(YR, bj)
             invT_T = compute_invT_from_W(W, Y[:bj, :bj])
-            A[j:, j + bj:] = upd_A(Y, A[j:, j + bj:], invT_T)
+            Ct = A[j:, j + bj:].T                                   # row-major view of C
+            YT = _trsv(invT_T, (Ct @ Y).T, lower=True)              # upd_A :259-271
+            Ct -= YT.T @ Y.T                                        # :275, in place
             j += b
         else:
             break
