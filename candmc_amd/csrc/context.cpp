@@ -140,7 +140,16 @@ int cqr_create(cqr_ctx** ctx, int device) {
   if (hipSetDevice(device) != hipSuccess) return CQR_ERR_DEVICE;
   cqr_ctx* c = new cqr_ctx();
   c->device = device;
-  if (hipMalloc(&c->d_flag, sizeof(int)) != hipSuccess) {
+  if (hipMalloc(&c->d_flag, sizeof(int)) != hipSuccess ||
+      hipMalloc(&c->d_cnt, sizeof(uint64_t) * CQR_MAX_TREE_NODES) != hipSuccess ||
+      hipMalloc(&c->d_terr, sizeof(int)) != hipSuccess ||
+      hipMemset(c->d_terr, 0, sizeof(int)) != hipSuccess ||
+      hipMemset(c->d_cnt, 0, sizeof(uint64_t) * CQR_MAX_TREE_NODES) != hipSuccess ||
+      hipMemset(c->d_flag, 0, sizeof(int)) != hipSuccess) {
+    (void)hipGetLastError();
+    if (c->d_flag) (void)hipFree(c->d_flag);
+    if (c->d_cnt) (void)hipFree(c->d_cnt);
+    if (c->d_terr) (void)hipFree(c->d_terr);
     delete c;
     return CQR_ERR_NOMEM;
   }
@@ -159,6 +168,8 @@ int cqr_destroy(cqr_ctx* c) {
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
   if (c->ws) (void)hipFree(c->ws);
   if (c->d_flag) (void)hipFree(c->d_flag);
+  if (c->d_cnt) (void)hipFree(c->d_cnt);
+  if (c->d_terr) (void)hipFree(c->d_terr);
   delete c;
   return CQR_OK;
 }
@@ -174,6 +185,13 @@ void* cqr_get_stream(const cqr_ctx* c) { return c ? (void*)c->stream : nullptr; 
 int cqr_sync(cqr_ctx* c) {
   if (!c) return CQR_ERR_INVALID;
   CQR_HIP(c, hipStreamSynchronize(c->stream));
+  // a pipelined tree node that gave up waiting for a child leaves its flag
+  int terr = 0;
+  CQR_HIP(c, hipMemcpy(&terr, c->d_terr, sizeof(int), hipMemcpyDeviceToHost));
+  if (terr) {
+    CQR_HIP(c, hipMemset(c->d_terr, 0, sizeof(int)));
+    return set_err(c, CQR_ERR_DEVICE, "TSQR tree node timed out waiting for a child");
+  }
   return CQR_OK;
 }
 

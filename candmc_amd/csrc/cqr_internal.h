@@ -33,7 +33,10 @@ struct cqr_ctx {
   size_t ws_bytes = 0;
   int check_finite = 0;
   int64_t leaf_rows = 0;        // 0 = automatic
-  int* d_flag = nullptr;        // device flag for non-finite scans
+  int* d_flag = nullptr;        // device flag for non-finite scans / tree timeouts
+  uint64_t* d_cnt = nullptr;    // per-node column counters of the pipelined tree
+  int* d_terr = nullptr;        // set by a tree node whose wait for a child timed out
+  uint64_t tree_epoch = 0;      // launch epoch folded into the counters
   std::string err;
 };
 
@@ -123,6 +126,13 @@ inline BMat bmat(const double* p, int64_t ld, int64_t zs = 0, int mode = BM_STRI
   BMat b; b.p = p; b.ld = ld; b.zs = zs; b.mode = mode; return b;
 }
 
+#define CQR_MAX_TREE_NODES 8192
+
+// Column-pipelined tree over all levels (ptree_kernel.hip): node outputs R
+// in nodeR (nnodes×b×b), V2/T/tau per node in forward level order.
+int launch_ptree(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
+                 int L, int nnodes, int b);
+
 // Kernel launchers (tsqr_kernels.hip)
 int leaf_qr(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int b, double* R, double* T,
             double* tau);
@@ -156,6 +166,12 @@ enum { SM_NONE = 0, SM_UNIT_LOWER = 1, SM_UPPER = 2, SM_LOWER = 3, SM_IDENTITY =
 int gemm_f64(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha,
              const double* A, int64_t lda, int smA, const double* B, int64_t ldb,
              double beta, double* C, int64_t ldc);
+// Batched MFMA GEMM: operands addressed per batch entry z as in BMat; m < 0
+// means "rows of leaf z" (max_m bounds them).  smC = SM_UPPER zeroes the
+// strictly lower part of each C.
+int gemm_f64_batched(cqr_ctx* c, int ta, int tb, int m, int n, int k, double alpha, const BMat& A, int smA,
+                     const BMat& B, int smB, double beta, const BMat& C, int smC, int batch, int64_t pm, int pL,
+                     int zoff, int max_m, int kind);
 // Same with an explicit split-K scratch (nullptr: no split) and a mask on B.
 int gemm_f64_ws(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha,
                 const double* A, int64_t lda, int smA, const double* B, int64_t ldb, int smB,

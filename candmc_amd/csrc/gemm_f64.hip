@@ -27,7 +27,23 @@ struct GemmArgs {
   int64_t ldc;
   int64_t kchunk;        // rows of K per split (grid.z splits)
   int64_t split_stride;  // > 0: write raw partial sums to C + z·split_stride
+  // batched form (grid.z = batch index, no split): per-z operand addresses
+  int batched;
+  int a_mode, b_mode, c_mode;  // BM_STRIDED / BM_LEAFROWS / BM_SHIFTED
+  int64_t a_zs, b_zs, c_zs;
+  int64_t pm;                  // panel rows for BM_LEAFROWS
+  int pL, zoff;
+  int m_leaf;                  // m = rows of leaf z
+  int smC;                     // SM_UPPER: store zeros below the diagonal
 };
+
+__device__ __forceinline__ int64_t leaf_row0(int64_t z, int64_t m, int64_t L) { return (z * m) / L; }
+
+__device__ __forceinline__ const double* zaddr(const double* p, int mode, int64_t zs, int z, const GemmArgs& g) {
+  if (mode == BM_LEAFROWS) return p + leaf_row0(z, g.pm, g.pL);
+  if (mode == BM_SHIFTED) return p + (int64_t)(z + g.zoff) * zs;
+  return p + (int64_t)z * zs;
+}
 
 __device__ __forceinline__ double gmask(double v, int64_t r, int64_t c, int sm) {
   switch (sm) {
@@ -61,11 +77,24 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 64;
   const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
-  const int64_t kbeg = (int64_t)blockIdx.z * g.kchunk;
-  const int64_t kend = min(g.k, kbeg + g.kchunk);
   const double* __restrict__ A = g.A;
   const double* __restrict__ B = g.B;
-  const bool full_mn = (m0 + BM <= g.m) && (n0 + BN <= g.n);
+  double* C = g.C;
+  int64_t Mrows = g.m, kbeg, kend;
+  if (g.batched) {
+    const int z = blockIdx.z;
+    A = zaddr(g.A, g.a_mode, g.a_zs, z, g);
+    B = zaddr(g.B, g.b_mode, g.b_zs, z, g);
+    C = const_cast<double*>(zaddr(g.C, g.c_mode, g.c_zs, z, g));
+    if (g.m_leaf) Mrows = leaf_row0(z + 1, g.pm, g.pL) - leaf_row0(z, g.pm, g.pL);
+    kbeg = 0;
+    kend = g.k;
+    if (m0 >= Mrows) return;   // whole block out of this batch entry's rows
+  } else {
+    kbeg = (int64_t)blockIdx.z * g.kchunk;
+    kend = min(g.k, kbeg + g.kchunk);
+  }
+  const bool full_mn = (m0 + BM <= Mrows) && (n0 + BN <= g.n);
 
   // Each thread loads 8 elements of the A slab and 8 of the B slab:
   //   K-contiguous operand: k = k0 + (tid & 15), index = base + (tid >> 4) + 16r
@@ -108,7 +137,7 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
     for (int r = 0; r < 8; ++r) {
       const int64_t gm = m0 + am + (TA ? 16 * r : 0), gk = k0 + ak + (TA ? 0 : 2 * r);
       double v = 0.0;
-      if (gm < g.m && gk < kend) {
+      if (gm < Mrows && gk < kend) {
         v = qa[r * a_rstep];
         if (mA) v = gmask(v, TA ? gk : gm, TA ? gm : gk, g.smA);
       }
@@ -158,7 +187,6 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
   }
 
   // lane l, register v holds C[m0+wm0+16i+(l&15)][n0+wn0+16j+(l>>4)+4v]
-  double* C = g.C;
   const bool split = g.split_stride > 0;
   if (split) C += (int64_t)blockIdx.z * g.split_stride;
   const bool has_beta = !split && g.beta != 0.0;
@@ -166,7 +194,7 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int64_t gm = m0 + wm0 + 16 * i + fc;
-    if (!full_mn && gm >= g.m) continue;
+    if (!full_mn && gm >= Mrows) continue;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
 #pragma unroll
@@ -176,6 +204,7 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
         double* cp = C + gm + gn * g.ldc;
         double val = alpha * acc[i][j][v];
         if (has_beta) val += g.beta * *cp;
+        if (g.smC == SM_UPPER && gm > gn) val = 0.0;
         *cp = val;
       }
     }
@@ -252,6 +281,29 @@ static int gemm_f64_impl(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64
                      C, ldc, alpha, beta);
   prof_end(c, ev, PK_REDUCE, (double)ns * m * n, 8.0 * (ns + (beta != 0.0 ? 2.0 : 1.0)) * m * n);
   return check_launch(c, "splitk_reduce");
+}
+
+int gemm_f64_batched(cqr_ctx* c, int ta, int tb, int m, int n, int k, double alpha, const BMat& A, int smA,
+                     const BMat& B, int smB, double beta, const BMat& C, int smC, int batch, int64_t pm, int pL,
+                     int zoff, int max_m, int kind) {
+  if (batch <= 0 || n <= 0 || max_m <= 0) return CQR_OK;
+  GemmArgs g{};
+  g.m = m < 0 ? max_m : m; g.n = n; g.k = k; g.alpha = alpha; g.beta = beta;
+  g.A = A.p; g.lda = A.ld; g.smA = smA; g.B = B.p; g.ldb = B.ld; g.smB = smB; g.C = const_cast<double*>(C.p);
+  g.ldc = C.ld; g.kchunk = k; g.split_stride = 0;
+  g.batched = 1;
+  g.a_mode = A.mode; g.b_mode = B.mode; g.c_mode = C.mode;
+  g.a_zs = A.zs; g.b_zs = B.zs; g.c_zs = C.zs;
+  g.pm = pm; g.pL = pL; g.zoff = zoff; g.m_leaf = m < 0; g.smC = smC;
+  dim3 grid((unsigned)((max_m + 127) / 128), (unsigned)((n + 127) / 128), (unsigned)batch);
+  hipEvent_t ev = prof_begin(c);
+  if (!ta && !tb) hipLaunchKernelGGL((mfma_gemm_f64<false, false>), grid, dim3(256), 0, c->stream, g);
+  else if (ta && !tb) hipLaunchKernelGGL((mfma_gemm_f64<true, false>), grid, dim3(256), 0, c->stream, g);
+  else if (!ta && tb) hipLaunchKernelGGL((mfma_gemm_f64<false, true>), grid, dim3(256), 0, c->stream, g);
+  else hipLaunchKernelGGL((mfma_gemm_f64<true, true>), grid, dim3(256), 0, c->stream, g);
+  prof_end(c, ev, kind, 2.0 * batch * (double)max_m * n * k,
+           8.0 * batch * ((double)max_m * k + (double)k * n + 2.0 * max_m * n));
+  return check_launch(c, "mfma_gemm_f64_batched");
 }
 
 size_t gemm_f64_scratch_doubles(int64_t m, int64_t n, int64_t k) {

@@ -1,0 +1,329 @@
+// Column-pipelined TSQR reduction tree (all levels in one launch).
+//
+// The tree is the reference's binary schedule (alg/QR/tsqr/bitree_tsqr.cxx:
+// 172-232): on a level with n live ranks, receiver z stacks its R on the R
+// of sender z + ceil(n/2) and refactors the 2b×b triangle pair (dtpqrt with
+// l = b, :20-34).  Column i of a node's output R depends only on columns
+// 0..i of its two inputs, so every node runs left-looking and publishes each
+// finished column through a monotone counter; a parent starts column i as
+// soon as both children have published it.  All levels then advance
+// together and the tree costs about (b + depth) column steps instead of
+// b × depth.
+//
+// One workgroup (512 threads) per node.  Node ids follow forward level
+// order, so every producer has a smaller block id than its consumer and is
+// dispatched first; waits are bounded (a timeout raises a device flag and
+// the node finishes with whatever it has).
+//
+// Per column i of a node with reflectors [e_p; v2_p], p < i, and compact-WY
+// factor T (Q_node = I − V·T·Vᵀ, V = [I; V2]):
+//   s = c_top(0:i) + V2(:,0:i)ᵀ·c_bot,  u = Tᵀ·s,
+//   c_top(0:i) −= u,  c_bot −= V2(:,0:i)·u          (apply H_{i-1}···H_0)
+//   dlarfg on (c_top(i), c_bot(0:i)) → β, τ_i, v2_i; R_out(:,i) = [c_top(0:i); β]
+//   T(0:i, i) = −τ_i·T(0:i,0:i)·(V2(:,0:i)ᵀ·v2_i),  T(i,i) = τ_i.
+#include "cqr_device.h"
+#include "cqr_internal.h"
+
+namespace cqr {
+
+// Producer of rank r's current R when entering level `lv` (level sizes
+// n_0 = L, n_{k+1} = ceil(n_k / 2)): the node of the last earlier level at
+// which r was a receiver, or leaf r.  Returns a node id, or −1 for a leaf.
+__device__ inline int producer_of(int r, int lv, int L) {
+  int n = L, node0 = 0, best = -1;
+  for (int k = 0; k < lv; ++k) {
+    const int pairs = n / 2;
+    if (r < pairs) best = node0 + r;
+    node0 += pairs;
+    n = n / 2 + (n & 1);
+  }
+  return best;
+}
+
+__device__ inline uint64_t load_acquire_u64(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void store_release_u64(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Coherent (L1-bypassing) read of data another workgroup published.
+__device__ inline double load_agent(const double* p) {
+  const uint64_t u = __hip_atomic_load(reinterpret_cast<const uint64_t*>(p), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double((long long)u);
+}
+
+template <int CPW, int RPL>
+__global__ void __launch_bounds__(512)
+ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, double* __restrict__ V2out,
+             double* __restrict__ Tout, double* __restrict__ tauout, uint64_t* __restrict__ cnt,
+             uint64_t epoch, int L, int b, int* __restrict__ err) {
+  constexpr int WAVES = 8;
+  constexpr int SH = xreduce_shift<CPW>();
+  const int nid = blockIdx.x;
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int64_t bb = (int64_t)b * b;
+
+  // locate this node: level lv, pair z, live ranks n
+  int lv = 0, n = L, node0 = 0;
+  while (nid >= node0 + n / 2) {
+    node0 += n / 2;
+    n = n / 2 + (n & 1);
+    ++lv;
+  }
+  const int z = nid - node0, snd = z + (n + 1) / 2;
+  const int ptop = producer_of(z, lv, L), pbot = producer_of(snd, lv, L);
+  const double* Rt = ptop >= 0 ? nodeR + ptop * bb : leafR + z * bb;
+  const double* Rb = pbot >= 0 ? nodeR + pbot * bb : leafR + snd * bb;
+  double* Ro = nodeR + nid * bb;
+
+  __shared__ double sbuf[128], ubuf[128], zbuf[128], stau[128];
+  __shared__ double red[WAVES][128];
+  __shared__ int s_err;
+
+  double v2[CPW][RPL], t[CPW][RPL];
+#pragma unroll
+  for (int cl = 0; cl < CPW; ++cl)
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) v2[cl][s] = t[cl][s] = 0.0;
+  if (threadIdx.x == 0) s_err = 0;
+
+  for (int i = 0; i < b; ++i) {
+    // ---- wait for column i of both inputs
+    if (threadIdx.x == 0) {
+      const uint64_t want = epoch * 256 + (uint64_t)(i + 1);
+      for (int which = 0; which < 2; ++which) {
+        const int p = which ? pbot : ptop;
+        if (p < 0) continue;
+        long spins = 0;
+        while (load_acquire_u64(cnt + p) < want) {
+          __builtin_amdgcn_s_sleep(2);
+          if (++spins > (1L << 22)) {
+            s_err = 1;
+            break;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (s_err) break;
+    double ctop[RPL], cbot[RPL];
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+      const int row = lane + 64 * s;
+      const bool in = row <= i;
+      ctop[s] = in ? load_agent(Rt + row + (int64_t)i * b) : 0.0;
+      cbot[s] = in ? load_agent(Rb + row + (int64_t)i * b) : 0.0;
+    }
+    // ---- s = c_top + V2ᵀ c_bot   (columns q < i)
+    {
+      double part[CPW];
+#pragma unroll
+      for (int cl = 0; cl < CPW; ++cl) {
+        double a = 0.0;
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) a += v2[cl][s] * cbot[s];
+        part[cl] = a;
+      }
+      const double r = xreduce<CPW>(part, lane);
+      const int q = w + WAVES * (lane >> SH);
+      if ((lane & ((1 << SH) - 1)) == 0 && q < i) sbuf[q] = r + load_agent(Rt + q + (int64_t)i * b);
+    }
+    __syncthreads();
+    // ---- u = Tᵀ s
+    {
+      double sv[RPL];
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) {
+        const int row = lane + 64 * s;
+        sv[s] = row < i ? sbuf[row] : 0.0;
+      }
+      double part[CPW];
+#pragma unroll
+      for (int cl = 0; cl < CPW; ++cl) {
+        double a = 0.0;
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) a += t[cl][s] * sv[s];
+        part[cl] = a;
+      }
+      const double r = xreduce<CPW>(part, lane);
+      const int q = w + WAVES * (lane >> SH);
+      if ((lane & ((1 << SH) - 1)) == 0 && q < i) ubuf[q] = r;
+    }
+    __syncthreads();
+    // ---- c_top −= u; c_bot −= V2·u (partials per wave, summed through LDS)
+    {
+      double pb[RPL];
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) pb[s] = 0.0;
+#pragma unroll
+      for (int cl = 0; cl < CPW; ++cl) {
+        const int q = w + WAVES * cl;
+        if (q < i) {
+          const double uq = ubuf[q];
+#pragma unroll
+          for (int s = 0; s < RPL; ++s) pb[s] += v2[cl][s] * uq;
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) red[w][lane + 64 * s] = pb[s];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+      const int row = lane + 64 * s;
+      double acc = 0.0;
+#pragma unroll
+      for (int ww = 0; ww < WAVES; ++ww) acc += red[ww][row];
+      cbot[s] -= acc;
+      if (row < i) ctop[s] -= ubuf[row];
+    }
+    // ---- reflector for column i (every wave computes the same scalars)
+    double ss = 0.0, piv = 0.0;
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+      const int row = lane + 64 * s;
+      if (row <= i) ss += cbot[s] * cbot[s];
+      if (s == (i >> 6)) piv = ctop[s];
+    }
+    ss = wave_sum(ss);
+    const double alpha = readlane_d(piv, i & 63);
+    double xnorm;
+    if (ss_needs_rescale(ss)) {
+      double mx = 0.0;
+#pragma unroll
+      for (int s = 0; s < RPL; ++s)
+        if (lane + 64 * s <= i) mx = fmax(mx, fabs(cbot[s]));
+      mx = wave_max(mx);
+      if (mx == 0.0) {
+        xnorm = 0.0;
+      } else {
+        double s2 = 0.0;
+#pragma unroll
+        for (int s = 0; s < RPL; ++s)
+          if (lane + 64 * s <= i) { const double tt = cbot[s] / mx; s2 += tt * tt; }
+        xnorm = mx * sqrt(wave_sum(s2));
+      }
+    } else {
+      xnorm = sqrt(ss);
+    }
+    const Reflector h = dlarfg_scalar(alpha, xnorm);
+    double v[RPL];
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+      const int row = lane + 64 * s;
+      double x = cbot[s];
+      if (h.tau != 0.0 && row <= i) {
+        for (int k = 0; k < h.knt; ++k) x *= 0x1p969;
+        x *= h.scal;
+      }
+      v[s] = row <= i ? x : 0.0;
+    }
+    if (w == (i % WAVES)) {
+#pragma unroll
+      for (int cl = 0; cl < CPW; ++cl)
+        if (cl == i / WAVES)
+#pragma unroll
+          for (int s = 0; s < RPL; ++s) v2[cl][s] = v[s];
+    }
+    // ---- publish column i of R: [c_top(0:i); β]
+    if (w == 0) {
+      for (int s = 0; s < RPL; ++s) {
+        const int row = lane + 64 * s;
+        if (row < b) Ro[row + (int64_t)i * b] = row < i ? ctop[s] : (row == i ? h.beta : 0.0);
+      }
+      __threadfence();
+    }
+    if (threadIdx.x == 0) stau[i] = h.tau;
+    // ---- z = V2(:,0:i)ᵀ v (columns q < i)
+    {
+      double part[CPW];
+#pragma unroll
+      for (int cl = 0; cl < CPW; ++cl) {
+        double a = 0.0;
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) a += v2[cl][s] * v[s];
+        part[cl] = a;
+      }
+      const double r = xreduce<CPW>(part, lane);
+      const int q = w + WAVES * (lane >> SH);
+      if ((lane & ((1 << SH) - 1)) == 0 && q < i) zbuf[q] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) store_release_u64(cnt + nid, epoch * 256 + (uint64_t)(i + 1));
+    // ---- T(0:i, i) = −τ T(0:i,0:i) z: partial over each wave's columns
+    {
+      double py[RPL];
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) py[s] = 0.0;
+#pragma unroll
+      for (int cl = 0; cl < CPW; ++cl) {
+        const int q = w + WAVES * cl;
+        if (q < i) {
+          const double zq = zbuf[q];
+#pragma unroll
+          for (int s = 0; s < RPL; ++s) py[s] += t[cl][s] * zq;
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) red[w][lane + 64 * s] = py[s];
+    }
+    __syncthreads();
+    if (w == (i % WAVES)) {
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) {
+        const int row = lane + 64 * s;
+        double acc = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < WAVES; ++ww) acc += red[ww][row];
+        const double tv = row < i ? -h.tau * acc : (row == i ? h.tau : 0.0);
+#pragma unroll
+        for (int cl = 0; cl < CPW; ++cl)
+          if (cl == i / WAVES) t[cl][s] = tv;
+      }
+    }
+    __syncthreads();
+  }
+  if (s_err && threadIdx.x == 0) *err = 1;
+  // ---- V2, T, tau of this node
+  double* V2 = V2out + nid * bb;
+  double* Tn = Tout + nid * bb;
+#pragma unroll
+  for (int cl = 0; cl < CPW; ++cl) {
+    const int col = w + WAVES * cl;
+    if (col >= b) continue;
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+      const int row = lane + 64 * s;
+      if (row < b) {
+        V2[row + (int64_t)col * b] = row <= col ? v2[cl][s] : 0.0;
+        Tn[row + (int64_t)col * b] = row <= col ? t[cl][s] : 0.0;
+      }
+    }
+  }
+  for (int p = threadIdx.x; p < b; p += 512) tauout[(int64_t)nid * b + p] = stau[p];
+}
+
+template <int CPW, int RPL>
+static int ptree_launch(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
+                        uint64_t* cnt, int L, int nnodes, int b) {
+  const uint64_t epoch = ++c->tree_epoch;
+  hipEvent_t ev = prof_begin(c);
+  hipLaunchKernelGGL((ptree_kernel<CPW, RPL>), dim3(nnodes), dim3(512), 0, c->stream, leafR, nodeR, V2, T, tau,
+                     cnt, epoch, L, b, c->d_terr);
+  prof_end(c, ev, PK_TREE, nnodes * (2.0 / 3.0) * b * (double)b * b, nnodes * 40.0 * b * b);
+  return check_launch(c, "ptree");
+}
+
+int launch_ptree(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
+                 int L, int nnodes, int b) {
+  if (nnodes <= 0) return CQR_OK;
+  if (nnodes > CQR_MAX_TREE_NODES) return set_err(c, CQR_ERR_UNSUPPORTED, "tree of %d nodes", nnodes);
+  uint64_t* cnt = c->d_cnt;
+  if (b <= 16) return ptree_launch<2, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b);
+  if (b <= 32) return ptree_launch<4, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b);
+  if (b <= 64) return ptree_launch<8, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b);
+  return ptree_launch<16, 2>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b);
+}
+
+}  // namespace cqr

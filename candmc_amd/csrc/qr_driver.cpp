@@ -12,6 +12,7 @@ int wy_update_f64_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, c
                      double* A, int64_t lda, int64_t m, int64_t n, int b, double* ws);
 size_t gemm_f64_scratch_doubles(int64_t m, int64_t n, int64_t k);
 int launch_tril_halve(cqr_ctx* c, const double* G, int b, double* T, int64_t ldt);
+int launch_baxpy(cqr_ctx* c, double* C, const double* X, int64_t bb, int batch, double alpha);
 int launch_copy2d(cqr_ctx* c, const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m,
                   int64_t n);
 
@@ -19,7 +20,7 @@ int launch_copy2d(cqr_ctx* c, const double* src, int64_t lds, double* dst, int64
 // Panel workspace: every buffer TSQR + HR needs for one m×b panel.
 // ---------------------------------------------------------------------------
 struct PanelWS {
-  size_t leafR, leafT, leafTau, nodeV2, nodeT, nodeTau, C, X, Z, M, Y, Uinv, Winv, LinvT, signs,
+  size_t leafR, leafT, leafTau, nodeR, nodeV2, nodeT, nodeTau, C, X, Z, M, Y, Uinv, Winv, LinvT, signs,
       Wtmp, Ttmp;
 };
 
@@ -29,6 +30,7 @@ static PanelWS panel_layout(const TsqrPlan& p, Sizer& s) {
   w.leafR = s.add(8 * L * bb);
   w.leafT = s.add(8 * L * bb);
   w.leafTau = s.add(8 * L * b);
+  w.nodeR = s.add(8 * nn * bb);
   w.nodeV2 = s.add(8 * nn * bb);
   w.nodeT = s.add(8 * nn * bb);
   w.nodeTau = s.add(8 * nn * b);
@@ -64,21 +66,12 @@ static int reverse_tree(cqr_ctx* c, const TsqrPlan& p, const double* nodeV2, con
     const int npairs = (int)p.levels[lv].size();
     const int n = (int)(npairs == 0 ? 1 : p.levels[lv].back().second + 1);
     const int64_t node0 = p.level_node0[lv];
-    BgemmArgs g = bg(b, b, b, 1.0, 0.0);
-    g.A = bmat(nodeT + node0 * bb, b, bb); g.smA = SM_UPPER;
-    g.B = bmat(C, b, bb);
-    g.C = bmat(X, b, bb);
-    CQR_TRY(bgemm(c, g, npairs, b));
-    g = bg(b, b, b, -1.0, 0.0);
-    g.A = bmat(nodeV2 + node0 * bb, b, bb); g.smA = SM_UPPER;
-    g.B = bmat(X, b, bb);
-    g.C = bmat(C, b, bb, BM_SHIFTED); g.zoff = (n + 1) / 2; g.smC = SM_UPPER;
-    CQR_TRY(bgemm(c, g, npairs, b));
-    g = bg(b, b, b, -1.0, 1.0);
-    g.A = bmat(nullptr, b); g.smA = SM_IDENTITY;
-    g.B = bmat(X, b, bb);
-    g.C = bmat(C, b, bb);
-    CQR_TRY(bgemm(c, g, npairs, b));
+    CQR_TRY(gemm_f64_batched(c, 0, 0, b, b, b, 1.0, bmat(nodeT + node0 * bb, b, bb), SM_UPPER, bmat(C, b, bb),
+                             SM_NONE, 0.0, bmat(X, b, bb), SM_NONE, npairs, 0, 1, 0, b, PK_BGEMM));
+    CQR_TRY(gemm_f64_batched(c, 0, 0, b, b, b, -1.0, bmat(nodeV2 + node0 * bb, b, bb), SM_UPPER, bmat(X, b, bb),
+                             SM_NONE, 0.0, bmat(C, b, bb, BM_SHIFTED), SM_UPPER, npairs, 0, 1, (n + 1) / 2, b,
+                             PK_BGEMM));
+    CQR_TRY(launch_baxpy(c, C, X, bb, npairs, -1.0));
   }
   return CQR_OK;
 }
@@ -88,24 +81,14 @@ static int leaf_apply(cqr_ctx* c, const TsqrPlan& p, const double* A, int64_t ld
                       const double* C, double* Z, double* M, double* Q, int64_t ldq) {
   const int b = (int)p.b, L = (int)p.nleaves;
   const int64_t bb = (int64_t)b * b;
-  BgemmArgs g = bg(b, b, b, 1.0, 0.0);
-  g.A = bmat(A, lda, 0, BM_LEAFROWS); g.ta = 1; g.smA = SM_UNIT_LOWER;
-  g.B = bmat(C, b, bb);
-  g.C = bmat(Z, b, bb);
-  g.pm = p.m; g.pL = L;
-  CQR_TRY(bgemm(c, g, L, b));
-  g = bg(b, b, b, 1.0, 0.0);
-  g.A = bmat(leafT, b, bb); g.smA = SM_UPPER;
-  g.B = bmat(Z, b, bb);
-  g.C = bmat(M, b, bb);
-  CQR_TRY(bgemm(c, g, L, b));
+  CQR_TRY(gemm_f64_batched(c, 1, 0, b, b, b, 1.0, bmat(A, lda, 0, BM_LEAFROWS), SM_UNIT_LOWER, bmat(C, b, bb),
+                           SM_NONE, 0.0, bmat(Z, b, bb), SM_NONE, L, p.m, L, 0, b, PK_BGEMM));
+  CQR_TRY(gemm_f64_batched(c, 0, 0, b, b, b, 1.0, bmat(leafT, b, bb), SM_UPPER, bmat(Z, b, bb), SM_NONE, 0.0,
+                           bmat(M, b, bb), SM_NONE, L, p.m, L, 0, b, PK_BGEMM));
   CQR_TRY(init_q1(c, Q, ldq, p.m, L, b, C));
-  g = bg(-1, b, b, -1.0, 1.0);
-  g.A = bmat(A, lda, 0, BM_LEAFROWS); g.smA = SM_UNIT_LOWER;
-  g.B = bmat(M, b, bb);
-  g.C = bmat(Q, ldq, 0, BM_LEAFROWS);
-  g.pm = p.m; g.pL = L;
-  return bgemm(c, g, L, (int)p.max_leaf_rows);
+  return gemm_f64_batched(c, 0, 0, -1, b, b, -1.0, bmat(A, lda, 0, BM_LEAFROWS), SM_UNIT_LOWER, bmat(M, b, bb),
+                          SM_NONE, 1.0, bmat(Q, ldq, 0, BM_LEAFROWS), SM_NONE, L, p.m, L, 0,
+                          (int)p.max_leaf_rows, PK_BGEMM);
 }
 
 size_t panel_ws_bytes(int64_t m, int64_t b, int64_t nleaves, int64_t leaf_rows) {
@@ -134,11 +117,9 @@ int hh_recon_panel(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, dou
   double* nodeT = at(w.nodeT);
   // K1: leaves (bitree_tsqr.cxx:156)
   CQR_TRY(leaf_qr(c, A, lda, m, L, bi, leafR, leafT, at(w.leafTau)));
-  // K2: tree levels (bitree_tsqr.cxx:172-232)
-  for (size_t lv = 0; lv < p.levels.size(); ++lv) {
-    const int n = p.levels[lv].back().second + 1;
-    CQR_TRY(tree_level(c, leafR, n, (int)p.level_node0[lv], bi, nodeV2, nodeT, at(w.nodeTau)));
-  }
+  // K2: the reduction tree, all levels pipelined in one launch (bitree_tsqr.cxx:172-232)
+  CQR_TRY(launch_ptree(c, leafR, at(w.nodeR), nodeV2, nodeT, at(w.nodeTau), L, (int)p.nnodes, bi));
+  const double* Rroot = p.nnodes > 0 ? at(w.nodeR) + (p.nnodes - 1) * bb : leafR;
   // K3: construct_Q1 (construct_tsqr_Q.cxx:34-242) into the Y scratch
   CQR_TRY(reverse_tree(c, p, nodeV2, nodeT, at(w.C), at(w.X)));
   double* Y = at(w.Y);
@@ -146,38 +127,21 @@ int hh_recon_panel(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, dou
   // K4: LU(Q1_top − S), R := S·R, W = U·S (hh_recon.cxx:282-308)
   double* Wd = W ? W : at(w.Wtmp);
   int64_t* sg = signs_out ? signs_out : reinterpret_cast<int64_t*>(base + w.signs);
-  CQR_TRY(hr_core(c, Y, m, leafR, A, lda, Wd, at(w.Uinv), at(w.Winv), at(w.LinvT), sg, bi));
+  CQR_TRY(hr_core(c, Y, m, Rroot, A, lda, Wd, at(w.Uinv), at(w.Winv), at(w.LinvT), sg, bi));
   // Y = Q1·U⁻¹ below the top b rows (hh_recon.cxx:294-301)
-  if (m > b) {
-    BgemmArgs g = bg((int)(m - b), bi, bi, 1.0, 0.0);
-    g.A = bmat(Y + b, m);
-    g.B = bmat(at(w.Uinv), b); g.smB = SM_UPPER;
-    g.C = bmat(A + b, lda);
-    CQR_TRY(bgemm(c, g, 1, (int)(m - b)));
-  }
+  if (m > b)
+    CQR_TRY(gemm_f64_ws(c, 0, 0, m - b, b, b, 1.0, Y + b, m, SM_NONE, at(w.Uinv), b, SM_UPPER, 0.0, A + b, lda,
+                        nullptr));
   // T = −W·L⁻ᵀ, Tᵀ = −L⁻¹·Wᵀ, T⁻ᵀ = −W⁻ᵀ·L (W = −T·Y1ᵀ)
-  if (Tt) {
-    BgemmArgs g = bg(bi, bi, bi, -1.0, 0.0);
-    g.A = bmat(at(w.LinvT), b); g.ta = 1; g.smA = SM_UNIT_UPPER;
-    g.B = bmat(Wd, b); g.tb = 1; g.smB = SM_UPPER;
-    g.C = bmat(Tt, b);
-    CQR_TRY(bgemm(c, g, 1, bi));
-  }
-  if (T) {
-    BgemmArgs g = bg(bi, bi, bi, -1.0, 0.0);
-    g.A = bmat(Wd, b); g.smA = SM_UPPER;
-    g.B = bmat(at(w.LinvT), b); g.smB = SM_UNIT_UPPER;
-    g.C = bmat(T, b); g.smC = SM_UPPER;
-    CQR_TRY(bgemm(c, g, 1, bi));
-  }
-  if (invT_T) {
-    BgemmArgs g = bg(bi, bi, bi, -1.0, 0.0);
-    g.A = bmat(at(w.Winv), b); g.ta = 1; g.smA = SM_UPPER;
-    g.B = bmat(A, lda); g.smB = SM_UNIT_LOWER;
-    g.C = bmat(invT_T, b);
-    CQR_TRY(bgemm(c, g, 1, bi));
-  }
-  (void)bb;
+  if (Tt)
+    CQR_TRY(gemm_f64_ws(c, 1, 1, b, b, b, -1.0, at(w.LinvT), b, SM_UNIT_UPPER, Wd, b, SM_UPPER, 0.0, Tt, b,
+                        nullptr));
+  if (T)
+    CQR_TRY(gemm_f64_ws(c, 0, 0, b, b, b, -1.0, Wd, b, SM_UPPER, at(w.LinvT), b, SM_UNIT_UPPER, 0.0, T, b,
+                        nullptr));
+  if (invT_T)
+    CQR_TRY(gemm_f64_ws(c, 1, 0, b, b, b, -1.0, at(w.Winv), b, SM_UPPER, A, lda, SM_UNIT_LOWER, 0.0, invT_T, b,
+                        nullptr));
   return CQR_OK;
 }
 
@@ -304,18 +268,16 @@ int cqr_bitree_tsqr(cqr_ctx* c, double* A, int64_t lda, double* R, double* tau, 
   if (s != CQR_OK) return set_err(c, s, "bitree_tsqr: cannot plan");
   CQR_TRY(finite_check(c, A, lda, m, b));
   const int64_t bb = b * b, L = p.nleaves, nn = p.nnodes;
-  CQR_TRY(ensure_ws(c, 8 * (L * bb) + 256));
+  CQR_TRY(ensure_ws(c, 8 * ((L + nn) * bb) + 512));
   double* leafR = static_cast<double*>(c->ws);
+  double* nodeR = leafR + L * bb;
   double* leafT = tree_data;
   double* nodeV2 = leafT + L * bb;
   double* nodeT = nodeV2 + nn * bb;
   double* nodeTau = nodeT + nn * bb;
   CQR_TRY(leaf_qr(c, A, lda, m, (int)L, (int)b, leafR, leafT, tau));
-  for (size_t lv = 0; lv < p.levels.size(); ++lv) {
-    const int n = p.levels[lv].back().second + 1;
-    CQR_TRY(tree_level(c, leafR, n, (int)p.level_node0[lv], (int)b, nodeV2, nodeT, nodeTau));
-  }
-  return launch_copy2d(c, leafR, b, R, b, b, b);
+  CQR_TRY(launch_ptree(c, leafR, nodeR, nodeV2, nodeT, nodeTau, (int)L, (int)nn, (int)b));
+  return launch_copy2d(c, nn > 0 ? nodeR + (nn - 1) * bb : leafR, b, R, b, b, b);
 }
 
 int cqr_construct_Q1(cqr_ctx* c, const double* Y, int64_t lda_Y, const double* tau, double* Q1,

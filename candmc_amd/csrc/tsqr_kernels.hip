@@ -455,27 +455,73 @@ __global__ void __launch_bounds__(256) bgemm_kernel(BgemmArgs g) {
 // s_i = −1 if the pivot is positive else +1, pivot −= s_i, column scaled by
 // the pivot (division), rank-1 update of the trailing block.
 // ---------------------------------------------------------------------------
-template <int NT>
-__device__ void signed_lu_lds(double* M, int b, double* sgn) {
+// Signed LU of a b×b block (alg/QR/hh_recon/hh_recon.cxx:43-70), b ≤ 128,
+// 512 threads: the block lives in registers (wave w owns columns
+// w, w+8, ..., lane l rows l and l+64).  Step i: the owner wave of column i
+// picks s_i (−1 if the pivot is positive, else +1), subtracts it, divides
+// the column below the pivot by the pivot and publishes it through LDS; one
+// barrier later every wave applies the rank-1 update to its columns, the
+// pivot-row entries coming from lane i%64 by readlane.  On exit the L\U
+// factors are in M (LDS, ld b) and the signs in sgn.
+__device__ void signed_lu_regs(const double* __restrict__ src, int64_t ld, double* M, double* sgn,
+                               double* lbuf, int b) {
+  constexpr int WAVES = 8, CPW = 16, RPL = 2;
   const int lane = threadIdx.x & 63, w = wave_id();
-  constexpr int NW = NT / 64;
-  for (int i = 0; i < b; ++i) {
-    const double s = M[i + i * b] > 0.0 ? -1.0 : 1.0;
-    const double piv = M[i + i * b] - s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      M[i + i * b] = piv;
-      sgn[i] = s;
+  double a[CPW][RPL];
+#pragma unroll
+  for (int cl = 0; cl < CPW; ++cl) {
+    const int col = w + WAVES * cl;
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+      const int row = lane + 64 * s;
+      a[cl][s] = (col < b && row < b) ? src[row + (int64_t)col * ld] : 0.0;
     }
-    if (w == 0)
-      for (int rr = i + 1 + lane; rr < b; rr += 64) M[rr + i * b] = M[rr + i * b] / piv;
-    __syncthreads();
-    for (int cc = i + 1 + w; cc < b; cc += NW) {
-      const double u = M[i + cc * b];
-      for (int rr = i + 1 + lane; rr < b; rr += 64) M[rr + cc * b] -= M[rr + i * b] * u;
-    }
-    __syncthreads();
   }
+  for (int i = 0; i < b; ++i) {
+    const int ow = i % WAVES, ocl = i / WAVES, plane = i & 63, pslot = i >> 6;
+    double* lb = lbuf + (i & 1) * 128;
+    if (w == ow) {
+#pragma unroll
+      for (int cl = 0; cl < CPW; ++cl) {
+        if (cl != ocl) continue;
+        const double pv = readlane_d(pslot ? a[cl][1] : a[cl][0], plane);
+        const double sg = pv > 0.0 ? -1.0 : 1.0;
+        const double piv = pv - sg;
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) {
+          const int row = lane + 64 * s;
+          if (row > i) a[cl][s] = a[cl][s] / piv;
+          if (row == i) a[cl][s] = piv;
+          lb[row] = row > i ? a[cl][s] : 0.0;
+        }
+        if (lane == 0) sgn[i] = sg;
+      }
+    }
+    __syncthreads();
+    double l[RPL];
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) l[s] = lb[lane + 64 * s];
+#pragma unroll
+    for (int cl = 0; cl < CPW; ++cl) {
+      const int col = w + WAVES * cl;
+      if (col > i && col < b) {
+        const double u = readlane_d(pslot ? a[cl][1] : a[cl][0], plane);
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) a[cl][s] -= l[s] * u;
+      }
+    }
+  }
+#pragma unroll
+  for (int cl = 0; cl < CPW; ++cl) {
+    const int col = w + WAVES * cl;
+    if (col >= b) continue;
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) {
+      const int row = lane + 64 * s;
+      if (row < b) M[row + col * b] = a[cl][s];
+    }
+  }
+  __syncthreads();
 }
 
 // K4: Householder reconstruction core (one workgroup).
@@ -489,10 +535,9 @@ hr_core_kernel(const double* __restrict__ Q1top, int64_t ldq, const double* __re
   double* M = smem;            // b×b
   double* sgn = M + b * b;     // b
   double* d = sgn + b;         // b
+  double* lbuf = d + b;        // 2 × 128
   const int tid = threadIdx.x;
-  FOR_RC(rr, cc, b, NT) M[rr + cc * b] = Q1top[rr + (int64_t)cc * ldq];
-  __syncthreads();
-  signed_lu_lds<NT>(M, b, sgn);
+  signed_lu_regs(Q1top, ldq, M, sgn, lbuf, b);
   // outputs: L below the diagonal of Atop, S·R on and above it, W = U·S
   FOR_RC(rr, cc, b, NT) {
     const int e = rr + cc * b;
@@ -528,10 +573,9 @@ signed_nlu_kernel(double* __restrict__ A, int64_t lda, double* __restrict__ R, i
   extern __shared__ double smem[];
   double* M = smem;
   double* sgn = M + b * b;
+  double* lbuf = sgn + b;
   const int tid = threadIdx.x;
-  FOR_RC(rr, cc, b, NT) M[rr + cc * b] = A[rr + (int64_t)cc * lda];
-  __syncthreads();
-  signed_lu_lds<NT>(M, b, sgn);
+  signed_lu_regs(A, lda, M, sgn, lbuf, b);
   FOR_RC(rr, cc, b, NT) {
     A[rr + cc * lda] = M[rr + cc * b];
     if (R && rr <= cc && sgn[rr] < 0) R[rr + cc * ldr] = -R[rr + cc * ldr];
@@ -665,6 +709,13 @@ __global__ void diag_kernel(const double* __restrict__ T, int64_t ldt, double* _
 // C -= X element-wise over z-batched b×b blocks is done with bgemm
 // (SM_IDENTITY); no extra kernel needed.
 
+// C_z += alpha·X_z over z-batched b×b blocks (reverse-tree C_r −= X)
+__global__ void baxpy_kernel(double* __restrict__ C, const double* __restrict__ X, int64_t bb, double alpha) {
+  const int z = blockIdx.y;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < bb; e += (int64_t)gridDim.x * blockDim.x)
+    C[z * bb + e] += alpha * X[z * bb + e];
+}
+
 // T := tril(G) with the diagonal halved (qr_2d.cxx:42-51)
 __global__ void tril_halve_kernel(const double* __restrict__ G, int b, double* __restrict__ T, int64_t ldt) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -759,7 +810,7 @@ int hr_core(cqr_ctx* c, const double* Q1top, int64_t ldq, const double* Rroot, d
             int64_t lda, double* W, double* Uinv, double* Winv, double* LinvT, int64_t* signs, int b) {
   static bool attr = false;
   if (!attr) { CQR_TRY(big_lds_attr(c, (const void*)hr_core_kernel)); attr = true; }
-  const size_t lds = sizeof(double) * ((size_t)b * b + 2 * b);
+  const size_t lds = sizeof(double) * ((size_t)b * b + 2 * b + 256);
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(hr_core_kernel, dim3(1), dim3(512), lds, c->stream, Q1top, ldq, Rroot, Atop,
                      lda, W, Uinv, Winv, LinvT, signs, b);
@@ -770,7 +821,7 @@ int hr_core(cqr_ctx* c, const double* Q1top, int64_t ldq, const double* Rroot, d
 int launch_signed_nlu(cqr_ctx* c, double* A, int64_t lda, double* R, int64_t lda_R, int64_t* signs, int b) {
   static bool attr = false;
   if (!attr) { CQR_TRY(big_lds_attr(c, (const void*)signed_nlu_kernel)); attr = true; }
-  const size_t lds = sizeof(double) * ((size_t)b * b + b);
+  const size_t lds = sizeof(double) * ((size_t)b * b + b + 256);
   hipLaunchKernelGGL(signed_nlu_kernel, dim3(1), dim3(512), lds, c->stream, A, lda, R, lda_R, signs, b);
   return check_launch(c, "signed_nlu");
 }
@@ -829,6 +880,13 @@ int launch_finite_scan(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int6
   if (gx <= 0) return CQR_OK;
   hipLaunchKernelGGL(finite_scan_kernel, dim3(gx), dim3(256), 0, c->stream, A, lda, m, n, c->d_flag);
   return check_launch(c, "finite_scan");
+}
+
+int launch_baxpy(cqr_ctx* c, double* C, const double* X, int64_t bb, int batch, double alpha) {
+  if (batch <= 0) return CQR_OK;
+  const int gx = (int)std::min<int64_t>(64, (bb + 255) / 256);
+  hipLaunchKernelGGL(baxpy_kernel, dim3(gx, batch), dim3(256), 0, c->stream, C, X, bb, alpha);
+  return check_launch(c, "baxpy");
 }
 
 int launch_tril_halve(cqr_ctx* c, const double* G, int b, double* T, int64_t ldt) {

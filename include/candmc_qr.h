@@ -41,6 +41,9 @@ int cqr_create(cqr_ctx** ctx, int device);
 int cqr_destroy(cqr_ctx* ctx);
 int cqr_set_stream(cqr_ctx* ctx, void* hip_stream);   /* NULL = default stream */
 void* cqr_get_stream(const cqr_ctx* ctx);
+/* Waits for the context's stream; returns CQR_ERR_DEVICE if a TSQR tree
+ * node launched since the last sync gave up waiting for a child (its
+ * results are then invalid). */
 int cqr_sync(cqr_ctx* ctx);
 int cqr_reserve(cqr_ctx* ctx, size_t bytes);          /* pre-grow the arena   */
 size_t cqr_workspace_bytes(const cqr_ctx* ctx);

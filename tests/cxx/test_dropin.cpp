@@ -1,0 +1,94 @@
+// Drop-in check in the style of the reference's test/QR/test_hh_recon.cxx
+// and test/QR/test_qr_2d.cxx: the same per-element drand48 inputs, calls
+// through the reference signatures (host arrays), and the reference's
+// acceptance metric Σ_i ‖(A − QR)(:,i)‖/‖A(:,i)‖ ≤ 1e-9 computed on the
+// host with plain loops.  Exit status 0 on success.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "candmc/CANDMC_QR.h"
+
+static double colrel_residual(const std::vector<double>& A, const std::vector<double>& YR, int64_t m,
+                              int64_t n, bool flip_last) {
+  std::vector<double> QR((size_t)m * n, 0.0);
+  for (int64_t j = 0; j < n; ++j)
+    for (int64_t i = 0; i <= j && i < m; ++i) QR[i + j * m] = YR[i + j * m];
+  if (flip_last) QR[(m - 1) + (n - 1) * m] *= -1.0;  // test/QR/test_qr_2d.cxx:135-137
+  for (int64_t i = std::min(m, n) - 1; i >= 0; --i) {  // Q = H_1 ... H_n
+    double nrm = 0.0;
+    for (int64_t r = i + 1; r < m; ++r) nrm += YR[r + i * m] * YR[r + i * m];
+    double tau = 2.0 / (nrm + 1.0);
+    if (tau == 2.0) tau = 0.0;                          // alg/shared/util.cxx:451-456
+    for (int64_t j = 0; j < n; ++j) {
+      double w = QR[i + j * m];
+      for (int64_t r = i + 1; r < m; ++r) w += YR[r + i * m] * QR[r + j * m];
+      QR[i + j * m] -= tau * w;
+      for (int64_t r = i + 1; r < m; ++r) QR[r + j * m] -= tau * w * YR[r + i * m];
+    }
+  }
+  double s = 0.0;
+  for (int64_t j = 0; j < n; ++j) {
+    double d = 0.0, a = 0.0;
+    for (int64_t r = 0; r < m; ++r) {
+      d += (A[r + j * m] - QR[r + j * m]) * (A[r + j * m] - QR[r + j * m]);
+      a += A[r + j * m] * A[r + j * m];
+    }
+    s += std::sqrt(d / a);
+  }
+  return s;
+}
+
+static double seeded(int64_t row, int64_t col, int64_t m) {
+  srand48(99900 + (row + col * m) * 61);
+  return drand48();
+}
+
+int main() {
+  int fails = 0;
+  CommData_t cdt{nullptr, 1, 0, 0, 1};
+  {  // test_hh_recon.cxx defaults at 4 ranks: b = 17, m = 34·4
+    const int64_t P = 4, b = 17, m = 34 * P;
+    std::vector<double> A((size_t)m * b), YR, W((size_t)b * b);
+    for (int64_t c = 0; c < b; ++c)
+      for (int64_t r = 0; r < m; ++r) A[r + c * m] = seeded(r, c, m);
+    YR = A;
+    hh_recon_qr(YR.data(), m, m, b, W.data(), 0, P, 0, 0, cdt);
+    const double norm = colrel_residual(A, YR, m, b, false);
+    printf("hh_recon_qr m=%lld b=%lld: ||A-QR||_2 = %.2E\n", (long long)m, (long long)b, norm);
+    if (!(norm <= 1e-9)) { printf("TEST FAILED!!!\n"); ++fails; } else printf("Test successful.\n");
+  }
+  {  // test_qr_2d.cxx input (drand48 − 0.5) through QR_2D and QR_2D_2D
+    const int64_t m = 96, k = 48;
+    std::vector<double> A((size_t)m * k);
+    for (int64_t c = 0; c < k; ++c)
+      for (int64_t r = 0; r < m; ++r) A[r + c * m] = seeded(r, c, m) - 0.5;
+    pview pv{};
+    pv.crow = cdt; pv.ccol = cdt; pv.cworld = cdt;
+    std::vector<double> YR = A;
+    QR_2D(YR.data(), m, m, k, 8, &pv, nullptr, 0);
+    double norm = colrel_residual(A, YR, m, k, m == k);
+    printf("QR_2D m=%lld k=%lld b=8: ||A-QR||_2 = %.2E\n", (long long)m, (long long)k, norm);
+    if (!(norm <= 1e-9)) { printf("TEST FAILED!!!\n"); ++fails; } else printf("Test successful.\n");
+    YR = A;
+    QR_2D_2D(YR.data(), m, m, k, 16, 8, &pv, nullptr, 0);
+    norm = colrel_residual(A, YR, m, k, m == k);
+    printf("QR_2D_2D m=%lld k=%lld b2=16 b=8: ||A-QR||_2 = %.2E\n", (long long)m, (long long)k, norm);
+    if (!(norm <= 1e-9)) { printf("TEST FAILED!!!\n"); ++fails; } else printf("Test successful.\n");
+  }
+  {  // square: the reference flips the last R entry before comparing
+    const int64_t m = 64, k = 64;
+    std::vector<double> A((size_t)m * k);
+    srand48(7);
+    for (auto& x : A) x = drand48() - 0.5;
+    pview pv{};
+    pv.crow = cdt; pv.ccol = cdt; pv.cworld = cdt;
+    std::vector<double> YR = A;
+    QR_2D_pipe(YR.data(), m, m, k, 16, &pv);
+    const double norm = colrel_residual(A, YR, m, k, true);
+    printf("QR_2D_pipe m=k=64 b=16: ||A-QR||_2 = %.2E\n", norm);
+    if (!(norm <= 1e-9)) { printf("TEST FAILED!!!\n"); ++fails; } else printf("Test successful.\n");
+  }
+  return fails ? 1 : 0;
+}
