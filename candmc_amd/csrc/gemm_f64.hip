@@ -6,7 +6,9 @@
 //   A −= Y·Z'          (:275)
 // One kernel template serves all three: 128×128 block tile, 4 waves of
 // 64×64, v_mfma_f64_16x16x4f64, LDS double buffering with a register
-// prefetch of the next K slab.
+// prefetch of the next K slab.  The ROLE template argument only tags the
+// instantiation (0 general, 1 Z = Yᵀ·A, 2 A −= Y·Z', 3 batched per-leaf
+// products) so that kernel traces attribute time to each use.
 #include "cqr_device.h"
 #include "cqr_internal.h"
 
@@ -67,7 +69,7 @@ __device__ __forceinline__ bool mask_active(int sm, int64_t r_lo, int64_t r_hi, 
   }
 }
 
-template <bool TA, bool TB>
+template <bool TA, bool TB, int ROLE>
 __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
   constexpr int BM = 128, BN = 128, BK = 16, PAD = 4;
   constexpr int LDS_A = BM + PAD, LDS_B = BN + PAD;
@@ -227,13 +229,25 @@ __global__ void splitk_reduce_kernel(const double* __restrict__ P, int64_t strid
   }
 }
 
+template <int ROLE>
+static void launch_role(hipStream_t st, int ta, int tb, dim3 grid, const GemmArgs& g) {
+  if (!ta && !tb) hipLaunchKernelGGL((mfma_gemm_f64<false, false, ROLE>), grid, dim3(256), 0, st, g);
+  else if (ta && !tb) hipLaunchKernelGGL((mfma_gemm_f64<true, false, ROLE>), grid, dim3(256), 0, st, g);
+  else if (!ta && tb) hipLaunchKernelGGL((mfma_gemm_f64<false, true, ROLE>), grid, dim3(256), 0, st, g);
+  else hipLaunchKernelGGL((mfma_gemm_f64<true, true, ROLE>), grid, dim3(256), 0, st, g);
+}
+
+static void launch_kind(cqr_ctx* c, int ta, int tb, dim3 grid, const GemmArgs& g, int kind) {
+  if (kind == PK_GEMM_YTA) launch_role<1>(c->stream, ta, tb, grid, g);
+  else if (kind == PK_GEMM_UPD) launch_role<2>(c->stream, ta, tb, grid, g);
+  else if (kind == PK_BGEMM) launch_role<3>(c->stream, ta, tb, grid, g);
+  else launch_role<0>(c->stream, ta, tb, grid, g);
+}
+
 static int launch_mfma(cqr_ctx* c, int ta, int tb, const GemmArgs& g, int nsplit, int kind) {
   hipEvent_t ev = prof_begin(c);
   dim3 grid((unsigned)((g.m + 127) / 128), (unsigned)((g.n + 127) / 128), (unsigned)nsplit);
-  if (!ta && !tb) hipLaunchKernelGGL((mfma_gemm_f64<false, false>), grid, dim3(256), 0, c->stream, g);
-  else if (ta && !tb) hipLaunchKernelGGL((mfma_gemm_f64<true, false>), grid, dim3(256), 0, c->stream, g);
-  else if (!ta && tb) hipLaunchKernelGGL((mfma_gemm_f64<false, true>), grid, dim3(256), 0, c->stream, g);
-  else hipLaunchKernelGGL((mfma_gemm_f64<true, true>), grid, dim3(256), 0, c->stream, g);
+  launch_kind(c, ta, tb, grid, g, kind);
   const double out = (double)g.m * g.n;
   const double bytes = 8.0 * ((double)g.m * g.k + (double)g.k * g.n +
                               (nsplit > 1 ? nsplit * out : (g.beta != 0.0 ? 2.0 : 1.0) * out));
@@ -297,10 +311,7 @@ int gemm_f64_batched(cqr_ctx* c, int ta, int tb, int m, int n, int k, double alp
   g.pm = pm; g.pL = pL; g.zoff = zoff; g.m_leaf = m < 0; g.smC = smC;
   dim3 grid((unsigned)((max_m + 127) / 128), (unsigned)((n + 127) / 128), (unsigned)batch);
   hipEvent_t ev = prof_begin(c);
-  if (!ta && !tb) hipLaunchKernelGGL((mfma_gemm_f64<false, false>), grid, dim3(256), 0, c->stream, g);
-  else if (ta && !tb) hipLaunchKernelGGL((mfma_gemm_f64<true, false>), grid, dim3(256), 0, c->stream, g);
-  else if (!ta && tb) hipLaunchKernelGGL((mfma_gemm_f64<false, true>), grid, dim3(256), 0, c->stream, g);
-  else hipLaunchKernelGGL((mfma_gemm_f64<true, true>), grid, dim3(256), 0, c->stream, g);
+  launch_kind(c, ta, tb, grid, g, kind);
   prof_end(c, ev, kind, 2.0 * batch * (double)max_m * n * k,
            8.0 * batch * ((double)max_m * k + (double)k * n + 2.0 * max_m * n));
   return check_launch(c, "mfma_gemm_f64_batched");

@@ -57,7 +57,7 @@ class Context:
     def set_check_finite(self, on: bool):
         self.check(self.lib.cqr_set_check_finite(self.h, 1 if on else 0))
 
-    PROFILE_KINDS = ("gemm_yta", "gemm_update", "gemm_small", "splitk_reduce", "leaf_qr", "tree_level",
+    PROFILE_KINDS = ("gemm_yta", "gemm_update", "gemm_small", "splitk_reduce", "leaf_qr", "tree",
                      "hr_core", "bgemm")
 
     def profile(self, on: bool):

@@ -54,7 +54,7 @@ int cqr_set_leaf_rows(cqr_ctx* ctx, int64_t rows);    /* 0 = automatic        */
  * the stream and returns, for one kernel class, the summed event time,
  * the algorithmic flops and HBM bytes of those launches and their count.
  * Classes: 0 Yᵀ·A split-K GEMM, 1 A −= Y·Z' update GEMM, 2 small b×b
- * GEMMs on MFMA, 3 split-K reduction, 4 leaf QR, 5 tree level, 6 HR core,
+ * GEMMs on MFMA, 3 split-K reduction, 4 leaf QR, 5 TSQR tree, 6 HR core,
  * 7 batched small GEMMs of the panel.                                    */
 int cqr_profile_enable(cqr_ctx* ctx, int on);
 int cqr_profile_read(cqr_ctx* ctx, int kind, double* ms, double* flops, double* bytes,
