@@ -143,6 +143,8 @@ int cqr_create(cqr_ctx** ctx, int device) {
   if (hipMalloc(&c->d_flag, sizeof(int)) != hipSuccess ||
       hipMalloc(&c->d_cnt, sizeof(uint64_t) * CQR_MAX_TREE_NODES) != hipSuccess ||
       hipMalloc(&c->d_terr, sizeof(int)) != hipSuccess ||
+      hipMalloc(&c->d_qflags, sizeof(uint64_t) * CQR_MAX_TREE_NODES * 8) != hipSuccess ||
+      hipMemset(c->d_qflags, 0, sizeof(uint64_t) * CQR_MAX_TREE_NODES * 8) != hipSuccess ||
       hipMemset(c->d_terr, 0, sizeof(int)) != hipSuccess ||
       hipMemset(c->d_cnt, 0, sizeof(uint64_t) * CQR_MAX_TREE_NODES) != hipSuccess ||
       hipMemset(c->d_flag, 0, sizeof(int)) != hipSuccess) {
@@ -150,6 +152,7 @@ int cqr_create(cqr_ctx** ctx, int device) {
     if (c->d_flag) (void)hipFree(c->d_flag);
     if (c->d_cnt) (void)hipFree(c->d_cnt);
     if (c->d_terr) (void)hipFree(c->d_terr);
+    if (c->d_qflags) (void)hipFree(c->d_qflags);
     delete c;
     return CQR_ERR_NOMEM;
   }
@@ -170,6 +173,7 @@ int cqr_destroy(cqr_ctx* c) {
   if (c->d_flag) (void)hipFree(c->d_flag);
   if (c->d_cnt) (void)hipFree(c->d_cnt);
   if (c->d_terr) (void)hipFree(c->d_terr);
+  if (c->d_qflags) (void)hipFree(c->d_qflags);
   delete c;
   return CQR_OK;
 }

@@ -50,42 +50,80 @@ __device__ inline double wave_max(double x) {
   return readlane_d(x, 63);
 }
 
-// Reduce N per-lane partial values (N a power of two ≤ 64) across the 64
-// lanes of a wave with log2(64) exchange steps in total: the first log2(N)
-// steps halve the number of values each lane carries (each lane sends the
-// half it does not keep), the rest are plain butterflies.  On return every
-// lane holds the full sum of value index ((lane >> log2(64/N)) & (N-1)).
-template <int C, int O, int N>
-__device__ inline void xreduce_stage(double (&v)[N], int lane) {
-  if constexpr (C > 1) {
-    const bool hi = (lane & O) != 0;
+// Reduce N per-lane partial values (N = 1, 2, 4, 8 or 16) across the 64
+// lanes of a wave.  The first log2(N) steps halve the number of values each
+// lane carries: lanes pair up through a DPP pattern that flips one lane bit
+// (row mirror: bit 3, half-row mirror: bit 2, quad permutes: bits 1 and 0)
+// and each lane sends the half its partner keeps.  The remaining lane bits
+// are summed with DPP row rotations within a row of 16 and two cross-row
+// exchanges.  On return lane l holds the full sum of value index l & (N−1);
+// lane k (k < N) is the designated holder of index k.
+template <int CTRL, int HB, int C, int N>
+__device__ inline void xr_halve(double (&v)[N], int lane) {
+  const bool hi = ((lane >> HB) & 1) != 0;
 #pragma unroll
-    for (int t = 0; t < C / 2; ++t) {
-      const double send = hi ? v[t] : v[t + C / 2];
-      const double keep = hi ? v[t + C / 2] : v[t];
-      v[t] = keep + __shfl_xor(send, O, 64);
-    }
-    xreduce_stage<C / 2, O / 2, N>(v, lane);
+  for (int t = 0; t < C / 2; ++t) {
+    const double send = hi ? v[t] : v[t + C / 2];
+    const double keep = hi ? v[t + C / 2] : v[t];
+    v[t] = keep + dpp_d<CTRL, 0xf>(send);
   }
 }
-template <int O>
-__device__ inline double xreduce_tail(double s) {
-  if constexpr (O > 0) {
-    s += __shfl_xor(s, O, 64);
-    return xreduce_tail<O / 2>(s);
-  } else {
-    return s;
-  }
+__device__ inline double xr_cross_rows(double s) {
+  s += __shfl_xor(s, 16, 64);
+  s += __shfl_xor(s, 32, 64);
+  return s;
 }
 template <int N>
 __device__ inline double xreduce(double (&v)[N], int lane) {
-  static_assert(N >= 1 && N <= 64 && (N & (N - 1)) == 0, "N power of two");
-  xreduce_stage<N, 32, N>(v, lane);
-  return xreduce_tail<32 / N>(v[0]);
+  static_assert(N == 1 || N == 2 || N == 4 || N == 8 || N == 16, "N in {1,2,4,8,16}");
+  if constexpr (N == 16) xr_halve<0x140, 3, 16, N>(v, lane);  // row_mirror
+  if constexpr (N >= 8) xr_halve<0x141, 2, (N >= 16 ? 8 : N), N>(v, lane);  // row_half_mirror
+  if constexpr (N >= 4) xr_halve<0x4E, 1, (N >= 8 ? 4 : N), N>(v, lane);    // quad_perm [2,3,0,1]
+  if constexpr (N >= 2) xr_halve<0xB1, 0, 2, N>(v, lane);                    // quad_perm [1,0,3,2]
+  double s = v[0];
+  if constexpr (N == 1) s += dpp_d<0xB1, 0xf>(s);
+  if constexpr (N <= 2) s += dpp_d<0x4E, 0xf>(s);
+  if constexpr (N <= 4) s += dpp_d<0x124, 0xf>(s);  // row_ror:4
+  if constexpr (N <= 8) s += dpp_d<0x128, 0xf>(s);  // row_ror:8
+  return xr_cross_rows(s);
 }
-template <int N>
-__host__ __device__ constexpr int xreduce_shift() {
-  return N >= 64 ? 0 : N >= 32 ? 1 : N >= 16 ? 2 : N >= 8 ? 3 : N >= 4 ? 4 : N >= 2 ? 5 : 6;
+
+// ---------------------------------------------------------------------------
+// Hand-off of data between workgroups of one launch without cache-wide
+// fences.  Every value that crosses workgroups is written and read with
+// device-scope (agent) relaxed atomics, which the hardware keeps coherent
+// across the XCDs' L2 caches (write-through on store, no stale hit on
+// load).  A producer stores its data, waits for all of its memory
+// operations to complete (s_waitcnt 0 in every wave, then a barrier) and
+// only then stores the flag; a consumer polls the flag, meets its
+// workgroup at a barrier and then loads the data.  This replaces the
+// release/acquire fences, which write back and invalidate whole L2 caches.
+// ---------------------------------------------------------------------------
+__device__ inline double load_coherent(const double* p) {
+  const uint64_t u = __hip_atomic_load(reinterpret_cast<const uint64_t*>(p), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double((long long)u);
+}
+__device__ inline void store_coherent(double* p, double v) {
+  __hip_atomic_store(reinterpret_cast<uint64_t*>(p), (uint64_t)__double_as_longlong(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint64_t flag_load(const uint64_t* f) {
+  return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void flag_store(uint64_t* f, uint64_t v) {
+  __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// All of this wave's outstanding memory operations have completed.
+__device__ inline void wait_memory() { __builtin_amdgcn_s_waitcnt(0); }
+// Bounded wait (by one thread) for *f ≥ want; false on timeout.
+__device__ inline bool flag_wait(const uint64_t* f, uint64_t want) {
+  long spins = 0;
+  while (flag_load(f) < want) {
+    __builtin_amdgcn_s_sleep(2);
+    if (++spins > (1L << 22)) return false;
+  }
+  return true;
 }
 
 // ---------------------------------------------------------------------------

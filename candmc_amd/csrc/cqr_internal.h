@@ -36,6 +36,8 @@ struct cqr_ctx {
   int* d_flag = nullptr;        // device flag for non-finite scans / tree timeouts
   uint64_t* d_cnt = nullptr;    // per-node column counters of the pipelined tree
   int* d_terr = nullptr;        // set by a tree node whose wait for a child timed out
+  uint64_t* d_qflags = nullptr; // per (node, column block) flags of construct_Q1
+  uint64_t q1_epoch = 0;
   uint64_t tree_epoch = 0;      // launch epoch folded into the counters
   std::string err;
 };
@@ -132,6 +134,11 @@ inline BMat bmat(const double* p, int64_t ld, int64_t zs = 0, int mode = BM_STRI
 // in nodeR (nnodes×b×b), V2/T/tau per node in forward level order.
 int launch_ptree(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
                  int L, int nnodes, int b);
+
+// construct_Q1 over the whole tree and the leaves in one launch
+// (q1_kernel.hip): C is scratch of L×b×b; Q receives the m×b Q1 (ld ldq).
+int launch_q1(cqr_ctx* c, const double* nodeV2, const double* nodeT, const double* leafT, const double* V,
+              int64_t ldv, int64_t m, double* C, double* Q, int64_t ldq, int L, int nnodes, int b);
 
 // Kernel launchers (tsqr_kernels.hip)
 int leaf_qr(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int b, double* R, double* T,

@@ -40,26 +40,12 @@ __device__ inline int producer_of(int r, int lv, int L) {
   return best;
 }
 
-__device__ inline uint64_t load_acquire_u64(const uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void store_release_u64(uint64_t* p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-// Coherent (L1-bypassing) read of data another workgroup published.
-__device__ inline double load_agent(const double* p) {
-  const uint64_t u = __hip_atomic_load(reinterpret_cast<const uint64_t*>(p), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-  return __longlong_as_double((long long)u);
-}
-
 template <int CPW, int RPL>
 __global__ void __launch_bounds__(512)
 ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, double* __restrict__ V2out,
              double* __restrict__ Tout, double* __restrict__ tauout, uint64_t* __restrict__ cnt,
              uint64_t epoch, int L, int b, int* __restrict__ err) {
   constexpr int WAVES = 8;
-  constexpr int SH = xreduce_shift<CPW>();
   const int nid = blockIdx.x;
   const int lane = threadIdx.x & 63, w = wave_id();
   const int64_t bb = (int64_t)b * b;
@@ -86,35 +72,32 @@ ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, doubl
   for (int cl = 0; cl < CPW; ++cl)
 #pragma unroll
     for (int s = 0; s < RPL; ++s) v2[cl][s] = t[cl][s] = 0.0;
-  if (threadIdx.x == 0) s_err = 0;
-
-  for (int i = 0; i < b; ++i) {
-    // ---- wait for column i of both inputs
-    if (threadIdx.x == 0) {
-      const uint64_t want = epoch * 256 + (uint64_t)(i + 1);
-      for (int which = 0; which < 2; ++which) {
-        const int p = which ? pbot : ptop;
-        if (p < 0) continue;
-        long spins = 0;
-        while (load_acquire_u64(cnt + p) < want) {
-          __builtin_amdgcn_s_sleep(2);
-          if (++spins > (1L << 22)) {
-            s_err = 1;
-            break;
-          }
-        }
-      }
-    }
-    __syncthreads();
-    if (s_err) break;
-    double ctop[RPL], cbot[RPL];
+  // Thread 0 waits until both inputs have published column i (bounded).
+  auto wait_inputs = [&](int i) {
+    const uint64_t want = epoch * 256 + (uint64_t)(i + 1);
+    if (ptop >= 0 && !flag_wait(cnt + ptop, want)) s_err = 1;
+    if (pbot >= 0 && !flag_wait(cnt + pbot, want)) s_err = 1;
+  };
+  // Column i of both inputs (rows ≤ i), device-coherent loads.
+  double ctop[RPL], cbot[RPL];
+  auto load_inputs = [&](int i) {
 #pragma unroll
     for (int s = 0; s < RPL; ++s) {
       const int row = lane + 64 * s;
       const bool in = row <= i;
-      ctop[s] = in ? load_agent(Rt + row + (int64_t)i * b) : 0.0;
-      cbot[s] = in ? load_agent(Rb + row + (int64_t)i * b) : 0.0;
+      ctop[s] = in ? load_coherent(Rt + row + (int64_t)i * b) : 0.0;
+      cbot[s] = in ? load_coherent(Rb + row + (int64_t)i * b) : 0.0;
     }
+  };
+  if (threadIdx.x == 0) {
+    s_err = 0;
+    wait_inputs(0);
+  }
+  __syncthreads();
+  if (!s_err) load_inputs(0);
+
+  for (int i = 0; i < b && !s_err; ++i) {
+    // ctop/cbot hold column i; column i+1 is loaded during the T update below
     // ---- s = c_top + V2ᵀ c_bot   (columns q < i)
     {
       double part[CPW];
@@ -126,8 +109,8 @@ ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, doubl
         part[cl] = a;
       }
       const double r = xreduce<CPW>(part, lane);
-      const int q = w + WAVES * (lane >> SH);
-      if ((lane & ((1 << SH) - 1)) == 0 && q < i) sbuf[q] = r + load_agent(Rt + q + (int64_t)i * b);
+      const int q = w + WAVES * (lane & (CPW - 1));
+      if (lane < CPW && q < i) sbuf[q] = r;
     }
     __syncthreads();
     // ---- u = Tᵀ s
@@ -136,7 +119,7 @@ ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, doubl
 #pragma unroll
       for (int s = 0; s < RPL; ++s) {
         const int row = lane + 64 * s;
-        sv[s] = row < i ? sbuf[row] : 0.0;
+        sv[s] = row < i ? sbuf[row] + ctop[s] : 0.0;  // s = c_top + V2ᵀ·c_bot
       }
       double part[CPW];
 #pragma unroll
@@ -147,8 +130,8 @@ ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, doubl
         part[cl] = a;
       }
       const double r = xreduce<CPW>(part, lane);
-      const int q = w + WAVES * (lane >> SH);
-      if ((lane & ((1 << SH) - 1)) == 0 && q < i) ubuf[q] = r;
+      const int q = w + WAVES * (lane & (CPW - 1));
+      if (lane < CPW && q < i) ubuf[q] = r;
     }
     __syncthreads();
     // ---- c_top −= u; c_bot −= V2·u (partials per wave, summed through LDS)
@@ -230,9 +213,8 @@ ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, doubl
     if (w == 0) {
       for (int s = 0; s < RPL; ++s) {
         const int row = lane + 64 * s;
-        if (row < b) Ro[row + (int64_t)i * b] = row < i ? ctop[s] : (row == i ? h.beta : 0.0);
+        if (row < b) store_coherent(Ro + row + (int64_t)i * b, row < i ? ctop[s] : (row == i ? h.beta : 0.0));
       }
-      __threadfence();
     }
     if (threadIdx.x == 0) stau[i] = h.tau;
     // ---- z = V2(:,0:i)ᵀ v (columns q < i)
@@ -246,11 +228,15 @@ ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, doubl
         part[cl] = a;
       }
       const double r = xreduce<CPW>(part, lane);
-      const int q = w + WAVES * (lane >> SH);
-      if ((lane & ((1 << SH) - 1)) == 0 && q < i) zbuf[q] = r;
+      const int q = w + WAVES * (lane & (CPW - 1));
+      if (lane < CPW && q < i) zbuf[q] = r;
     }
+    if (w == 0) wait_memory();  // the R column stores above have completed
     __syncthreads();
-    if (threadIdx.x == 0) store_release_u64(cnt + nid, epoch * 256 + (uint64_t)(i + 1));
+    if (threadIdx.x == 0) {
+      flag_store(cnt + nid, epoch * 256 + (uint64_t)(i + 1));
+      if (i + 1 < b) wait_inputs(i + 1);
+    }
     // ---- T(0:i, i) = −τ T(0:i,0:i) z: partial over each wave's columns
     {
       double py[RPL];
@@ -269,6 +255,7 @@ ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, doubl
       for (int s = 0; s < RPL; ++s) red[w][lane + 64 * s] = py[s];
     }
     __syncthreads();
+    if (i + 1 < b && !s_err) load_inputs(i + 1);  // in flight during the T update
     if (w == (i % WAVES)) {
 #pragma unroll
       for (int s = 0; s < RPL; ++s) {

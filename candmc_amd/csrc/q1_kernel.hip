@@ -1,0 +1,269 @@
+// construct_Q1 in one launch: the explicit first b columns of the TSQR Q
+// (alg/QR/tsqr/construct_tsqr_Q.cxx:34-242).
+//
+// The reference walks the tree from the root down: the root starts from
+// C = [I; 0]; every node (receiver z, sender s) turns the b×b block of its
+// receiver into [C_z; C_s] = Q_node·[C_z; 0] with Q_node = I − [I;V2]·T·[I;V2]ᵀ
+// (the dtprfb of :195-208):
+//     X = T·C_z,   C_s = −V2·X,   C_z = C_z − X;
+// and every leaf l finally expands its block with its own reflectors (the
+// dormqr of :229-232):
+//     Q1_l = [C_l; 0] − V_l·(T_l·(V_topᵀ·C_l)).
+// Columns never mix, so the work splits into (node, 16-column block) and
+// (leaf, 16-column block) items.  Items are numbered root first, then down
+// the tree, then the leaves: an item only waits for the item of its parent
+// node and column block, which has a smaller block id and is therefore
+// dispatched earlier.  Blocks C pass between items through device-coherent
+// atomics and a flag per (node, column block) (see cqr_device.h).  Waits
+// are bounded; a timeout raises a flag that the host reads at the next
+// synchronisation.
+//
+// All blocks C are upper triangular (the root's identity is, and T, V2 keep
+// it so), so column block [j0, j0+16) only has rows below R = j0 + 16.
+#include "cqr_device.h"
+#include "cqr_internal.h"
+
+namespace cqr {
+
+namespace {
+
+constexpr int CW = 16;     // columns per work item
+constexpr int NT = 256;    // threads per item
+constexpr int LDS_LD = CW + 1;
+
+// The node consuming rank r's block above level `lv` (lv = −1 for a leaf):
+// the first level k > lv at which r is a receiver (node z = r) or a sender
+// (node z = r − ⌈n_k/2⌉); −1 when none exists (the root, or a one-leaf tree).
+__device__ inline int parent_node(int r, int lv, int L) {
+  int n = L, node0 = 0;
+  for (int k = 0; n > 1; ++k) {
+    const int pairs = n / 2, half = (n + 1) / 2;
+    if (k > lv) {
+      if (r < pairs) return node0 + r;
+      if (r >= half && r < n) return node0 + (r - half);
+    }
+    node0 += pairs;
+    n = half;
+  }
+  return -1;
+}
+
+// out[i][j] (the thread's row i and its 8 columns) = Σ_{k<R} M[i][k]·S[k][j]
+// with M read from global memory column-major (ld).  The k loop runs in
+// batches of 16 loads with the next batch in flight while the current one
+// is consumed, so the chain costs about one memory latency, not R.
+__device__ inline void rowprod(const double* __restrict__ M, int64_t ld, const double (*S)[LDS_LD], int R,
+                               int i, int jh, double (&acc)[8]) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc[j] = 0.0;
+  if (i >= R) return;
+  const double* mp = M + i;
+  double cur[16], nxt[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) cur[t] = t < R ? mp[(int64_t)t * ld] : 0.0;
+  for (int k0 = 0; k0 < R; k0 += 16) {
+    const int kn = k0 + 16;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) nxt[t] = kn + t < R ? mp[(int64_t)(kn + t) * ld] : 0.0;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int k = min(k0 + t, 127);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[j] += cur[t] * S[k][jh * 8 + j];
+    }
+#pragma unroll
+    for (int t = 0; t < 16; ++t) cur[t] = nxt[t];
+  }
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(NT)
+q1_kernel(const double* __restrict__ nodeV2, const double* __restrict__ nodeT, const double* __restrict__ leafT,
+          const double* __restrict__ V, int64_t ldv, int64_t m, double* __restrict__ C,
+          double* __restrict__ Q, int64_t ldq, uint64_t* __restrict__ flags, uint64_t epoch, int L,
+          int nnodes, int b, int* __restrict__ err) {
+  __shared__ double S0[128][LDS_LD];
+  __shared__ double S1[128][LDS_LD];
+  __shared__ double Vs[CW][128 + 1];
+  __shared__ int s_ok;
+
+  const int ncb = (b + CW - 1) / CW;
+  const int item = blockIdx.x;
+  const int cb = item % ncb;
+  const int j0 = cb * CW, nj = min(CW, b - j0);
+  const int R = min(b, j0 + CW);
+  const int tid = threadIdx.x, i = tid & 127, jh = tid >> 7;
+  const int64_t bb = (int64_t)b * b;
+
+  const bool is_node = item < nnodes * ncb;
+  int z, parent, nid = -1, snd = -1;
+  if (is_node) {
+    nid = nnodes - 1 - item / ncb;
+    int lv = 0, n = L, node0 = 0;  // level, live ranks and first node of nid's level
+    while (nid >= node0 + n / 2) {
+      node0 += n / 2;
+      n = n / 2 + (n & 1);
+      ++lv;
+    }
+    z = nid - node0;
+    snd = z + (n + 1) / 2;
+    parent = parent_node(z, lv, L);
+  } else {
+    z = item / ncb - nnodes;
+    parent = parent_node(z, -1, L);
+  }
+
+  // ---- input block: C_z[:, j0:j0+16] (identity at the root)
+  if (tid == 0) s_ok = 1;
+  if (parent >= 0 && tid == 0) {
+    if (!flag_wait(flags + (int64_t)parent * ncb + cb, epoch)) s_ok = 0;
+  }
+  __syncthreads();
+  if (!s_ok) {
+    if (tid == 0) *err = 1;
+    return;
+  }
+  double* Cz = C + (int64_t)z * bb;
+  {
+    // thread (k = tid & 127, half jh) loads 8 columns of row k, all in flight
+    double cv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int jj = jh * 8 + j;
+      double v = 0.0;
+      if (i < R && jj < nj)
+        v = parent >= 0 ? load_coherent(Cz + i + (int64_t)(j0 + jj) * b) : (i == j0 + jj ? 1.0 : 0.0);
+      cv[j] = v;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) S0[i][jh * 8 + j] = cv[j];
+  }
+  __syncthreads();
+
+  double acc[8];
+  if (is_node) {
+    const double* Tn = nodeT + (int64_t)nid * bb;
+    const double* V2 = nodeV2 + (int64_t)nid * bb;
+    // X = T·C_z
+    rowprod(Tn, b, S0, R, i, jh, acc);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) S1[i][jh * 8 + j] = i < R ? acc[j] : 0.0;
+    __syncthreads();
+    // C_s = −V2·X, C_z −= X
+    rowprod(V2, b, S1, R, i, jh, acc);
+    double* Cs = C + (int64_t)snd * bb;
+    if (i < b) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int jj = jh * 8 + j;
+        if (jj < nj) {
+          store_coherent(Cs + i + (int64_t)(j0 + jj) * b, i < R ? -acc[j] : 0.0);
+          store_coherent(Cz + i + (int64_t)(j0 + jj) * b, i < R ? S0[i][jj] - S1[i][jj] : 0.0);
+        }
+      }
+    }
+    wait_memory();
+    __syncthreads();
+    if (tid == 0) flag_store(flags + (int64_t)nid * ncb + cb, epoch);
+    return;
+  }
+
+  // ---- leaf z: rows [r0, r1) of V = [V\R] (unit lower in its top b×b)
+  const int64_t r0 = ((int64_t)z * m) / L, r1 = ((int64_t)(z + 1) * m) / L;
+  const double* Vl = V + r0;
+  // Z = V_topᵀ·C_l, staged 16 rows of V_top at a time (rows k ≥ column i)
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc[j] = 0.0;
+  // thread (kk = tid & 15, columns tid/16 + 16·e) stages V_top rows k0+kk
+  double pf[8];
+  auto vtop = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int kk = tid & (CW - 1), col = (tid >> 4) + 16 * e, k = k0 + kk;
+      double v = 0.0;
+      if (k < R && col < b) v = k > col ? Vl[k + (int64_t)col * ldv] : (k == col ? 1.0 : 0.0);
+      pf[e] = v;
+    }
+  };
+  vtop(0);
+  for (int k0 = 0; k0 < R; k0 += CW) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; ++e) Vs[tid & (CW - 1)][(tid >> 4) + 16 * e] = pf[e];
+    __syncthreads();
+    if (k0 + CW < R) vtop(k0 + CW);  // next slab in flight during this one
+    if (i < b) {
+      const int kend = min(CW, R - k0);
+      for (int kk = 0; kk < kend; ++kk) {
+        const double vk = Vs[kk][i];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[j] += vk * S0[k0 + kk][jh * 8 + j];
+      }
+    }
+  }
+  __syncthreads();
+  // Z[i] = Σ_{i≤k<R} V_top[k][i]·C[k] vanishes for i ≥ R, and so does
+  // M = T_l·Z (T_l upper): both have R rows.
+#pragma unroll
+  for (int j = 0; j < 8; ++j) S1[i][jh * 8 + j] = i < R ? acc[j] : 0.0;
+  __syncthreads();
+  const double* Tl = leafT + (int64_t)z * bb;
+  rowprod(Tl, b, S1, R, i, jh, acc);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 8; ++j) S1[i][jh * 8 + j] = i < R ? acc[j] : 0.0;  // M (R×16)
+  __syncthreads();
+  // Q1_l = [C_l; 0] − V_l·M over the leaf rows, 128 rows per pass
+  const int64_t ml = r1 - r0;
+  for (int64_t rb = 0; rb < ml; rb += 128) {
+    const int64_t r = rb + i;
+    if (r >= ml) continue;
+    double q[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) q[j] = r < R ? S0[r][jh * 8 + j] : 0.0;
+    // V_l[r][k]: 1 at k = r, 0 above (k > r), so only k ≤ r is read
+    const double* vp = Vl + r;
+    double cur[16], nxt[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) cur[t] = (t < R && t < r) ? vp[(int64_t)t * ldv] : (t == r ? 1.0 : 0.0);
+    for (int k0 = 0; k0 < R; k0 += 16) {
+      const int kn = k0 + 16;
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const int k = kn + t;
+        nxt[t] = (k < R && k < r) ? vp[(int64_t)k * ldv] : (k == r ? 1.0 : 0.0);
+      }
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const int k = min(k0 + t, 127);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) q[j] -= cur[t] * S1[k][jh * 8 + j];
+      }
+#pragma unroll
+      for (int t = 0; t < 16; ++t) cur[t] = nxt[t];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int jj = jh * 8 + j;
+      if (jj < nj) Q[r0 + r + (int64_t)(j0 + jj) * ldq] = q[j];
+    }
+  }
+}
+
+int launch_q1(cqr_ctx* c, const double* nodeV2, const double* nodeT, const double* leafT, const double* V,
+              int64_t ldv, int64_t m, double* C, double* Q, int64_t ldq, int L, int nnodes, int b) {
+  if (b > 128 || b < 1) return set_err(c, CQR_ERR_UNSUPPORTED, "construct_Q1: b = %d", b);
+  const int ncb = (b + CW - 1) / CW;
+  if ((int64_t)nnodes * ncb > CQR_MAX_TREE_NODES * 8)
+    return set_err(c, CQR_ERR_UNSUPPORTED, "construct_Q1: tree of %d nodes", nnodes);
+  const uint64_t epoch = ++c->q1_epoch;
+  hipEvent_t ev = prof_begin(c);
+  hipLaunchKernelGGL(q1_kernel, dim3((unsigned)((nnodes + L) * ncb)), dim3(NT), 0, c->stream, nodeV2, nodeT,
+                     leafT, V, ldv, m, C, Q, ldq, c->d_qflags, epoch, L, nnodes, b, c->d_terr);
+  prof_end(c, ev, PK_BGEMM, 4.0 * nnodes * (double)b * b * b + 4.0 * (double)m * b * b,
+           8.0 * (3.0 * nnodes * b * b + 2.0 * m * b + 2.0 * L * b * b));
+  return check_launch(c, "q1_kernel");
+}
+
+}  // namespace cqr

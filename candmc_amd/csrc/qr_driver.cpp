@@ -121,9 +121,8 @@ int hh_recon_panel(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, dou
   CQR_TRY(launch_ptree(c, leafR, at(w.nodeR), nodeV2, nodeT, at(w.nodeTau), L, (int)p.nnodes, bi));
   const double* Rroot = p.nnodes > 0 ? at(w.nodeR) + (p.nnodes - 1) * bb : leafR;
   // K3: construct_Q1 (construct_tsqr_Q.cxx:34-242) into the Y scratch
-  CQR_TRY(reverse_tree(c, p, nodeV2, nodeT, at(w.C), at(w.X)));
   double* Y = at(w.Y);
-  CQR_TRY(leaf_apply(c, p, A, lda, leafT, at(w.C), at(w.Z), at(w.M), Y, m));
+  CQR_TRY(launch_q1(c, nodeV2, nodeT, leafT, A, lda, m, at(w.C), Y, m, L, (int)p.nnodes, bi));
   // K4: LU(Q1_top − S), R := S·R, W = U·S (hh_recon.cxx:282-308)
   double* Wd = W ? W : at(w.Wtmp);
   int64_t* sg = signs_out ? signs_out : reinterpret_cast<int64_t*>(base + w.signs);
@@ -301,8 +300,8 @@ int cqr_construct_Q1(cqr_ctx* c, const double* Y, int64_t lda_Y, const double* t
   const double* leafT = tree_data;
   const double* nodeV2 = leafT + L * bb;
   const double* nodeT = nodeV2 + nn * bb;
-  CQR_TRY(reverse_tree(c, p, nodeV2, nodeT, at(oC), at(oX)));
-  return leaf_apply(c, p, Y, lda_Y, leafT, at(oC), at(oZ), at(oM), Q1, lda_Q);
+  (void)oX; (void)oZ; (void)oM;
+  return launch_q1(c, nodeV2, nodeT, leafT, Y, lda_Y, m, at(oC), Q1, lda_Q, (int)L, (int)nn, (int)b);
 }
 
 int cqr_signed_nlu(cqr_ctx* c, double* A, int64_t lda, double* R, int64_t lda_R, int64_t b, int64_t* signs) {

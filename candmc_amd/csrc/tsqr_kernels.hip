@@ -35,7 +35,6 @@ __global__ void __launch_bounds__(64 * WAVES)
 leaf_qr_kernel(double* __restrict__ A, int64_t lda, int64_t m, int L, int b,
                double* __restrict__ Rout, double* __restrict__ Tout, double* __restrict__ tauout) {
   constexpr int NT = 64 * WAVES;
-  constexpr int SH = xreduce_shift<CPW>();
   const int leaf = blockIdx.x;
   const int64_t r0 = leaf_off(leaf, m, L);
   const int r = (int)(leaf_off(leaf + 1, m, L) - r0);
@@ -136,14 +135,14 @@ leaf_qr_kernel(double* __restrict__ A, int64_t lda, int64_t m, int L, int b,
     const double mine = xreduce<CPW>(part, lane);
     {
       // Gram entries of finished columns: one lane per reduced value
-      const int col = w + WAVES * (lane >> SH);
-      if ((lane & ((1 << SH) - 1)) == 0 && col < i) G[col + i * b] = mine;
+      const int col = w + WAVES * (lane & (CPW - 1));
+      if (lane < CPW && col < i) G[col + i * b] = mine;
     }
 #pragma unroll
     for (int cl = 0; cl < CPW; ++cl) {
       const int col = w + WAVES * cl;
       if (col > i && col < b) {
-        const double f = tau * readlane_d(mine, cl << SH);
+        const double f = tau * readlane_d(mine, cl);
 #pragma unroll
         for (int s = 0; s < RPL; ++s) a[cl][s] -= f * v[s];
       }
@@ -186,7 +185,6 @@ __global__ void __launch_bounds__(64 * WAVES)
 tree_node_kernel(double* __restrict__ Rleaf, int n, int node0, int b, double* __restrict__ V2out,
                  double* __restrict__ Tout, double* __restrict__ tauout) {
   constexpr int NT = 64 * WAVES;
-  constexpr int SH = xreduce_shift<CPW>();
   const int z = blockIdx.x;
   const int snd = z + (n + 1) / 2;
   const int64_t bb = (int64_t)b * b;
@@ -284,8 +282,8 @@ tree_node_kernel(double* __restrict__ Rleaf, int n, int node0, int b, double* __
     }
     double d = xreduce<CPW>(part, lane);
     {
-      const int col = w + WAVES * (lane >> SH);
-      if ((lane & ((1 << SH) - 1)) == 0 && col < b) {
+      const int col = w + WAVES * (lane & (CPW - 1));
+      if (lane < CPW && col < b) {
         if (col > i) {
           const double r1 = R1[pu(i, col)];
           d += r1;
@@ -299,7 +297,7 @@ tree_node_kernel(double* __restrict__ Rleaf, int n, int node0, int b, double* __
     for (int cl = 0; cl < CPW; ++cl) {
       const int col = w + WAVES * cl;
       if (col > i && col < b) {
-        const double f = tau * readlane_d(d, cl << SH);
+        const double f = tau * readlane_d(d, cl);
 #pragma unroll
         for (int s = 0; s < RPL; ++s) a2[cl][s] -= f * v[s];
       }
