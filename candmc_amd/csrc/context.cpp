@@ -78,6 +78,7 @@ int ensure_ws(cqr_ctx* c, size_t bytes) {
   // Growing replaces the arena: wait for work that may still use it.
   if (c->ws) {
     CQR_HIP(c, hipStreamSynchronize(c->stream));
+    CQR_HIP(c, hipStreamSynchronize(c->side));
     CQR_HIP(c, hipFree(c->ws));
     c->ws = nullptr;
     c->ws_bytes = 0;
@@ -147,12 +148,18 @@ int cqr_create(cqr_ctx** ctx, int device) {
       hipMemset(c->d_qflags, 0, sizeof(uint64_t) * CQR_MAX_TREE_NODES * 8) != hipSuccess ||
       hipMemset(c->d_terr, 0, sizeof(int)) != hipSuccess ||
       hipMemset(c->d_cnt, 0, sizeof(uint64_t) * CQR_MAX_TREE_NODES) != hipSuccess ||
-      hipMemset(c->d_flag, 0, sizeof(int)) != hipSuccess) {
+      hipMemset(c->d_flag, 0, sizeof(int)) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
     (void)hipGetLastError();
     if (c->d_flag) (void)hipFree(c->d_flag);
     if (c->d_cnt) (void)hipFree(c->d_cnt);
     if (c->d_terr) (void)hipFree(c->d_terr);
     if (c->d_qflags) (void)hipFree(c->d_qflags);
+    if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     delete c;
     return CQR_ERR_NOMEM;
   }
@@ -164,6 +171,7 @@ int cqr_destroy(cqr_ctx* c) {
   if (!c) return CQR_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  if (c->side) (void)hipStreamSynchronize(c->side);
   for (const ProfRec& r : c->recs) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
@@ -174,6 +182,9 @@ int cqr_destroy(cqr_ctx* c) {
   if (c->d_cnt) (void)hipFree(c->d_cnt);
   if (c->d_terr) (void)hipFree(c->d_terr);
   if (c->d_qflags) (void)hipFree(c->d_qflags);
+  if (c->side) (void)hipStreamDestroy(c->side);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   delete c;
   return CQR_OK;
 }
@@ -196,6 +207,12 @@ int cqr_sync(cqr_ctx* c) {
     CQR_HIP(c, hipMemset(c->d_terr, 0, sizeof(int)));
     return set_err(c, CQR_ERR_DEVICE, "TSQR tree node timed out waiting for a child");
   }
+  return CQR_OK;
+}
+
+int cqr_set_lookahead(cqr_ctx* c, int on) {
+  if (!c) return CQR_ERR_INVALID;
+  c->lookahead = on ? 1 : 0;
   return CQR_OK;
 }
 

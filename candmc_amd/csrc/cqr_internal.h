@@ -39,6 +39,9 @@ struct cqr_ctx {
   uint64_t* d_qflags = nullptr; // per (node, column block) flags of construct_Q1
   uint64_t q1_epoch = 0;
   uint64_t tree_epoch = 0;      // launch epoch folded into the counters
+  hipStream_t side = nullptr;   // second stream of the QR_2D lookahead
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int lookahead = 1;
   std::string err;
 };
 

@@ -400,11 +400,14 @@ int cqr_update_A(cqr_ctx* c, const double* Y, int64_t lda_Y, double* A, int64_t 
 
 // Workspace of QR_2D for an m×k matrix with panel width b: the maximum over
 // all panels (the split-K scratch of the update is not monotone in j).
-static size_t max_update_ws(int64_t m, int64_t k, int64_t b) {
+// part 0: the whole trailing block; 1: the next panel's columns; 2: the rest.
+static size_t max_update_ws(int64_t m, int64_t k, int64_t b, int part = 0) {
   size_t mx = 0;
   for (int64_t j = 0; j < k; j += b) {
     const int64_t mj = m - j, kj = k - j, bj = std::min(kj, b);
-    if (kj - b > 0 && mj - b > 0) mx = std::max(mx, wy_update_ws_doubles(mj, kj - bj, (int)bj));
+    if (!(kj - b > 0 && mj - b > 0)) continue;
+    const int64_t bn = std::min(b, kj - bj), cols = part == 0 ? kj - bj : part == 1 ? bn : kj - bj - bn;
+    if (cols > 0) mx = std::max(mx, wy_update_ws_doubles(mj, cols, (int)bj));
   }
   return mx;
 }
@@ -415,35 +418,82 @@ static size_t max_panel_ws(int64_t m, int64_t k, int64_t b, int64_t leaf_rows) {
   return mx;
 }
 
-static size_t qr2d_ws(int64_t m, int64_t k, int64_t b, int64_t leaf_rows, size_t* o_panel, size_t* o_upd,
-                      size_t* o_W, size_t* o_Tt) {
+struct Qr2dWS {
+  size_t panel, upd_near, upd_far, W, Tt;
+};
+
+static size_t qr2d_ws(int64_t m, int64_t k, int64_t b, int64_t leaf_rows, Qr2dWS* o) {
   Sizer s;
   const int64_t b0 = std::min(k, b);
-  *o_panel = s.add(max_panel_ws(m, k, b, leaf_rows));
-  *o_upd = s.add(8 * (max_update_ws(m, k, b) + 64));
-  *o_W = s.add(8 * b0 * b0);
-  *o_Tt = s.add(8 * b0 * b0);
+  o->panel = s.add(max_panel_ws(m, k, b, leaf_rows));
+  o->upd_near = s.add(8 * (std::max(max_update_ws(m, k, b, 0), max_update_ws(m, k, b, 1)) + 64));
+  o->upd_far = s.add(8 * (max_update_ws(m, k, b, 2) + 64));
+  o->W = s.add(8 * b0 * b0);
+  o->Tt = s.add(8 * 2 * b0 * b0);
   return s.bytes;
 }
 
+// Issues the kernels of its scope on the context's side stream.
+struct OnSide {
+  cqr_ctx* c;
+  hipStream_t main;
+  explicit OnSide(cqr_ctx* ctx) : c(ctx), main(ctx->stream) { c->stream = c->side; }
+  ~OnSide() { c->stream = main; }
+};
+
 static int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b, double* aggreg_Y,
-                    int64_t lda_aY, char* base, size_t o_panel, size_t o_upd, size_t o_W, size_t o_Tt) {
-  double* W = reinterpret_cast<double*>(base + o_W);
-  double* Tt = reinterpret_cast<double*>(base + o_Tt);
-  double* ws_upd = reinterpret_cast<double*>(base + o_upd);
-  for (int64_t j = 0;; j += b) {                         // qr_2d.cxx:299-363
+                    int64_t lda_aY, char* base, const Qr2dWS& o) {
+  double* W = reinterpret_cast<double*>(base + o.W);
+  double* Tt2[2] = {reinterpret_cast<double*>(base + o.Tt), reinterpret_cast<double*>(base + o.Tt) + b * b};
+  double* ws_near = reinterpret_cast<double*>(base + o.upd_near);
+  double* ws_far = reinterpret_cast<double*>(base + o.upd_far);
+  const bool la = c->lookahead && k > 2 * b && m > 2 * b;
+  if (la) {  // the side stream starts after everything already queued
+    CQR_HIP(c, hipEventRecord(c->ev_fork, c->stream));
+    CQR_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+  }
+  int rc = CQR_OK;
+  for (int64_t j = 0, p = 0;; j += b, ++p) {             // qr_2d.cxx:299-363
     const int64_t mj = m - j, kj = k - j, bj = std::min(kj, b);
     double* P = A + j + j * lda;
-    CQR_TRY(hh_recon_panel(c, P, lda, mj, bj, W, nullptr, nullptr, Tt, 0, base + o_panel, nullptr));
-    if (aggreg_Y)
-      CQR_TRY(launch_copy_unit_lower(c, P, lda, aggreg_Y + j + j * lda_aY, lda_aY, mj, bj));
-    if (kj - b > 0 && mj - b > 0) {
-      CQR_TRY(wy_update_f64_ws(c, P, lda, 1, Tt, P + bj * lda, lda, mj, kj - bj, (int)bj, ws_upd));
-    } else {
+    double* Tt = Tt2[p & 1];
+    if ((rc = hh_recon_panel(c, P, lda, mj, bj, W, nullptr, nullptr, Tt, 0, base + o.panel, nullptr))) break;
+    if (aggreg_Y && (rc = launch_copy_unit_lower(c, P, lda, aggreg_Y + j + j * lda_aY, lda_aY, mj, bj))) break;
+    if (!(kj - b > 0 && mj - b > 0)) break;
+    if (!la) {
+      if ((rc = wy_update_f64_ws(c, P, lda, 1, Tt, P + bj * lda, lda, mj, kj - bj, (int)bj, ws_near))) break;
+      continue;
+    }
+    // Lookahead (qr_2d.cxx:378-635): the next panel's b columns first, on
+    // the main stream, once the far update of the previous panel (which
+    // covered them) has finished; then the far columns on the side stream,
+    // overlapping the next panel's factorization.  Tᵀ alternates between
+    // two buffers because the side stream still reads the previous one.
+    const int64_t bn = std::min(b, kj - bj), rest = kj - bj - bn;
+    if (p > 0 && (rc = hipStreamWaitEvent(c->stream, c->ev_join, 0)) != hipSuccess) {
+      rc = set_err(c, CQR_ERR_DEVICE, "QR_2D: stream wait failed");
       break;
     }
+    if ((rc = wy_update_f64_ws(c, P, lda, 1, Tt, P + bj * lda, lda, mj, bn, (int)bj, ws_near))) break;
+    if (rest > 0) {
+      if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess ||
+          hipStreamWaitEvent(c->side, c->ev_fork, 0) != hipSuccess) {
+        rc = set_err(c, CQR_ERR_DEVICE, "QR_2D: stream fork failed");
+        break;
+      }
+      OnSide g(c);
+      if ((rc = wy_update_f64_ws(c, P, lda, 1, Tt, P + (bj + bn) * lda, lda, mj, rest, (int)bj, ws_far))) break;
+      if (hipEventRecord(c->ev_join, c->side) != hipSuccess) {
+        rc = set_err(c, CQR_ERR_DEVICE, "QR_2D: event record failed");
+        break;
+      }
+    }
   }
-  return CQR_OK;
+  if (la) {  // join: work queued on the caller's stream after this sees all of it
+    CQR_HIP(c, hipEventRecord(c->ev_join, c->side));
+    CQR_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  }
+  return rc;
 }
 
 int cqr_qr_2d(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b, double* aggreg_Y,
@@ -453,10 +503,10 @@ int cqr_qr_2d(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t 
     return set_err(c, CQR_ERR_INVALID, "QR_2D: bad arguments (needs m >= k, lda >= m)");
   if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "QR_2D: b > %d", CQR_MAX_B);
   CQR_TRY(finite_check(c, A, lda, m, k));
-  size_t op, ou, ow, ot;
-  const size_t need = qr2d_ws(m, k, b, c->leaf_rows, &op, &ou, &ow, &ot);
+  Qr2dWS o;
+  const size_t need = qr2d_ws(m, k, b, c->leaf_rows, &o);
   CQR_TRY(ensure_ws(c, need));
-  return qr2d_run(c, A, lda, m, k, b, aggreg_Y, lda_aY, (char*)c->ws, op, ou, ow, ot);
+  return qr2d_run(c, A, lda, m, k, b, aggreg_Y, lda_aY, (char*)c->ws, o);
 }
 
 int cqr_qr_2d_2d(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b, int64_t b_sub,
@@ -468,8 +518,8 @@ int cqr_qr_2d_2d(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64
   if (b_sub > CQR_MAX_B || b > CQR_MAX_B)
     return set_err(c, CQR_ERR_UNSUPPORTED, "QR_2D_2D: b > %d (aggregated T is one b×b block)", CQR_MAX_B);
   CQR_TRY(finite_check(c, A, lda, m, k));
-  size_t op, ou, ow, ot;
-  const size_t q = qr2d_ws(m, std::min(k, b), b_sub, c->leaf_rows, &op, &ou, &ow, &ot);
+  Qr2dWS o;
+  const size_t q = qr2d_ws(m, std::min(k, b), b_sub, c->leaf_rows, &o);
   Sizer s;
   s.bytes = q;
   const int64_t bb = b * b;
@@ -483,7 +533,7 @@ int cqr_qr_2d_2d(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64
     double* P = A + j + j * lda;
     double* aY = aggreg_Y ? aggreg_Y + j + j * lda_aY : nullptr;
     if (kj - b > 0 && mj - b > 0) {
-      CQR_TRY(qr2d_run(c, P, lda, mj, b, b_sub, aY, lda_aY, base, op, ou, ow, ot));
+      CQR_TRY(qr2d_run(c, P, lda, mj, b, b_sub, aY, lda_aY, base, o));
       // aggregated T⁻ᵀ = tril(YᵀY), diagonal halved (compute_invT_from_Y)
       CQR_TRY(gemm_f64_ws(c, 1, 0, b, b, mj, 1.0, P, lda, SM_UNIT_LOWER, P, lda, SM_UNIT_LOWER, 0.0, at(oG), b,
                           at(oS)));
@@ -491,7 +541,7 @@ int cqr_qr_2d_2d(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64
       CQR_TRY(Tt_from_invTT(c, at(oI), b, at(oTt), (int)b));
       CQR_TRY(wy_update_f64_ws(c, P, lda, 1, at(oTt), P + b * lda, lda, mj, kj - b, (int)b, at(oS)));
     } else {
-      return qr2d_run(c, P, lda, mj, kj, b_sub, aY, lda_aY, base, op, ou, ow, ot);
+      return qr2d_run(c, P, lda, mj, kj, b_sub, aY, lda_aY, base, o);
     }
   }
 }

@@ -54,6 +54,9 @@ class Context:
     def set_leaf_rows(self, rows: int):
         self.check(self.lib.cqr_set_leaf_rows(self.h, int(rows)))
 
+    def set_lookahead(self, on: bool):
+        self.check(self.lib.cqr_set_lookahead(self.h, 1 if on else 0))
+
     def set_check_finite(self, on: bool):
         self.check(self.lib.cqr_set_check_finite(self.h, 1 if on else 0))
 

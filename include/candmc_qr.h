@@ -49,6 +49,12 @@ int cqr_reserve(cqr_ctx* ctx, size_t bytes);          /* pre-grow the arena   */
 size_t cqr_workspace_bytes(const cqr_ctx* ctx);
 int cqr_set_check_finite(cqr_ctx* ctx, int on);       /* scan inputs first    */
 int cqr_set_leaf_rows(cqr_ctx* ctx, int64_t rows);    /* 0 = automatic        */
+/* QR_2D lookahead (on by default): the update of the next panel's columns
+ * runs first, then the rest of the trailing update proceeds on a second
+ * stream of the context while the next panel is factored (the pipeline of
+ * alg/QR/qr_2d/qr_2d.cxx:378-635, QR_2D_pipe).  Results differ from the
+ * non-lookahead order only in the summation order of Yᵀ·A.               */
+int cqr_set_lookahead(cqr_ctx* ctx, int on);
 /* Kernel timing: while enabled, the library records a HIP event pair on
  * its stream around every kernel it launches; cqr_profile_read waits for
  * the stream and returns, for one kernel class, the summed event time,

@@ -254,6 +254,28 @@ def test_qr_2d(Q, m, k, b):
         assert rel(YR, qr) < 1e-11                             # = dgeqrf for m > k
 
 
+@pytest.mark.parametrize("m,k,b", [(1500, 900, 64), (1024, 1024, 128)])
+def test_qr_2d_lookahead_on_and_off(Q, m, k, b):
+    """The two-stream lookahead order (qr_2d.cxx:378-635) and the plain
+    right-looking order give the same factorization up to round-off."""
+    A = O.gaussian(m, k, 7 * m + k)
+    want = O.QR_2D(A, b)
+    ctx = Q.context()
+    out = []
+    for la in (False, True):
+        ctx.set_lookahead(la)
+        try:
+            dA = Q.to_gpu(A)
+            Q.QR_2D(dA, m, m, k, b)
+            out.append(Q.from_gpu(dA, m, k))
+        finally:
+            ctx.set_lookahead(True)
+    for YR in out:
+        assert rel(YR, want) < 1e-11
+    assert O.column_relative_residual(A, lapack_signs(out[1], m, k)) < 1e-9
+    assert rel(out[0], out[1]) < 1e-12
+
+
 def test_qr_2d_reference_default_input(Q):
     g = gold("qr2d_96x48.npz")                                 # singular: residual only
     dA = Q.to_gpu(g["A"])
