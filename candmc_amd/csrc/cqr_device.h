@@ -183,14 +183,38 @@ __device__ inline bool ss_needs_rescale(double ss) {
 }
 
 // ---------------------------------------------------------------------------
+// One 16×16 tile of P = X·Y on FP64 MFMA (v_mfma_f64_16x16x4f64), whole
+// wave.  x(m, k) and y(k, n) return the operands (0 outside the product);
+// k runs over [k0, k1) in steps of 4.  Lane l, register v of the result
+// holds P[l & 15][(l >> 4) + 4v].
+// ---------------------------------------------------------------------------
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+template <class FX, class FY>
+__device__ inline dbl4 mfma_tile16(const FX& x, const FY& y, int k0, int k1, int lane) {
+  dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+  const int r = lane & 15, q = lane >> 4;
+  for (int k = k0; k < k1; k += 4) {
+    const double xa = x(r, k + q);  // X[m = r][k]
+    const double yb = y(k + q, r);  // Y[k][n = r]
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(yb, xa, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
 // In-LDS inverse of an upper triangular b×b matrix (column-major, ld),
 // whole workgroup.  On entry the strictly upper part of M holds N and
 // d[i] the reciprocal diagonal; the matrix inverted is diag(1/d) + N, and
 // the recurrence used (X[p][i] = −d_i·Σ_{p≤q<i} X[p][q]·N[q][i]) also
 // yields LAPACK's compact-WY T when d = tau and N = striu(VᵀV) — including
 // tau = 0.  Diagonal 16×16 blocks are done by one wave each (rows in lanes,
-// wave-synchronous, so in-place is safe); blocks are then merged pairwise
-// with X12 = −X11·N12·X22.  The strictly lower part is used as scratch.
+// wave-synchronous, so in-place is safe); blocks are then merged pairwise,
+// X12 = −X11·(N12·X22), both products as 16×16 MFMA tiles spread over the
+// waves.  The intermediate N12·X22 is kept in the strictly lower mirror
+// block (rows of the second block, columns of the first), which no earlier
+// merge touched; the strictly lower part may hold such scratch, so every
+// read of a triangle masks it.
 // ---------------------------------------------------------------------------
 template <int NT>
 __device__ void tri_inv_upper_lds(double* M, int ld, const double* d, int b) {
@@ -212,30 +236,52 @@ __device__ void tri_inv_upper_lds(double* M, int ld, const double* d, int b) {
     }
   }
   __syncthreads();
-  for (int h = 16, lh = 4; h < b; h *= 2, ++lh) {
-    // pairs of blocks [a, a+h) and [a+h, a+2h)
-    // step 1: Xs = N12·X22 stored in the strictly lower mirror block
-    for (int e = tid; e < (b / (2 * h) + 1) * h * h; e += NT) {
-      const int pair = e >> (2 * lh), r = e & (h * h - 1);
-      const int p = r & (h - 1), j = r >> lh;
-      const int a = pair * 2 * h;
-      const int h2 = min(h, b - a - h);
-      if (a + h >= b || j >= h2) continue;
-      double acc = 0.0;
-      for (int q = 0; q <= j; ++q) acc += M[(a + p) + (a + h + q) * ld] * M[(a + h + q) + (a + h + j) * ld];
-      M[(a + h + j) + (a + p) * ld] = acc;  // lower mirror, (row a+h+j, col a+p)
+  for (int h = 16; h < b; h *= 2) {
+    const int npairs = (b - h + 2 * h - 1) / (2 * h);   // pairs with a second block
+    const int tm = h / 16, tn = (h + 15) / 16;
+    const int ntiles = npairs * tm * tn;
+    // step 1: P = N12·X22 → lower mirror M[a+h+j][a+p]
+    for (int t = wave; t < ntiles; t += NW) {
+      const int pr = t / (tm * tn), ti = (t / tn) % tm, tj = t % tn;
+      const int a = pr * 2 * h, h2 = min(h, b - a - h);
+      if (16 * tj >= h2) continue;
+      const double* N12 = M + a + (a + h) * ld;        // (p, q) p < h, q < h2
+      const double* X22 = M + (a + h) + (a + h) * ld;  // (q, j) q ≤ j < h2
+      const int i0 = 16 * ti, j0 = 16 * tj;
+      auto x = [&](int r, int k) { return k < h2 ? N12[(i0 + r) + k * ld] : 0.0; };
+      auto y = [&](int k, int c) {
+        const int j = j0 + c;
+        return (k <= j && j < h2) ? X22[k + j * ld] : 0.0;
+      };
+      const dbl4 acc = mfma_tile16(x, y, 0, min(h2, j0 + 16), lane);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int pp = i0 + (lane & 15), j = j0 + (lane >> 4) + 4 * v;
+        if (j < h2) M[(a + h + j) + (a + pp) * ld] = acc[v];
+      }
     }
     __syncthreads();
-    // step 2: X12 = −X11·Xs
-    for (int e = tid; e < (b / (2 * h) + 1) * h * h; e += NT) {
-      const int pair = e >> (2 * lh), r = e & (h * h - 1);
-      const int p = r & (h - 1), j = r >> lh;
-      const int a = pair * 2 * h;
-      const int h2 = min(h, b - a - h);
-      if (a + h >= b || j >= h2) continue;
-      double acc = 0.0;
-      for (int q = p; q < h; ++q) acc += M[(a + p) + (a + q) * ld] * M[(a + h + j) + (a + q) * ld];
-      M[(a + p) + (a + h + j) * ld] = -acc;
+    // step 2: X12 = −X11·P → M[a+p][a+h+j]
+    for (int t = wave; t < ntiles; t += NW) {
+      const int pr = t / (tm * tn), ti = (t / tn) % tm, tj = t % tn;
+      const int a = pr * 2 * h, h2 = min(h, b - a - h);
+      if (16 * tj >= h2) continue;
+      const double* X11 = M + a + a * ld;              // (p, q) p ≤ q < h
+      const int i0 = 16 * ti, j0 = 16 * tj;
+      auto x = [&](int r, int k) {
+        const int pp = i0 + r;
+        return (pp <= k && k < h) ? X11[pp + k * ld] : 0.0;
+      };
+      auto y = [&](int k, int c) {
+        const int j = j0 + c;
+        return (k < h && j < h2) ? M[(a + h + j) + (a + k) * ld] : 0.0;
+      };
+      const dbl4 acc = mfma_tile16(x, y, i0, h, lane);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int pp = i0 + (lane & 15), j = j0 + (lane >> 4) + 4 * v;
+        if (j < h2) M[(a + pp) + (a + h + j) * ld] = -acc[v];
+      }
     }
     __syncthreads();
   }

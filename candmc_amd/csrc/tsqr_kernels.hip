@@ -752,10 +752,15 @@ static int leaf_launch(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int
 
 int leaf_qr(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int b, double* R, double* T,
             double* tau) {
+  // rows per lane sized to the largest leaf (the register tile is RPL·64 rows)
+  const int64_t rows = (m + L - 1) / L;
   if (b <= 16) return leaf_launch<4, 4, 16>(c, A, lda, m, L, b, R, T, tau);
-  if (b <= 32) return leaf_launch<8, 4, 16>(c, A, lda, m, L, b, R, T, tau);
-  if (b <= 64) return leaf_launch<8, 8, 8>(c, A, lda, m, L, b, R, T, tau);
-  return leaf_launch<8, 16, 4>(c, A, lda, m, L, b, R, T, tau);
+  if (b <= 32) return rows <= 512 ? leaf_launch<8, 4, 8>(c, A, lda, m, L, b, R, T, tau)
+                                  : leaf_launch<8, 4, 16>(c, A, lda, m, L, b, R, T, tau);
+  if (b <= 64) return rows <= 256 ? leaf_launch<8, 8, 4>(c, A, lda, m, L, b, R, T, tau)
+                                  : leaf_launch<8, 8, 8>(c, A, lda, m, L, b, R, T, tau);
+  return rows <= 128 ? leaf_launch<8, 16, 2>(c, A, lda, m, L, b, R, T, tau)
+                     : leaf_launch<8, 16, 4>(c, A, lda, m, L, b, R, T, tau);
 }
 
 int64_t leaf_capacity_rows(int64_t b) {
