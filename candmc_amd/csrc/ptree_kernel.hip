@@ -79,47 +79,100 @@ ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, doubl
     if (pbot >= 0 && !flag_wait(cnt + pbot, want)) s_err = 1;
   };
   // Column i of both inputs (rows ≤ i), device-coherent loads.
-  double ctop[RPL], cbot[RPL];
-  auto load_inputs = [&](int i) {
+  double ctop[RPL], cbot[RPL], ntop[RPL], nbot[RPL];
+  auto load_inputs = [&](int i, double (&top)[RPL], double (&bot)[RPL]) {
 #pragma unroll
     for (int s = 0; s < RPL; ++s) {
       const int row = lane + 64 * s;
       const bool in = row <= i;
-      ctop[s] = in ? load_coherent(Rt + row + (int64_t)i * b) : 0.0;
-      cbot[s] = in ? load_coherent(Rb + row + (int64_t)i * b) : 0.0;
+      top[s] = in ? load_coherent(Rt + row + (int64_t)i * b) : 0.0;
+      bot[s] = in ? load_coherent(Rb + row + (int64_t)i * b) : 0.0;
     }
   };
+  // z = V2(:, 0:c)ᵀ·x over this wave's columns q < c, reduced into buf[q]
+  auto v2t_times = [&](const double (&x)[RPL], int c, double* buf) {
+    double part[CPW];
+#pragma unroll
+    for (int cl = 0; cl < CPW; ++cl) {
+      double a = 0.0;
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) a += v2[cl][s] * x[s];
+      part[cl] = a;
+    }
+    const double r = xreduce<CPW>(part, lane);
+    const int q = w + WAVES * (lane & (CPW - 1));
+    if (lane < CPW && q < c) buf[q] = r;
+  };
+  // T(0:c, c) = −τ·T(0:c,0:c)·z: this wave's partial sums over its columns
+  auto t_partials = [&](int c) {
+    double py[RPL];
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) py[s] = 0.0;
+    // T's columns q ≥ c are still zero and zbuf holds finite values, so the
+    // sum needs no mask (and no branch per column)
+#pragma unroll
+    for (int cl = 0; cl < CPW; ++cl) {
+      const double zq = zbuf[w + WAVES * cl];
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) py[s] += t[cl][s] * zq;
+    }
+#pragma unroll
+    for (int s = 0; s < RPL; ++s) red[w][lane + 64 * s] = py[s];
+  };
+  // the owner of column c sums the partials into its T column
+  auto t_finish = [&](int c, double tau_c) {
+    if (w == (c % WAVES)) {
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) {
+        const int row = lane + 64 * s;
+        double acc = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < WAVES; ++ww) acc += red[ww][row];
+        const double tv = row < c ? -tau_c * acc : (row == c ? tau_c : 0.0);
+        const int cc = c / WAVES;
+#pragma unroll
+        for (int cl = 0; cl < CPW; ++cl) t[cl][s] = cl == cc ? tv : t[cl][s];
+      }
+    }
+  };
+
+  for (int p = threadIdx.x; p < 128; p += 512) sbuf[p] = ubuf[p] = zbuf[p] = 0.0;
   if (threadIdx.x == 0) {
     s_err = 0;
     wait_inputs(0);
   }
   __syncthreads();
-  if (!s_err) load_inputs(0);
+  if (!s_err) load_inputs(0, ctop, cbot);
 
+  // Software pipeline: iteration i finishes the T column of i−1 alongside
+  // the start of column i, so each column costs four barriers.
+  double vprev[RPL];
+#pragma unroll
+  for (int s = 0; s < RPL; ++s) vprev[s] = 0.0;
+  double tau_prev = 0.0;
+  int done = 0;  // columns whose reflector exists
   for (int i = 0; i < b && !s_err; ++i) {
-    // ctop/cbot hold column i; column i+1 is loaded during the T update below
-    // ---- s = c_top + V2ᵀ c_bot   (columns q < i)
-    {
-      double part[CPW];
-#pragma unroll
-      for (int cl = 0; cl < CPW; ++cl) {
-        double a = 0.0;
-#pragma unroll
-        for (int s = 0; s < RPL; ++s) a += v2[cl][s] * cbot[s];
-        part[cl] = a;
-      }
-      const double r = xreduce<CPW>(part, lane);
-      const int q = w + WAVES * (lane & (CPW - 1));
-      if (lane < CPW && q < i) sbuf[q] = r;
+    // ---- A: z_{i−1} = V2ᵀ·v_{i−1} and s_i = V2ᵀ·c_bot (columns q < i−1, q < i)
+    if (i > 0) v2t_times(vprev, i - 1, zbuf);
+    v2t_times(cbot, i, sbuf);
+    if (w == 0) wait_memory();  // column i−1 of R has reached memory
+    __syncthreads();                                                       // B1
+    if (threadIdx.x == 0) {
+      if (i > 0) flag_store(cnt + nid, epoch * 256 + (uint64_t)i);
+      if (i + 1 < b) wait_inputs(i + 1);
     }
-    __syncthreads();
-    // ---- u = Tᵀ s
+    // ---- B: partial sums of T's column i−1
+    if (i > 0) t_partials(i - 1);
+    __syncthreads();                                                       // B2
+    if (i + 1 < b && !s_err) load_inputs(i + 1, ntop, nbot);  // in flight from here on
+    if (i > 0) t_finish(i - 1, tau_prev);
+    // ---- C: u = Tᵀ·(c_top + V2ᵀ·c_bot)
     {
       double sv[RPL];
 #pragma unroll
       for (int s = 0; s < RPL; ++s) {
         const int row = lane + 64 * s;
-        sv[s] = row < i ? sbuf[row] + ctop[s] : 0.0;  // s = c_top + V2ᵀ·c_bot
+        sv[s] = row < i ? sbuf[row] + ctop[s] : 0.0;
       }
       double part[CPW];
 #pragma unroll
@@ -133,25 +186,23 @@ ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, doubl
       const int q = w + WAVES * (lane & (CPW - 1));
       if (lane < CPW && q < i) ubuf[q] = r;
     }
-    __syncthreads();
-    // ---- c_top −= u; c_bot −= V2·u (partials per wave, summed through LDS)
+    __syncthreads();                                                       // B3
+    // ---- D: c_bot −= V2·u (partials per wave, summed through LDS), c_top −= u
     {
       double pb[RPL];
 #pragma unroll
       for (int s = 0; s < RPL; ++s) pb[s] = 0.0;
+      // V2's columns q ≥ i are still zero: no mask needed (see t_partials)
 #pragma unroll
       for (int cl = 0; cl < CPW; ++cl) {
-        const int q = w + WAVES * cl;
-        if (q < i) {
-          const double uq = ubuf[q];
+        const double uq = ubuf[w + WAVES * cl];
 #pragma unroll
-          for (int s = 0; s < RPL; ++s) pb[s] += v2[cl][s] * uq;
-        }
+        for (int s = 0; s < RPL; ++s) pb[s] += v2[cl][s] * uq;
       }
 #pragma unroll
       for (int s = 0; s < RPL; ++s) red[w][lane + 64 * s] = pb[s];
     }
-    __syncthreads();
+    __syncthreads();                                                       // B4
 #pragma unroll
     for (int s = 0; s < RPL; ++s) {
       const int row = lane + 64 * s;
@@ -161,7 +212,7 @@ ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, doubl
       cbot[s] -= acc;
       if (row < i) ctop[s] -= ubuf[row];
     }
-    // ---- reflector for column i (every wave computes the same scalars)
+    // ---- E: reflector for column i (every wave computes the same scalars)
     double ss = 0.0, piv = 0.0;
 #pragma unroll
     for (int s = 0; s < RPL; ++s) {
@@ -191,7 +242,6 @@ ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, doubl
       xnorm = sqrt(ss);
     }
     const Reflector h = dlarfg_scalar(alpha, xnorm);
-    double v[RPL];
 #pragma unroll
     for (int s = 0; s < RPL; ++s) {
       const int row = lane + 64 * s;
@@ -200,16 +250,16 @@ ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, doubl
         for (int k = 0; k < h.knt; ++k) x *= 0x1p969;
         x *= h.scal;
       }
-      v[s] = row <= i ? x : 0.0;
+      vprev[s] = row <= i ? x : 0.0;
     }
     if (w == (i % WAVES)) {
+      const int ci = i / WAVES;
 #pragma unroll
       for (int cl = 0; cl < CPW; ++cl)
-        if (cl == i / WAVES)
 #pragma unroll
-          for (int s = 0; s < RPL; ++s) v2[cl][s] = v[s];
+        for (int s = 0; s < RPL; ++s) v2[cl][s] = cl == ci ? vprev[s] : v2[cl][s];
     }
-    // ---- publish column i of R: [c_top(0:i); β]
+    // column i of R: [c_top(0:i); β], published after the next barrier
     if (w == 0) {
       for (int s = 0; s < RPL; ++s) {
         const int row = lane + 64 * s;
@@ -217,60 +267,26 @@ ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, doubl
       }
     }
     if (threadIdx.x == 0) stau[i] = h.tau;
-    // ---- z = V2(:,0:i)ᵀ v (columns q < i)
-    {
-      double part[CPW];
+    tau_prev = h.tau;
+    done = i + 1;
 #pragma unroll
-      for (int cl = 0; cl < CPW; ++cl) {
-        double a = 0.0;
-#pragma unroll
-        for (int s = 0; s < RPL; ++s) a += v2[cl][s] * v[s];
-        part[cl] = a;
-      }
-      const double r = xreduce<CPW>(part, lane);
-      const int q = w + WAVES * (lane & (CPW - 1));
-      if (lane < CPW && q < i) zbuf[q] = r;
+    for (int s = 0; s < RPL; ++s) {
+      ctop[s] = ntop[s];
+      cbot[s] = nbot[s];
     }
-    if (w == 0) wait_memory();  // the R column stores above have completed
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      flag_store(cnt + nid, epoch * 256 + (uint64_t)(i + 1));
-      if (i + 1 < b) wait_inputs(i + 1);
-    }
-    // ---- T(0:i, i) = −τ T(0:i,0:i) z: partial over each wave's columns
-    {
-      double py[RPL];
-#pragma unroll
-      for (int s = 0; s < RPL; ++s) py[s] = 0.0;
-#pragma unroll
-      for (int cl = 0; cl < CPW; ++cl) {
-        const int q = w + WAVES * cl;
-        if (q < i) {
-          const double zq = zbuf[q];
-#pragma unroll
-          for (int s = 0; s < RPL; ++s) py[s] += t[cl][s] * zq;
-        }
-      }
-#pragma unroll
-      for (int s = 0; s < RPL; ++s) red[w][lane + 64 * s] = py[s];
-    }
-    __syncthreads();
-    if (i + 1 < b && !s_err) load_inputs(i + 1);  // in flight during the T update
-    if (w == (i % WAVES)) {
-#pragma unroll
-      for (int s = 0; s < RPL; ++s) {
-        const int row = lane + 64 * s;
-        double acc = 0.0;
-#pragma unroll
-        for (int ww = 0; ww < WAVES; ++ww) acc += red[ww][row];
-        const double tv = row < i ? -h.tau * acc : (row == i ? h.tau : 0.0);
-#pragma unroll
-        for (int cl = 0; cl < CPW; ++cl)
-          if (cl == i / WAVES) t[cl][s] = tv;
-      }
-    }
-    __syncthreads();
   }
+  // ---- finish the last column: publish it and complete its T column
+  if (done > 0) {
+    const int c = done - 1;
+    v2t_times(vprev, c, zbuf);
+    if (w == 0) wait_memory();
+    __syncthreads();
+    if (threadIdx.x == 0) flag_store(cnt + nid, epoch * 256 + (uint64_t)done);
+    t_partials(c);
+    __syncthreads();
+    t_finish(c, tau_prev);
+  }
+  __syncthreads();
   if (s_err && threadIdx.x == 0) *err = 1;
   // ---- V2, T, tau of this node
   double* V2 = V2out + nid * bb;
