@@ -51,6 +51,11 @@ SIGNATURES = {
     "cqr_qr_2d": (_I, [_P, _P, _I64, _I64, _I64, _I64, _P, _I64]),
     "cqr_qr_2d_2d": (_I, [_P, _P, _I64, _I64, _I64, _I64, _I64, _P, _I64]),
     "cqr_tau_recon": (_I, [_P, _c.c_char, _I64, _I64, _I64, _P, _P]),
+    "cqr_geqrf2d": (_I, [_P, _P, _I64, _I64, _I64, _I64, _P, _I]),
+    "cqr_apply_qt": (_I, [_P, _P, _I64, _P, _I64, _I64, _I64, _P, _I64, _I64, _I]),
+    "cqr_trsm_r": (_I, [_P, _P, _I64, _I64, _P, _I64, _I64, _I]),
+    "cqr_lstsq": (_I, [_P, _P, _I64, _I64, _I64, _P, _I64, _I64, _P, _I64, _I64, _I, _I, _D,
+                       _c.POINTER(_I), _c.POINTER(_D)]),
     "cqr_pack_upper": (_I, [_P, _P, _P, _I64, _I64]),
     "cqr_unpack_upper": (_I, [_P, _P, _P, _I64, _I64]),
     "cqr_pack_lower": (_I, [_P, _P, _P, _I64, _I64, _I]),

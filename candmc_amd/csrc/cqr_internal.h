@@ -190,8 +190,44 @@ int gemm_f64_ws(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64_t k, dou
 // top b×b (stored as [Y\R] in the panel), Tt = Tᵀ (b×b, ld b).
 int wy_update_f64(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit,
                   const double* Tt, double* A, int64_t lda, int64_t m, int64_t n, int b);
+// Same with workspace (wy_update_ws_doubles(m, n, b) doubles), the choice of
+// Qᵀ (t_trans = 0: Z' = Tᵀ·Z) or Q (t_trans = 1: Z' = T·Z), and the
+// arithmetic of the two large products (CQR_PREC_*).
+size_t wy_update_ws_doubles(int64_t m, int64_t n, int b);
+int wy_update_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const double* Tt, int t_trans,
+                 double* A, int64_t lda, int64_t m, int64_t n, int b, double* ws, int prec);
+int wy_update_f64_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const double* Tt,
+                     double* A, int64_t lda, int64_t m, int64_t n, int b, double* ws);
+size_t gemm_f64_scratch_doubles(int64_t m, int64_t n, int64_t k);
 
-// Drivers (qr_driver.hip)
+// Least-squares kernels (lsq_kernels.hip): C = alpha·op(A)·X + beta·C for
+// X with nr ≤ 16 columns; the transposed form needs
+// thin_gemm_scratch_doubles(m, k, nr) of scratch.
+int64_t thin_t_slab_rows(int64_t m, int64_t k);
+size_t thin_gemm_scratch_doubles(int64_t m, int64_t k, int nr);
+int thin_gemm(cqr_ctx* c, int ta, int64_t m, int64_t k, double alpha, const double* A, int64_t lda,
+              const double* X, int64_t ldx, int nr, double beta, double* C, int64_t ldc, double* scratch);
+int launch_axpby(cqr_ctx* c, int64_t m, int64_t n, double alpha, const double* X, int64_t ldx, double beta,
+                 double* Y, int64_t ldy);
+int launch_colnorm(cqr_ctx* c, const double* X, int64_t ldx, int64_t m, int n, double* out);
+int launch_copy2d(cqr_ctx* c, const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m,
+                  int64_t n);
+
+// Drivers (qr_driver.cpp)
+#define CQR_CHECK_CTX(c)                     \
+  do {                                       \
+    if (!(c)) return CQR_ERR_INVALID;        \
+    (void)hipSetDevice((c)->device);         \
+  } while (0)
+int finite_check(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n);
+// QR_2D workspace layout (byte offsets into one arena) and the driver; the
+// arena base is passed so callers can place their own buffers behind it.
+struct Qr2dWS {
+  size_t panel, upd_near, upd_far, W, Tt;
+};
+size_t qr2d_ws(int64_t m, int64_t k, int64_t b, int64_t leaf_rows, Qr2dWS* o);
+int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b, double* aggreg_Y,
+             int64_t lda_aY, char* base, const Qr2dWS& o, double* Tt_out, int prec);
 int hh_recon_core(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, double* W,
                   double* T, double* invT_T, double* Tt, int64_t nleaves);
 

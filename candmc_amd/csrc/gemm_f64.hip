@@ -12,6 +12,8 @@
 #include "cqr_device.h"
 #include "cqr_internal.h"
 
+#include <type_traits>
+
 namespace cqr {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -69,12 +71,21 @@ __device__ __forceinline__ bool mask_active(int sm, int64_t r_lo, int64_t r_hi, 
   }
 }
 
-template <bool TA, bool TB, int ROLE>
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// PREC = CQR_PREC_F64: FP64 MFMA.  CQR_PREC_F32: the FP64 operands are
+// rounded to FP32 while they are staged in LDS and multiplied on FP32 MFMA
+// (v_mfma_f32_16x16x4f32) with FP32 accumulation; the result is added to
+// the FP64 output in FP64.
+template <bool TA, bool TB, int ROLE, int PREC = CQR_PREC_F64>
 __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
   constexpr int BM = 128, BN = 128, BK = 16, PAD = 4;
   constexpr int LDS_A = BM + PAD, LDS_B = BN + PAD;
-  __shared__ double As[2][BK][LDS_A];
-  __shared__ double Bs[2][BK][LDS_B];
+  constexpr bool LP = PREC != CQR_PREC_F64;
+  typedef typename std::conditional<LP, float, double>::type ST;
+  typedef typename std::conditional<LP, f4v, d4>::type AT;
+  __shared__ ST As[2][BK][LDS_A];
+  __shared__ ST Bs[2][BK][LDS_B];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 64;
@@ -112,11 +123,11 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
   const int64_t a_kstep = TA ? 1 : g.lda;                 // per unit of k
   const int64_t b_kstep = TB ? g.ldb : 1;
 
-  d4 acc[4][4];
+  AT acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < 4; ++j) acc[i][j] = AT{0, 0, 0, 0};
 
   double ra[8], rb[8];
   auto load = [&](int64_t k0) {
@@ -156,8 +167,8 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
   auto store = [&](int buf) {
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-      As[buf][ak + (TA ? 0 : 2 * r)][am + (TA ? 16 * r : 0)] = ra[r];
-      Bs[buf][bk + (TB ? 2 * r : 0)][bn + (TB ? 0 : 16 * r)] = rb[r];
+      As[buf][ak + (TA ? 0 : 2 * r)][am + (TA ? 16 * r : 0)] = (ST)ra[r];
+      Bs[buf][bk + (TB ? 2 * r : 0)][bn + (TB ? 0 : 16 * r)] = (ST)rb[r];
     }
   };
 
@@ -173,7 +184,7 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
     if (t + 1 < ntiles) load(kbeg + (t + 1) * BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      double af[4], bf[4];
+      ST af[4], bf[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) af[i] = As[buf][kk + fr][wm0 + i * 16 + fc];
 #pragma unroll
@@ -181,14 +192,20 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 4; ++j) {
+          if constexpr (LP)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[j], af[i], acc[i][j], 0, 0, 0);
+          else
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+        }
     }
     if (t + 1 < ntiles) store(buf ^ 1);
     __syncthreads();
   }
 
-  // lane l, register v holds C[m0+wm0+16i+(l&15)][n0+wn0+16j+(l>>4)+4v]
+  // lane l, register v holds C[m0+wm0+16i+(l&15)][n0+wn0+16j+c(l,v)] with
+  // c = (l>>4)+4v for the FP64 MFMA and c = 4(l>>4)+v for the FP32 one
+  // (operands are passed swapped, so each lane's values share one row)
   const bool split = g.split_stride > 0;
   if (split) C += (int64_t)blockIdx.z * g.split_stride;
   const bool has_beta = !split && g.beta != 0.0;
@@ -201,10 +218,10 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
     for (int j = 0; j < 4; ++j) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t gn = n0 + wn0 + 16 * j + fr + 4 * v;
+        const int64_t gn = n0 + wn0 + 16 * j + (LP ? 4 * fr + v : fr + 4 * v);
         if (!full_mn && gn >= g.n) continue;
         double* cp = C + gm + gn * g.ldc;
-        double val = alpha * acc[i][j][v];
+        double val = alpha * (double)acc[i][j][v];
         if (has_beta) val += g.beta * *cp;
         if (g.smC == SM_UPPER && gm > gn) val = 0.0;
         *cp = val;
@@ -229,25 +246,33 @@ __global__ void splitk_reduce_kernel(const double* __restrict__ P, int64_t strid
   }
 }
 
-template <int ROLE>
+template <int ROLE, int PREC = CQR_PREC_F64>
 static void launch_role(hipStream_t st, int ta, int tb, dim3 grid, const GemmArgs& g) {
-  if (!ta && !tb) hipLaunchKernelGGL((mfma_gemm_f64<false, false, ROLE>), grid, dim3(256), 0, st, g);
-  else if (ta && !tb) hipLaunchKernelGGL((mfma_gemm_f64<true, false, ROLE>), grid, dim3(256), 0, st, g);
-  else if (!ta && tb) hipLaunchKernelGGL((mfma_gemm_f64<false, true, ROLE>), grid, dim3(256), 0, st, g);
-  else hipLaunchKernelGGL((mfma_gemm_f64<true, true, ROLE>), grid, dim3(256), 0, st, g);
+  if (!ta && !tb) hipLaunchKernelGGL((mfma_gemm_f64<false, false, ROLE, PREC>), grid, dim3(256), 0, st, g);
+  else if (ta && !tb) hipLaunchKernelGGL((mfma_gemm_f64<true, false, ROLE, PREC>), grid, dim3(256), 0, st, g);
+  else if (!ta && tb) hipLaunchKernelGGL((mfma_gemm_f64<false, true, ROLE, PREC>), grid, dim3(256), 0, st, g);
+  else hipLaunchKernelGGL((mfma_gemm_f64<true, true, ROLE, PREC>), grid, dim3(256), 0, st, g);
 }
 
-static void launch_kind(cqr_ctx* c, int ta, int tb, dim3 grid, const GemmArgs& g, int kind) {
+static void launch_kind(cqr_ctx* c, int ta, int tb, dim3 grid, const GemmArgs& g, int kind,
+                        int prec = CQR_PREC_F64) {
+  if (prec == CQR_PREC_F32) {
+    // only the two trailing-update products run in reduced precision
+    if (kind == PK_GEMM_YTA) launch_role<1, CQR_PREC_F32>(c->stream, ta, tb, grid, g);
+    else launch_role<2, CQR_PREC_F32>(c->stream, ta, tb, grid, g);
+    return;
+  }
   if (kind == PK_GEMM_YTA) launch_role<1>(c->stream, ta, tb, grid, g);
   else if (kind == PK_GEMM_UPD) launch_role<2>(c->stream, ta, tb, grid, g);
   else if (kind == PK_BGEMM) launch_role<3>(c->stream, ta, tb, grid, g);
   else launch_role<0>(c->stream, ta, tb, grid, g);
 }
 
-static int launch_mfma(cqr_ctx* c, int ta, int tb, const GemmArgs& g, int nsplit, int kind) {
+static int launch_mfma(cqr_ctx* c, int ta, int tb, const GemmArgs& g, int nsplit, int kind,
+                       int prec = CQR_PREC_F64) {
   hipEvent_t ev = prof_begin(c);
   dim3 grid((unsigned)((g.m + 127) / 128), (unsigned)((g.n + 127) / 128), (unsigned)nsplit);
-  launch_kind(c, ta, tb, grid, g, kind);
+  launch_kind(c, ta, tb, grid, g, kind, prec);
   const double out = (double)g.m * g.n;
   const double bytes = 8.0 * ((double)g.m * g.k + (double)g.k * g.n +
                               (nsplit > 1 ? nsplit * out : (g.beta != 0.0 ? 2.0 : 1.0) * out));
@@ -276,7 +301,8 @@ static int choose_splits(int64_t m, int64_t n, int64_t k, int64_t* kchunk) {
 // doubles when splitting (callers size it with gemm_f64_scratch).
 static int gemm_f64_impl(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha,
                          const double* A, int64_t lda, int smA, const double* B, int64_t ldb, int smB,
-                         double beta, double* C, int64_t ldc, double* scratch, int kind = PK_GEMM_SMALL) {
+                         double beta, double* C, int64_t ldc, double* scratch, int kind = PK_GEMM_SMALL,
+                         int prec = CQR_PREC_F64) {
   if (m <= 0 || n <= 0) return CQR_OK;
   GemmArgs g{};
   g.m = m; g.n = n; g.k = k; g.alpha = alpha; g.beta = beta;
@@ -285,10 +311,10 @@ static int gemm_f64_impl(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64
   const int ns = scratch ? choose_splits(m, n, k, &kc) : 1;
   if (ns <= 1) {
     g.C = C; g.ldc = ldc; g.kchunk = std::max<int64_t>(k, 1); g.split_stride = 0;
-    return launch_mfma(c, ta, tb, g, 1, kind);
+    return launch_mfma(c, ta, tb, g, 1, kind, prec);
   }
   g.C = scratch; g.ldc = m; g.kchunk = kc; g.split_stride = m * n;
-  CQR_TRY(launch_mfma(c, ta, tb, g, ns, kind));
+  CQR_TRY(launch_mfma(c, ta, tb, g, ns, kind, prec));
   const int gx = (int)std::min<int64_t>(2048, (m * n + 255) / 256);
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(splitk_reduce_kernel, dim3(gx), dim3(256), 0, c->stream, scratch, m * n, ns, m, n,
@@ -346,8 +372,8 @@ size_t wy_update_ws_doubles(int64_t m, int64_t n, int b) {
   return gemm_f64_scratch_doubles(b, n, m) + 2 * (size_t)b * n + 64;
 }
 
-int wy_update_f64_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const double* Tt,
-                     double* A, int64_t lda, int64_t m, int64_t n, int b, double* ws) {
+int wy_update_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const double* Tt, int t_trans,
+                 double* A, int64_t lda, int64_t m, int64_t n, int b, double* ws, int prec) {
   if (m <= 0 || n <= 0 || b <= 0) return CQR_OK;
   const int smY = y_implicit ? SM_UNIT_LOWER : SM_NONE;
   double* part = ws;
@@ -356,12 +382,18 @@ int wy_update_f64_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, c
   double* Zp = Z + ((size_t)b * n + 31) / 32 * 32;
   // Z = Yᵀ·A  (b×n)
   CQR_TRY(gemm_f64_impl(c, 1, 0, b, n, m, 1.0, Y, ldy, smY, A, lda, SM_NONE, 0.0, Z, b,
-                        np ? part : nullptr, PK_GEMM_YTA));
-  // Z' = Tᵀ·Z
-  CQR_TRY(gemm_f64_impl(c, 0, 0, b, n, b, 1.0, Tt, b, SM_LOWER, Z, b, SM_NONE, 0.0, Zp, b, nullptr));
+                        np ? part : nullptr, PK_GEMM_YTA, prec));
+  // Z' = Tᵀ·Z (Qᵀ·A), or T·Z with t_trans (Q·A); Tt holds Tᵀ (lower), FP64
+  CQR_TRY(gemm_f64_impl(c, t_trans ? 1 : 0, 0, b, n, b, 1.0, Tt, b, SM_LOWER, Z, b, SM_NONE, 0.0, Zp, b,
+                        nullptr));
   // A −= Y·Z'
   return gemm_f64_impl(c, 0, 0, m, n, b, -1.0, Y, ldy, smY, Zp, b, SM_NONE, 1.0, A, lda, nullptr,
-                       PK_GEMM_UPD);
+                       PK_GEMM_UPD, prec);
+}
+
+int wy_update_f64_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const double* Tt,
+                     double* A, int64_t lda, int64_t m, int64_t n, int b, double* ws) {
+  return wy_update_ws(c, Y, ldy, y_implicit, Tt, 0, A, lda, m, n, b, ws, CQR_PREC_F64);
 }
 
 int wy_update_f64(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const double* Tt,

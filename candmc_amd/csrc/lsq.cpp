@@ -1,0 +1,178 @@
+// Least-squares path on top of the 2D QR: Qᵀ/Q application with the stored
+// compact-WY factors, block triangular solves with R, and the least-squares
+// driver with iterative refinement on FP64 residuals.
+//
+// The Qᵀ application follows apply_tsqr_QT (alg/QR/tsqr/apply_tsqr_QT.cxx:
+// 31-165) in spirit: the reflectors are applied panel by panel as blocked
+// (WY) updates, here with the panel factors of QR_2D instead of the tree
+// factors.  The solve and the refinement have no counterpart in the
+// reference.  Refinement is the augmented-system scheme of Björck (1967):
+// with r = b − A·x the residual, each step solves
+//     [I  A] [dr]   [f]     f = b − r − A·x,
+//     [Aᵀ 0] [dx] = [g],    g = −Aᵀ·r,
+// through the computed factors (h = R⁻ᵀg, d = Qᵀf, dx = R⁻¹(d₁ − h),
+// dr = Q·[h; d₂]) while f and g are formed in FP64.  Its fixed point is
+// the exact least-squares solution, so a factorization computed with an
+// FP32 trailing update converges to FP64 accuracy when κ(A)·u₃₂ < 1.
+#include <cmath>
+#include <vector>
+
+#include "cqr_internal.h"
+
+namespace cqr {
+
+// B (m×nr) := Qᵀ·B (trans = 1) or Q·B (trans = 0), Q = H_1···H_k from the
+// [Y\R] panels of a QR_2D of width b (ldy) and their Tᵀ blocks (b×b each,
+// ld = the panel's width).
+int apply_q_ws(cqr_ctx* c, const double* YR, int64_t ldy, const double* Tt, int64_t m, int64_t k, int64_t b,
+               double* B, int64_t ldb, int64_t nr, int trans, double* ws) {
+  const int64_t np = (k + b - 1) / b;
+  for (int64_t q = 0; q < np; ++q) {
+    const int64_t p = trans ? q : np - 1 - q;
+    const int64_t j = p * b, bj = std::min(b, k - j);
+    if (m - j <= 0) continue;
+    CQR_TRY(wy_update_ws(c, YR + j + j * ldy, ldy, 1, Tt + p * b * b, trans ? 0 : 1, B + j, ldb, m - j, nr,
+                         (int)bj, ws, CQR_PREC_F64));
+  }
+  return CQR_OK;
+}
+
+size_t apply_q_ws_doubles(int64_t m, int64_t b, int64_t nr) { return wy_update_ws_doubles(m, nr, (int)b) + 64; }
+
+// Inverses of the diagonal blocks of R (upper triangle of YR), 128 wide.
+int r_block_inverses(cqr_ctx* c, const double* YR, int64_t ldy, int64_t n, double* Rinv) {
+  const int64_t nb = (n + 127) / 128;
+  for (int64_t p = 0; p < nb; ++p) {
+    const int64_t j = p * 128, bj = std::min<int64_t>(128, n - j);
+    CQR_TRY(launch_tri_inverse(c, YR + j + j * ldy, ldy, Rinv + p * 128 * 128, bj, (int)bj, 1, 0));
+  }
+  return CQR_OK;
+}
+
+// B(0:n, :) := R⁻¹·B (trans = 0) or R⁻ᵀ·B (trans = 1), blocked with the
+// diagonal-block inverses; tmp holds 128×nr doubles.
+int trsm_r_ws(cqr_ctx* c, const double* YR, int64_t ldy, int64_t n, const double* Rinv, double* B, int64_t ldb,
+              int64_t nr, int trans, double* tmp) {
+  const int64_t nb = (n + 127) / 128;
+  for (int64_t q = 0; q < nb; ++q) {
+    const int64_t p = trans ? q : nb - 1 - q;
+    const int64_t j = p * 128, bj = std::min<int64_t>(128, n - j);
+    const double* Ri = Rinv + p * 128 * 128;
+    // x_p = R_pp⁻¹·y_p  (or R_pp⁻ᵀ·y_p)
+    CQR_TRY(gemm_f64_ws(c, trans ? 1 : 0, 0, bj, nr, bj, 1.0, Ri, bj, SM_UPPER, B + j, ldb, SM_NONE, 0.0, tmp,
+                        bj, nullptr));
+    CQR_TRY(launch_copy2d(c, tmp, bj, B + j, ldb, bj, nr));
+    if (!trans && j > 0)        // y(0:j) −= R(0:j, p)·x_p
+      CQR_TRY(gemm_f64_ws(c, 0, 0, j, nr, bj, -1.0, YR + j * ldy, ldy, SM_NONE, B + j, ldb, SM_NONE, 1.0, B,
+                          ldb, nullptr));
+    if (trans && j + bj < n)    // y(j+bj:n) −= R(p, j+bj:n)ᵀ·x_p
+      CQR_TRY(gemm_f64_ws(c, 1, 0, n - j - bj, nr, bj, -1.0, YR + j + (j + bj) * ldy, ldy, SM_NONE, B + j, ldb,
+                          SM_NONE, 1.0, B + j + bj, ldb, nullptr));
+  }
+  return CQR_OK;
+}
+
+}  // namespace cqr
+
+using namespace cqr;
+
+extern "C" {
+
+int cqr_apply_qt(cqr_ctx* c, const double* YR, int64_t ldy, const double* Tt, int64_t m, int64_t k, int64_t b,
+                 double* B, int64_t ldb, int64_t nrhs, int trans) {
+  CQR_CHECK_CTX(c);
+  if (!YR || !Tt || !B || m < 1 || k < 1 || k > m || b < 1 || ldy < m || ldb < m || nrhs < 0)
+    return set_err(c, CQR_ERR_INVALID, "apply_qt: bad arguments");
+  if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "apply_qt: b > %d", CQR_MAX_B);
+  if (nrhs == 0) return CQR_OK;
+  CQR_TRY(ensure_ws(c, 8 * apply_q_ws_doubles(m, b, nrhs)));
+  return apply_q_ws(c, YR, ldy, Tt, m, k, b, B, ldb, nrhs, trans ? 1 : 0, static_cast<double*>(c->ws));
+}
+
+int cqr_trsm_r(cqr_ctx* c, const double* YR, int64_t ldy, int64_t n, double* B, int64_t ldb, int64_t nrhs,
+               int trans) {
+  CQR_CHECK_CTX(c);
+  if (!YR || !B || n < 1 || ldy < n || ldb < n || nrhs < 0) return set_err(c, CQR_ERR_INVALID, "trsm_r: bad arguments");
+  if (nrhs == 0) return CQR_OK;
+  const int64_t nb = (n + 127) / 128;
+  Sizer s;
+  const size_t oR = s.add(8 * nb * 128 * 128), oT = s.add(8 * 128 * nrhs);
+  CQR_TRY(ensure_ws(c, s.bytes));
+  char* base = static_cast<char*>(c->ws);
+  double* Rinv = reinterpret_cast<double*>(base + oR);
+  CQR_TRY(r_block_inverses(c, YR, ldy, n, Rinv));
+  return trsm_r_ws(c, YR, ldy, n, Rinv, B, ldb, nrhs, trans ? 1 : 0, reinterpret_cast<double*>(base + oT));
+}
+
+int cqr_lstsq(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, const double* B, int64_t ldb,
+              int64_t nrhs, double* X, int64_t ldx, int64_t b, int prec, int max_iter, double tol, int* iters,
+              double* last_update) {
+  CQR_CHECK_CTX(c);
+  if (iters) *iters = 0;
+  if (last_update) *last_update = 0.0;
+  if (!A || !B || !X || m < 1 || n < 1 || m < n || lda < m || ldb < m || ldx < n || nrhs < 1 || b < 1 ||
+      max_iter < 0)
+    return set_err(c, CQR_ERR_INVALID, "lstsq: bad arguments (needs m >= n >= 1, nrhs >= 1)");
+  if (nrhs > 16) return set_err(c, CQR_ERR_UNSUPPORTED, "lstsq: at most 16 right-hand sides");
+  if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "lstsq: b > %d", CQR_MAX_B);
+  if (prec != CQR_PREC_F64 && prec != CQR_PREC_F32) return set_err(c, CQR_ERR_UNSUPPORTED, "lstsq: precision %d", prec);
+  CQR_TRY(finite_check(c, A, lda, m, n));
+  const int64_t np = (n + b - 1) / b, nb = (n + 127) / 128, nr = nrhs;
+  Qr2dWS o;
+  Sizer s;
+  s.bytes = qr2d_ws(m, n, b, c->leaf_rows, &o);
+  const size_t oAf = s.add(8 * (size_t)m * n), oTt = s.add(8 * np * b * b), oRi = s.add(8 * nb * 128 * 128),
+               oR = s.add(8 * (size_t)m * nr), oF = s.add(8 * (size_t)m * nr), oG = s.add(8 * (size_t)n * nr),
+               oH = s.add(8 * (size_t)n * nr), oTmp = s.add(8 * 128 * nr),
+               oQ = s.add(8 * std::max(apply_q_ws_doubles(m, b, nr), thin_gemm_scratch_doubles(m, n, (int)nr))),
+               oN = s.add(8 * 2 * nr);
+  CQR_TRY(ensure_ws(c, s.bytes));
+  char* base = static_cast<char*>(c->ws);
+  auto at = [&](size_t off) { return reinterpret_cast<double*>(base + off); };
+  double *Af = at(oAf), *Tt = at(oTt), *Rinv = at(oRi), *r = at(oR), *f = at(oF), *g = at(oG), *h = at(oH),
+         *tmp = at(oTmp), *wsq = at(oQ), *nrm = at(oN);
+  // factor a copy of A (the caller's A stays the FP64 reference operator)
+  CQR_TRY(launch_copy2d(c, A, lda, Af, m, m, n));
+  CQR_TRY(qr2d_run(c, Af, m, m, n, b, nullptr, 0, base, o, Tt, prec));
+  CQR_TRY(r_block_inverses(c, Af, m, n, Rinv));
+  // x0 = R⁻¹·(Qᵀb)(0:n)
+  CQR_TRY(launch_copy2d(c, B, ldb, f, m, m, nr));
+  CQR_TRY(apply_q_ws(c, Af, m, Tt, m, n, b, f, m, nr, 1, wsq));
+  CQR_TRY(trsm_r_ws(c, Af, m, n, Rinv, f, m, nr, 0, tmp));
+  CQR_TRY(launch_copy2d(c, f, m, X, ldx, n, nr));
+  if (max_iter == 0) return CQR_OK;
+  // r = b − A·x
+  CQR_TRY(launch_copy2d(c, B, ldb, r, m, m, nr));
+  CQR_TRY(thin_gemm(c, 0, m, n, -1.0, A, lda, X, ldx, (int)nr, 1.0, r, m, nullptr));
+  std::vector<double> hn(2 * nr);
+  for (int it = 1; it <= max_iter; ++it) {
+    // f = b − r − A·x,  g = −Aᵀ·r   (FP64)
+    CQR_TRY(launch_copy2d(c, B, ldb, f, m, m, nr));
+    CQR_TRY(launch_axpby(c, m, nr, -1.0, r, m, 1.0, f, m));
+    CQR_TRY(thin_gemm(c, 0, m, n, -1.0, A, lda, X, ldx, (int)nr, 1.0, f, m, nullptr));
+    CQR_TRY(thin_gemm(c, 1, m, n, -1.0, A, lda, r, m, (int)nr, 0.0, g, n, wsq));
+    // h = R⁻ᵀ·g;  d = Qᵀ·f;  dx = R⁻¹·(d₁ − h);  dr = Q·[h; d₂]
+    CQR_TRY(trsm_r_ws(c, Af, m, n, Rinv, g, n, nr, 1, tmp));               // g := h
+    CQR_TRY(apply_q_ws(c, Af, m, Tt, m, n, b, f, m, nr, 1, wsq));           // f := d
+    CQR_TRY(launch_copy2d(c, f, m, h, n, n, nr));                            // h := d₁
+    CQR_TRY(launch_axpby(c, n, nr, -1.0, g, n, 1.0, h, n));                  // h := d₁ − h
+    CQR_TRY(trsm_r_ws(c, Af, m, n, Rinv, h, n, nr, 0, tmp));                // h := dx
+    CQR_TRY(launch_copy2d(c, g, n, f, m, n, nr));                            // f := [h; d₂]
+    CQR_TRY(apply_q_ws(c, Af, m, Tt, m, n, b, f, m, nr, 0, wsq));           // f := dr
+    CQR_TRY(launch_axpby(c, n, nr, 1.0, h, n, 1.0, X, ldx));
+    CQR_TRY(launch_axpby(c, m, nr, 1.0, f, m, 1.0, r, m));
+    // stop when every column's update is below tol relative to the solution
+    CQR_TRY(launch_colnorm(c, h, n, n, (int)nr, nrm));
+    CQR_TRY(launch_colnorm(c, X, ldx, n, (int)nr, nrm + nr));
+    CQR_HIP(c, hipMemcpyAsync(hn.data(), nrm, sizeof(double) * 2 * nr, hipMemcpyDeviceToHost, c->stream));
+    CQR_HIP(c, hipStreamSynchronize(c->stream));
+    double rel = 0.0;
+    for (int64_t j = 0; j < nr; ++j) rel = std::max(rel, hn[j] / std::max(hn[nr + j], 1e-300));
+    if (iters) *iters = it;
+    if (last_update) *last_update = rel;
+    if (!(rel > tol)) break;  // also stops on NaN
+  }
+  return CQR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,205 @@
+// Kernels of the least-squares path: thin products with a few right-hand
+// sides (A·X and Aᵀ·X with X of at most 16 columns, FP64, bandwidth-bound
+// on A), element-wise updates and column norms.
+#include "cqr_device.h"
+#include "cqr_internal.h"
+
+namespace cqr {
+
+namespace {
+constexpr int TN_ROWS = 64;   // k-slab of X staged in LDS (A·X)
+constexpr int TT_ROWS = 32;   // row slab of A and X staged in LDS (Aᵀ·X)
+}  // namespace
+
+// C (m×nr) = alpha·A·X + beta·C, A m×k, X k×nr (nr ≤ NR); one row per thread.
+template <int NR>
+__global__ void __launch_bounds__(256)
+thin_gemm_n_kernel(const double* __restrict__ A, int64_t lda, int64_t m, int64_t k,
+                   const double* __restrict__ X, int64_t ldx, int nr, double alpha, double beta,
+                   double* __restrict__ C, int64_t ldc) {
+  __shared__ double Xs[TN_ROWS][NR];
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double acc[NR];
+#pragma unroll
+  for (int j = 0; j < NR; ++j) acc[j] = 0.0;
+  for (int64_t k0 = 0; k0 < k; k0 += TN_ROWS) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < TN_ROWS * NR; e += 256) {
+      const int kk = e % TN_ROWS, j = e / TN_ROWS;
+      Xs[kk][j] = (k0 + kk < k && j < nr) ? X[(k0 + kk) + (int64_t)j * ldx] : 0.0;
+    }
+    __syncthreads();
+    if (r < m) {
+      const int kend = (int)min<int64_t>(TN_ROWS, k - k0);
+      const double* ap = A + r + k0 * lda;
+      int kk = 0;
+      for (; kk + 4 <= kend; kk += 4) {
+        const double a0 = ap[(int64_t)kk * lda], a1 = ap[(int64_t)(kk + 1) * lda];
+        const double a2 = ap[(int64_t)(kk + 2) * lda], a3 = ap[(int64_t)(kk + 3) * lda];
+#pragma unroll
+        for (int j = 0; j < NR; ++j)
+          acc[j] += a0 * Xs[kk][j] + a1 * Xs[kk + 1][j] + a2 * Xs[kk + 2][j] + a3 * Xs[kk + 3][j];
+      }
+      for (; kk < kend; ++kk) {
+        const double a = ap[(int64_t)kk * lda];
+#pragma unroll
+        for (int j = 0; j < NR; ++j) acc[j] += a * Xs[kk][j];
+      }
+    }
+  }
+  if (r < m) {
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      if (j >= nr) continue;
+      double* cp = C + r + (int64_t)j * ldc;
+      *cp = alpha * acc[j] + (beta != 0.0 ? beta * *cp : 0.0);
+    }
+  }
+}
+
+// Partial P_z (k×nr, ld k) = A(rows of slab z, :)ᵀ·X(rows of slab z, :) for
+// a block of 128 columns of A; thread (column c = tid & 127, half h) owns
+// NR/2 right-hand sides.  Slabs are reduced afterwards in a fixed order.
+template <int NR>
+__global__ void __launch_bounds__(256)
+thin_gemm_t_kernel(const double* __restrict__ A, int64_t lda, int64_t m, int64_t k,
+                   const double* __restrict__ X, int64_t ldx, int nr, int64_t slab,
+                   double* __restrict__ P) {
+  constexpr int NH = NR > 1 ? NR / 2 : 1;
+  __shared__ double As[TT_ROWS][128 + 1];
+  __shared__ double Xs[TT_ROWS][NR];
+  const int tid = threadIdx.x, c = tid & 127, h = tid >> 7;
+  const int64_t c0 = (int64_t)blockIdx.x * 128, r0 = (int64_t)blockIdx.y * slab;
+  const int64_t r1 = min<int64_t>(m, r0 + slab);
+  double acc[NH];
+#pragma unroll
+  for (int j = 0; j < NH; ++j) acc[j] = 0.0;
+  for (int64_t rb = r0; rb < r1; rb += TT_ROWS) {
+    __syncthreads();
+    for (int e = tid; e < TT_ROWS * 128; e += 256) {
+      const int rr = e % TT_ROWS, cc = e / TT_ROWS;
+      const int64_t row = rb + rr, col = c0 + cc;
+      As[rr][cc] = (row < r1 && col < k) ? A[row + col * lda] : 0.0;
+    }
+    for (int e = tid; e < TT_ROWS * NR; e += 256) {
+      const int rr = e % TT_ROWS, j = e / TT_ROWS;
+      const int64_t row = rb + rr;
+      Xs[rr][j] = (row < r1 && j < nr) ? X[row + (int64_t)j * ldx] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int rr = 0; rr < TT_ROWS; ++rr) {
+      const double a = As[rr][c];
+#pragma unroll
+      for (int j = 0; j < NH; ++j) acc[j] += a * Xs[rr][(NR > 1 ? h * NH : 0) + j];
+    }
+  }
+  const int64_t col = c0 + c;
+  if (col < k && (NR > 1 || h == 0)) {
+    double* pz = P + (int64_t)blockIdx.y * k * nr;
+#pragma unroll
+    for (int j = 0; j < NH; ++j) {
+      const int jj = (NR > 1 ? h * NH : 0) + j;
+      if (jj < nr) pz[col + (int64_t)jj * k] = acc[j];
+    }
+  }
+}
+
+// C = alpha·Σ_z P_z + beta·C (fixed order).
+__global__ void slab_reduce_kernel(const double* __restrict__ P, int nslab, int64_t k, int nr, double alpha,
+                                   double beta, double* __restrict__ C, int64_t ldc) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < k * nr; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e % k, j = e / k;
+    double s = 0.0;
+    for (int z = 0; z < nslab; ++z) s += P[(int64_t)z * k * nr + e];
+    double* cp = C + r + j * ldc;
+    *cp = alpha * s + (beta != 0.0 ? beta * *cp : 0.0);
+  }
+}
+
+// Y = alpha·X + beta·Y on an m×n block.
+__global__ void axpby_kernel(int64_t m, int64_t n, double alpha, const double* __restrict__ X, int64_t ldx,
+                             double beta, double* __restrict__ Y, int64_t ldy) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m * n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e % m, j = e / m;
+    double* yp = Y + r + j * ldy;
+    *yp = alpha * X[r + j * ldx] + (beta != 0.0 ? beta * *yp : 0.0);
+  }
+}
+
+// out[j] = ‖X(:, j)‖₂ (scaled sum of squares in FP64), one workgroup per column.
+__global__ void __launch_bounds__(256)
+colnorm_kernel(const double* __restrict__ X, int64_t ldx, int64_t m, double* __restrict__ out) {
+  __shared__ double part[4];
+  const int j = blockIdx.x;
+  const double* x = X + (int64_t)j * ldx;
+  double mx = 0.0;
+  for (int64_t r = threadIdx.x; r < m; r += 256) mx = fmax(mx, fabs(x[r]));
+  mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  mx = fmax(fmax(part[0], part[1]), fmax(part[2], part[3]));
+  __syncthreads();
+  double ss = 0.0;
+  if (mx > 0.0)
+    for (int64_t r = threadIdx.x; r < m; r += 256) { const double t = x[r] / mx; ss += t * t; }
+  ss = wave_sum(ss);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = ss;
+  __syncthreads();
+  if (threadIdx.x == 0) out[j] = mx * sqrt(part[0] + part[1] + part[2] + part[3]);
+}
+
+// ---------------------------------------------------------------------------
+int thin_gemm(cqr_ctx* c, int ta, int64_t m, int64_t k, double alpha, const double* A, int64_t lda,
+              const double* X, int64_t ldx, int nr, double beta, double* C, int64_t ldc, double* scratch) {
+  if (nr < 1 || nr > 16) return set_err(c, CQR_ERR_UNSUPPORTED, "thin product with %d columns", nr);
+  if (m <= 0 || k <= 0) return CQR_OK;
+  if (!ta) {
+    const dim3 grid((unsigned)((m + 255) / 256));
+    if (nr == 1) hipLaunchKernelGGL(thin_gemm_n_kernel<1>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, alpha, beta, C, ldc);
+    else if (nr <= 4) hipLaunchKernelGGL(thin_gemm_n_kernel<4>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, alpha, beta, C, ldc);
+    else hipLaunchKernelGGL(thin_gemm_n_kernel<16>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, alpha, beta, C, ldc);
+    return check_launch(c, "thin_gemm_n");
+  }
+  // C (k×nr) = alpha·Aᵀ·X + beta·C, A m×k; slabs of rows, then a fixed-order sum
+  const int64_t slab = thin_t_slab_rows(m, k);
+  const int nslab = (int)((m + slab - 1) / slab);
+  const dim3 grid((unsigned)((k + 127) / 128), (unsigned)nslab);
+  if (nr == 1) hipLaunchKernelGGL(thin_gemm_t_kernel<1>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, slab, scratch);
+  else if (nr <= 4) hipLaunchKernelGGL(thin_gemm_t_kernel<4>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, slab, scratch);
+  else hipLaunchKernelGGL(thin_gemm_t_kernel<16>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, slab, scratch);
+  CQR_TRY(check_launch(c, "thin_gemm_t"));
+  const int gx = (int)std::min<int64_t>(1024, (k * nr + 255) / 256);
+  hipLaunchKernelGGL(slab_reduce_kernel, dim3(gx), dim3(256), 0, c->stream, scratch, nslab, k, nr, alpha, beta, C, ldc);
+  return check_launch(c, "slab_reduce");
+}
+
+// Rows per slab of the Aᵀ·X product: about 512 workgroups in total.
+int64_t thin_t_slab_rows(int64_t m, int64_t k) {
+  const int64_t cblocks = (k + 127) / 128;
+  int64_t nslab = std::max<int64_t>(1, 512 / cblocks);
+  int64_t slab = (m + nslab - 1) / nslab;
+  slab = std::max<int64_t>(TT_ROWS, (slab + TT_ROWS - 1) / TT_ROWS * TT_ROWS);
+  return slab;
+}
+
+size_t thin_gemm_scratch_doubles(int64_t m, int64_t k, int nr) {
+  const int64_t slab = thin_t_slab_rows(m, k);
+  return (size_t)((m + slab - 1) / slab) * k * nr;
+}
+
+int launch_axpby(cqr_ctx* c, int64_t m, int64_t n, double alpha, const double* X, int64_t ldx, double beta,
+                 double* Y, int64_t ldy) {
+  if (m <= 0 || n <= 0) return CQR_OK;
+  const int gx = (int)std::min<int64_t>(2048, (m * n + 255) / 256);
+  hipLaunchKernelGGL(axpby_kernel, dim3(gx), dim3(256), 0, c->stream, m, n, alpha, X, ldx, beta, Y, ldy);
+  return check_launch(c, "axpby");
+}
+
+int launch_colnorm(cqr_ctx* c, const double* X, int64_t ldx, int64_t m, int n, double* out) {
+  if (n <= 0) return CQR_OK;
+  hipLaunchKernelGGL(colnorm_kernel, dim3(n), dim3(256), 0, c->stream, X, ldx, m, out);
+  return check_launch(c, "colnorm");
+}
+
+}  // namespace cqr

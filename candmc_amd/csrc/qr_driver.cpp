@@ -7,10 +7,6 @@
 
 namespace cqr {
 
-size_t wy_update_ws_doubles(int64_t m, int64_t n, int b);
-int wy_update_f64_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const double* Tt,
-                     double* A, int64_t lda, int64_t m, int64_t n, int b, double* ws);
-size_t gemm_f64_scratch_doubles(int64_t m, int64_t n, int64_t k);
 int launch_tril_halve(cqr_ctx* c, const double* G, int b, double* T, int64_t ldt);
 int launch_baxpy(cqr_ctx* c, double* C, const double* X, int64_t bb, int batch, double alpha);
 int launch_copy2d(cqr_ctx* c, const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m,
@@ -174,12 +170,6 @@ static int invTT_from_W(cqr_ctx* c, const double* W, int b, const double* Y1, in
 }  // namespace cqr
 
 using namespace cqr;
-
-#define CQR_CHECK_CTX(c)                     \
-  do {                                       \
-    if (!(c)) return CQR_ERR_INVALID;        \
-    (void)hipSetDevice((c)->device);         \
-  } while (0)
 
 extern "C" {
 
@@ -398,6 +388,10 @@ int cqr_update_A(cqr_ctx* c, const double* Y, int64_t lda_Y, double* A, int64_t 
   return CQR_OK;
 }
 
+}  // extern "C"
+
+namespace cqr {
+
 // Workspace of QR_2D for an m×k matrix with panel width b: the maximum over
 // all panels (the split-K scratch of the update is not monotone in j).
 // part 0: the whole trailing block; 1: the next panel's columns; 2: the rest.
@@ -418,11 +412,7 @@ static size_t max_panel_ws(int64_t m, int64_t k, int64_t b, int64_t leaf_rows) {
   return mx;
 }
 
-struct Qr2dWS {
-  size_t panel, upd_near, upd_far, W, Tt;
-};
-
-static size_t qr2d_ws(int64_t m, int64_t k, int64_t b, int64_t leaf_rows, Qr2dWS* o) {
+size_t qr2d_ws(int64_t m, int64_t k, int64_t b, int64_t leaf_rows, Qr2dWS* o) {
   Sizer s;
   const int64_t b0 = std::min(k, b);
   o->panel = s.add(max_panel_ws(m, k, b, leaf_rows));
@@ -441,8 +431,8 @@ struct OnSide {
   ~OnSide() { c->stream = main; }
 };
 
-static int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b, double* aggreg_Y,
-                    int64_t lda_aY, char* base, const Qr2dWS& o) {
+int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b, double* aggreg_Y,
+             int64_t lda_aY, char* base, const Qr2dWS& o, double* Tt_out, int prec) {
   double* W = reinterpret_cast<double*>(base + o.W);
   double* Tt2[2] = {reinterpret_cast<double*>(base + o.Tt), reinterpret_cast<double*>(base + o.Tt) + b * b};
   double* ws_near = reinterpret_cast<double*>(base + o.upd_near);
@@ -456,12 +446,13 @@ static int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, in
   for (int64_t j = 0, p = 0;; j += b, ++p) {             // qr_2d.cxx:299-363
     const int64_t mj = m - j, kj = k - j, bj = std::min(kj, b);
     double* P = A + j + j * lda;
-    double* Tt = Tt2[p & 1];
+    double* Tt = Tt_out ? Tt_out + p * b * b : Tt2[p & 1];
     if ((rc = hh_recon_panel(c, P, lda, mj, bj, W, nullptr, nullptr, Tt, 0, base + o.panel, nullptr))) break;
     if (aggreg_Y && (rc = launch_copy_unit_lower(c, P, lda, aggreg_Y + j + j * lda_aY, lda_aY, mj, bj))) break;
     if (!(kj - b > 0 && mj - b > 0)) break;
     if (!la) {
-      if ((rc = wy_update_f64_ws(c, P, lda, 1, Tt, P + bj * lda, lda, mj, kj - bj, (int)bj, ws_near))) break;
+      if ((rc = wy_update_ws(c, P, lda, 1, Tt, 0, P + bj * lda, lda, mj, kj - bj, (int)bj, ws_near, prec)))
+        break;
       continue;
     }
     // Lookahead (qr_2d.cxx:378-635): the next panel's b columns first, on
@@ -474,7 +465,7 @@ static int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, in
       rc = set_err(c, CQR_ERR_DEVICE, "QR_2D: stream wait failed");
       break;
     }
-    if ((rc = wy_update_f64_ws(c, P, lda, 1, Tt, P + bj * lda, lda, mj, bn, (int)bj, ws_near))) break;
+    if ((rc = wy_update_ws(c, P, lda, 1, Tt, 0, P + bj * lda, lda, mj, bn, (int)bj, ws_near, prec))) break;
     if (rest > 0) {
       if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess ||
           hipStreamWaitEvent(c->side, c->ev_fork, 0) != hipSuccess) {
@@ -482,7 +473,8 @@ static int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, in
         break;
       }
       OnSide g(c);
-      if ((rc = wy_update_f64_ws(c, P, lda, 1, Tt, P + (bj + bn) * lda, lda, mj, rest, (int)bj, ws_far))) break;
+      if ((rc = wy_update_ws(c, P, lda, 1, Tt, 0, P + (bj + bn) * lda, lda, mj, rest, (int)bj, ws_far, prec)))
+        break;
       if (hipEventRecord(c->ev_join, c->side) != hipSuccess) {
         rc = set_err(c, CQR_ERR_DEVICE, "QR_2D: event record failed");
         break;
@@ -496,6 +488,10 @@ static int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, in
   return rc;
 }
 
+}  // namespace cqr
+
+extern "C" {
+
 int cqr_qr_2d(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b, double* aggreg_Y,
               int64_t lda_aY) {
   CQR_CHECK_CTX(c);
@@ -506,7 +502,21 @@ int cqr_qr_2d(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t 
   Qr2dWS o;
   const size_t need = qr2d_ws(m, k, b, c->leaf_rows, &o);
   CQR_TRY(ensure_ws(c, need));
-  return qr2d_run(c, A, lda, m, k, b, aggreg_Y, lda_aY, (char*)c->ws, o);
+  return qr2d_run(c, A, lda, m, k, b, aggreg_Y, lda_aY, (char*)c->ws, o, nullptr, CQR_PREC_F64);
+}
+
+int cqr_geqrf2d(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b, double* Tt, int prec) {
+  CQR_CHECK_CTX(c);
+  if (!A || m < 1 || k < 1 || b < 1 || lda < m || m < k)
+    return set_err(c, CQR_ERR_INVALID, "geqrf2d: bad arguments (needs m >= k, lda >= m)");
+  if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "geqrf2d: b > %d", CQR_MAX_B);
+  if (prec != CQR_PREC_F64 && prec != CQR_PREC_F32)
+    return set_err(c, CQR_ERR_UNSUPPORTED, "geqrf2d: precision %d", prec);
+  CQR_TRY(finite_check(c, A, lda, m, k));
+  Qr2dWS o;
+  const size_t need = qr2d_ws(m, k, b, c->leaf_rows, &o);
+  CQR_TRY(ensure_ws(c, need));
+  return qr2d_run(c, A, lda, m, k, b, nullptr, 0, (char*)c->ws, o, Tt, prec);
 }
 
 int cqr_qr_2d_2d(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b, int64_t b_sub,
@@ -533,7 +543,7 @@ int cqr_qr_2d_2d(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64
     double* P = A + j + j * lda;
     double* aY = aggreg_Y ? aggreg_Y + j + j * lda_aY : nullptr;
     if (kj - b > 0 && mj - b > 0) {
-      CQR_TRY(qr2d_run(c, P, lda, mj, b, b_sub, aY, lda_aY, base, o));
+      CQR_TRY(qr2d_run(c, P, lda, mj, b, b_sub, aY, lda_aY, base, o, nullptr, CQR_PREC_F64));
       // aggregated T⁻ᵀ = tril(YᵀY), diagonal halved (compute_invT_from_Y)
       CQR_TRY(gemm_f64_ws(c, 1, 0, b, b, mj, 1.0, P, lda, SM_UNIT_LOWER, P, lda, SM_UNIT_LOWER, 0.0, at(oG), b,
                           at(oS)));
@@ -541,7 +551,7 @@ int cqr_qr_2d_2d(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64
       CQR_TRY(Tt_from_invTT(c, at(oI), b, at(oTt), (int)b));
       CQR_TRY(wy_update_f64_ws(c, P, lda, 1, at(oTt), P + b * lda, lda, mj, kj - b, (int)b, at(oS)));
     } else {
-      return qr2d_run(c, P, lda, mj, kj, b_sub, aY, lda_aY, base, o);
+      return qr2d_run(c, P, lda, mj, kj, b_sub, aY, lda_aY, base, o, nullptr, CQR_PREC_F64);
     }
   }
 }

@@ -250,6 +250,45 @@ def QR_2D_2D(A, lda_A, m, k, b, b_sub, pv=None, aggreg_Y=None, lda_aY=0):
     ctx.check(ctx.lib.cqr_qr_2d_2d(ctx.h, _pd(A), lda_A, m, k, b, b_sub, _pd(aggreg_Y), lda_aY))
 
 
+# --- least-squares path ------------------------------------------------------
+PREC_F64, PREC_F32 = 0, 1
+
+
+def geqrf2d(A, lda, m, k, b, Tt=None, prec=PREC_F64):
+    """QR_2D that also returns the panel factors Tᵀ (⌈k/b⌉ blocks of b·b)
+    and runs the trailing update in `prec`.  Returns Tt."""
+    ctx = context()
+    if Tt is None:
+        Tt = torch.zeros(((k + b - 1) // b) * b * b, dtype=torch.float64, device=A.device)
+    ctx.check(ctx.lib.cqr_geqrf2d(ctx.h, _pd(A), lda, m, k, b, _pd(Tt), int(prec)))
+    return Tt
+
+
+def apply_qt(YR, ldy, Tt, m, k, b, B, ldb, nrhs, trans=True):
+    """B := Qᵀ·B (trans) or Q·B, Q from geqrf2d (apply_tsqr_QT.cxx:31-165 analogue)."""
+    ctx = context()
+    ctx.check(ctx.lib.cqr_apply_qt(ctx.h, _pd(YR), ldy, _pd(Tt), m, k, b, _pd(B), ldb, nrhs,
+                                   1 if trans else 0))
+
+
+def trsm_r(YR, ldy, n, B, ldb, nrhs, trans=False):
+    """B(0:n) := R⁻¹·B (or R⁻ᵀ·B), R the upper triangle of YR."""
+    ctx = context()
+    ctx.check(ctx.lib.cqr_trsm_r(ctx.h, _pd(YR), ldy, n, _pd(B), ldb, nrhs, 1 if trans else 0))
+
+
+def lstsq(A, lda, m, n, B, ldb, nrhs, X=None, ldx=None, b=128, prec=PREC_F64, max_iter=10, tol=1e-14):
+    """min ‖A·X − B‖ with refinement on FP64 residuals; returns (X, iters, last_update)."""
+    ctx = context()
+    ldx = n if ldx is None else ldx
+    if X is None:
+        X = torch.zeros(ldx * nrhs, dtype=torch.float64, device=A.device)
+    it, upd = ctypes.c_int(), ctypes.c_double()
+    ctx.check(ctx.lib.cqr_lstsq(ctx.h, _pd(A), lda, m, n, _pd(B), ldb, nrhs, _pd(X), ldx, b, int(prec),
+                                int(max_iter), float(tol), ctypes.byref(it), ctypes.byref(upd)))
+    return X, it.value, upd.value
+
+
 # --- alg/shared/util ---------------------------------------------------------
 def tau_recon(structure, ncol, nrow, lda, Y, tau):
     ctx = context()

@@ -31,6 +31,12 @@ enum cqr_status {
   CQR_ERR_NONFINITE = 5    /* input holds NaN/Inf (only with check_finite)  */
 };
 
+/* Arithmetic of the trailing (compact-WY) update; panels are always FP64. */
+enum cqr_prec {
+  CQR_PREC_F64 = 0,  /* FP64 MFMA                                          */
+  CQR_PREC_F32 = 1   /* FP32 MFMA, FP32 accumulation, FP64 matrix in HBM    */
+};
+
 /* Largest panel width the FP64 panel kernels accept. */
 #define CQR_MAX_B 128
 
@@ -172,6 +178,32 @@ int cqr_pack_lower(cqr_ctx* ctx, const double* T, double* P, int64_t b, int64_t 
                    int has_diag);
 int cqr_unpack_lower(cqr_ctx* ctx, const double* P, double* T, int64_t b, int64_t lda,
                      int has_diag);
+
+/* ---- f1 least-squares path (no reference counterpart except the
+ *      implicit Qᵀ application of alg/QR/tsqr/apply_tsqr_QT.cxx:31-165) --- */
+/* QR_2D of an m×k matrix (m ≥ k) that also returns the panel factors:
+ * Tt (⌈k/b⌉ blocks of b·b doubles; block p holds Tᵀ of panel p with ld equal
+ * to that panel's width), and runs the trailing update in `prec`
+ * (CQR_PREC_F64 or CQR_PREC_F32; the panels are FP64 either way).        */
+int cqr_geqrf2d(cqr_ctx* ctx, double* A, int64_t lda, int64_t m, int64_t k, int64_t b,
+                double* Tt, int prec);
+/* B (m×nrhs, ld ldb) := Qᵀ·B (trans = 1) or Q·B (trans = 0), with Q from
+ * cqr_geqrf2d ([Y\R] in YR, its Tt, the same m, k, b).                  */
+int cqr_apply_qt(cqr_ctx* ctx, const double* YR, int64_t ldy, const double* Tt, int64_t m,
+                 int64_t k, int64_t b, double* B, int64_t ldb, int64_t nrhs, int trans);
+/* B(0:n, :) := R⁻¹·B (trans = 0) or R⁻ᵀ·B (trans = 1), R the upper
+ * triangle of the leading n×n block of YR.                               */
+int cqr_trsm_r(cqr_ctx* ctx, const double* YR, int64_t ldy, int64_t n, double* B, int64_t ldb,
+               int64_t nrhs, int trans);
+/* min ‖A·X − B‖ for m ≥ n, 1 ≤ nrhs ≤ 16: QR of a copy of A (A itself is
+ * not modified) with the trailing update in `prec`, then up to max_iter
+ * steps of iterative refinement with FP64 residuals (augmented system),
+ * stopping when every column's update is ≤ tol relative to the solution.
+ * iters / last_update (nullable) report the steps taken and the last
+ * relative update.                                                       */
+int cqr_lstsq(cqr_ctx* ctx, const double* A, int64_t lda, int64_t m, int64_t n, const double* B,
+              int64_t ldb, int64_t nrhs, double* X, int64_t ldx, int64_t b, int prec, int max_iter,
+              double tol, int* iters, double* last_update);
 
 #ifdef __cplusplus
 }

@@ -480,6 +480,16 @@ def apply_QT(YR, B, b=None):
     return B
 
 
+def conditioned_matrix(m, n, kappa, seed):
+    """A = U·diag(s)·Vᵀ with U (m×n) and V (n×n) orthonormal from FP64 QR of
+    seeded Gaussians and s geometric from 1 to 1/κ (the mixed-precision
+    least-squares workload)."""
+    U, _ = np.linalg.qr(gaussian(m, n, seed))
+    V, _ = np.linalg.qr(gaussian(n, n, seed + 1))
+    s = np.geomspace(1.0, 1.0 / kappa, n)
+    return np.asfortranarray((U * s) @ V.T)
+
+
 def lstsq_qr(A, B, b=128):
     """x = R⁻¹·(Qᵀb)[0:n] through QR_2D (FP64)."""
     YR = QR_2D(A, b)

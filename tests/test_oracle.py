@@ -247,3 +247,21 @@ def test_lstsq_matches_scipy():
     x = O.lstsq_qr(A, B, b=16)
     xs = sl.lstsq(A, B)[0]
     assert np.abs(x - xs).max() < 1e-12
+
+
+def test_apply_qt_against_dorgqr_and_norm():
+    """Qᵀ·B from the reflectors: its top k rows are Q1ᵀ·B with Q1 from
+    LAPACK dorgqr, and Qᵀ is orthogonal (norm preserved column by column)."""
+    A = O.gaussian(200, 48, 37)
+    B = O.gaussian(200, 3, 38)
+    YR = O.QR_2D(A, 16)
+    QtB = O.apply_QT(YR, B)
+    Q1 = O.explicit_q(YR)
+    assert np.abs(QtB[:48] - Q1.T @ B).max() < 1e-12
+    assert np.allclose(np.linalg.norm(QtB, axis=0), np.linalg.norm(B, axis=0), rtol=1e-13)
+
+
+def test_conditioned_matrix_has_the_stated_spectrum():
+    A = O.conditioned_matrix(300, 40, 1e6, 5)
+    s = np.linalg.svd(A, compute_uv=False)
+    assert abs(s[0] - 1.0) < 1e-12 and abs(s[-1] * 1e6 - 1.0) < 1e-6
