@@ -77,15 +77,29 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 // rounded to FP32 while they are staged in LDS and multiplied on FP32 MFMA
 // (v_mfma_f32_16x16x4f32) with FP32 accumulation; the result is added to
 // the FP64 output in FP64.
+typedef short s4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short to_bf16_bits(double x) {
+  return __builtin_bit_cast(unsigned short, (__bf16)(float)x);
+}
+
+// CQR_PREC_BF16: operands rounded to BF16 (k-contiguous LDS rows so that a
+// lane's four consecutive k values are one 8-byte read), multiplied on
+// v_mfma_f32_16x16x16bf16_1k with FP32 accumulation.
 template <bool TA, bool TB, int ROLE, int PREC = CQR_PREC_F64>
 __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
-  constexpr int BM = 128, BN = 128, BK = 16, PAD = 4;
+  constexpr int BM = 128, BN = 128, BK = 16, PAD = 4, KP = BK + 4;
   constexpr int LDS_A = BM + PAD, LDS_B = BN + PAD;
-  constexpr bool LP = PREC != CQR_PREC_F64;
+  constexpr bool LP = PREC != CQR_PREC_F64, BF = PREC == CQR_PREC_BF16;
   typedef typename std::conditional<LP, float, double>::type ST;
   typedef typename std::conditional<LP, f4v, d4>::type AT;
-  __shared__ ST As[2][BK][LDS_A];
-  __shared__ ST Bs[2][BK][LDS_B];
+  constexpr size_t SZ_STD = 2 * BK * (LDS_A + LDS_B) * sizeof(ST);
+  constexpr size_t SZ_BF = 2 * (BM + BN) * KP * sizeof(unsigned short);
+  __shared__ __align__(16) unsigned char smem[BF ? SZ_BF : SZ_STD];
+  auto As = reinterpret_cast<ST (*)[BK][LDS_A]>(smem);
+  auto Bs = reinterpret_cast<ST (*)[BK][LDS_B]>(smem + 2 * BK * LDS_A * sizeof(ST));
+  auto As16 = reinterpret_cast<unsigned short (*)[BM][KP]>(smem);
+  auto Bs16 = reinterpret_cast<unsigned short (*)[BN][KP]>(smem + 2 * BM * KP * sizeof(unsigned short));
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 64;
@@ -167,8 +181,13 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
   auto store = [&](int buf) {
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-      As[buf][ak + (TA ? 0 : 2 * r)][am + (TA ? 16 * r : 0)] = (ST)ra[r];
-      Bs[buf][bk + (TB ? 2 * r : 0)][bn + (TB ? 0 : 16 * r)] = (ST)rb[r];
+      if constexpr (BF) {
+        As16[buf][am + (TA ? 16 * r : 0)][ak + (TA ? 0 : 2 * r)] = to_bf16_bits(ra[r]);
+        Bs16[buf][bn + (TB ? 0 : 16 * r)][bk + (TB ? 2 * r : 0)] = to_bf16_bits(rb[r]);
+      } else {
+        As[buf][ak + (TA ? 0 : 2 * r)][am + (TA ? 16 * r : 0)] = (ST)ra[r];
+        Bs[buf][bk + (TB ? 2 * r : 0)][bn + (TB ? 0 : 16 * r)] = (ST)rb[r];
+      }
     }
   };
 
@@ -182,6 +201,19 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
   for (int64_t t = 0; t < ntiles; ++t) {
     const int buf = (int)(t & 1);
     if (t + 1 < ntiles) load(kbeg + (t + 1) * BK);
+    if constexpr (BF) {
+      // one K = 16 product per tile pair: lane l supplies k = 4(l>>4)..+3
+      s4v af[4], bf[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = *reinterpret_cast<const s4v*>(&As16[buf][wm0 + i * 16 + fc][4 * fr]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bf[j] = *reinterpret_cast<const s4v*>(&Bs16[buf][wn0 + j * 16 + fc][4 * fr]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf[j], af[i], acc[i][j], 0, 0, 0);
+    } else
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       ST af[4], bf[4];
@@ -260,6 +292,11 @@ static void launch_kind(cqr_ctx* c, int ta, int tb, dim3 grid, const GemmArgs& g
     // only the two trailing-update products run in reduced precision
     if (kind == PK_GEMM_YTA) launch_role<1, CQR_PREC_F32>(c->stream, ta, tb, grid, g);
     else launch_role<2, CQR_PREC_F32>(c->stream, ta, tb, grid, g);
+    return;
+  }
+  if (prec == CQR_PREC_BF16) {
+    if (kind == PK_GEMM_YTA) launch_role<1, CQR_PREC_BF16>(c->stream, ta, tb, grid, g);
+    else launch_role<2, CQR_PREC_BF16>(c->stream, ta, tb, grid, g);
     return;
   }
   if (kind == PK_GEMM_YTA) launch_role<1>(c->stream, ta, tb, grid, g);

@@ -115,7 +115,7 @@ int cqr_lstsq(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, co
     return set_err(c, CQR_ERR_INVALID, "lstsq: bad arguments (needs m >= n >= 1, nrhs >= 1)");
   if (nrhs > 16) return set_err(c, CQR_ERR_UNSUPPORTED, "lstsq: at most 16 right-hand sides");
   if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "lstsq: b > %d", CQR_MAX_B);
-  if (prec != CQR_PREC_F64 && prec != CQR_PREC_F32) return set_err(c, CQR_ERR_UNSUPPORTED, "lstsq: precision %d", prec);
+  if (prec != CQR_PREC_F64 && prec != CQR_PREC_F32 && prec != CQR_PREC_BF16) return set_err(c, CQR_ERR_UNSUPPORTED, "lstsq: precision %d", prec);
   CQR_TRY(finite_check(c, A, lda, m, n));
   const int64_t np = (n + b - 1) / b, nb = (n + 127) / 128, nr = nrhs;
   Qr2dWS o;

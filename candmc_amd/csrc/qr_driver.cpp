@@ -510,7 +510,7 @@ int cqr_geqrf2d(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_
   if (!A || m < 1 || k < 1 || b < 1 || lda < m || m < k)
     return set_err(c, CQR_ERR_INVALID, "geqrf2d: bad arguments (needs m >= k, lda >= m)");
   if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "geqrf2d: b > %d", CQR_MAX_B);
-  if (prec != CQR_PREC_F64 && prec != CQR_PREC_F32)
+  if (prec != CQR_PREC_F64 && prec != CQR_PREC_F32 && prec != CQR_PREC_BF16)
     return set_err(c, CQR_ERR_UNSUPPORTED, "geqrf2d: precision %d", prec);
   CQR_TRY(finite_check(c, A, lda, m, k));
   Qr2dWS o;

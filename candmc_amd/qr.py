@@ -251,7 +251,7 @@ def QR_2D_2D(A, lda_A, m, k, b, b_sub, pv=None, aggreg_Y=None, lda_aY=0):
 
 
 # --- least-squares path ------------------------------------------------------
-PREC_F64, PREC_F32 = 0, 1
+PREC_F64, PREC_F32, PREC_BF16 = 0, 1, 2
 
 
 def geqrf2d(A, lda, m, k, b, Tt=None, prec=PREC_F64):

@@ -34,7 +34,8 @@ enum cqr_status {
 /* Arithmetic of the trailing (compact-WY) update; panels are always FP64. */
 enum cqr_prec {
   CQR_PREC_F64 = 0,  /* FP64 MFMA                                          */
-  CQR_PREC_F32 = 1   /* FP32 MFMA, FP32 accumulation, FP64 matrix in HBM    */
+  CQR_PREC_F32 = 1,  /* FP32 MFMA, FP32 accumulation, FP64 matrix in HBM    */
+  CQR_PREC_BF16 = 2  /* BF16 MFMA, FP32 accumulation, FP64 matrix in HBM    */
 };
 
 /* Largest panel width the FP64 panel kernels accept. */
@@ -184,7 +185,7 @@ int cqr_unpack_lower(cqr_ctx* ctx, const double* P, double* T, int64_t b, int64_
 /* QR_2D of an m×k matrix (m ≥ k) that also returns the panel factors:
  * Tt (⌈k/b⌉ blocks of b·b doubles; block p holds Tᵀ of panel p with ld equal
  * to that panel's width), and runs the trailing update in `prec`
- * (CQR_PREC_F64 or CQR_PREC_F32; the panels are FP64 either way).        */
+ * (CQR_PREC_F64, CQR_PREC_F32 or CQR_PREC_BF16; panels are always FP64). */
 int cqr_geqrf2d(cqr_ctx* ctx, double* A, int64_t lda, int64_t m, int64_t k, int64_t b,
                 double* Tt, int prec);
 /* B (m×nrhs, ld ldb) := Qᵀ·B (trans = 1) or Q·B (trans = 0), with Q from

@@ -117,3 +117,19 @@ def test_lstsq_errors(Q):
     with pytest.raises(CQRError) as e:
         Q.lstsq(A, 100, 100, 10, B, 100, 17)
     assert e.value.status == 2
+
+
+def test_lstsq_bf16_update_refined_to_fp64(Q):
+    """BF16 trailing update (u = 2⁻⁸) converges under refinement when
+    κ·u < 1: κ = 1e2 reaches the FP64 solution, in more steps than FP32."""
+    m, n, nrhs = 4096, 256, 1
+    A = O.conditioned_matrix(m, n, 1e2, 31)
+    rng = np.random.default_rng(32)
+    B = np.asfortranarray(A @ rng.standard_normal((n, nrhs)) + 1e-6 * rng.standard_normal((m, nrhs)))
+    want = sl.lstsq(A, B)[0]
+    X, iters, upd = Q.lstsq(Q.to_gpu(A), m, m, n, Q.to_gpu(B), m, nrhs, b=128, prec=Q.PREC_BF16,
+                            max_iter=80, tol=1e-12)
+    assert rel(Q.from_gpu(X, n, nrhs), want) < 1e-10, (iters, upd)
+    _, it32, _ = Q.lstsq(Q.to_gpu(A), m, m, n, Q.to_gpu(B), m, nrhs, b=128, prec=Q.PREC_F32,
+                         max_iter=80, tol=1e-12)
+    assert it32 < iters < 80
