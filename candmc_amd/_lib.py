@@ -52,6 +52,16 @@ SIGNATURES = {
     "cqr_qr_2d_2d": (_I, [_P, _P, _I64, _I64, _I64, _I64, _I64, _P, _I64]),
     "cqr_tau_recon": (_I, [_P, _c.c_char, _I64, _I64, _I64, _P, _P]),
     "cqr_geqrf2d": (_I, [_P, _P, _I64, _I64, _I64, _I64, _P, _I]),
+    "cqr_batch_create": (_I, [_P, _c.POINTER(_P), _I64, _P, _P]),
+    "cqr_batch_destroy": (_I, [_P, _P]),
+    "cqr_batch_offsets": (_I, [_P, _P, _P, _c.POINTER(_I64), _c.POINTER(_I64)]),
+    "cqr_batch_groups": (_I, [_P]),
+    "cqr_batch_group_info": (_I, [_P, _I, _c.POINTER(_I64), _c.POINTER(_I64), _c.POINTER(_I64),
+                                  _c.POINTER(_I64), _c.POINTER(_I64)]),
+    "cqr_batch_set_leaf_rows": (_I, [_P, _P, _I, _I64]),
+    "cqr_batch_qr": (_I, [_P, _P, _P, _P, _I, _I]),
+    "cqr_batch_qr_loop": (_I, [_P, _P, _P, _P]),
+    "cqr_batch_tune": (_I, [_P, _P, _P, _I]),
     "cqr_apply_qt": (_I, [_P, _P, _I64, _P, _I64, _I64, _I64, _P, _I64, _I64, _I]),
     "cqr_trsm_r": (_I, [_P, _P, _I64, _I64, _P, _I64, _I64, _I]),
     "cqr_lstsq": (_I, [_P, _P, _I64, _I64, _I64, _P, _I64, _I64, _P, _I64, _I64, _I, _I, _D,

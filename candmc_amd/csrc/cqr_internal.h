@@ -111,7 +111,10 @@ int make_plan(int64_t m, int64_t b, int64_t nleaves_hint, int64_t leaf_rows,
 //   BM_LEAFROWS p + rowoff(z)     (leaf z's rows inside a column-major panel)
 //   BM_SHIFTED  p + (z+zoff)·zs   (the sender of tree pair z)
 // ---------------------------------------------------------------------------
-enum { BM_STRIDED = 0, BM_LEAFROWS = 1, BM_SHIFTED = 2 };
+//   BM_PROB_Y   p + bp[z].yoff + prow, ld bp[z].m   (problem z's Q1 block)
+//   BM_PROB_A   p + bp[z].aoff + prow, ld bp[z].m   (problem z's A block)
+//   BM_PROB_T   p + bp[z].toff                     (problem z's Tᵀ block)
+enum { BM_STRIDED = 0, BM_LEAFROWS = 1, BM_SHIFTED = 2, BM_PROB_Y = 3, BM_PROB_A = 4, BM_PROB_T = 5 };
 struct BMat {
   const double* p;
   int64_t ld;
@@ -131,6 +134,34 @@ inline BMat bmat(const double* p, int64_t ld, int64_t zs = 0, int mode = BM_STRI
   BMat b; b.p = p; b.ld = ld; b.zs = zs; b.mode = mode; return b;
 }
 
+// ---------------------------------------------------------------------------
+// Batched TSQR + Householder reconstruction: one row per problem of a group
+// that shares the panel width b.  All offsets index the group's packed
+// buffers: the problem's A block starts at Abase + aoff (column-major, ld
+// lda = m), its leaves are global leaves leaf0.., its tree nodes global
+// nodes node0.., its construct_Q1 items item0.., its Q1 block starts at
+// row offset yoff (doubles) of the Y scratch (ld m), its Tᵀ at toff.
+// Kernels find their problem by binary search on leaf0 / node0 / item0.
+// ---------------------------------------------------------------------------
+struct BatchProb {
+  int64_t aoff, m, yoff, toff;
+  int L, leaf0, node0, item0;
+};
+enum { BP_LEAF = 0, BP_NODE = 1, BP_ITEM = 2 };
+__host__ __device__ inline int bp_first(const BatchProb& P, int which) {
+  return which == BP_LEAF ? P.leaf0 : which == BP_NODE ? P.node0 : P.item0;
+}
+// problem owning global index idx (bp_first non-decreasing in p)
+__device__ inline int bp_find(const BatchProb* bp, int nprob, int idx, int which) {
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (bp_first(bp[mid], which) <= idx) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 #define CQR_MAX_TREE_NODES 8192
 
 // Column-pipelined tree over all levels (ptree_kernel.hip): node outputs R
@@ -142,6 +173,20 @@ int launch_ptree(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, dou
 // (q1_kernel.hip): C is scratch of L×b×b; Q receives the m×b Q1 (ld ldq).
 int launch_q1(cqr_ctx* c, const double* nodeV2, const double* nodeT, const double* leafT, const double* V,
               int64_t ldv, int64_t m, double* C, double* Q, int64_t ldq, int L, int nnodes, int b);
+
+// Batched launchers (one group of problems sharing b)
+int leaf_qr_batched(cqr_ctx* c, double* Abase, int64_t total_m, int L, int64_t rows, int b, double* R, double* T,
+                    double* tau, const BatchProb* bp, int nprob);
+int launch_ptree_batched(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
+                         uint64_t* cnt, int total_nodes, int b, const BatchProb* bp, int nprob);
+int launch_q1_batched(cqr_ctx* c, const double* nodeV2, const double* nodeT, const double* leafT, const double* Vbase,
+                      double* C, double* Y, uint64_t* flags, int total_items, int b, const BatchProb* bp, int nprob);
+int q1_items_per_problem(int L, int b);
+int hr_core_batched(cqr_ctx* c, const BatchProb* bp, int nprob, const double* Y, const double* leafR,
+                    const double* nodeR, double* Abase, double* W, double* Uinv, double* Winv, double* LinvT, int b);
+int gemm_f64_batched_prob(cqr_ctx* c, int ta, int tb, int m, int n, int k, double alpha, const BMat& A, int smA,
+                          const BMat& B, int smB, double beta, const BMat& C, int smC, int nprob, const BatchProb* bp,
+                          int64_t prow, int m_prob, int max_m);
 
 // Kernel launchers (tsqr_kernels.hip)
 int leaf_qr(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int b, double* R, double* T,
@@ -214,6 +259,9 @@ int launch_copy2d(cqr_ctx* c, const double* src, int64_t lds, double* dst, int64
                   int64_t n);
 
 // Drivers (qr_driver.cpp)
+size_t panel_ws_bytes(int64_t m, int64_t b, int64_t nleaves, int64_t leaf_rows);
+int hh_recon_panel(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, double* W, double* T,
+                   double* invT_T, double* Tt, int64_t nleaves, char* base, int64_t* signs_out);
 #define CQR_CHECK_CTX(c)                     \
   do {                                       \
     if (!(c)) return CQR_ERR_INVALID;        \

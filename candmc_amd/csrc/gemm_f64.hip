@@ -39,6 +39,10 @@ struct GemmArgs {
   int pL, zoff;
   int m_leaf;                  // m = rows of leaf z
   int smC;                     // SM_UPPER: store zeros below the diagonal
+  // batched by problem (BM_PROB_*): per-z addresses and sizes from a table
+  const BatchProb* bp;
+  int64_t prow;                // row shift inside the problem's block
+  int m_prob;                  // m = rows of problem z − prow
 };
 
 __device__ __forceinline__ int64_t leaf_row0(int64_t z, int64_t m, int64_t L) { return (z * m) / L; }
@@ -46,7 +50,14 @@ __device__ __forceinline__ int64_t leaf_row0(int64_t z, int64_t m, int64_t L) { 
 __device__ __forceinline__ const double* zaddr(const double* p, int mode, int64_t zs, int z, const GemmArgs& g) {
   if (mode == BM_LEAFROWS) return p + leaf_row0(z, g.pm, g.pL);
   if (mode == BM_SHIFTED) return p + (int64_t)(z + g.zoff) * zs;
+  if (mode == BM_PROB_Y) return p + g.bp[z].yoff + g.prow;
+  if (mode == BM_PROB_A) return p + g.bp[z].aoff + g.prow;
+  if (mode == BM_PROB_T) return p + g.bp[z].toff;
   return p + (int64_t)z * zs;
+}
+// leading dimension of a per-problem operand (its block is m×·, ld m)
+__device__ __forceinline__ int64_t zld(int64_t ld, int mode, int z, const GemmArgs& g) {
+  return (mode == BM_PROB_Y || mode == BM_PROB_A) ? g.bp[z].m : ld;
 }
 
 __device__ __forceinline__ double gmask(double v, int64_t r, int64_t c, int sm) {
@@ -108,12 +119,17 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
   const double* __restrict__ B = g.B;
   double* C = g.C;
   int64_t Mrows = g.m, kbeg, kend;
+  int64_t lda = g.lda, ldb = g.ldb, ldc = g.ldc;
   if (g.batched) {
     const int z = blockIdx.z;
     A = zaddr(g.A, g.a_mode, g.a_zs, z, g);
     B = zaddr(g.B, g.b_mode, g.b_zs, z, g);
     C = const_cast<double*>(zaddr(g.C, g.c_mode, g.c_zs, z, g));
+    lda = zld(lda, g.a_mode, z, g);
+    ldb = zld(ldb, g.b_mode, z, g);
+    ldc = zld(ldc, g.c_mode, z, g);
     if (g.m_leaf) Mrows = leaf_row0(z + 1, g.pm, g.pL) - leaf_row0(z, g.pm, g.pL);
+    if (g.m_prob) Mrows = g.bp[z].m - g.prow;
     kbeg = 0;
     kend = g.k;
     if (m0 >= Mrows) return;   // whole block out of this batch entry's rows
@@ -130,12 +146,12 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
   const int am = TA ? (tid >> 4) : (tid & 127);
   const int bk = TB ? (tid >> 7) : (tid & 15);
   const int bn = TB ? (tid & 127) : (tid >> 4);
-  const int64_t a_rstep = TA ? 16 * g.lda : 2 * g.lda;   // between the 8 elements
-  const int64_t b_rstep = TB ? 2 * g.ldb : 16 * g.ldb;
-  const double* pa = TA ? A + (m0 + am) * g.lda + ak : A + (m0 + am) + (int64_t)ak * g.lda;
-  const double* pb = TB ? B + (n0 + bn) + (int64_t)bk * g.ldb : B + bk + (n0 + bn) * g.ldb;
-  const int64_t a_kstep = TA ? 1 : g.lda;                 // per unit of k
-  const int64_t b_kstep = TB ? g.ldb : 1;
+  const int64_t a_rstep = TA ? 16 * lda : 2 * lda;   // between the 8 elements
+  const int64_t b_rstep = TB ? 2 * ldb : 16 * ldb;
+  const double* pa = TA ? A + (m0 + am) * lda + ak : A + (m0 + am) + (int64_t)ak * lda;
+  const double* pb = TB ? B + (n0 + bn) + (int64_t)bk * ldb : B + bk + (n0 + bn) * ldb;
+  const int64_t a_kstep = TA ? 1 : lda;                 // per unit of k
+  const int64_t b_kstep = TB ? ldb : 1;
 
   AT acc[4][4];
 #pragma unroll
@@ -252,7 +268,7 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
       for (int v = 0; v < 4; ++v) {
         const int64_t gn = n0 + wn0 + 16 * j + (LP ? 4 * fr + v : fr + 4 * v);
         if (!full_mn && gn >= g.n) continue;
-        double* cp = C + gm + gn * g.ldc;
+        double* cp = C + gm + gn * ldc;
         double val = alpha * (double)acc[i][j][v];
         if (has_beta) val += g.beta * *cp;
         if (g.smC == SM_UPPER && gm > gn) val = 0.0;
@@ -378,6 +394,27 @@ int gemm_f64_batched(cqr_ctx* c, int ta, int tb, int m, int n, int k, double alp
   prof_end(c, ev, kind, 2.0 * batch * (double)max_m * n * k,
            8.0 * batch * ((double)max_m * k + (double)k * n + 2.0 * max_m * n));
   return check_launch(c, "mfma_gemm_f64_batched");
+}
+
+// Batched over the problems of a group: operand modes BM_PROB_* and
+// BM_STRIDED; m_prob selects per-problem rows (m of problem z − prow).
+int gemm_f64_batched_prob(cqr_ctx* c, int ta, int tb, int m, int n, int k, double alpha, const BMat& A, int smA,
+                          const BMat& B, int smB, double beta, const BMat& C, int smC, int nprob, const BatchProb* bp,
+                          int64_t prow, int m_prob, int max_m) {
+  if (nprob <= 0 || n <= 0 || max_m <= 0) return CQR_OK;
+  GemmArgs g{};
+  g.m = m_prob ? max_m : m; g.n = n; g.k = k; g.alpha = alpha; g.beta = beta;
+  g.A = A.p; g.lda = A.ld; g.smA = smA; g.B = B.p; g.ldb = B.ld; g.smB = smB; g.C = const_cast<double*>(C.p);
+  g.ldc = C.ld; g.kchunk = k; g.split_stride = 0;
+  g.batched = 1;
+  g.a_mode = A.mode; g.b_mode = B.mode; g.c_mode = C.mode;
+  g.a_zs = A.zs; g.b_zs = B.zs; g.c_zs = C.zs;
+  g.smC = smC; g.bp = bp; g.prow = prow; g.m_prob = m_prob;
+  dim3 grid((unsigned)((max_m + 127) / 128), (unsigned)((n + 127) / 128), (unsigned)nprob);
+  hipEvent_t ev = prof_begin(c);
+  launch_kind(c, ta, tb, grid, g, PK_GEMM_SMALL);
+  prof_end(c, ev, PK_GEMM_SMALL, 0.0, 0.0);
+  return check_launch(c, "mfma_gemm_f64_batched_prob");
 }
 
 size_t gemm_f64_scratch_doubles(int64_t m, int64_t n, int64_t k) {

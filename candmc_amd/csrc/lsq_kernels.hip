@@ -202,4 +202,32 @@ int launch_colnorm(cqr_ctx* c, const double* X, int64_t ldx, int64_t m, int n, d
   return check_launch(c, "colnorm");
 }
 
+// Element-wise helpers of the batched path (kernels, so that they can live
+// inside a captured graph).
+__global__ void fill_u64_kernel(uint64_t* __restrict__ p, int64_t n) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    p[e] = 0;
+}
+
+__global__ void convert_kernel(const void* __restrict__ src, void* __restrict__ dst, int64_t n, int to_f64) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    if (to_f64) static_cast<double*>(dst)[e] = (double)static_cast<const float*>(src)[e];
+    else static_cast<float*>(dst)[e] = (float)static_cast<const double*>(src)[e];
+  }
+}
+
+int fill_u64(cqr_ctx* c, uint64_t* p, int64_t n) {
+  if (n <= 0) return CQR_OK;
+  const int gx = (int)std::min<int64_t>(1024, (n + 255) / 256);
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(gx), dim3(256), 0, c->stream, p, n);
+  return check_launch(c, "fill_u64");
+}
+
+int convert_f32_f64(cqr_ctx* c, const void* src, void* dst, int64_t n, int to_f64) {
+  if (n <= 0) return CQR_OK;
+  const int gx = (int)std::min<int64_t>(4096, (n + 255) / 256);
+  hipLaunchKernelGGL(convert_kernel, dim3(gx), dim3(256), 0, c->stream, src, dst, n, to_f64);
+  return check_launch(c, "convert");
+}
+
 }  // namespace cqr

@@ -44,11 +44,22 @@ template <int CPW, int RPL>
 __global__ void __launch_bounds__(512)
 ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, double* __restrict__ V2out,
              double* __restrict__ Tout, double* __restrict__ tauout, uint64_t* __restrict__ cnt,
-             uint64_t epoch, int L, int b, int* __restrict__ err) {
+             uint64_t epoch, int L, int b, int* __restrict__ err, const BatchProb* __restrict__ bp, int nprob) {
   constexpr int WAVES = 8;
-  const int nid = blockIdx.x;
+  int nid = blockIdx.x;
   const int lane = threadIdx.x & 63, w = wave_id();
   const int64_t bb = (int64_t)b * b;
+  if (bp) {  // batched: this problem's leaves and nodes are contiguous slices
+    const BatchProb P = bp[bp_find(bp, nprob, nid, BP_NODE)];
+    nid -= P.node0;
+    L = P.L;
+    leafR += (int64_t)P.leaf0 * bb;
+    nodeR += (int64_t)P.node0 * bb;
+    V2out += (int64_t)P.node0 * bb;
+    Tout += (int64_t)P.node0 * bb;
+    tauout += (int64_t)P.node0 * b;
+    cnt += P.node0;
+  }
 
   // locate this node: level lv, pair z, live ranks n
   int lv = 0, n = L, node0 = 0;
@@ -309,24 +320,36 @@ ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, doubl
 
 template <int CPW, int RPL>
 static int ptree_launch(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
-                        uint64_t* cnt, int L, int nnodes, int b) {
-  const uint64_t epoch = ++c->tree_epoch;
+                        uint64_t* cnt, int L, int nnodes, int b, uint64_t epoch, const BatchProb* bp, int nprob) {
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL((ptree_kernel<CPW, RPL>), dim3(nnodes), dim3(512), 0, c->stream, leafR, nodeR, V2, T, tau,
-                     cnt, epoch, L, b, c->d_terr);
+                     cnt, epoch, L, b, c->d_terr, bp, nprob);
   prof_end(c, ev, PK_TREE, nnodes * (2.0 / 3.0) * b * (double)b * b, nnodes * 40.0 * b * b);
   return check_launch(c, "ptree");
+}
+
+static int ptree_dispatch(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
+                          uint64_t* cnt, int L, int nnodes, int b, uint64_t epoch, const BatchProb* bp, int nprob) {
+  if (b <= 16) return ptree_launch<2, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
+  if (b <= 32) return ptree_launch<4, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
+  if (b <= 64) return ptree_launch<8, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
+  return ptree_launch<16, 2>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
 }
 
 int launch_ptree(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
                  int L, int nnodes, int b) {
   if (nnodes <= 0) return CQR_OK;
   if (nnodes > CQR_MAX_TREE_NODES) return set_err(c, CQR_ERR_UNSUPPORTED, "tree of %d nodes", nnodes);
-  uint64_t* cnt = c->d_cnt;
-  if (b <= 16) return ptree_launch<2, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b);
-  if (b <= 32) return ptree_launch<4, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b);
-  if (b <= 64) return ptree_launch<8, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b);
-  return ptree_launch<16, 2>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b);
+  return ptree_dispatch(c, leafR, nodeR, V2, T, tau, c->d_cnt, L, nnodes, b, ++c->tree_epoch, nullptr, 0);
+}
+
+// Batched: all trees of a group in one launch; cnt (one word per node) must
+// hold values below 256 (the batch driver zeroes it with a kernel before
+// every run, so the launch is graph-replayable with epoch 1).
+int launch_ptree_batched(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
+                         uint64_t* cnt, int total_nodes, int b, const BatchProb* bp, int nprob) {
+  if (total_nodes <= 0) return CQR_OK;
+  return ptree_dispatch(c, leafR, nodeR, V2, T, tau, cnt, 0, total_nodes, b, 1, bp, nprob);
 }
 
 }  // namespace cqr

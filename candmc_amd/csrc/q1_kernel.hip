@@ -82,14 +82,31 @@ __global__ void __launch_bounds__(NT)
 q1_kernel(const double* __restrict__ nodeV2, const double* __restrict__ nodeT, const double* __restrict__ leafT,
           const double* __restrict__ V, int64_t ldv, int64_t m, double* __restrict__ C,
           double* __restrict__ Q, int64_t ldq, uint64_t* __restrict__ flags, uint64_t epoch, int L,
-          int nnodes, int b, int* __restrict__ err) {
+          int nnodes, int b, int* __restrict__ err, const BatchProb* __restrict__ bp, int nprob) {
   __shared__ double S0[128][LDS_LD];
   __shared__ double S1[128][LDS_LD];
   __shared__ double Vs[CW][128 + 1];
   __shared__ int s_ok;
 
   const int ncb = (b + CW - 1) / CW;
-  const int item = blockIdx.x;
+  int item = blockIdx.x;
+  if (bp) {  // batched: V is the group's base; the problem's slices of every buffer
+    const int64_t bb0 = (int64_t)b * b;
+    const BatchProb P = bp[bp_find(bp, nprob, item, BP_ITEM)];
+    item -= P.item0;
+    L = P.L;
+    nnodes = P.L - 1;
+    nodeV2 += (int64_t)P.node0 * bb0;
+    nodeT += (int64_t)P.node0 * bb0;
+    leafT += (int64_t)P.leaf0 * bb0;
+    V += P.aoff;
+    ldv = P.m;
+    m = P.m;
+    C += (int64_t)P.leaf0 * bb0;
+    Q += P.yoff;
+    ldq = P.m;
+    flags += (int64_t)P.node0 * ncb;
+  }
   const int cb = item % ncb;
   const int j0 = cb * CW, nj = min(CW, b - j0);
   const int R = min(b, j0 + CW);
@@ -260,10 +277,24 @@ int launch_q1(cqr_ctx* c, const double* nodeV2, const double* nodeT, const doubl
   const uint64_t epoch = ++c->q1_epoch;
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(q1_kernel, dim3((unsigned)((nnodes + L) * ncb)), dim3(NT), 0, c->stream, nodeV2, nodeT,
-                     leafT, V, ldv, m, C, Q, ldq, c->d_qflags, epoch, L, nnodes, b, c->d_terr);
+                     leafT, V, ldv, m, C, Q, ldq, c->d_qflags, epoch, L, nnodes, b, c->d_terr, nullptr, 0);
   prof_end(c, ev, PK_BGEMM, 4.0 * nnodes * (double)b * b * b + 4.0 * (double)m * b * b,
            8.0 * (3.0 * nnodes * b * b + 2.0 * m * b + 2.0 * L * b * b));
   return check_launch(c, "q1_kernel");
 }
+
+// Batched: every problem of a group (items numbered problem by problem);
+// flags must be zero before the launch (epoch 1, graph-replayable).
+int launch_q1_batched(cqr_ctx* c, const double* nodeV2, const double* nodeT, const double* leafT, const double* Vbase,
+                      double* C, double* Y, uint64_t* flags, int total_items, int b, const BatchProb* bp, int nprob) {
+  if (total_items <= 0) return CQR_OK;
+  hipEvent_t ev = prof_begin(c);
+  hipLaunchKernelGGL(q1_kernel, dim3((unsigned)total_items), dim3(NT), 0, c->stream, nodeV2, nodeT, leafT, Vbase,
+                     (int64_t)0, (int64_t)0, C, Y, (int64_t)0, flags, (uint64_t)1, 0, 0, b, c->d_terr, bp, nprob);
+  prof_end(c, ev, PK_BGEMM, 0.0, 0.0);
+  return check_launch(c, "q1_kernel_batched");
+}
+
+int q1_items_per_problem(int L, int b) { return L * ((b + CW - 1) / CW) + (L - 1) * ((b + CW - 1) / CW); }
 
 }  // namespace cqr

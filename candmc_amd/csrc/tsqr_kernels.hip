@@ -33,15 +33,25 @@ __device__ __forceinline__ int64_t leaf_off(int64_t l, int64_t m, int64_t L) { r
 template <int WAVES, int CPW, int RPL>
 __global__ void __launch_bounds__(64 * WAVES)
 leaf_qr_kernel(double* __restrict__ A, int64_t lda, int64_t m, int L, int b,
-               double* __restrict__ Rout, double* __restrict__ Tout, double* __restrict__ tauout) {
+               double* __restrict__ Rout, double* __restrict__ Tout, double* __restrict__ tauout,
+               const BatchProb* __restrict__ bp, int nprob) {
   constexpr int NT = 64 * WAVES;
-  const int leaf = blockIdx.x;
+  const int gleaf = blockIdx.x;  // index of the outputs
+  int leaf = gleaf;
+  if (bp) {                      // batched: A is the group's base, the problem's block at aoff
+    const BatchProb P = bp[bp_find(bp, nprob, gleaf, BP_LEAF)];
+    A += P.aoff;
+    lda = P.m;
+    m = P.m;
+    L = P.L;
+    leaf = gleaf - P.leaf0;
+  }
   const int64_t r0 = leaf_off(leaf, m, L);
   const int r = (int)(leaf_off(leaf + 1, m, L) - r0);
   double* Al = A + r0;
-  double* Rl = Rout + (int64_t)leaf * b * b;
-  double* Tl = Tout + (int64_t)leaf * b * b;
-  double* taul = tauout + (int64_t)leaf * b;
+  double* Rl = Rout + (int64_t)gleaf * b * b;
+  double* Tl = Tout + (int64_t)gleaf * b * b;
+  double* taul = tauout + (int64_t)gleaf * b;
 
   const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
   extern __shared__ double smem[];
@@ -523,11 +533,10 @@ __device__ void signed_lu_regs(const double* __restrict__ src, int64_t ld, doubl
 }
 
 // K4: Householder reconstruction core (one workgroup).
-__global__ void __launch_bounds__(512)
-hr_core_kernel(const double* __restrict__ Q1top, int64_t ldq, const double* __restrict__ Rroot,
-               double* __restrict__ Atop, int64_t lda, double* __restrict__ W,
-               double* __restrict__ Uinv, double* __restrict__ Winv, double* __restrict__ LinvT,
-               int64_t* __restrict__ signs, int b) {
+__device__ void hr_core_body(const double* __restrict__ Q1top, int64_t ldq, const double* __restrict__ Rroot,
+                             double* __restrict__ Atop, int64_t lda, double* __restrict__ W,
+                             double* __restrict__ Uinv, double* __restrict__ Winv, double* __restrict__ LinvT,
+                             int64_t* __restrict__ signs, int b) {
   constexpr int NT = 512;
   extern __shared__ double smem[];
   double* M = smem;            // b×b
@@ -561,6 +570,27 @@ hr_core_kernel(const double* __restrict__ Q1top, int64_t ldq, const double* __re
   __syncthreads();
   tri_inv_upper_lds<NT>(M, b, d, b);
   FOR_RC(rr, cc, b, NT) LinvT[rr + cc * b] = rr <= cc ? M[rr + cc * b] : 0.0;
+}
+
+__global__ void __launch_bounds__(512)
+hr_core_kernel(const double* __restrict__ Q1top, int64_t ldq, const double* __restrict__ Rroot,
+               double* __restrict__ Atop, int64_t lda, double* __restrict__ W,
+               double* __restrict__ Uinv, double* __restrict__ Winv, double* __restrict__ LinvT,
+               int64_t* __restrict__ signs, int b) {
+  hr_core_body(Q1top, ldq, Rroot, Atop, lda, W, Uinv, Winv, LinvT, signs, b);
+}
+
+// Batched: workgroup p handles problem p (its Q1 top in Y, its root R).
+__global__ void __launch_bounds__(512)
+hr_core_batched_kernel(const BatchProb* __restrict__ bp, const double* __restrict__ Y,
+                       const double* __restrict__ leafR, const double* __restrict__ nodeR,
+                       double* __restrict__ Abase, double* __restrict__ W, double* __restrict__ Uinv,
+                       double* __restrict__ Winv, double* __restrict__ LinvT, int b) {
+  const BatchProb P = bp[blockIdx.x];
+  const int64_t bb = (int64_t)b * b, p = blockIdx.x;
+  const double* Rroot = P.L > 1 ? nodeR + (int64_t)(P.node0 + P.L - 2) * bb : leafR + (int64_t)P.leaf0 * bb;
+  hr_core_body(Y + P.yoff, P.m, Rroot, Abase + P.aoff, P.m, W + p * bb, Uinv + p * bb, Winv + p * bb,
+               LinvT + p * bb, nullptr, b);
 }
 
 // a5 standalone: signed LU of a b×b block with R scaling.
@@ -736,7 +766,7 @@ __global__ void copy2d_kernel(const double* __restrict__ S, int64_t lds, double*
 // ===========================================================================
 template <int WAVES, int CPW, int RPL>
 static int leaf_launch(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int b,
-                       double* R, double* T, double* tau) {
+                       double* R, double* T, double* tau, const BatchProb* bp = nullptr, int nprob = 0) {
   const size_t lds = sizeof(double) * ((size_t)b * b + 2 * 64 * RPL + 128);
   auto k = leaf_qr_kernel<WAVES, CPW, RPL>;
   static bool attr = false;
@@ -745,9 +775,22 @@ static int leaf_launch(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int
     attr = true;
   }
   hipEvent_t ev = prof_begin(c);
-  hipLaunchKernelGGL(k, dim3(L), dim3(64 * WAVES), lds, c->stream, A, lda, m, L, b, R, T, tau);
+  hipLaunchKernelGGL(k, dim3(L), dim3(64 * WAVES), lds, c->stream, A, lda, m, L, b, R, T, tau, bp, nprob);
   prof_end(c, ev, PK_LEAF, 2.0 * m * b * b - (2.0 / 3.0) * L * (double)b * b * b, 16.0 * m * b);
   return check_launch(c, "leaf_qr");
+}
+
+// Batched: L = total leaves of the group, m = its total rows, rows = the
+// largest leaf of any problem (selects the register tile).
+int leaf_qr_batched(cqr_ctx* c, double* Abase, int64_t total_m, int L, int64_t rows, int b, double* R, double* T,
+                    double* tau, const BatchProb* bp, int nprob) {
+  if (b <= 16) return leaf_launch<4, 4, 16>(c, Abase, 0, total_m, L, b, R, T, tau, bp, nprob);
+  if (b <= 32) return rows <= 512 ? leaf_launch<8, 4, 8>(c, Abase, 0, total_m, L, b, R, T, tau, bp, nprob)
+                                  : leaf_launch<8, 4, 16>(c, Abase, 0, total_m, L, b, R, T, tau, bp, nprob);
+  if (b <= 64) return rows <= 256 ? leaf_launch<8, 8, 4>(c, Abase, 0, total_m, L, b, R, T, tau, bp, nprob)
+                                  : leaf_launch<8, 8, 8>(c, Abase, 0, total_m, L, b, R, T, tau, bp, nprob);
+  return rows <= 128 ? leaf_launch<8, 16, 2>(c, Abase, 0, total_m, L, b, R, T, tau, bp, nprob)
+                     : leaf_launch<8, 16, 4>(c, Abase, 0, total_m, L, b, R, T, tau, bp, nprob);
 }
 
 int leaf_qr(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int b, double* R, double* T,
@@ -819,6 +862,18 @@ int hr_core(cqr_ctx* c, const double* Q1top, int64_t ldq, const double* Rroot, d
                      lda, W, Uinv, Winv, LinvT, signs, b);
   prof_end(c, ev, PK_HR, (2.0 / 3.0) * b * (double)b * b, 64.0 * b * b);
   return check_launch(c, "hr_core");
+}
+
+int hr_core_batched(cqr_ctx* c, const BatchProb* bp, int nprob, const double* Y, const double* leafR,
+                    const double* nodeR, double* Abase, double* W, double* Uinv, double* Winv, double* LinvT, int b) {
+  static bool attr = false;
+  if (!attr) { CQR_TRY(big_lds_attr(c, (const void*)hr_core_batched_kernel)); attr = true; }
+  const size_t lds = sizeof(double) * ((size_t)b * b + 2 * b + 256);
+  hipEvent_t ev = prof_begin(c);
+  hipLaunchKernelGGL(hr_core_batched_kernel, dim3(nprob), dim3(512), lds, c->stream, bp, Y, leafR, nodeR, Abase, W,
+                     Uinv, Winv, LinvT, b);
+  prof_end(c, ev, PK_HR, nprob * (2.0 / 3.0) * b * (double)b * b, nprob * 64.0 * b * b);
+  return check_launch(c, "hr_core_batched");
 }
 
 int launch_signed_nlu(cqr_ctx* c, double* A, int64_t lda, double* R, int64_t lda_R, int64_t* signs, int b) {

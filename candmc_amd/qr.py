@@ -289,6 +289,84 @@ def lstsq(A, lda, m, n, B, ldb, nrhs, X=None, ldx=None, b=128, prec=PREC_F64, ma
     return X, it.value, upd.value
 
 
+# --- batched tall-skinny QR ----------------------------------------------------
+class Batch:
+    """Many independent m_p×n_p problems (n_p ≤ 128), packed column-major one
+    after another; each is factored as one hh_recon_qr panel ([Y\R] in
+    place, Tᵀ in a second packed buffer)."""
+
+    def __init__(self, m, n):
+        self.ctx = context()
+        self.m = np.asarray(m, dtype=np.int64)
+        self.n = np.asarray(n, dtype=np.int64)
+        assert self.m.shape == self.n.shape and self.m.ndim == 1
+        self.np = len(self.m)
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx.lib.cqr_batch_create(self.ctx.h, ctypes.byref(h), self.np,
+                                                     self.m.ctypes.data, self.n.ctypes.data))
+        self.h = h
+        self.aoff = np.zeros(self.np, dtype=np.int64)
+        self.toff = np.zeros(self.np, dtype=np.int64)
+        ta, tt = ctypes.c_int64(), ctypes.c_int64()
+        self.ctx.check(self.ctx.lib.cqr_batch_offsets(self.h, self.aoff.ctypes.data, self.toff.ctypes.data,
+                                                      ctypes.byref(ta), ctypes.byref(tt)))
+        self.total_a, self.total_t = ta.value, tt.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.cqr_batch_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def groups(self):
+        out = []
+        for g in range(self.ctx.lib.cqr_batch_groups(self.h)):
+            v = [ctypes.c_int64() for _ in range(5)]
+            self.ctx.check(self.ctx.lib.cqr_batch_group_info(self.h, g, *[ctypes.byref(x) for x in v]))
+            out.append(dict(zip(("b", "nprob", "leaf_rows", "leaves", "nodes"), [x.value for x in v])))
+        return out
+
+    def set_leaf_rows(self, group, rows):
+        self.ctx.check(self.ctx.lib.cqr_batch_set_leaf_rows(self.ctx.h, self.h, group, rows))
+
+    def pack(self, mats, dtype=torch.float64, device=0):
+        buf = np.empty(self.total_a, dtype=np.float64)
+        for p, a in enumerate(mats):
+            a = np.asarray(a, dtype=np.float64)
+            buf[self.aoff[p]: self.aoff[p] + a.size] = a.reshape(-1, order="F")
+        return torch.from_numpy(buf).to(device=f"cuda:{device}", dtype=dtype)
+
+    def unpack(self, A):
+        h = A.detach().double().cpu().numpy()
+        return [np.asfortranarray(h[self.aoff[p]: self.aoff[p] + self.m[p] * self.n[p]]
+                                  .reshape(self.n[p], self.m[p]).T) for p in range(self.np)]
+
+    def unpack_t(self, Tt):
+        h = Tt.detach().double().cpu().numpy()
+        return [np.asfortranarray(h[self.toff[p]: self.toff[p] + self.n[p] ** 2]
+                                  .reshape(self.n[p], self.n[p]).T) for p in range(self.np)]
+
+    def new_t(self, A):
+        return torch.zeros(self.total_t, dtype=A.dtype, device=A.device)
+
+    def qr(self, A, Tt, use_graph=False):
+        dt = {torch.float64: 0, torch.float32: 1}[A.dtype]
+        assert Tt.dtype == A.dtype
+        self.ctx.check(self.ctx.lib.cqr_batch_qr(self.ctx.h, self.h, _p(A), _p(Tt), dt, 1 if use_graph else 0))
+
+    def qr_loop(self, A, Tt):
+        self.ctx.check(self.ctx.lib.cqr_batch_qr_loop(self.ctx.h, self.h, _pd(A), _pd(Tt)))
+
+    def tune(self, A, reps=2):
+        self.ctx.check(self.ctx.lib.cqr_batch_tune(self.ctx.h, self.h, _pd(A), reps))
+        return self.groups()
+
+
 # --- alg/shared/util ---------------------------------------------------------
 def tau_recon(structure, ncol, nrow, lda, Y, tau):
     ctx = context()

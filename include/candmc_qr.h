@@ -206,6 +206,34 @@ int cqr_lstsq(cqr_ctx* ctx, const double* A, int64_t lda, int64_t m, int64_t n, 
               int64_t ldb, int64_t nrhs, double* X, int64_t ldx, int64_t b, int prec, int max_iter,
               double tol, int* iters, double* last_update);
 
+/* ---- batched tall-skinny QR (no reference counterpart: a batch of
+ *      independent hh_recon_qr panels, alg/QR/hh_recon/hh_recon.cxx:129-321,
+ *      each problem being one panel of width n_p) ------------------------
+ * Problems p = 0..nprob−1 of m_p × n_p (m_p ≥ n_p, n_p ≤ 128) are packed
+ * one after another, column-major with ld = m_p, at the offsets
+ * cqr_batch_offsets returns; their Tᵀ blocks (n_p × n_p) likewise.      */
+typedef struct cqr_batch cqr_batch;
+int cqr_batch_create(cqr_ctx* ctx, cqr_batch** batch, int64_t nprob, const int64_t* m,
+                     const int64_t* n);
+int cqr_batch_destroy(cqr_ctx* ctx, cqr_batch* batch);
+int cqr_batch_offsets(const cqr_batch* batch, int64_t* aoff, int64_t* toff, int64_t* total_a,
+                      int64_t* total_t);
+int cqr_batch_groups(const cqr_batch* batch);   /* problems grouped by n   */
+int cqr_batch_group_info(const cqr_batch* batch, int group, int64_t* b, int64_t* nprob,
+                         int64_t* leaf_rows, int64_t* leaves, int64_t* nodes);
+int cqr_batch_set_leaf_rows(cqr_ctx* ctx, cqr_batch* batch, int group, int64_t leaf_rows);
+/* A := [Y\R] of every problem, Tt := their Tᵀ.  dtype 0: FP64 buffers;
+ * 1: FP32 buffers (computed in FP64 through a staging copy).  use_graph:
+ * the first call with given buffers runs eagerly and captures the launch
+ * sequence in a HIP graph; later calls with the same buffers replay it.  */
+int cqr_batch_qr(cqr_ctx* ctx, cqr_batch* batch, void* A, void* Tt, int dtype, int use_graph);
+/* The same results through one hh_recon_qr launch sequence per problem
+ * (the per-problem loop the batched form is measured against).           */
+int cqr_batch_qr_loop(cqr_ctx* ctx, cqr_batch* batch, double* A, double* Tt);
+/* Autotuner: per group, times every leaf height on a copy of A and keeps
+ * the fastest (reps timed runs after a warm-up).                         */
+int cqr_batch_tune(cqr_ctx* ctx, cqr_batch* batch, const double* A, int reps);
+
 #ifdef __cplusplus
 }
 #endif
