@@ -52,6 +52,13 @@ SIGNATURES = {
     "cqr_qr_2d_2d": (_I, [_P, _P, _I64, _I64, _I64, _I64, _I64, _P, _I64]),
     "cqr_tau_recon": (_I, [_P, _c.c_char, _I64, _I64, _I64, _P, _P]),
     "cqr_geqrf2d": (_I, [_P, _P, _I64, _I64, _I64, _I64, _P, _I]),
+    "cqr_lstsq_workspace_bytes": (_c.c_size_t, [_I64, _I64, _I64, _I64]),
+    "cqr_server_create": (_I, [_c.POINTER(_P), _I, _I, _c.c_size_t]),
+    "cqr_server_destroy": (_I, [_P]),
+    "cqr_server_submit": (_I64, [_P, _P]),
+    "cqr_server_wait": (_I, [_P, _I64, _P]),
+    "cqr_server_drain": (_I, [_P, _P, _I, _c.POINTER(_I)]),
+    "cqr_server_pending": (_I, [_P, _c.POINTER(_I64), _c.POINTER(_c.c_size_t)]),
     "cqr_batch_create": (_I, [_P, _c.POINTER(_P), _I64, _P, _P]),
     "cqr_batch_destroy": (_I, [_P, _P]),
     "cqr_batch_offsets": (_I, [_P, _P, _P, _c.POINTER(_I64), _c.POINTER(_I64)]),

@@ -276,6 +276,10 @@ struct Qr2dWS {
 size_t qr2d_ws(int64_t m, int64_t k, int64_t b, int64_t leaf_rows, Qr2dWS* o);
 int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b, double* aggreg_Y,
              int64_t lda_aY, char* base, const Qr2dWS& o, double* Tt_out, int prec);
+struct LsqWS {
+  size_t Af, Tt, Ri, R, F, G, H, Tmp, Q, N;
+};
+size_t lstsq_layout(int64_t m, int64_t n, int64_t nr, int64_t b, int64_t leaf_rows, Qr2dWS* o, LsqWS* w);
 int hh_recon_core(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, double* W,
                   double* T, double* invT_T, double* Tt, int64_t nleaves);
 

@@ -72,11 +72,37 @@ int trsm_r_ws(cqr_ctx* c, const double* YR, int64_t ldy, int64_t n, const double
   return CQR_OK;
 }
 
+// Workspace of the least-squares driver: the QR_2D arena followed by the
+// factor copy, the panel factors, R's block inverses and the vectors.
+size_t lstsq_layout(int64_t m, int64_t n, int64_t nr, int64_t b, int64_t leaf_rows, Qr2dWS* o, LsqWS* w) {
+  const int64_t np = (n + b - 1) / b, nb = (n + 127) / 128;
+  Sizer s;
+  s.bytes = qr2d_ws(m, n, b, leaf_rows, o);
+  w->Af = s.add(8 * (size_t)m * n);
+  w->Tt = s.add(8 * np * b * b);
+  w->Ri = s.add(8 * nb * 128 * 128);
+  w->R = s.add(8 * (size_t)m * nr);
+  w->F = s.add(8 * (size_t)m * nr);
+  w->G = s.add(8 * (size_t)n * nr);
+  w->H = s.add(8 * (size_t)n * nr);
+  w->Tmp = s.add(8 * 128 * nr);
+  w->Q = s.add(8 * std::max(apply_q_ws_doubles(m, b, nr), thin_gemm_scratch_doubles(m, n, (int)nr)));
+  w->N = s.add(8 * 2 * nr);
+  return s.bytes;
+}
+
 }  // namespace cqr
 
 using namespace cqr;
 
 extern "C" {
+
+size_t cqr_lstsq_workspace_bytes(int64_t m, int64_t n, int64_t nrhs, int64_t b) {
+  if (m < 1 || n < 1 || m < n || nrhs < 1 || b < 1 || b > CQR_MAX_B) return 0;
+  Qr2dWS o;
+  LsqWS w;
+  return lstsq_layout(m, n, nrhs, b, 0, &o, &w);
+}
 
 int cqr_apply_qt(cqr_ctx* c, const double* YR, int64_t ldy, const double* Tt, int64_t m, int64_t k, int64_t b,
                  double* B, int64_t ldb, int64_t nrhs, int trans) {
@@ -117,20 +143,14 @@ int cqr_lstsq(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, co
   if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "lstsq: b > %d", CQR_MAX_B);
   if (prec != CQR_PREC_F64 && prec != CQR_PREC_F32 && prec != CQR_PREC_BF16) return set_err(c, CQR_ERR_UNSUPPORTED, "lstsq: precision %d", prec);
   CQR_TRY(finite_check(c, A, lda, m, n));
-  const int64_t np = (n + b - 1) / b, nb = (n + 127) / 128, nr = nrhs;
+  const int64_t nr = nrhs;
   Qr2dWS o;
-  Sizer s;
-  s.bytes = qr2d_ws(m, n, b, c->leaf_rows, &o);
-  const size_t oAf = s.add(8 * (size_t)m * n), oTt = s.add(8 * np * b * b), oRi = s.add(8 * nb * 128 * 128),
-               oR = s.add(8 * (size_t)m * nr), oF = s.add(8 * (size_t)m * nr), oG = s.add(8 * (size_t)n * nr),
-               oH = s.add(8 * (size_t)n * nr), oTmp = s.add(8 * 128 * nr),
-               oQ = s.add(8 * std::max(apply_q_ws_doubles(m, b, nr), thin_gemm_scratch_doubles(m, n, (int)nr))),
-               oN = s.add(8 * 2 * nr);
-  CQR_TRY(ensure_ws(c, s.bytes));
+  LsqWS w;
+  CQR_TRY(ensure_ws(c, lstsq_layout(m, n, nr, b, c->leaf_rows, &o, &w)));
   char* base = static_cast<char*>(c->ws);
   auto at = [&](size_t off) { return reinterpret_cast<double*>(base + off); };
-  double *Af = at(oAf), *Tt = at(oTt), *Rinv = at(oRi), *r = at(oR), *f = at(oF), *g = at(oG), *h = at(oH),
-         *tmp = at(oTmp), *wsq = at(oQ), *nrm = at(oN);
+  double *Af = at(w.Af), *Tt = at(w.Tt), *Rinv = at(w.Ri), *r = at(w.R), *f = at(w.F), *g = at(w.G),
+         *h = at(w.H), *tmp = at(w.Tmp), *wsq = at(w.Q), *nrm = at(w.N);
   // factor a copy of A (the caller's A stays the FP64 reference operator)
   CQR_TRY(launch_copy2d(c, A, lda, Af, m, m, n));
   CQR_TRY(qr2d_run(c, Af, m, m, n, b, nullptr, 0, base, o, Tt, prec));

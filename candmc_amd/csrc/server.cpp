@@ -1,0 +1,218 @@
+// Least-squares request server with continuous batching on one GPU.
+//
+// Requests (A, B resident in device memory, X written back to device
+// memory) enter a FIFO queue.  A fixed set of slots serves them; each slot
+// owns a library context (its own HIP stream and workspace arena) and a host
+// thread that drives it.  A slot takes the oldest queued request as soon as
+// it is idle and the request's HBM footprint fits the remaining budget, so
+// requests that arrive while others run join immediately and their kernels
+// overlap on the device.  Every result depends only on its own request
+// (each slot factors into its own arena; all reductions have a fixed
+// order), so a request gives the same bits alone and in any mix.
+//
+// Two accuracy tiers: FP64 (trailing update in FP64, refined to 1e-12) and
+// mixed (FP32 trailing update, refined with FP64 residuals to 1e-10).
+// The serving loop has no counterpart in the reference.
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "cqr_internal.h"
+
+extern "C" {
+int cqr_lstsq(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, const double* B, int64_t ldb,
+              int64_t nrhs, double* X, int64_t ldx, int64_t b, int prec, int max_iter, double tol, int* iters,
+              double* last_update);
+size_t cqr_lstsq_workspace_bytes(int64_t m, int64_t n, int64_t nrhs, int64_t b);
+}
+
+namespace {
+
+int64_t now_ns() {
+  return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+struct Job {
+  int64_t id;
+  cqr_request req;
+  size_t bytes;
+  cqr_result res;
+};
+
+}  // namespace
+
+struct cqr_server {
+  int device = 0;
+  size_t budget = 0, in_use = 0;
+  int64_t next_id = 0;
+  bool stop = false;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Job> queue;
+  std::unordered_map<int64_t, cqr_result> done;   // finished, not yet collected
+  std::vector<int64_t> done_order;
+  std::vector<cqr_ctx*> ctxs;
+  std::vector<hipStream_t> streams;
+  std::vector<std::thread> workers;
+};
+
+namespace {
+
+void serve(cqr_server* s, int slot) {
+  cqr_ctx* c = s->ctxs[slot];
+  (void)hipSetDevice(s->device);
+  for (;;) {
+    Job job;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      // FIFO: the head waits until its footprint fits (no overtaking)
+      s->cv_work.wait(lk, [&] { return s->stop || (!s->queue.empty() && s->in_use + s->queue.front().bytes <= s->budget); });
+      if (s->stop) return;
+      job = s->queue.front();
+      s->queue.pop_front();
+      s->in_use += job.bytes;
+    }
+    const cqr_request& r = job.req;
+    job.res.t_start_ns = now_ns();
+    const bool mixed = r.tier == CQR_TIER_MIXED;
+    int iters = 0;
+    double upd = 0.0;
+    int st = cqr_lstsq(c, r.A, r.lda, r.m, r.n, r.B, r.ldb, r.nrhs, r.X, r.ldx, 128,
+                       mixed ? CQR_PREC_F32 : CQR_PREC_F64, mixed ? 30 : 4, mixed ? 1e-10 : 1e-12, &iters, &upd);
+    if (st == CQR_OK) st = cqr_sync(c);
+    job.res.t_end_ns = now_ns();
+    job.res.status = st;
+    job.res.iters = iters;
+    job.res.last_update = upd;
+    job.res.slot = slot;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->in_use -= job.bytes;
+      s->done[job.id] = job.res;
+      s->done_order.push_back(job.id);
+    }
+    s->cv_work.notify_all();
+    s->cv_done.notify_all();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cqr_server_create(cqr_server** out, int device, int nslots, size_t hbm_budget) {
+  if (!out || nslots < 1 || nslots > 64 || hbm_budget == 0) return CQR_ERR_INVALID;
+  *out = nullptr;
+  cqr_server* s = new cqr_server();
+  s->device = device;
+  s->budget = hbm_budget;
+  for (int i = 0; i < nslots; ++i) {
+    cqr_ctx* c = nullptr;
+    int st = cqr_create(&c, device);
+    hipStream_t stream = nullptr;
+    if (st == CQR_OK && hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) st = CQR_ERR_DEVICE;
+    if (st == CQR_OK) st = cqr_set_stream(c, stream);
+    if (st != CQR_OK) {
+      if (c) cqr_destroy(c);
+      if (stream) (void)hipStreamDestroy(stream);
+      for (size_t k = 0; k < s->ctxs.size(); ++k) {
+        cqr_destroy(s->ctxs[k]);
+        (void)hipStreamDestroy(s->streams[k]);
+      }
+      delete s;
+      return st;
+    }
+    s->ctxs.push_back(c);
+    s->streams.push_back(stream);
+  }
+  for (int i = 0; i < nslots; ++i) s->workers.emplace_back(serve, s, i);
+  *out = s;
+  return CQR_OK;
+}
+
+int cqr_server_destroy(cqr_server* s) {
+  if (!s) return CQR_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  for (std::thread& t : s->workers) t.join();
+  for (size_t k = 0; k < s->ctxs.size(); ++k) {
+    cqr_destroy(s->ctxs[k]);
+    (void)hipStreamDestroy(s->streams[k]);
+  }
+  delete s;
+  return CQR_OK;
+}
+
+int64_t cqr_server_submit(cqr_server* s, const cqr_request* r) {
+  if (!s || !r) return -CQR_ERR_INVALID;
+  if (!r->A || !r->B || !r->X || r->m < r->n || r->n < 1 || r->nrhs < 1 || r->nrhs > 16 || r->lda < r->m ||
+      r->ldb < r->m || r->ldx < r->n || (r->tier != CQR_TIER_F64 && r->tier != CQR_TIER_MIXED))
+    return -CQR_ERR_INVALID;
+  const size_t bytes = cqr_lstsq_workspace_bytes(r->m, r->n, r->nrhs, 128);
+  if (bytes == 0 || bytes > s->budget) return -CQR_ERR_NOMEM;  // can never be admitted
+  Job j;
+  j.req = *r;
+  j.bytes = bytes;
+  j.res = cqr_result{};
+  j.res.t_submit_ns = now_ns();
+  j.res.bytes = (int64_t)bytes;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    j.id = s->next_id++;
+    j.res.id = j.id;
+    s->queue.push_back(j);
+  }
+  s->cv_work.notify_all();
+  return j.id;
+}
+
+int cqr_server_wait(cqr_server* s, int64_t id, cqr_result* out) {
+  if (!s || id < 0) return CQR_ERR_INVALID;
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (id >= s->next_id) return CQR_ERR_INVALID;
+  s->cv_done.wait(lk, [&] { return s->done.count(id) > 0; });
+  if (out) *out = s->done[id];
+  s->done.erase(id);
+  for (size_t k = 0; k < s->done_order.size(); ++k)
+    if (s->done_order[k] == id) {
+      s->done_order.erase(s->done_order.begin() + (long)k);
+      break;
+    }
+  return CQR_OK;
+}
+
+int cqr_server_drain(cqr_server* s, cqr_result* out, int max, int* count) {
+  if (!s || (max > 0 && !out) || !count) return CQR_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
+  int k = 0;
+  std::vector<int64_t> keep;
+  for (int64_t id : s->done_order) {
+    if (k < max) {
+      out[k++] = s->done[id];
+      s->done.erase(id);
+    } else {
+      keep.push_back(id);
+    }
+  }
+  s->done_order.swap(keep);
+  *count = k;
+  return CQR_OK;
+}
+
+int cqr_server_pending(cqr_server* s, int64_t* queued, size_t* bytes_in_use) {
+  if (!s) return CQR_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (queued) *queued = (int64_t)s->queue.size();
+  if (bytes_in_use) *bytes_in_use = s->in_use;
+  return CQR_OK;
+}
+
+}  // extern "C"

@@ -367,6 +367,87 @@ class Batch:
         return self.groups()
 
 
+# --- least-squares request server ---------------------------------------------
+TIER_F64, TIER_MIXED = 0, 1
+
+
+class _Request(ctypes.Structure):
+    _fields_ = [("A", ctypes.c_void_p), ("lda", ctypes.c_int64), ("m", ctypes.c_int64), ("n", ctypes.c_int64),
+                ("B", ctypes.c_void_p), ("ldb", ctypes.c_int64), ("nrhs", ctypes.c_int64),
+                ("X", ctypes.c_void_p), ("ldx", ctypes.c_int64), ("tier", ctypes.c_int)]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int64), ("status", ctypes.c_int), ("iters", ctypes.c_int), ("slot", ctypes.c_int),
+                ("last_update", ctypes.c_double), ("t_submit_ns", ctypes.c_int64), ("t_start_ns", ctypes.c_int64),
+                ("t_end_ns", ctypes.c_int64), ("bytes", ctypes.c_int64)]
+
+
+def _res(r):
+    return {k: getattr(r, k) for k, _ in _Result._fields_}
+
+
+def lstsq_workspace_bytes(m, n, nrhs, b=128):
+    return int(load().cqr_lstsq_workspace_bytes(m, n, nrhs, b))
+
+
+class Server:
+    """Least-squares request server: `nslots` slots (own stream and arena
+    each) admit queued requests FIFO when their footprint fits the budget."""
+
+    def __init__(self, nslots=4, hbm_budget=32 << 30, device=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        st = self.lib.cqr_server_create(ctypes.byref(h), device, nslots, hbm_budget)
+        if st:
+            raise CQRError(st, "server_create")
+        self.h = h
+        self._keep = {}
+
+    def submit(self, A, lda, m, n, B, ldb, nrhs, X, ldx, tier=TIER_MIXED):
+        r = _Request(_pd(A), lda, m, n, _pd(B), ldb, nrhs, _pd(X), ldx, int(tier))
+        rid = int(self.lib.cqr_server_submit(self.h, ctypes.byref(r)))
+        if rid < 0:
+            raise CQRError(-rid, "server_submit")
+        self._keep[rid] = (A, B, X)      # tensors stay alive until completion
+        return rid
+
+    def wait(self, rid):
+        r = _Result()
+        st = self.lib.cqr_server_wait(self.h, rid, ctypes.byref(r))
+        if st:
+            raise CQRError(st, "server_wait")
+        self._keep.pop(rid, None)
+        return _res(r)
+
+    def drain(self, max_results=1024):
+        arr = (_Result * max_results)()
+        cnt = ctypes.c_int()
+        st = self.lib.cqr_server_drain(self.h, arr, max_results, ctypes.byref(cnt))
+        if st:
+            raise CQRError(st, "server_drain")
+        out = [_res(arr[i]) for i in range(cnt.value)]
+        for r in out:
+            self._keep.pop(r["id"], None)
+        return out
+
+    def pending(self):
+        q, b = ctypes.c_int64(), ctypes.c_size_t()
+        self.lib.cqr_server_pending(self.h, ctypes.byref(q), ctypes.byref(b))
+        return q.value, b.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cqr_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # --- alg/shared/util ---------------------------------------------------------
 def tau_recon(structure, ncol, nrow, lda, Y, tau):
     ctx = context()

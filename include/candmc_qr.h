@@ -234,6 +234,42 @@ int cqr_batch_qr_loop(cqr_ctx* ctx, cqr_batch* batch, double* A, double* Tt);
  * the fastest (reps timed runs after a warm-up).                         */
 int cqr_batch_tune(cqr_ctx* ctx, cqr_batch* batch, const double* A, int reps);
 
+/* ---- least-squares request server (no reference counterpart) ----------
+ * nslots host threads each drive their own context and stream; a request
+ * is admitted (FIFO) when a slot is idle and its workspace footprint fits
+ * the HBM budget.  A, B and X live in device memory and must stay valid
+ * until the request completes.                                           */
+enum cqr_tier { CQR_TIER_F64 = 0, CQR_TIER_MIXED = 1 };
+typedef struct cqr_request {
+  const double* A;
+  int64_t lda, m, n;
+  const double* B;
+  int64_t ldb, nrhs;
+  double* X;
+  int64_t ldx;
+  int tier;          /* CQR_TIER_F64: FP64 to 1e-12; CQR_TIER_MIXED: FP32
+                        trailing update, FP64 refinement to 1e-10          */
+} cqr_request;
+typedef struct cqr_result {
+  int64_t id;
+  int status, iters, slot;
+  double last_update;
+  int64_t t_submit_ns, t_start_ns, t_end_ns;   /* steady clock            */
+  int64_t bytes;                               /* admitted footprint      */
+} cqr_result;
+typedef struct cqr_server cqr_server;
+/* Workspace bytes one least-squares solve needs (its admission footprint). */
+size_t cqr_lstsq_workspace_bytes(int64_t m, int64_t n, int64_t nrhs, int64_t b);
+int cqr_server_create(cqr_server** server, int device, int nslots, size_t hbm_budget);
+int cqr_server_destroy(cqr_server* server);      /* waits for the slots     */
+/* Returns the request id (≥ 0) or −status (e.g. −CQR_ERR_NOMEM when the
+ * footprint exceeds the whole budget).                                   */
+int64_t cqr_server_submit(cqr_server* server, const cqr_request* request);
+int cqr_server_wait(cqr_server* server, int64_t id, cqr_result* result);   /* consumes */
+/* Moves up to max finished results (in completion order) into out.      */
+int cqr_server_drain(cqr_server* server, cqr_result* out, int max, int* count);
+int cqr_server_pending(cqr_server* server, int64_t* queued, size_t* bytes_in_use);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,96 @@
+"""GPU tests of the least-squares request server: each request gives the
+same bits alone and in a concurrent mix, both accuracy tiers meet their
+targets against scipy's FP64 lstsq, admission respects the HBM budget and
+oversized requests are refused."""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+import torch
+
+from oracle import candmc_oracle as O
+
+pytestmark = pytest.mark.gpu
+EPS = np.finfo(np.float64).eps
+
+
+@pytest.fixture(scope="module")
+def Q():
+    from candmc_amd import qr
+    assert torch.cuda.is_available(), "GPU tests need an MI355X"
+    return qr
+
+
+SIZES = [(3000, 200, 3, 1e3), (800, 8, 1, 1e2), (5000, 256, 16, 1e6), (1200, 130, 2, 1e4),
+         (4096, 64, 5, 1e5), (600, 40, 7, 10.0)]
+
+
+def make_requests(Q):
+    reqs = []
+    for i, (m, n, k, kappa) in enumerate(SIZES):
+        A = O.conditioned_matrix(m, n, kappa, 100 + i)
+        rng = np.random.default_rng(200 + i)
+        B = np.asfortranarray(A @ rng.standard_normal((n, k)) + 1e-6 * rng.standard_normal((m, k)))
+        for tier in (Q.TIER_F64, Q.TIER_MIXED):
+            reqs.append((A, B, m, n, k, kappa, tier))
+    return reqs
+
+
+def run(Q, srv, reqs, together):
+    xs, res = [], []
+    dev = []
+    for A, B, m, n, k, kappa, tier in reqs:
+        dev.append((Q.to_gpu(A), Q.to_gpu(B), torch.zeros(n * k, dtype=torch.float64, device="cuda")))
+    torch.cuda.synchronize()
+    ids = []
+    for (A, B, m, n, k, kappa, tier), (dA, dB, dX) in zip(reqs, dev):
+        rid = srv.submit(dA, m, m, n, dB, m, k, dX, n, tier)
+        if not together:
+            res.append(srv.wait(rid))
+        ids.append(rid)
+    if together:
+        res = [srv.wait(rid) for rid in ids]
+    for (A, B, m, n, k, kappa, tier), (dA, dB, dX) in zip(reqs, dev):
+        xs.append(Q.from_gpu(dX, n, k))
+    return xs, res
+
+
+def test_same_result_alone_and_in_mix(Q):
+    reqs = make_requests(Q)
+    alone = Q.Server(nslots=1)
+    xa, ra = run(Q, alone, reqs, together=False)
+    alone.close()
+    mix = Q.Server(nslots=4)
+    xm, rm = run(Q, mix, reqs, together=True)
+    mix.close()
+    assert all(r["status"] == 0 for r in ra + rm)
+    assert len({r["slot"] for r in rm}) > 1                  # the mix really ran concurrently
+    for (A, B, m, n, k, kappa, tier), a, b in zip(reqs, xa, xm):
+        assert np.array_equal(a, b)                         # bit-identical alone and in the mix
+        want = sl.lstsq(A, B)[0]
+        err = np.abs(a - want).max() / np.abs(want).max()
+        target = (1e-12 if tier == Q.TIER_F64 else 1e-10)
+        assert err < max(target, 50 * kappa * EPS), (m, n, k, kappa, tier, err)
+
+
+def test_admission_respects_budget(Q):
+    reqs = make_requests(Q)[:6]
+    foot = max(Q.lstsq_workspace_bytes(m, n, k) for _, _, m, n, k, _, _ in reqs)
+    budget = int(foot * 1.5)                                 # the largest fits, not two of them
+    srv = Q.Server(nslots=4, hbm_budget=budget)
+    _, res = run(Q, srv, reqs, together=True)
+    srv.close()
+    for r in res:                                           # footprints running at each start
+        live = sum(q["bytes"] for q in res if q["t_start_ns"] <= r["t_start_ns"] < q["t_end_ns"])
+        assert live <= budget
+
+
+def test_oversized_request_refused(Q):
+    from candmc_amd._lib import CQRError
+    srv = Q.Server(nslots=1, hbm_budget=1 << 20)
+    A = Q.to_gpu(O.gaussian(2000, 100, 1))
+    B = Q.to_gpu(O.gaussian(2000, 1, 2))
+    X = torch.zeros(100, dtype=torch.float64, device="cuda")
+    with pytest.raises(CQRError) as e:
+        srv.submit(A, 2000, 2000, 100, B, 2000, 1, X, 100)
+    assert e.value.status == 3
+    srv.close()
