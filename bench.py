@@ -102,6 +102,167 @@ def cpu_baseline(A, b, max_seconds):
             "kind": "port", "sample": sample, "seconds": round(dt, 3)}
 
 
+# --------------------------------------------------------------------------
+# Secondary legs: mixed-precision least squares, batched QR, serving
+# --------------------------------------------------------------------------
+def _timed(fn, torch):
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return out, (time.perf_counter() - t) * 1e3
+
+
+def _orthonormal(qr, torch, m, n, seed):
+    """Q (m×n) with orthonormal columns: our QR of a Gaussian, Q·[I; 0]."""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    G = torch.randn(n, m, generator=g, dtype=torch.float64, device="cuda").reshape(-1)
+    Tt = qr.geqrf2d(G, m, m, n, 128)
+    E = torch.zeros(n, m, dtype=torch.float64, device="cuda")
+    E[:, :n] = torch.eye(n, dtype=torch.float64, device="cuda")
+    E = E.reshape(-1)
+    qr.apply_qt(G, m, Tt, m, n, 128, E, m, n, trans=False)
+    return E.reshape(n, m).T            # column-major view as m×n
+
+
+def lsq_leg(qr, torch, m=65536, n=2048, kappas=(1e2, 1e6, 1e8), seed=11):
+    """Mixed-precision least squares: A = U·diag(s)·Vᵀ (s geometric 1..1/κ),
+    b = A·x_true + 1e-6·N(0,1); FP64 solve vs FP32-update and BF16-update
+    solves refined on FP64 residuals; error against the FP64 GPU solution."""
+    U = _orthonormal(qr, torch, m, n, seed)
+    V = _orthonormal(qr, torch, n, n, seed + 1)
+    g = torch.Generator(device="cuda").manual_seed(seed + 2)
+    xt = torch.randn(n, 1, generator=g, dtype=torch.float64, device="cuda")
+    noise = torch.randn(m, 1, generator=g, dtype=torch.float64, device="cuda")
+    out = {"m": m, "n": n, "nrhs": 1, "seed": seed, "tol_mixed": 1e-10, "cases": []}
+    flops = qr_flops(m, n)
+    for kappa in kappas:
+        s = torch.logspace(0, -np.log10(kappa), n, dtype=torch.float64, device="cuda")
+        A = ((U * s) @ V.T).T.contiguous().reshape(-1)        # column-major m×n
+        B = ((U * s) @ (V.T @ xt) + 1e-6 * noise).reshape(-1)
+        (X64, it64, _), t64 = _timed(lambda: qr.lstsq(A, m, m, n, B, m, 1, prec=qr.PREC_F64, max_iter=2,
+                                                      tol=1e-14), torch)
+        x64 = X64.double()
+        case = {"kappa": kappa, "fp64_ms": round(t64, 2), "fp64_iters": it64}
+        for name, prec in (("fp32", qr.PREC_F32), ("bf16", qr.PREC_BF16)):
+            (Xm, itm, upd), tm = _timed(lambda: qr.lstsq(A, m, m, n, B, m, 1, prec=prec, max_iter=40,
+                                                          tol=1e-11), torch)
+            err = float((Xm - x64).abs().max() / x64.abs().max())
+            case[name] = {"ms": round(tm, 2), "iters": itm, "rel_err_vs_fp64": float(f"{err:.3g}"),
+                          "met_1e-10": bool(err <= 1e-10), "speedup_vs_fp64": round(t64 / tm, 3),
+                          "gflops_factor_equiv": round(flops / (tm * 1e-3) / 1e9, 1)}
+        out["cases"].append(case)
+        del A, B
+    return out
+
+
+def batch_leg(qr, torch, nprob=2048, seed=13, loop=True):
+    """2048 tall-skinny QRs, m ~ U[256, 16384], n ∈ {16, 32, 64, 128}:
+    batched launches (autotuned leaf heights, graph-replayed) against the
+    per-problem launch loop, FP64 and FP32 buffers."""
+    rng = np.random.default_rng(seed)
+    m = rng.integers(256, 16385, nprob)
+    n = rng.choice([16, 32, 64, 128], nprob)
+    bt = qr.Batch(m, n)
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    A0 = torch.randn(bt.total_a, generator=g, dtype=torch.float64, device="cuda")
+    A = A0.clone()
+    Tt = bt.new_t(A)
+    F = float(np.sum(2.0 * m * n.astype(np.float64) ** 2 - (2.0 / 3.0) * n.astype(np.float64) ** 3))
+    byt = float(np.sum(2.0 * m * n * 8.0))
+    t_roof = float(np.sum(np.maximum(2.0 * m * n * 8.0 / (PEAK_HBM_GBS * 1e9),
+                                     (2.0 * m * n.astype(np.float64) ** 2) / (PEAK_FP64_TFLOPS * 1e12))))
+    tuned = bt.tune(A0, reps=1)
+    res = {"nprob": nprob, "seed": seed, "flops": F, "bytes": byt, "roofline_ms_fp64": round(t_roof * 1e3, 3),
+           "groups": tuned}
+
+    def timed_runs(fn, reps=3):
+        ts = []
+        for _ in range(reps):
+            A.copy_(A0)
+            _, t = _timed(fn, torch)
+            ts.append(t)
+        return min(ts)
+
+    bt.qr(A, Tt)                                   # warm
+    t_eager = timed_runs(lambda: bt.qr(A, Tt))
+    A.copy_(A0)
+    bt.qr(A, Tt, use_graph=True)                   # capture
+    t_graph = timed_runs(lambda: bt.qr(A, Tt, use_graph=True))
+    res["fp64"] = {"batched_ms": round(t_eager, 2), "graph_ms": round(t_graph, 2),
+                   "gflops": round(F / (t_graph * 1e-3) / 1e9, 1),
+                   "roofline_frac": round(t_roof * 1e3 / t_graph, 4)}
+    if loop:
+        t_loop = timed_runs(lambda: bt.qr_loop(A, Tt), reps=1)
+        res["fp64"]["per_problem_loop_ms"] = round(t_loop, 2)
+        res["fp64"]["speedup_vs_loop"] = round(t_loop / t_graph, 2)
+    A32 = A0.float()
+    T32 = bt.new_t(A32)
+    A32w = A32.clone()
+    bt.qr(A32w, T32)
+    ts = []
+    for _ in range(3):
+        A32w.copy_(A32)
+        _, t = _timed(lambda: bt.qr(A32w, T32), torch)
+        ts.append(t)
+    res["fp32_buffers"] = {"batched_ms": round(min(ts), 2), "note": "FP32 storage, computed in FP64"}
+    bt.close()
+    return res
+
+
+def serve_leg(qr, torch, nreq=160, rates=(100.0, 400.0, 1600.0), nslots=4, seed=17):
+    """Poisson stream of least-squares requests: m lognormal (median 4096,
+    σ = 1, capped at 65536), n ~ U[8, 256], 1-16 right-hand sides, column
+    scaling to κ ≤ 1e6; both tiers; p50/p99 latency and solves/s per rate."""
+    rng = np.random.default_rng(seed)
+    ms = np.minimum(65536, np.maximum(1, rng.lognormal(np.log(4096), 1.0, nreq))).astype(np.int64)
+    ns = rng.integers(8, 257, nreq)
+    ms = np.maximum(ms, ns)
+    ks = rng.integers(1, 17, nreq)
+    kap = 10.0 ** rng.uniform(0, 6, nreq)
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    reqs = []
+    for m, n, k, kp in zip(ms, ns, ks, kap):
+        scale = torch.logspace(0, -np.log10(kp), int(n), dtype=torch.float64, device="cuda")
+        A = (torch.randn(int(n), int(m), generator=g, dtype=torch.float64, device="cuda") * scale[:, None]).reshape(-1)
+        B = torch.randn(int(k) * int(m), generator=g, dtype=torch.float64, device="cuda")
+        X = torch.zeros(int(n) * int(k), dtype=torch.float64, device="cuda")
+        reqs.append((A, B, X, int(m), int(n), int(k)))
+    torch.cuda.synchronize()
+    out = {"nreq": nreq, "nslots": nslots, "seed": seed,
+           "m": "lognormal(median 4096, sigma 1) capped at 65536", "n": "U[8,256]", "nrhs": "U{1..16}",
+           "kappa": "column scaling 1..1e-6 (log-uniform)", "tiers": {}}
+    srv = qr.Server(nslots=nslots, hbm_budget=64 << 30)
+    for tier, tname in ((qr.TIER_MIXED, "mixed_1e-10"), (qr.TIER_F64, "fp64_1e-12")):
+        # warm every slot's arena with the largest request
+        big = int(np.argmax(ms * ns))
+        ids = [srv.submit(reqs[big][0], reqs[big][3], reqs[big][3], reqs[big][4], reqs[big][1], reqs[big][3],
+                          reqs[big][5], reqs[big][2], reqs[big][4], tier) for _ in range(nslots)]
+        for i in ids:
+            srv.wait(i)
+        rows = []
+        for rate in rates:
+            gaps = rng.exponential(1.0 / rate, nreq)
+            t0 = time.perf_counter()
+            due = t0 + np.cumsum(gaps)
+            ids = []
+            for (A, B, X, m, n, k), d in zip(reqs, due):
+                while time.perf_counter() < d:
+                    pass
+                ids.append(srv.submit(A, m, m, n, B, m, k, X, n, tier))
+            res = [srv.wait(i) for i in ids]
+            lat = np.array([(r["t_end_ns"] - r["t_submit_ns"]) * 1e-6 for r in res])
+            span = (max(r["t_end_ns"] for r in res) - min(r["t_submit_ns"] for r in res)) * 1e-9
+            rows.append({"offered_qps": rate, "solves_per_s": round(nreq / span, 1),
+                         "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                         "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                         "mean_iters": round(float(np.mean([r["iters"] for r in res])), 2),
+                         "failed": int(sum(r["status"] != 0 for r in res))})
+        out["tiers"][tname] = rows
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +275,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-lsq", action="store_true", help="skip the mixed-precision least-squares leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched tall-skinny QR leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-loop", action="store_true", help="skip the per-problem loop baseline of the batch")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -205,6 +370,17 @@ def main():
         res, orth = accuracy(A_host, YR)
         out["accuracy"] = {"resid_over_n_eps": float(f"{res:.4g}"), "orth": float(f"{orth:.4g}"),
                            "probe": "randomized ‖Ax−QRx‖/(‖A‖‖x‖nε), ‖QᵀQv−v‖/‖v‖"}
+    del A, A0
+    torch.cuda.empty_cache()
+    if not args.no_lsq:
+        out["lsq_mixed"] = lsq_leg(qr, torch)
+        torch.cuda.empty_cache()
+    if not args.no_batch:
+        out["batch"] = batch_leg(qr, torch, loop=not args.no_loop)
+        torch.cuda.empty_cache()
+    if not args.no_serve:
+        out["serving"] = serve_leg(qr, torch)
+        torch.cuda.empty_cache()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A_host, b, args.cpu_seconds)
     print(json.dumps(out), flush=True)

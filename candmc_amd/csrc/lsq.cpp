@@ -165,6 +165,7 @@ int cqr_lstsq(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, co
   CQR_TRY(launch_copy2d(c, B, ldb, r, m, m, nr));
   CQR_TRY(thin_gemm(c, 0, m, n, -1.0, A, lda, X, ldx, (int)nr, 1.0, r, m, nullptr));
   std::vector<double> hn(2 * nr);
+  double prev_rel = 0.0;
   for (int it = 1; it <= max_iter; ++it) {
     // f = b − r − A·x,  g = −Aᵀ·r   (FP64)
     CQR_TRY(launch_copy2d(c, B, ldb, f, m, m, nr));
@@ -191,6 +192,10 @@ int cqr_lstsq(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, co
     if (iters) *iters = it;
     if (last_update) *last_update = rel;
     if (!(rel > tol)) break;  // also stops on NaN
+    // the updates stopped shrinking: the attainable accuracy (about κ·ε of
+    // the FP64 residual) is reached, or κ·u of the factorization is ≥ 1
+    if (it > 1 && rel > 0.9 * prev_rel) break;
+    prev_rel = rel;
   }
   return CQR_OK;
 }
