@@ -329,6 +329,15 @@ def main():
     is_mfma = dom.startswith("gemm")
     achieved = (dfl / (dms * 1e-3) / 1e12) if is_mfma else (dby / (dms * 1e-3) / 1e9)
     peak = PEAK_FP64_TFLOPS if is_mfma else PEAK_HBM_GBS
+    # HBM bytes per launch from the committed counter runs (FETCH_SIZE and
+    # WRITE_SIZE, calibrated on a 1 GiB copy) when they cover this kernel
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "qr2d_32768x4096_b128_update_traffic.json")
+    if os.path.exists(tf) and (m, n, b) == (32768, 4096, 128):
+        with open(tf) as fh:
+            tj = json.load(fh)
+        if tj.get("kernel_class") == dom:
+            traffic = tj["traffic_bytes_per_launch"]
     roofline = {
         "bound": "mfma" if is_mfma else "hbm",
         "kernel": dom,
@@ -336,7 +345,8 @@ def main():
         "peak": peak,
         "unit": "TFLOP/s" if is_mfma else "GB/s",
         "frac": round(achieved / peak, 4),
-        "traffic": None,
+        "traffic": traffic,
+        "algorithmic_bytes_per_launch": round(dby / dn),
         "avg_launch_ms": round(dms / dn, 5),
         "launches_per_step": dn / args.steps,
     }
