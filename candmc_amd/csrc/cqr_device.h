@@ -202,6 +202,28 @@ __device__ inline dbl4 mfma_tile16(const FX& x, const FY& y, int k0, int k1, int
   return acc;
 }
 
+// Same tile with the operands of 16 k steps fetched before their MFMAs, so
+// that global-memory latency is paid once per 64 values of k, not per step.
+template <class FX, class FY>
+__device__ inline dbl4 mfma_tile16_k128(const FX& x, const FY& y, int k0, int k1, int lane) {
+  const int r = lane & 15, q = lane >> 4;
+  dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int kc = k0; kc < k1; kc += 64) {   // chunks of 16 k steps, loaded up front
+    double xa[16], yb[16];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const int k = kc + 4 * s;
+      const bool in = k < k1;
+      xa[s] = in ? x(r, k + q) : 0.0;
+      yb[s] = in ? y(k + q, r) : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < 16; ++s)
+      if (kc + 4 * s < k1) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[s], xa[s], acc, 0, 0, 0);
+  }
+  return acc;
+}
+
 // ---------------------------------------------------------------------------
 // In-LDS inverse of an upper triangular b×b matrix (column-major, ld),
 // whole workgroup.  On entry the strictly upper part of M holds N and

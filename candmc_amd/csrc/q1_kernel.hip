@@ -48,34 +48,6 @@ __device__ inline int parent_node(int r, int lv, int L) {
   return -1;
 }
 
-// out[i][j] (the thread's row i and its 8 columns) = Σ_{k<R} M[i][k]·S[k][j]
-// with M read from global memory column-major (ld).  The k loop runs in
-// batches of 16 loads with the next batch in flight while the current one
-// is consumed, so the chain costs about one memory latency, not R.
-__device__ inline void rowprod(const double* __restrict__ M, int64_t ld, const double (*S)[LDS_LD], int R,
-                               int i, int jh, double (&acc)[8]) {
-#pragma unroll
-  for (int j = 0; j < 8; ++j) acc[j] = 0.0;
-  if (i >= R) return;
-  const double* mp = M + i;
-  double cur[16], nxt[16];
-#pragma unroll
-  for (int t = 0; t < 16; ++t) cur[t] = t < R ? mp[(int64_t)t * ld] : 0.0;
-  for (int k0 = 0; k0 < R; k0 += 16) {
-    const int kn = k0 + 16;
-#pragma unroll
-    for (int t = 0; t < 16; ++t) nxt[t] = kn + t < R ? mp[(int64_t)(kn + t) * ld] : 0.0;
-#pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      const int k = min(k0 + t, 127);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) acc[j] += cur[t] * S[k][jh * 8 + j];
-    }
-#pragma unroll
-    for (int t = 0; t < 16; ++t) cur[t] = nxt[t];
-  }
-}
-
 }  // namespace
 
 __global__ void __launch_bounds__(NT)
@@ -85,7 +57,6 @@ q1_kernel(const double* __restrict__ nodeV2, const double* __restrict__ nodeT, c
           int nnodes, int b, int* __restrict__ err, const BatchProb* __restrict__ bp, int nprob) {
   __shared__ double S0[128][LDS_LD];
   __shared__ double S1[128][LDS_LD];
-  __shared__ double Vs[CW][128 + 1];
   __shared__ int s_ok;
 
   const int ncb = (b + CW - 1) / CW;
@@ -158,25 +129,47 @@ q1_kernel(const double* __restrict__ nodeV2, const double* __restrict__ nodeT, c
   }
   __syncthreads();
 
-  double acc[8];
+  // Products on FP64 MFMA (16×16 tiles, a wave per tile row): the 16-column
+  // block is one tile wide; wave w takes tile rows w, w+4, ...  Fragment
+  // layout as in mfma_tile16: lane l returns row (l & 15), columns
+  // (l >> 4) + 4v.
+  const int lane = tid & 63, w = tid >> 6;
+  const int tr = lane & 15, tc = lane >> 4;
+  auto fromS = [&](const double (*S)[LDS_LD], int K) {
+    return [=](int k, int c) { return k < K ? S[k][c] : 0.0; };
+  };
+  const int nrt = (R + 15) / 16;  // tile rows of an R-row result
   if (is_node) {
     const double* Tn = nodeT + (int64_t)nid * bb;
     const double* V2 = nodeV2 + (int64_t)nid * bb;
-    // X = T·C_z
-    rowprod(Tn, b, S0, R, i, jh, acc);
+    // X = T·C_z (T upper: row tile t needs k ≥ 16t)
+    for (int t = w; t < nrt; t += 4) {
+      const int m0 = 16 * t;
+      auto xT = [&](int r, int k) { return (m0 + r < b && k < R) ? Tn[(m0 + r) + (int64_t)k * b] : 0.0; };
+      const dbl4 acc = mfma_tile16_k128(xT, fromS(S0, R), m0, R, lane);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) S1[i][jh * 8 + j] = i < R ? acc[j] : 0.0;
+      for (int v = 0; v < 4; ++v) S1[m0 + tr][tc + 4 * v] = m0 + tr < R ? acc[v] : 0.0;
+    }
+    for (int t = nrt + w; t < 8; t += 4)   // rows ≥ R of X are zero
+#pragma unroll
+      for (int v = 0; v < 4; ++v) S1[16 * t + tr][tc + 4 * v] = 0.0;
     __syncthreads();
     // C_s = −V2·X, C_z −= X
-    rowprod(V2, b, S1, R, i, jh, acc);
     double* Cs = C + (int64_t)snd * bb;
-    if (i < b) {
+    for (int t = w; t * 16 < b; t += 4) {
+      const int m0 = 16 * t;
+      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+      if (t < nrt) {
+        auto xV = [&](int r, int k) { return (m0 + r < b && k < R) ? V2[(m0 + r) + (int64_t)k * b] : 0.0; };
+        acc = mfma_tile16_k128(xV, fromS(S1, R), m0, R, lane);
+      }
+      const int row = m0 + tr;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int jj = jh * 8 + j;
-        if (jj < nj) {
-          store_coherent(Cs + i + (int64_t)(j0 + jj) * b, i < R ? -acc[j] : 0.0);
-          store_coherent(Cz + i + (int64_t)(j0 + jj) * b, i < R ? S0[i][jj] - S1[i][jj] : 0.0);
+      for (int v = 0; v < 4; ++v) {
+        const int jj = tc + 4 * v;
+        if (row < b && jj < nj) {
+          store_coherent(Cs + row + (int64_t)(j0 + jj) * b, row < R ? -acc[v] : 0.0);
+          store_coherent(Cz + row + (int64_t)(j0 + jj) * b, row < R ? S0[row][jj] - S1[row][jj] : 0.0);
         }
       }
     }
@@ -189,81 +182,58 @@ q1_kernel(const double* __restrict__ nodeV2, const double* __restrict__ nodeT, c
   // ---- leaf z: rows [r0, r1) of V = [V\R] (unit lower in its top b×b)
   const int64_t r0 = ((int64_t)z * m) / L, r1 = ((int64_t)(z + 1) * m) / L;
   const double* Vl = V + r0;
-  // Z = V_topᵀ·C_l, staged 16 rows of V_top at a time (rows k ≥ column i)
+  const int64_t ml = r1 - r0;
+  // Z = V_topᵀ·C_l: Z[i] = Σ_{i≤k<R} V_top[k][i]·C[k], R rows
+  for (int t = w; t < nrt; t += 4) {
+    const int m0 = 16 * t;
+    auto xVt = [&](int r, int k) {
+      const int col = m0 + r;
+      return (k < R && col < b) ? (k > col ? Vl[k + (int64_t)col * ldv] : (k == col ? 1.0 : 0.0)) : 0.0;
+    };
+    const dbl4 acc = mfma_tile16_k128(xVt, fromS(S0, R), m0, R, lane);
 #pragma unroll
-  for (int j = 0; j < 8; ++j) acc[j] = 0.0;
-  // thread (kk = tid & 15, columns tid/16 + 16·e) stages V_top rows k0+kk
-  double pf[8];
-  auto vtop = [&](int k0) {
+    for (int v = 0; v < 4; ++v) S1[m0 + tr][tc + 4 * v] = m0 + tr < R ? acc[v] : 0.0;
+  }
+  __syncthreads();
+  // M = T_l·Z, overwriting Z in S1 once every tile is computed (a wave
+  // holds at most two of the eight tile rows)
+  const double* Tl = leafT + (int64_t)z * bb;
+  dbl4 macc[2];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int kk = tid & (CW - 1), col = (tid >> 4) + 16 * e, k = k0 + kk;
-      double v = 0.0;
-      if (k < R && col < b) v = k > col ? Vl[k + (int64_t)col * ldv] : (k == col ? 1.0 : 0.0);
-      pf[e] = v;
-    }
-  };
-  vtop(0);
-  for (int k0 = 0; k0 < R; k0 += CW) {
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < 8; ++e) Vs[tid & (CW - 1)][(tid >> 4) + 16 * e] = pf[e];
-    __syncthreads();
-    if (k0 + CW < R) vtop(k0 + CW);  // next slab in flight during this one
-    if (i < b) {
-      const int kend = min(CW, R - k0);
-      for (int kk = 0; kk < kend; ++kk) {
-        const double vk = Vs[kk][i];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) acc[j] += vk * S0[k0 + kk][jh * 8 + j];
-      }
+  for (int q = 0; q < 2; ++q) {
+    const int t = w + 4 * q, m0 = 16 * t;
+    macc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
+    if (t < nrt) {
+      auto xT = [&](int r, int k) { return (m0 + r < b && k < R) ? Tl[(m0 + r) + (int64_t)k * b] : 0.0; };
+      macc[q] = mfma_tile16_k128(xT, fromS(S1, R), m0, R, lane);
     }
   }
   __syncthreads();
-  // Z[i] = Σ_{i≤k<R} V_top[k][i]·C[k] vanishes for i ≥ R, and so does
-  // M = T_l·Z (T_l upper): both have R rows.
 #pragma unroll
-  for (int j = 0; j < 8; ++j) S1[i][jh * 8 + j] = i < R ? acc[j] : 0.0;
+  for (int q = 0; q < 2; ++q) {
+    const int t = w + 4 * q, m0 = 16 * t;
+    if (t < nrt)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) S1[m0 + tr][tc + 4 * v] = m0 + tr < R ? macc[q][v] : 0.0;
+  }
   __syncthreads();
-  const double* Tl = leafT + (int64_t)z * bb;
-  rowprod(Tl, b, S1, R, i, jh, acc);
-  __syncthreads();
+  // Q1_l = [C_l; 0] − V_l·M over the leaf rows (V_l unit lower on top)
+  for (int64_t t = w; t * 16 < ml; t += 4) {
+    const int64_t m0 = 16 * t;
+    auto xV = [&](int r, int k) {
+      const int64_t row = m0 + r;
+      if (row >= ml || k >= R || k > row) return 0.0;
+      return k == row ? 1.0 : Vl[row + (int64_t)k * ldv];
+    };
+    const int kmax = (int)min<int64_t>(R, m0 + 16);
+    const dbl4 acc = mfma_tile16_k128(xV, fromS(S1, R), 0, kmax, lane);
+    const int64_t row = m0 + tr;
+    if (row < ml) {
 #pragma unroll
-  for (int j = 0; j < 8; ++j) S1[i][jh * 8 + j] = i < R ? acc[j] : 0.0;  // M (R×16)
-  __syncthreads();
-  // Q1_l = [C_l; 0] − V_l·M over the leaf rows, 128 rows per pass
-  const int64_t ml = r1 - r0;
-  for (int64_t rb = 0; rb < ml; rb += 128) {
-    const int64_t r = rb + i;
-    if (r >= ml) continue;
-    double q[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) q[j] = r < R ? S0[r][jh * 8 + j] : 0.0;
-    // V_l[r][k]: 1 at k = r, 0 above (k > r), so only k ≤ r is read
-    const double* vp = Vl + r;
-    double cur[16], nxt[16];
-#pragma unroll
-    for (int t = 0; t < 16; ++t) cur[t] = (t < R && t < r) ? vp[(int64_t)t * ldv] : (t == r ? 1.0 : 0.0);
-    for (int k0 = 0; k0 < R; k0 += 16) {
-      const int kn = k0 + 16;
-#pragma unroll
-      for (int t = 0; t < 16; ++t) {
-        const int k = kn + t;
-        nxt[t] = (k < R && k < r) ? vp[(int64_t)k * ldv] : (k == r ? 1.0 : 0.0);
+      for (int v = 0; v < 4; ++v) {
+        const int jj = tc + 4 * v;
+        if (jj < nj) Q[r0 + row + (int64_t)(j0 + jj) * ldq] = (row < R ? S0[row][jj] : 0.0) - acc[v];
       }
-#pragma unroll
-      for (int t = 0; t < 16; ++t) {
-        const int k = min(k0 + t, 127);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) q[j] -= cur[t] * S1[k][jh * 8 + j];
-      }
-#pragma unroll
-      for (int t = 0; t < 16; ++t) cur[t] = nxt[t];
-    }
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int jj = jh * 8 + j;
-      if (jj < nj) Q[r0 + r + (int64_t)(j0 + jj) * ldq] = q[j];
     }
   }
 }
