@@ -258,23 +258,33 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
   if (split) C += (int64_t)blockIdx.z * g.split_stride;
   const bool has_beta = !split && g.beta != 0.0;
   const double alpha = split ? 1.0 : g.alpha;
+  // C is read (beta ≠ 0) sixteen values at a time, all loads of a group
+  // issued before any of its stores, so a tile pays four memory latencies
+  // rather than one per element.
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int64_t gm = m0 + wm0 + 16 * i + fc;
-    if (!full_mn && gm >= Mrows) continue;
+    const bool row_ok = full_mn || gm < Mrows;
+    double cold[16];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int64_t gn = n0 + wn0 + 16 * j + (LP ? 4 * fr + v : fr + 4 * v);
-        if (!full_mn && gn >= g.n) continue;
-        double* cp = C + gm + gn * ldc;
-        double val = alpha * (double)acc[i][j][v];
-        if (has_beta) val += g.beta * *cp;
-        if (g.smC == SM_UPPER && gm > gn) val = 0.0;
-        *cp = val;
+        const bool ok = row_ok && (full_mn || gn < g.n);
+        cold[4 * j + v] = (has_beta && ok) ? C[gm + gn * ldc] : 0.0;
       }
-    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t gn = n0 + wn0 + 16 * j + (LP ? 4 * fr + v : fr + 4 * v);
+        if (!row_ok || !(full_mn || gn < g.n)) continue;
+        double val = alpha * (double)acc[i][j][v];
+        if (has_beta) val += g.beta * cold[4 * j + v];
+        if (g.smC == SM_UPPER && gm > gn) val = 0.0;
+        C[gm + gn * ldc] = val;
+      }
   }
 }
 
