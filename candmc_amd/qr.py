@@ -291,7 +291,7 @@ def lstsq(A, lda, m, n, B, ldb, nrhs, X=None, ldx=None, b=128, prec=PREC_F64, ma
 
 # --- batched tall-skinny QR ----------------------------------------------------
 class Batch:
-    """Many independent m_p×n_p problems (n_p ≤ 128), packed column-major one
+    r"""Many independent m_p×n_p problems (n_p ≤ 128), packed column-major one
     after another; each is factored as one hh_recon_qr panel ([Y\R] in
     place, Tᵀ in a second packed buffer)."""
 
