@@ -456,6 +456,41 @@ size_t wy_update_ws_doubles(int64_t m, int64_t n, int b) {
   return gemm_f64_scratch_doubles(b, n, m) + 2 * (size_t)b * n + 64;
 }
 
+int lean_gemm(cqr_ctx* c, int ak, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+              const double* B, int64_t ldb, double beta, double* C, int64_t ldc, double* scratch, int nsplit,
+              int kind, int* used);
+
+// FP64 update with the unmasked bulk on the lean kernel: rows [0, bt) of Y
+// (its unit-lower top block) go through the masked general kernel, rows
+// [bt, m) through lean_gemm; the Yᵀ·A partial sums of both are reduced in a
+// fixed order.
+static int wy_update_f64_split(cqr_ctx* c, const double* Y, int64_t ldy, int smY, const double* Tt, int t_trans,
+                               double* A, int64_t lda, int64_t m, int64_t n, int b, double* part, int ns, double* Z,
+                               double* Zp) {
+  const int64_t bt = std::min<int64_t>(b, m);
+  // slot 0: Y(0:bt)ᵀ·A(0:bt), masked
+  GemmArgs g{};
+  g.m = b; g.n = n; g.k = bt; g.alpha = 1.0; g.beta = 0.0;
+  g.A = Y; g.lda = ldy; g.smA = smY; g.B = A; g.ldb = lda; g.smB = SM_NONE;
+  g.C = part; g.ldc = b; g.kchunk = std::max<int64_t>(bt, 1); g.split_stride = (int64_t)b * n;
+  CQR_TRY(launch_mfma(c, 1, 0, g, 1, PK_GEMM_YTA));
+  // slots 1..: Y(bt:m)ᵀ·A(bt:m) on the lean kernel
+  int used = 0;
+  CQR_TRY(lean_gemm(c, 1, b, n, m - bt, 1.0, Y + bt, ldy, A + bt, lda, 0.0, nullptr, 0, part + (int64_t)b * n,
+                    ns - 1, PK_GEMM_YTA, &used));
+  const int nslots = 1 + used;
+  const int gx = (int)std::min<int64_t>(2048, ((int64_t)b * n + 255) / 256);
+  hipEvent_t ev = prof_begin(c);
+  hipLaunchKernelGGL(splitk_reduce_kernel, dim3(gx), dim3(256), 0, c->stream, part, (int64_t)b * n, nslots,
+                     (int64_t)b, n, Z, (int64_t)b, 1.0, 0.0);
+  prof_end(c, ev, PK_REDUCE, (double)nslots * b * n, 8.0 * (nslots + 1.0) * b * n);
+  CQR_TRY(check_launch(c, "splitk_reduce"));
+  CQR_TRY(gemm_f64_impl(c, t_trans ? 1 : 0, 0, b, n, b, 1.0, Tt, b, SM_LOWER, Z, b, SM_NONE, 0.0, Zp, b, nullptr));
+  // A(0:bt) −= Y(0:bt)·Z' (masked), A(bt:m) −= Y(bt:m)·Z' (lean)
+  CQR_TRY(gemm_f64_impl(c, 0, 0, bt, n, b, -1.0, Y, ldy, smY, Zp, b, SM_NONE, 1.0, A, lda, nullptr, PK_GEMM_UPD));
+  return lean_gemm(c, 0, m - bt, n, b, -1.0, Y + bt, ldy, Zp, b, 1.0, A + bt, lda, nullptr, 1, PK_GEMM_UPD, nullptr);
+}
+
 int wy_update_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const double* Tt, int t_trans,
                  double* A, int64_t lda, int64_t m, int64_t n, int b, double* ws, int prec) {
   if (m <= 0 || n <= 0 || b <= 0) return CQR_OK;
@@ -464,6 +499,11 @@ int wy_update_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const
   const size_t np = gemm_f64_scratch_doubles(b, n, m);
   double* Z = ws + ((np + 31) / 32) * 32;
   double* Zp = Z + ((size_t)b * n + 31) / 32 * 32;
+  // the lean kernel's 128×128 tiles pay off from three column tiles on
+  if (prec == CQR_PREC_F64 && np > 0 && m > 2 * b && n > 2 * 128) {
+    const int ns = (int)(np / ((size_t)b * n));   // partial slots of choose_splits
+    if (ns >= 2) return wy_update_f64_split(c, Y, ldy, smY, Tt, t_trans, A, lda, m, n, b, part, ns, Z, Zp);
+  }
   // Z = Yᵀ·A  (b×n)
   CQR_TRY(gemm_f64_impl(c, 1, 0, b, n, m, 1.0, Y, ldy, smY, A, lda, SM_NONE, 0.0, Z, b,
                         np ? part : nullptr, PK_GEMM_YTA, prec));
