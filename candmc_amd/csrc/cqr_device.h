@@ -68,11 +68,23 @@ __device__ inline void xr_halve(double (&v)[N], int lane) {
     v[t] = keep + dpp_d<CTRL, 0xf>(send);
   }
 }
-__device__ inline double xr_cross_rows(double s) {
-  s += __shfl_xor(s, 16, 64);
-  s += __shfl_xor(s, 32, 64);
-  return s;
+// Sum over the four rows of 16 lanes, lane by lane, with the gfx950 lane
+// swaps (VALU only, no LDS round trip): swapping odd rows of one copy with
+// even rows of another and adding the two results gives every lane the sum
+// of its row pair; the 32-lane swap then adds the two halves.
+__device__ inline double xr_swap_sum16(double s) {
+  const unsigned lo = (unsigned)__double2loint(s), hi = (unsigned)__double2hiint(s);
+  const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  return __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
 }
+__device__ inline double xr_swap_sum32(double s) {
+  const unsigned lo = (unsigned)__double2loint(s), hi = (unsigned)__double2hiint(s);
+  const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  return __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
+}
+__device__ inline double xr_cross_rows(double s) { return xr_swap_sum32(xr_swap_sum16(s)); }
 template <int N>
 __device__ inline double xreduce(double (&v)[N], int lane) {
   static_assert(N == 1 || N == 2 || N == 4 || N == 8 || N == 16, "N in {1,2,4,8,16}");
@@ -222,6 +234,33 @@ __device__ inline dbl4 mfma_tile16_k128(const FX& x, const FY& y, int k0, int k1
       if (kc + 4 * s < k1) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[s], xa[s], acc, 0, 0, 0);
   }
   return acc;
+}
+
+// Same tile with the k steps dealt round-robin to four accumulators, so
+// consecutive MFMAs are independent and issue back to back instead of each
+// waiting for the previous result (the chains are summed at the end).  The
+// accessors are called for every k of a 64-wide chunk, also k ≥ k1: they
+// must stay in bounds there and return 0 (branch-free accessors let the
+// 32 loads of a chunk issue together).
+template <class FX, class FY>
+__device__ inline dbl4 mfma_tile16_acc4(const FX& x, const FY& y, int k0, int k1, int lane) {
+  const int r = lane & 15, q = lane >> 4;
+  dbl4 acc[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) acc[a] = dbl4{0.0, 0.0, 0.0, 0.0};
+  for (int kc = k0; kc < k1; kc += 64) {
+    double xa[16], yb[16];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const int k = kc + 4 * s;
+      xa[s] = x(r, k + q);
+      yb[s] = y(k + q, r);
+    }
+#pragma unroll
+    for (int s = 0; s < 16; ++s)
+      if (kc + 4 * s < k1) acc[s & 3] = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[s], xa[s], acc[s & 3], 0, 0, 0);
+  }
+  return (acc[0] + acc[1]) + (acc[2] + acc[3]);
 }
 
 // ---------------------------------------------------------------------------

@@ -318,6 +318,367 @@ ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, doubl
   for (int p = threadIdx.x; p < b; p += 512) tauout[(int64_t)nid * b + p] = stau[p];
 }
 
+// ---------------------------------------------------------------------------
+// Blocked form of the same tree node (b a multiple of 8): the node advances
+// eight columns at a time, left-looking.  For block [c0, c0+8):
+//   S = C_top(0:c0) + V2(:,0:c0)ᵀ·C_bot,  U = T(0:c0,0:c0)ᵀ·S,
+//   R(0:c0, block) = C_top(0:c0) − U,  C_bot −= V2(:,0:c0)·U
+// (Qᵀ of the earlier columns applied to the block at once), then one wave
+// factors the 8 columns (dlarfg per column, reflectors applied within the
+// block), builds their 8×8 T and publishes the block of R.  T's columns of
+// block k−1 above its diagonal block, T(0:c, blk) = −T(0:c,0:c)·(V2(:,0:c)ᵀ·
+// V2_blk)·T_blk, are completed at the start of block k: V2ᵀ·V2_blk shares
+// the MFMA tiles of V2ᵀ·C_bot (columns 8..15 of the 16-wide tile).
+// V2 and T live in LDS as packed upper triangles; each product is one
+// 16-row FP64 MFMA tile per wave, so a block costs six barriers where the
+// column form spends four per column.
+// ---------------------------------------------------------------------------
+__device__ inline int pk_off(int q) { return q * (q + 1) / 2; }
+// MFMA operand accessors load from a clamped (in-bounds) address
+// unconditionally and mask the value, so no load sits behind a branch
+__device__ inline double sel(bool ok, double v) { return ok ? v : 0.0; }
+
+template <int BMAX>
+__global__ void __launch_bounds__(4 * BMAX)
+btree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, double* __restrict__ V2out,
+             double* __restrict__ Tout, double* __restrict__ tauout, uint64_t* __restrict__ cnt,
+             uint64_t epoch, int L, int b, int* __restrict__ err, const BatchProb* __restrict__ bp, int nprob) {
+  constexpr int NT = 4 * BMAX, NB = 8, RPL = (BMAX + 63) / 64;
+  constexpr int PACK = BMAX * (BMAX + 1) / 2;
+  __shared__ double V2p[PACK], Tp[PACK];   // packed upper triangles of the node's V2 and T
+  __shared__ double Cb[BMAX * NB];         // bottom rows of the block, row-major [r][8]
+  __shared__ double Xs[BMAX * NB];         // S
+  __shared__ double Xg[BMAX * NB];         // V2ᵀ·V2_blk, then U
+  __shared__ double Ctd[NB * NB];          // top 8×8 diagonal block of the input, row-major
+  __shared__ double stau[BMAX];
+  __shared__ int s_err;
+
+  int nid = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
+  const int64_t bb = (int64_t)b * b;
+  if (bp) {
+    const BatchProb P = bp[bp_find(bp, nprob, nid, BP_NODE)];
+    nid -= P.node0;
+    L = P.L;
+    leafR += (int64_t)P.leaf0 * bb;
+    nodeR += (int64_t)P.node0 * bb;
+    V2out += (int64_t)P.node0 * bb;
+    Tout += (int64_t)P.node0 * bb;
+    tauout += (int64_t)P.node0 * b;
+    cnt += P.node0;
+  }
+  int lv = 0, n = L, node0 = 0;
+  while (nid >= node0 + n / 2) {
+    node0 += n / 2;
+    n = n / 2 + (n & 1);
+    ++lv;
+  }
+  const int z = nid - node0, snd = z + (n + 1) / 2;
+  const int ptop = producer_of(z, lv, L), pbot = producer_of(snd, lv, L);
+  const double* Rt = ptop >= 0 ? nodeR + ptop * bb : leafR + z * bb;
+  const double* Rb = pbot >= 0 ? nodeR + pbot * bb : leafR + snd * bb;
+  double* Ro = nodeR + nid * bb;
+
+  // MFMA tile of this wave: rows t0..t0+15; in a result, lane l holds row
+  // tq = t0 + (l & 15) and columns q4 + 4v (v = 0..3)
+  const int r16 = lane & 15, q4 = lane >> 4;
+  const int t0 = 16 * w, tq = t0 + r16;
+
+  // T(0:cp, cp:cp+8) = −T(0:cp,0:cp)·G·T(cp:cp+8, cp:cp+8), G = V2(:,0:cp)ᵀ·V2(:,cp:cp+8) in Xg
+  auto t12_block = [&](int cp) {
+    if (t0 >= cp) return;
+    auto x = [&](int m, int p) {
+      const int q = t0 + m;
+      const bool ok = q <= p && p < cp;
+      return sel(ok, Tp[ok ? pk_off(p) + q : 0]);
+    };
+    auto y = [&](int p, int c) {
+      const bool ok = c < 8 && p < cp;
+      return sel(ok, Xg[ok ? p * 8 + c : 0]);
+    };
+    const dbl4 g2 = mfma_tile16_acc4(x, y, t0, cp, lane);
+    // G2·T_blk with G2 straight from the accumulator (its layout is the A operand's)
+    auto tb = [&](int kk, int c) {
+      const bool ok = c < 8 && kk <= c;
+      return sel(ok, Tp[ok ? pk_off(cp + c) + cp + kk : 0]);
+    };
+    dbl4 t12 = {0.0, 0.0, 0.0, 0.0};
+    t12 = __builtin_amdgcn_mfma_f64_16x16x4f64(tb(q4, r16), g2[0], t12, 0, 0, 0);
+    t12 = __builtin_amdgcn_mfma_f64_16x16x4f64(tb(4 + q4, r16), g2[1], t12, 0, 0, 0);
+    if (tq < cp) {
+      Tp[pk_off(cp + q4) + tq] = -t12[0];
+      Tp[pk_off(cp + q4 + 4) + tq] = -t12[1];
+    }
+  };
+
+  if (tid == 0) s_err = 0;
+  const int nblk = b / NB;
+  for (int k = 0; k < nblk; ++k) {
+    const int c0 = k * NB, cp = c0 - NB;
+    if (tid == 0) {
+      const uint64_t want = epoch * 256 + (uint64_t)(c0 + NB);
+      if (ptop >= 0 && !flag_wait(cnt + ptop, want)) s_err = 1;
+      if (pbot >= 0 && !flag_wait(cnt + pbot, want)) s_err = 1;
+    }
+    __syncthreads();                                                       // B0
+    if (s_err) break;
+    // ---- inputs of the block (rows ≤ column), zeros below the block's diagonal
+    for (int e = tid; e < BMAX * NB; e += NT) {
+      const int r = e >> 3, jj = e & 7;
+      Cb[e] = (r < b && r <= c0 + jj) ? load_coherent(Rb + r + (int64_t)(c0 + jj) * b) : 0.0;
+    }
+    if (tid < NB * NB) {
+      const int rr = tid >> 3, jj = tid & 7;
+      Ctd[tid] = rr <= jj ? load_coherent(Rt + (c0 + rr) + (int64_t)(c0 + jj) * b) : 0.0;
+    }
+    double ct0 = 0.0, ct1 = 0.0;
+    if (tq < c0) {
+      ct0 = load_coherent(Rt + tq + (int64_t)(c0 + q4) * b);
+      ct1 = load_coherent(Rt + tq + (int64_t)(c0 + q4 + 4) * b);
+    }
+    {
+      const int rows = b - c0 - NB;
+      for (int e = tid; e < rows * NB; e += NT) {
+        const int jj = e / rows, r = c0 + NB + (e - jj * rows);
+        Ro[r + (int64_t)(c0 + jj) * b] = 0.0;
+      }
+    }
+    __syncthreads();                                                       // B1
+    if (k > 0) {
+      // S = C_top + V2ᵀ·C_bot (tile columns 0..7) and V2ᵀ·V2_{k−1} (columns 8..15)
+      if (t0 < c0) {
+        auto x = [&](int m, int r) {
+          const int q = t0 + m;
+          const bool ok = r <= q && q < c0;
+          return sel(ok, V2p[ok ? pk_off(q) + r : 0]);
+        };
+        auto y = [&](int r, int c) {
+          const int col = cp + c - 8;
+          const bool ok = r < c0 && (c < 8 || r <= col);
+          const double* a = c < 8 ? Cb + r * 8 + c : V2p + pk_off(col) + r;
+          return sel(ok, *(ok ? a : Cb));
+        };
+        const dbl4 acc = mfma_tile16_acc4(x, y, 0, min(t0 + 16, c0), lane);
+        if (tq < c0) {
+          Xs[tq * 8 + q4] = ct0 + acc[0];
+          Xs[tq * 8 + q4 + 4] = ct1 + acc[1];
+          Xg[tq * 8 + q4] = acc[2];
+          Xg[tq * 8 + q4 + 4] = acc[3];
+        }
+      }
+      __syncthreads();                                                     // B2
+      // T columns of block k−1 above its diagonal block
+      if (cp > 0) t12_block(cp);
+      __syncthreads();                                                     // B3
+      // U = Tᵀ·S; R(0:c0, block) = C_top − U
+      if (t0 < c0) {
+        auto x = [&](int m, int p) {
+          const int q = t0 + m;
+          const bool ok = p <= q && q < c0;
+          return sel(ok, Tp[ok ? pk_off(q) + p : 0]);
+        };
+        auto y = [&](int p, int c) {
+          const bool ok = c < 8 && p < c0;
+          return sel(ok, Xs[ok ? p * 8 + c : 0]);
+        };
+        const dbl4 u = mfma_tile16_acc4(x, y, 0, min(t0 + 16, c0), lane);
+        if (tq < c0) {
+          Xg[tq * 8 + q4] = u[0];
+          Xg[tq * 8 + q4 + 4] = u[1];
+          store_coherent(Ro + tq + (int64_t)(c0 + q4) * b, ct0 - u[0]);
+          store_coherent(Ro + tq + (int64_t)(c0 + q4 + 4) * b, ct1 - u[1]);
+        }
+      }
+      __syncthreads();                                                     // B4
+      // C_bot −= V2·U
+      if (t0 < c0) {
+        auto x = [&](int m, int q) {
+          const int r = t0 + m;
+          const bool ok = r <= q && q < c0;
+          return sel(ok, V2p[ok ? pk_off(q) + r : 0]);
+        };
+        auto y = [&](int q, int c) {
+          const bool ok = c < 8 && q < c0;
+          return sel(ok, Xg[ok ? q * 8 + c : 0]);
+        };
+        const dbl4 d = mfma_tile16_acc4(x, y, t0, c0, lane);
+        if (tq < c0) {
+          Cb[tq * 8 + q4] -= d[0];
+          Cb[tq * 8 + q4 + 4] -= d[1];
+        }
+      }
+      __syncthreads();                                                     // B5
+    }
+    // ---- one wave factors the 8 columns
+    if (w == 0) {
+      double cb[RPL][NB];
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) {
+        const int r = lane + 64 * s;
+#pragma unroll
+        for (int jj = 0; jj < NB; ++jj) cb[s][jj] = r < BMAX ? Cb[r * 8 + jj] : 0.0;
+      }
+      double trow[NB];   // row `lane` of the block's T (lanes < 8)
+      double crow[NB];   // row jj of the input's top diagonal block
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        trow[i] = 0.0;
+        crow[i] = Ctd[i];
+      }
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj) {
+        double crn[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) crn[i] = jj + 1 < NB ? Ctd[(jj + 1) * 8 + i] : 0.0;
+        // one reduction of x·c_i over the 8 columns of the block, x = column
+        // jj below the pivot: i ≠ jj feeds the reflector's application and
+        // T, i = jj is ‖x‖²; none of it depends on the reflector itself
+        double dv[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+          double a = 0.0;
+#pragma unroll
+          for (int s = 0; s < RPL; ++s) a += cb[s][i] * cb[s][jj];
+          dv[i] = a;
+        }
+        const double red = xreduce<NB>(dv, lane);
+        const double ss = readlane_d(red, jj);
+        const double alpha = crow[jj];
+        Reflector h;
+        if (!ss_needs_rescale(ss) && fabs(alpha) < 0x1p480) {
+          // ‖x‖ and |alpha| in range: |beta| = sqrt(alpha² + ‖x‖²) ≥ 2^-480, no safmin scaling
+          const double bm = sqrt(fma(alpha, alpha, ss));
+          h.beta = -copysign(bm, alpha);
+          h.tau = (h.beta - alpha) / h.beta;
+          h.scal = 1.0 / (alpha - h.beta);
+          h.knt = 0;
+        } else {
+          double xnorm;
+          if (ss_needs_rescale(ss)) {
+            double mx = 0.0;
+#pragma unroll
+            for (int s = 0; s < RPL; ++s) mx = fmax(mx, fabs(cb[s][jj]));
+            mx = wave_max(mx);
+            if (mx == 0.0) {
+              xnorm = 0.0;
+            } else {
+              double s2 = 0.0;
+#pragma unroll
+              for (int s = 0; s < RPL; ++s) { const double tt = cb[s][jj] / mx; s2 += tt * tt; }
+              xnorm = mx * sqrt(wave_sum(s2));
+            }
+          } else {
+            xnorm = sqrt(ss);
+          }
+          h = dlarfg_scalar(alpha, xnorm);
+        }
+        // v = x scaled as dlarfg scales it; the reduced dot products of x take the same factors
+        double d[NB];   // i > jj: v·c_i; i < jj: v_i·v
+#pragma unroll
+        for (int i = 0; i < NB; ++i) d[i] = i == jj ? 0.0 : readlane_d(red, i);
+        if (h.knt == 0) {
+          const double f = h.tau != 0.0 ? h.scal : 1.0;
+#pragma unroll
+          for (int s = 0; s < RPL; ++s) cb[s][jj] *= f;
+#pragma unroll
+          for (int i = 0; i < NB; ++i) d[i] *= f;
+        } else {
+          auto scale = [&](double x) {
+            if (h.tau != 0.0) {
+              for (int kk = 0; kk < h.knt; ++kk) x *= 0x1p969;
+              x *= h.scal;
+            }
+            return x;
+          };
+#pragma unroll
+          for (int s = 0; s < RPL; ++s) cb[s][jj] = scale(cb[s][jj]);
+#pragma unroll
+          for (int i = 0; i < NB; ++i) d[i] = scale(d[i]);
+        }
+        double rval = lane == jj ? h.beta : 0.0;
+#pragma unroll
+        for (int i = jj + 1; i < NB; ++i) {
+          const double wv = crow[i] + d[i];
+#pragma unroll
+          for (int s = 0; s < RPL; ++s) cb[s][i] -= h.tau * wv * cb[s][jj];
+          rval = lane == i ? crow[i] - h.tau * wv : rval;
+        }
+        if (lane < NB) store_coherent(Ro + (c0 + jj) + (int64_t)(c0 + lane) * b, rval);
+        double tsum = 0.0;
+#pragma unroll
+        for (int i = 0; i < jj; ++i) tsum += trow[i] * d[i];
+        trow[jj] = lane < jj ? -h.tau * tsum : (lane == jj ? h.tau : 0.0);
+        if (lane == 0) stau[c0 + jj] = h.tau;
+#pragma unroll
+        for (int i = 0; i < NB; ++i) crow[i] = crn[i];
+      }
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) {
+        const int r = lane + 64 * s;
+#pragma unroll
+        for (int jj = 0; jj < NB; ++jj)
+          if (r <= c0 + jj) V2p[pk_off(c0 + jj) + r] = cb[s][jj];
+      }
+      if (lane < NB) {
+#pragma unroll
+        for (int jj = 0; jj < NB; ++jj)
+          if (lane <= jj) Tp[pk_off(c0 + jj) + c0 + lane] = trow[jj];
+      }
+    }
+    wait_memory();   // this wave's R stores have reached memory
+    __syncthreads();                                                       // B6
+    if (tid == 0) flag_store(cnt + nid, epoch * 256 + (uint64_t)(c0 + NB));
+  }
+  // ---- T columns of the last block above its diagonal block
+  const int cl = (nblk - 1) * NB;
+  if (!s_err && cl > 0) {
+    if (t0 < cl) {
+      auto x = [&](int m, int r) {
+        const int q = t0 + m;
+        const bool ok = r <= q && q < cl;
+        return sel(ok, V2p[ok ? pk_off(q) + r : 0]);
+      };
+      auto y = [&](int r, int c) {
+        const int col = cl + c;
+        const bool ok = c < 8 && r <= col;
+        return sel(ok, V2p[ok ? pk_off(col) + r : 0]);
+      };
+      const dbl4 g = mfma_tile16_acc4(x, y, 0, min(t0 + 16, cl), lane);
+      if (tq < cl) {
+        Xg[tq * 8 + q4] = g[0];
+        Xg[tq * 8 + q4 + 4] = g[1];
+      }
+    }
+    __syncthreads();
+    t12_block(cl);
+  }
+  __syncthreads();
+  if (s_err) {
+    if (tid == 0) *err = 1;
+    return;
+  }
+  // ---- V2, T (full b×b, zeros below the diagonal) and tau of this node
+  double* V2 = V2out + nid * bb;
+  double* Tn = Tout + nid * bb;
+  for (int e = tid; e < b * b; e += NT) {
+    const int col = e / b, row = e - col * b;
+    const bool up = row <= col;
+    V2[e] = up ? V2p[pk_off(col) + row] : 0.0;
+    Tn[e] = up ? Tp[pk_off(col) + row] : 0.0;
+  }
+  for (int p = tid; p < b; p += NT) tauout[(int64_t)nid * b + p] = stau[p];
+}
+
+template <int BMAX>
+static int btree_launch(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
+                        uint64_t* cnt, int L, int nnodes, int b, uint64_t epoch, const BatchProb* bp, int nprob) {
+  hipEvent_t ev = prof_begin(c);
+  hipLaunchKernelGGL((btree_kernel<BMAX>), dim3(nnodes), dim3(4 * BMAX), 0, c->stream, leafR, nodeR, V2, T, tau,
+                     cnt, epoch, L, b, c->d_terr, bp, nprob);
+  prof_end(c, ev, PK_TREE, nnodes * (2.0 / 3.0) * b * (double)b * b, nnodes * 40.0 * b * b);
+  return check_launch(c, "btree");
+}
+
 template <int CPW, int RPL>
 static int ptree_launch(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
                         uint64_t* cnt, int L, int nnodes, int b, uint64_t epoch, const BatchProb* bp, int nprob) {
@@ -330,6 +691,11 @@ static int ptree_launch(cqr_ctx* c, const double* leafR, double* nodeR, double* 
 
 static int ptree_dispatch(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
                           uint64_t* cnt, int L, int nnodes, int b, uint64_t epoch, const BatchProb* bp, int nprob) {
+  if (b % 8 == 0) {
+    if (b <= 32) return btree_launch<32>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
+    if (b <= 64) return btree_launch<64>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
+    if (b <= 128) return btree_launch<128>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
+  }
   if (b <= 16) return ptree_launch<2, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
   if (b <= 32) return ptree_launch<4, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
   if (b <= 64) return ptree_launch<8, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);

@@ -505,13 +505,19 @@ def column_relative_residual(A, YR):
     """test/QR/test_hh_recon.cxx:234-259: Σ_i ‖(A − Q·R)(:,i)‖/‖A(:,i)‖."""
     m, n = A.shape
     tau = tau_recon("L", YR, n, m)
-    R = np.zeros((m, n), order="F")
-    R[:n, :] = np.triu(YR[:n, :n])
-    QR = R.copy(order="F")
-    Y = unit_lower(YR, n)
-    for i in range(n - 1, -1, -1):
-        v = Y[:, i]
-        QR -= tau[i] * np.outer(v, v @ QR)
+    if m >= n:
+        # Q·R with Q = H_0···H_{n−1} formed by dorgqr (same product, blocked)
+        q, work, info = _lp.dorgqr(np.asfortranarray(YR.copy()), tau)
+        assert info == 0
+        QR = q @ np.triu(YR[:n, :n])
+    else:
+        R = np.zeros((m, n), order="F")
+        R[:n, :] = np.triu(YR[:n, :n])
+        QR = R.copy(order="F")
+        Y = unit_lower(YR, n)
+        for i in range(n - 1, -1, -1):
+            v = Y[:, i]
+            QR -= tau[i] * np.outer(v, v @ QR)
     D = A - QR
     return float(np.sum(np.sqrt(np.sum(D * D, 0) / np.sum(A * A, 0))))
 

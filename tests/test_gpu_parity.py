@@ -117,7 +117,8 @@ def test_tree_tsqr_random(Q, b):
 
 
 # ---------------------------------------------------------------- a3/a4 bitree + construct_Q1
-@pytest.mark.parametrize("m,b,P", [(8192, 64, 4), (32768, 128, 0), (1000, 50, 3), (3000, 17, 0), (640, 64, 5)])
+@pytest.mark.parametrize("m,b,P", [(8192, 64, 4), (32768, 128, 0), (1000, 50, 3), (3000, 17, 0), (640, 64, 5),
+                                   (2048, 40, 6), (1024, 24, 4), (4096, 120, 7)])
 def test_bitree_tsqr_and_construct_Q1(Q, m, b, P):
     A = O.hh_recon_test_matrix(m, b)
     dA = Q.to_gpu(A)
