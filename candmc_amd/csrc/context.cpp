@@ -142,12 +142,12 @@ int cqr_create(cqr_ctx** ctx, int device) {
   cqr_ctx* c = new cqr_ctx();
   c->device = device;
   if (hipMalloc(&c->d_flag, sizeof(int)) != hipSuccess ||
-      hipMalloc(&c->d_cnt, sizeof(uint64_t) * CQR_MAX_TREE_NODES) != hipSuccess ||
+      hipMalloc(&c->d_cnt, sizeof(uint64_t) * CQR_MAX_TREE_NODES * 2) != hipSuccess ||
       hipMalloc(&c->d_terr, sizeof(int)) != hipSuccess ||
       hipMalloc(&c->d_qflags, sizeof(uint64_t) * CQR_MAX_TREE_NODES * 8) != hipSuccess ||
       hipMemset(c->d_qflags, 0, sizeof(uint64_t) * CQR_MAX_TREE_NODES * 8) != hipSuccess ||
       hipMemset(c->d_terr, 0, sizeof(int)) != hipSuccess ||
-      hipMemset(c->d_cnt, 0, sizeof(uint64_t) * CQR_MAX_TREE_NODES) != hipSuccess ||
+      hipMemset(c->d_cnt, 0, sizeof(uint64_t) * CQR_MAX_TREE_NODES * 2) != hipSuccess ||
       hipMemset(c->d_flag, 0, sizeof(int)) != hipSuccess ||
       hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||

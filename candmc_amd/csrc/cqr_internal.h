@@ -34,7 +34,7 @@ struct cqr_ctx {
   int check_finite = 0;
   int64_t leaf_rows = 0;        // 0 = automatic
   int* d_flag = nullptr;        // device flag for non-finite scans / tree timeouts
-  uint64_t* d_cnt = nullptr;    // per-node column counters of the pipelined tree
+  uint64_t* d_cnt = nullptr;    // column counters: tree nodes, then (second half) leaves
   int* d_terr = nullptr;        // set by a tree node whose wait for a child timed out
   uint64_t* d_qflags = nullptr; // per (node, column block) flags of construct_Q1
   uint64_t q1_epoch = 0;
@@ -166,6 +166,8 @@ __device__ inline int bp_find(const BatchProb* bp, int nprob, int idx, int which
 
 // Column-pipelined tree over all levels (ptree_kernel.hip): node outputs R
 // in nodeR (nnodes×b×b), V2/T/tau per node in forward level order.
+int leaf_tree(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int b, double* leafR, double* leafT,
+              double* leafTau, double* nodeR, double* V2, double* T, double* tau, int nnodes);
 int launch_ptree(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
                  int L, int nnodes, int b);
 

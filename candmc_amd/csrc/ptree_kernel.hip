@@ -23,6 +23,7 @@
 //   T(0:i, i) = −τ_i·T(0:i,0:i)·(V2(:,0:i)ᵀ·v2_i),  T(i,i) = τ_i.
 #include "cqr_device.h"
 #include "cqr_internal.h"
+#include "leaf_body.h"
 
 namespace cqr {
 
@@ -338,22 +339,31 @@ __device__ inline int pk_off(int q) { return q * (q + 1) / 2; }
 // unconditionally and mask the value, so no load sits behind a branch
 __device__ inline double sel(bool ok, double v) { return ok ? v : 0.0; }
 
+// LDS of one node (doubles): packed V2 and T, three 8-wide row blocks, the
+// 8×8 diagonal input block and tau.
 template <int BMAX>
-__global__ void __launch_bounds__(4 * BMAX)
-btree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, double* __restrict__ V2out,
-             double* __restrict__ Tout, double* __restrict__ tauout, uint64_t* __restrict__ cnt,
-             uint64_t epoch, int L, int b, int* __restrict__ err, const BatchProb* __restrict__ bp, int nprob) {
+constexpr int btree_smem_doubles() { return BMAX * (BMAX + 1) + 3 * BMAX * 8 + 64 + BMAX; }
+
+// One tree node per workgroup (4·BMAX threads).  A producer that is a node
+// is waited for through cnt; a leaf producer is complete at launch unless
+// lcnt is set (fused launch), in which case lcnt[leaf] is waited for too.
+template <int BMAX>
+__device__ __forceinline__ void btree_body(const double* __restrict__ leafR, double* __restrict__ nodeR,
+                                  double* __restrict__ V2out, double* __restrict__ Tout,
+                                  double* __restrict__ tauout, uint64_t* __restrict__ cnt, uint64_t epoch, int L,
+                                  int b, int* __restrict__ err, const BatchProb* __restrict__ bp, int nprob, int nid,
+                                  double* smem, uint64_t* __restrict__ lcnt) {
   constexpr int NT = 4 * BMAX, NB = 8, RPL = (BMAX + 63) / 64;
   constexpr int PACK = BMAX * (BMAX + 1) / 2;
-  __shared__ double V2p[PACK], Tp[PACK];   // packed upper triangles of the node's V2 and T
-  __shared__ double Cb[BMAX * NB];         // bottom rows of the block, row-major [r][8]
-  __shared__ double Xs[BMAX * NB];         // S
-  __shared__ double Xg[BMAX * NB];         // V2ᵀ·V2_blk, then U
-  __shared__ double Ctd[NB * NB];          // top 8×8 diagonal block of the input, row-major
-  __shared__ double stau[BMAX];
+  double* V2p = smem;                      // packed upper triangles of the node's V2 and T
+  double* Tp = V2p + PACK;
+  double* Cb = Tp + PACK;                  // bottom rows of the block, row-major [r][8]
+  double* Xs = Cb + BMAX * NB;             // S
+  double* Xg = Xs + BMAX * NB;             // V2ᵀ·V2_blk, then U
+  double* Ctd = Xg + BMAX * NB;            // top 8×8 diagonal block of the input, row-major
+  double* stau = Ctd + NB * NB;
   __shared__ int s_err;
 
-  int nid = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
   const int64_t bb = (int64_t)b * b;
   if (bp) {
@@ -417,8 +427,16 @@ btree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, doubl
     const int c0 = k * NB, cp = c0 - NB;
     if (tid == 0) {
       const uint64_t want = epoch * 256 + (uint64_t)(c0 + NB);
-      if (ptop >= 0 && !flag_wait(cnt + ptop, want)) s_err = 1;
-      if (pbot >= 0 && !flag_wait(cnt + pbot, want)) s_err = 1;
+      if (ptop >= 0) {
+        if (!flag_wait(cnt + ptop, want)) s_err = 1;
+      } else if (lcnt && !flag_wait(lcnt + z, want)) {
+        s_err = 1;
+      }
+      if (pbot >= 0) {
+        if (!flag_wait(cnt + pbot, want)) s_err = 1;
+      } else if (lcnt && !flag_wait(lcnt + snd, want)) {
+        s_err = 1;
+      }
     }
     __syncthreads();                                                       // B0
     if (s_err) break;
@@ -670,11 +688,47 @@ btree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, doubl
 }
 
 template <int BMAX>
+__global__ void __launch_bounds__(4 * BMAX)
+btree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, double* __restrict__ V2out,
+             double* __restrict__ Tout, double* __restrict__ tauout, uint64_t* __restrict__ cnt,
+             uint64_t epoch, int L, int b, int* __restrict__ err, const BatchProb* __restrict__ bp, int nprob) {
+  extern __shared__ double smem[];
+  btree_body<BMAX>(leafR, nodeR, V2out, Tout, tauout, cnt, epoch, L, b, err, bp, nprob, blockIdx.x, smem, nullptr);
+}
+
+// Leaves and tree in one launch (b in (64, 128], leaves of at most 256
+// rows): workgroups [0, L) factor the leaves and publish their R eight
+// columns at a time, workgroups [L, L + nnodes) are the tree nodes, which
+// start on a leaf's first columns while the leaf is still running.  Block
+// ids order every producer before its consumers, so dispatch order alone
+// guarantees progress.
+template <int RPL>
+__global__ void __launch_bounds__(512)
+leaf_tree_kernel(double* __restrict__ A, int64_t lda, int64_t m, int L, int b, double* __restrict__ leafR,
+                 double* __restrict__ leafT, double* __restrict__ leafTau, double* __restrict__ nodeR,
+                 double* __restrict__ V2out, double* __restrict__ Tout, double* __restrict__ tauout,
+                 uint64_t* __restrict__ cnt, uint64_t* __restrict__ lcnt, uint64_t epoch, int* __restrict__ err) {
+  extern __shared__ double smem[];
+  if ((int)blockIdx.x < L)
+    leaf_qr_body<8, 16, RPL>(A, lda, m, L, b, leafR, leafT, leafTau, nullptr, 0, blockIdx.x, smem, lcnt, epoch);
+  else
+    btree_body<128>(leafR, nodeR, V2out, Tout, tauout, cnt, epoch, L, b, err, nullptr, 0, blockIdx.x - L, smem,
+                    lcnt);
+}
+
+template <int BMAX>
 static int btree_launch(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
                         uint64_t* cnt, int L, int nnodes, int b, uint64_t epoch, const BatchProb* bp, int nprob) {
+  const size_t lds = sizeof(double) * btree_smem_doubles<BMAX>();
+  auto k = btree_kernel<BMAX>;
+  static bool attr = false;
+  if (!attr) {
+    CQR_HIP(c, hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr = true;
+  }
   hipEvent_t ev = prof_begin(c);
-  hipLaunchKernelGGL((btree_kernel<BMAX>), dim3(nnodes), dim3(4 * BMAX), 0, c->stream, leafR, nodeR, V2, T, tau,
-                     cnt, epoch, L, b, c->d_terr, bp, nprob);
+  hipLaunchKernelGGL(k, dim3(nnodes), dim3(4 * BMAX), lds, c->stream, leafR, nodeR, V2, T, tau, cnt, epoch, L, b,
+                     c->d_terr, bp, nprob);
   prof_end(c, ev, PK_TREE, nnodes * (2.0 / 3.0) * b * (double)b * b, nnodes * 40.0 * b * b);
   return check_launch(c, "btree");
 }
@@ -700,6 +754,43 @@ static int ptree_dispatch(cqr_ctx* c, const double* leafR, double* nodeR, double
   if (b <= 32) return ptree_launch<4, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
   if (b <= 64) return ptree_launch<8, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
   return ptree_launch<16, 2>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
+}
+
+template <int RPL>
+static int leaf_tree_launch(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int b, double* leafR,
+                            double* leafT, double* leafTau, double* nodeR, double* V2, double* T, double* tau,
+                            int nnodes) {
+  const size_t lds = sizeof(double) * std::max<size_t>(btree_smem_doubles<128>(),
+                                                      (size_t)b * b + 2 * 64 * RPL + 128);
+  auto k = leaf_tree_kernel<RPL>;
+  static bool attr = false;
+  if (!attr) {
+    CQR_HIP(c, hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr = true;
+  }
+  const uint64_t epoch = ++c->tree_epoch;
+  hipEvent_t ev = prof_begin(c);
+  hipLaunchKernelGGL(k, dim3(L + nnodes), dim3(512), lds, c->stream, A, lda, m, L, b, leafR, leafT, leafTau, nodeR,
+                     V2, T, tau, c->d_cnt, c->d_cnt + CQR_MAX_TREE_NODES, epoch, c->d_terr);
+  prof_end(c, ev, PK_TREE,
+           2.0 * m * b * b - (2.0 / 3.0) * L * (double)b * b * b + nnodes * (2.0 / 3.0) * b * (double)b * b,
+           16.0 * m * b + nnodes * 40.0 * b * b);
+  return check_launch(c, "leaf_tree");
+}
+
+// Leaf QR of every row block, then the reduction tree; one launch when the
+// shapes allow the fused kernel, else the two kernels in stream order.
+int leaf_tree(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int b, double* leafR, double* leafT,
+              double* leafTau, double* nodeR, double* V2, double* T, double* tau, int nnodes) {
+  const int64_t rows = (m + L - 1) / L;
+  const bool fuse = nnodes > 0 && b > 64 && b <= 128 && b % 8 == 0 && rows <= 256 && L <= CQR_MAX_TREE_NODES &&
+                    nnodes <= CQR_MAX_TREE_NODES;
+  if (!fuse) {
+    CQR_TRY(leaf_qr(c, A, lda, m, L, b, leafR, leafT, leafTau));
+    return launch_ptree(c, leafR, nodeR, V2, T, tau, L, nnodes, b);
+  }
+  if (rows <= 128) return leaf_tree_launch<2>(c, A, lda, m, L, b, leafR, leafT, leafTau, nodeR, V2, T, tau, nnodes);
+  return leaf_tree_launch<4>(c, A, lda, m, L, b, leafR, leafT, leafTau, nodeR, V2, T, tau, nnodes);
 }
 
 int launch_ptree(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,

@@ -111,10 +111,10 @@ int hh_recon_panel(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, dou
   double* leafT = at(w.leafT);
   double* nodeV2 = at(w.nodeV2);
   double* nodeT = at(w.nodeT);
-  // K1: leaves (bitree_tsqr.cxx:156)
-  CQR_TRY(leaf_qr(c, A, lda, m, L, bi, leafR, leafT, at(w.leafTau)));
-  // K2: the reduction tree, all levels pipelined in one launch (bitree_tsqr.cxx:172-232)
-  CQR_TRY(launch_ptree(c, leafR, at(w.nodeR), nodeV2, nodeT, at(w.nodeTau), L, (int)p.nnodes, bi));
+  // K1 + K2: leaves (bitree_tsqr.cxx:156) and the reduction tree, all levels
+  // pipelined (bitree_tsqr.cxx:172-232), fused into one launch where possible
+  CQR_TRY(leaf_tree(c, A, lda, m, L, bi, leafR, leafT, at(w.leafTau), at(w.nodeR), nodeV2, nodeT, at(w.nodeTau),
+                    (int)p.nnodes));
   const double* Rroot = p.nnodes > 0 ? at(w.nodeR) + (p.nnodes - 1) * bb : leafR;
   // K3: construct_Q1 (construct_tsqr_Q.cxx:34-242) into the Y scratch
   double* Y = at(w.Y);
@@ -264,8 +264,7 @@ int cqr_bitree_tsqr(cqr_ctx* c, double* A, int64_t lda, double* R, double* tau, 
   double* nodeV2 = leafT + L * bb;
   double* nodeT = nodeV2 + nn * bb;
   double* nodeTau = nodeT + nn * bb;
-  CQR_TRY(leaf_qr(c, A, lda, m, (int)L, (int)b, leafR, leafT, tau));
-  CQR_TRY(launch_ptree(c, leafR, nodeR, nodeV2, nodeT, nodeTau, (int)L, (int)nn, (int)b));
+  CQR_TRY(leaf_tree(c, A, lda, m, (int)L, (int)b, leafR, leafT, tau, nodeR, nodeV2, nodeT, nodeTau, (int)nn));
   return launch_copy2d(c, nn > 0 ? nodeR + (nn - 1) * bb : leafR, b, R, b, b, b);
 }
 
