@@ -315,117 +315,201 @@ __global__ void __launch_bounds__(256) bgemm_kernel(BgemmArgs g) {
 }
 
 // ---------------------------------------------------------------------------
-// Signed LU of a b×b block in LDS (alg/QR/hh_recon/hh_recon.cxx:43-70):
-// s_i = −1 if the pivot is positive else +1, pivot −= s_i, column scaled by
-// the pivot (division), rank-1 update of the trailing block.
-// ---------------------------------------------------------------------------
 // Signed LU of a b×b block (alg/QR/hh_recon/hh_recon.cxx:43-70), b ≤ 128,
-// 512 threads: the block lives in registers (wave w owns columns
-// w, w+8, ..., lane l rows l and l+64).  Step i: the owner wave of column i
-// picks s_i (−1 if the pivot is positive, else +1), subtracts it, divides
-// the column below the pivot by the pivot and publishes it through LDS; one
-// barrier later every wave applies the rank-1 update to its columns, the
-// pivot-row entries coming from lane i%64 by readlane.  On exit the L\U
-// factors are in M (LDS, ld b) and the signs in sgn.
-__device__ void signed_lu_regs(const double* __restrict__ src, int64_t ld, double* M, double* sgn,
-                               double* lbuf, int b) {
-  constexpr int WAVES = 8, CPW = 16, RPL = 2;
-  const int lane = threadIdx.x & 63, w = wave_id();
-  double a[CPW][RPL];
+// 512 threads, in LDS (M, ld b).  Step i picks s_i (−1 if the pivot is
+// positive, else +1), subtracts it from the pivot, scales the column below
+// the pivot by its reciprocal (|pivot| ≥ 1) and updates the later columns.
+// Blocked by eight: one wave factors the block's columns from registers
+// (pivot-row entries by readlane, no barrier); one thread per later column
+// solves its eight block rows with the unit-lower diagonal block (U12);
+// the rank-8 update of the trailing matrix runs as FP64 MFMA tiles over all
+// waves.  On exit M holds L\U and sgn the signs.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void signed_lu_lds(const double* __restrict__ src, int64_t ld, double* M, int ldm, double* sgn, int b) {
+  constexpr int NT = 512, NB = 8, RPL = 2;
+  const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
+  {
+    // all 32 loads of a thread in flight at once (clamped addresses, then masked stores)
+    double v[16][2];
 #pragma unroll
-  for (int cl = 0; cl < CPW; ++cl) {
-    const int col = w + WAVES * cl;
+    for (int j = 0; j < 16; ++j)
 #pragma unroll
-    for (int s = 0; s < RPL; ++s) {
-      const int row = lane + 64 * s;
-      a[cl][s] = (col < b && row < b) ? src[row + (int64_t)col * ld] : 0.0;
-    }
+      for (int s = 0; s < 2; ++s) {
+        const int cc = w + 8 * j, rr = lane + 64 * s;
+        v[j][s] = src[(cc < b && rr < b) ? rr + (int64_t)cc * ld : 0];
+      }
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int cc = w + 8 * j, rr = lane + 64 * s;
+        if (cc < b && rr < b) M[rr + cc * ldm] = v[j][s];
+      }
   }
-  for (int i = 0; i < b; ++i) {
-    const int ow = i % WAVES, ocl = i / WAVES, plane = i & 63, pslot = i >> 6;
-    double* lb = lbuf + (i & 1) * 128;
-    if (w == ow) {
+  __syncthreads();
+  for (int c0 = 0; c0 < b; c0 += NB) {
+    const int nb = min(NB, b - c0);
+    if (w == 0) {
+      double pn[RPL][NB];
 #pragma unroll
-      for (int cl = 0; cl < CPW; ++cl) {
-        if (cl != ocl) continue;
-        const double pv = readlane_d(pslot ? a[cl][1] : a[cl][0], plane);
+      for (int s = 0; s < RPL; ++s) {
+        const int row = lane + 64 * s;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) pn[s][j] = (row >= c0 && row < b && j < nb) ? M[row + (c0 + j) * ldm] : 0.0;
+      }
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj) {
+        if (jj < nb) {
+        const int i = c0 + jj, hi = i >> 6, pl = i & 63;
+        const double pv = readlane_d(hi ? pn[1][jj] : pn[0][jj], pl);
         const double sg = pv > 0.0 ? -1.0 : 1.0;
         const double piv = pv - sg;
+        const double rp = 1.0 / piv;
 #pragma unroll
         for (int s = 0; s < RPL; ++s) {
           const int row = lane + 64 * s;
-          if (row > i) a[cl][s] = a[cl][s] / piv;
-          if (row == i) a[cl][s] = piv;
-          lb[row] = row > i ? a[cl][s] : 0.0;
+          if (row > i) pn[s][jj] *= rp;
+          if (row == i) pn[s][jj] = piv;
         }
         if (lane == 0) sgn[i] = sg;
+#pragma unroll
+        for (int j2 = jj + 1; j2 < NB; ++j2) {
+          if (j2 < nb) {
+            const double u = readlane_d(hi ? pn[1][j2] : pn[0][j2], pl);
+#pragma unroll
+            for (int s = 0; s < RPL; ++s)
+              if (lane + 64 * s > i) pn[s][j2] -= pn[s][jj] * u;
+          }
+        }
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < RPL; ++s) {
+        const int row = lane + 64 * s;
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+          if (row >= c0 && row < b && j < nb) M[row + (c0 + j) * ldm] = pn[s][j];
       }
     }
     __syncthreads();
-    double l[RPL];
+    const int j0 = c0 + nb;
+    if (j0 >= b) break;
+    // U12 = L11⁻¹·A12, one thread per later column
+    for (int j = j0 + tid; j < b; j += NT) {
+      double u[NB];
 #pragma unroll
-    for (int s = 0; s < RPL; ++s) l[s] = lb[lane + 64 * s];
+      for (int jj = 0; jj < NB; ++jj) u[jj] = jj < nb ? M[c0 + jj + j * ldm] : 0.0;
 #pragma unroll
-    for (int cl = 0; cl < CPW; ++cl) {
-      const int col = w + WAVES * cl;
-      if (col > i && col < b) {
-        const double u = readlane_d(pslot ? a[cl][1] : a[cl][0], plane);
+      for (int jj = 1; jj < NB; ++jj)
 #pragma unroll
-        for (int s = 0; s < RPL; ++s) a[cl][s] -= l[s] * u;
+        for (int p = 0; p < jj; ++p)
+          if (jj < nb) u[jj] -= M[c0 + jj + (c0 + p) * ldm] * u[p];
+#pragma unroll
+      for (int jj = 1; jj < NB; ++jj)
+        if (jj < nb) M[c0 + jj + j * ldm] = u[jj];
+    }
+    __syncthreads();
+    // A22 −= L21·U12 on 16×16 MFMA tiles (K = 8)
+    const int nt = (b - j0 + 15) / 16;
+    auto tile = [&](int t) {
+      const int r0 = j0 + 16 * (t % nt), q0 = j0 + 16 * (t / nt);
+      auto x = [&](int m, int k) {
+        const int r = r0 + m;
+        const bool ok = r < b && k < nb;
+        return ok ? M[r + (c0 + k) * ldm] : 0.0;
+      };
+      auto y = [&](int k, int n) {
+        const int q = q0 + n;
+        const bool ok = q < b && k < nb;
+        return ok ? M[c0 + k + q * ldm] : 0.0;
+      };
+      const dbl4 acc = mfma_tile16(x, y, 0, NB, lane);
+      const int r = r0 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int q = q0 + (lane >> 4) + 4 * v;
+        if (r < b && q < b) M[r + q * ldm] -= acc[v];
       }
+    };
+    // two tiles per wave in flight
+    for (int t = w; t < nt * nt; t += 2 * NT / 64) {
+      tile(t);
+      if (t + NT / 64 < nt * nt) tile(t + NT / 64);
     }
-  }
-#pragma unroll
-  for (int cl = 0; cl < CPW; ++cl) {
-    const int col = w + WAVES * cl;
-    if (col >= b) continue;
-#pragma unroll
-    for (int s = 0; s < RPL; ++s) {
-      const int row = lane + 64 * s;
-      if (row < b) M[row + col * b] = a[cl][s];
-    }
+    __syncthreads();
   }
   __syncthreads();
 }
 
-// K4: Householder reconstruction core (one workgroup).
-__device__ void hr_core_body(const double* __restrict__ Q1top, int64_t ldq, const double* __restrict__ Rroot,
+// K4: Householder reconstruction core (one workgroup).  M is kept in LDS
+// with leading dimension b + 1.
+__device__ __forceinline__ void hr_core_body(const double* __restrict__ Q1top, int64_t ldq, const double* __restrict__ Rroot,
                              double* __restrict__ Atop, int64_t lda, double* __restrict__ W,
                              double* __restrict__ Uinv, double* __restrict__ Winv, double* __restrict__ LinvT,
                              int64_t* __restrict__ signs, int b) {
   constexpr int NT = 512;
   extern __shared__ double smem[];
-  double* M = smem;            // b×b
-  double* sgn = M + b * b;     // b
+  const int ldm = b + 1;
+  double* M = smem;            // b×b, ld b+1
+  double* sgn = M + b * ldm;   // b
   double* d = sgn + b;         // b
-  double* lbuf = d + b;        // 2 × 128
-  const int tid = threadIdx.x;
-  signed_lu_regs(Q1top, ldq, M, sgn, lbuf, b);
+  const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
+  signed_lu_lds(Q1top, ldq, M, ldm, sgn, b);
   // outputs: L below the diagonal of Atop, S·R on and above it, W = U·S
-  FOR_RC(rr, cc, b, NT) {
-    const int e = rr + cc * b;
-    if (rr > cc) Atop[rr + cc * lda] = M[e];
-    else Atop[rr + cc * lda] = sgn[rr] * Rroot[e];
-    W[e] = rr <= cc ? M[e] * sgn[cc] : 0.0;
+  {
+    double rv[16][2];
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int cc = w + 8 * j, rr = lane + 64 * s;
+        rv[j][s] = Rroot[(cc < b && rr < b) ? rr + cc * b : 0];
+      }
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int cc = w + 8 * j, rr = lane + 64 * s;
+        if (cc < b && rr < b) {
+          const double mv = M[rr + cc * ldm];
+          Atop[rr + cc * lda] = rr > cc ? mv : sgn[rr] * rv[j][s];
+          W[rr + cc * b] = rr <= cc ? mv * sgn[cc] : 0.0;
+        }
+      }
   }
   if (signs)
     for (int p = tid; p < b; p += NT) signs[p] = (int64_t)sgn[p];
-  for (int p = tid; p < b; p += NT) d[p] = 1.0 / M[p + p * b];
+  for (int p = tid; p < b; p += NT) d[p] = 1.0 / M[p + p * ldm];
   __syncthreads();
-  tri_inv_upper_lds<NT>(M, b, d, b);
+  tri_inv_upper_lds<NT>(M, ldm, d, b);
   FOR_RC(rr, cc, b, NT) {
-    const int e = rr + cc * b;
-    const double u = rr <= cc ? M[e] : 0.0;
-    Uinv[e] = u;
-    Winv[e] = sgn[rr] * u;        // W⁻¹ = S·U⁻¹
+    const double u = rr <= cc ? M[rr + cc * ldm] : 0.0;
+    Uinv[rr + cc * b] = u;
+    Winv[rr + cc * b] = sgn[rr] * u;        // W⁻¹ = S·U⁻¹
   }
   __syncthreads();
-  // L⁻ᵀ: load Lᵀ (unit upper) and invert
-  FOR_RC(rr, cc, b, NT) M[rr + cc * b] = rr < cc ? Atop[cc + rr * lda] : 0.0;
+  // L⁻ᵀ: load Lᵀ (unit upper) from Atop and invert
+  {
+    double lv[16][2];
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int cc = w + 8 * j, rr = lane + 64 * s;
+        lv[j][s] = Atop[(cc < b && rr < cc) ? cc + rr * lda : 0];
+      }
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int cc = w + 8 * j, rr = lane + 64 * s;
+        if (cc < b && rr < b) M[rr + cc * ldm] = rr < cc ? lv[j][s] : 0.0;
+      }
+  }
   for (int p = tid; p < b; p += NT) d[p] = 1.0;
   __syncthreads();
-  tri_inv_upper_lds<NT>(M, b, d, b);
-  FOR_RC(rr, cc, b, NT) LinvT[rr + cc * b] = rr <= cc ? M[rr + cc * b] : 0.0;
+  tri_inv_upper_lds<NT>(M, ldm, d, b);
+  FOR_RC(rr, cc, b, NT) LinvT[rr + cc * b] = rr <= cc ? M[rr + cc * ldm] : 0.0;
 }
 
 __global__ void __launch_bounds__(512)
@@ -455,13 +539,13 @@ signed_nlu_kernel(double* __restrict__ A, int64_t lda, double* __restrict__ R, i
                   int64_t* __restrict__ signs, int b) {
   constexpr int NT = 512;
   extern __shared__ double smem[];
+  const int ldm = b + 1;
   double* M = smem;
-  double* sgn = M + b * b;
-  double* lbuf = sgn + b;
+  double* sgn = M + b * ldm;
   const int tid = threadIdx.x;
-  signed_lu_regs(A, lda, M, sgn, lbuf, b);
+  signed_lu_lds(A, lda, M, ldm, sgn, b);
   FOR_RC(rr, cc, b, NT) {
-    A[rr + cc * lda] = M[rr + cc * b];
+    A[rr + cc * lda] = M[rr + cc * ldm];
     if (R && rr <= cc && sgn[rr] < 0) R[rr + cc * ldr] = -R[rr + cc * ldr];
   }
   if (signs)
@@ -712,7 +796,7 @@ int hr_core(cqr_ctx* c, const double* Q1top, int64_t ldq, const double* Rroot, d
             int64_t lda, double* W, double* Uinv, double* Winv, double* LinvT, int64_t* signs, int b) {
   static bool attr = false;
   if (!attr) { CQR_TRY(big_lds_attr(c, (const void*)hr_core_kernel)); attr = true; }
-  const size_t lds = sizeof(double) * ((size_t)b * b + 2 * b + 256);
+  const size_t lds = sizeof(double) * ((size_t)b * (b + 1) + 2 * b);
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(hr_core_kernel, dim3(1), dim3(512), lds, c->stream, Q1top, ldq, Rroot, Atop,
                      lda, W, Uinv, Winv, LinvT, signs, b);
@@ -724,7 +808,7 @@ int hr_core_batched(cqr_ctx* c, const BatchProb* bp, int nprob, const double* Y,
                     const double* nodeR, double* Abase, double* W, double* Uinv, double* Winv, double* LinvT, int b) {
   static bool attr = false;
   if (!attr) { CQR_TRY(big_lds_attr(c, (const void*)hr_core_batched_kernel)); attr = true; }
-  const size_t lds = sizeof(double) * ((size_t)b * b + 2 * b + 256);
+  const size_t lds = sizeof(double) * ((size_t)b * (b + 1) + 2 * b);
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(hr_core_batched_kernel, dim3(nprob), dim3(512), lds, c->stream, bp, Y, leafR, nodeR, Abase, W,
                      Uinv, Winv, LinvT, b);
@@ -735,7 +819,7 @@ int hr_core_batched(cqr_ctx* c, const BatchProb* bp, int nprob, const double* Y,
 int launch_signed_nlu(cqr_ctx* c, double* A, int64_t lda, double* R, int64_t lda_R, int64_t* signs, int b) {
   static bool attr = false;
   if (!attr) { CQR_TRY(big_lds_attr(c, (const void*)signed_nlu_kernel)); attr = true; }
-  const size_t lds = sizeof(double) * ((size_t)b * b + b + 256);
+  const size_t lds = sizeof(double) * ((size_t)b * (b + 1) + b);
   hipLaunchKernelGGL(signed_nlu_kernel, dim3(1), dim3(512), lds, c->stream, A, lda, R, lda_R, signs, b);
   return check_launch(c, "signed_nlu");
 }
