@@ -20,9 +20,9 @@ __device__ __forceinline__ int64_t leaf_off(int64_t l, int64_t m, int64_t L) { r
 //
 // With `lcnt` set, the leaf also publishes its R while it runs: column i of
 // R (rows ≤ i) is final once its reflector exists, so the owner wave stores
-// it at once with device-coherent stores, and after every eighth column
-// lcnt[leaf] = epoch·256 + (columns done) tells a consumer in the same
-// launch that those columns may be read.
+// it at once with device-coherent stores; after every eighth column all
+// waves wait for their stores and lcnt[leaf] = epoch·256 + (columns done)
+// tells a consumer in the same launch that those columns may be read.
 template <int WAVES, int CPW, int RPL>
 __device__ __forceinline__ void leaf_qr_body(double* __restrict__ A, int64_t lda, int64_t m, int L, int b,
                                     double* __restrict__ Rout, double* __restrict__ Tout,
@@ -124,10 +124,12 @@ __device__ __forceinline__ void leaf_qr_body(double* __restrict__ A, int64_t lda
             const int row = lane + 64 * s;
             if (row <= i) store_coherent(Rl + row + (int64_t)i * b, a[cl][s]);
           }
-          wait_memory();
         }
       }
     }
+    // before a block of eight columns is published, every owner's R stores
+    // have completed
+    if (lcnt && ((i & 7) == 7 || i + 1 == b)) wait_memory();
     __syncthreads();
     if (lcnt && tid == 0 && ((i & 7) == 7 || i + 1 == b)) flag_store(lcnt + gleaf, epoch * 256 + (uint64_t)(i + 1));
     const double tau = stau[i];
