@@ -177,6 +177,7 @@ int cqr_destroy(cqr_ctx* c) {
     (void)hipEventDestroy(r.b);
   }
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
+  cqr::blas_destroy(c);
   if (c->ws) (void)hipFree(c->ws);
   if (c->d_flag) (void)hipFree(c->d_flag);
   if (c->d_cnt) (void)hipFree(c->d_cnt);
@@ -213,6 +214,12 @@ int cqr_sync(cqr_ctx* c) {
 int cqr_set_lookahead(cqr_ctx* c, int on) {
   if (!c) return CQR_ERR_INVALID;
   c->lookahead = on ? 1 : 0;
+  return CQR_OK;
+}
+
+int cqr_set_blas_update(cqr_ctx* c, int on) {
+  if (!c) return CQR_ERR_INVALID;
+  c->blas_update = on ? 1 : 0;
   return CQR_OK;
 }
 

@@ -40,12 +40,15 @@ struct cqr_ctx {
   uint64_t q1_epoch = 0;
   uint64_t tree_epoch = 0;      // launch epoch folded into the counters
   hipStream_t side = nullptr;   // second stream of the QR_2D lookahead
+  int blas_update = 1;          // bulk of A −= Y·Z' through the vendor BLAS
+  void* blas = nullptr;         // its handle, created on first use
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int lookahead = 1;
   std::string err;
 };
 
 namespace cqr {
+void blas_destroy(cqr_ctx* c);
 
 // ---------------------------------------------------------------------------
 // Errors

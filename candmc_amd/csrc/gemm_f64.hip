@@ -459,6 +459,8 @@ size_t wy_update_ws_doubles(int64_t m, int64_t n, int b) {
 int lean_gemm(cqr_ctx* c, int ak, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
               const double* B, int64_t ldb, double beta, double* C, int64_t ldc, double* scratch, int nsplit,
               int kind, int* used);
+int blas_update(cqr_ctx* c, int64_t m, int64_t n, int64_t k, const double* Y, int64_t ldy, const double* Z,
+                int64_t ldz, double* A, int64_t lda);
 
 // FP64 update with the unmasked bulk on the lean kernel: rows [0, bt) of Y
 // (its unit-lower top block) go through the masked general kernel, rows
@@ -488,6 +490,7 @@ static int wy_update_f64_split(cqr_ctx* c, const double* Y, int64_t ldy, int smY
   CQR_TRY(gemm_f64_impl(c, t_trans ? 1 : 0, 0, b, n, b, 1.0, Tt, b, SM_LOWER, Z, b, SM_NONE, 0.0, Zp, b, nullptr));
   // A(0:bt) −= Y(0:bt)·Z' (masked), A(bt:m) −= Y(bt:m)·Z' (lean)
   CQR_TRY(gemm_f64_impl(c, 0, 0, bt, n, b, -1.0, Y, ldy, smY, Zp, b, SM_NONE, 1.0, A, lda, nullptr, PK_GEMM_UPD));
+  if (c->blas_update) return blas_update(c, m - bt, n, b, Y + bt, ldy, Zp, b, A + bt, lda);
   return lean_gemm(c, 0, m - bt, n, b, -1.0, Y + bt, ldy, Zp, b, 1.0, A + bt, lda, nullptr, 1, PK_GEMM_UPD, nullptr);
 }
 

@@ -57,6 +57,11 @@ class Context:
     def set_lookahead(self, on: bool):
         self.check(self.lib.cqr_set_lookahead(self.h, 1 if on else 0))
 
+    def set_blas_update(self, on):
+        """Bulk of the FP64 trailing update through rocBLAS dgemm (True, the
+        default) or the library's own MFMA kernel (False)."""
+        self.check(self.lib.cqr_set_blas_update(self.h, 1 if on else 0))
+
     def set_check_finite(self, on: bool):
         self.check(self.lib.cqr_set_check_finite(self.h, 1 if on else 0))
 

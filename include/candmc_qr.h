@@ -62,6 +62,12 @@ int cqr_set_leaf_rows(cqr_ctx* ctx, int64_t rows);    /* 0 = automatic        */
  * alg/QR/qr_2d/qr_2d.cxx:378-635, QR_2D_pipe).  Results differ from the
  * non-lookahead order only in the summation order of Yᵀ·A.               */
 int cqr_set_lookahead(cqr_ctx* ctx, int on);
+/* The unmasked bulk of the FP64 trailing update A −= Y·(T·Yᵀ·A), a plain
+ * GEMM once Z' = T·Yᵀ·A is formed, runs through the vendor BLAS (rocBLAS
+ * dgemm) when on (the default), else through the library's own MFMA kernel.
+ * The structured parts (unit-lower Y block, triangular T) always use the
+ * library's kernels.  No counterpart in the reference (a tuning switch).  */
+int cqr_set_blas_update(cqr_ctx* ctx, int on);
 /* Kernel timing: while enabled, the library records a HIP event pair on
  * its stream around every kernel it launches; cqr_profile_read waits for
  * the stream and returns, for one kernel class, the summed event time,

@@ -277,6 +277,27 @@ def test_qr_2d_lookahead_on_and_off(Q, m, k, b):
     assert rel(out[0], out[1]) < 1e-12
 
 
+@pytest.mark.parametrize("m,k,b", [(2048, 1024, 128), (1500, 900, 64)])
+def test_qr_2d_update_through_blas_or_own_kernel(Q, m, k, b):
+    """The bulk of the trailing update through rocBLAS dgemm or through the
+    library's MFMA kernel: the same factorization up to round-off."""
+    A = O.gaussian(m, k, 3 * m + k)
+    want = O.QR_2D(A, b)
+    ctx = Q.context()
+    out = []
+    for on in (False, True):
+        ctx.set_blas_update(on)
+        try:
+            dA = Q.to_gpu(A)
+            Q.QR_2D(dA, m, m, k, b)
+            out.append(Q.from_gpu(dA, m, k))
+        finally:
+            ctx.set_blas_update(True)
+    for YR in out:
+        assert rel(YR, want) < 1e-11
+    assert rel(out[0], out[1]) < 1e-12
+
+
 def test_qr_2d_reference_default_input(Q):
     g = gold("qr2d_96x48.npz")                                 # singular: residual only
     dA = Q.to_gpu(g["A"])
