@@ -345,7 +345,7 @@ def main():
     # HBM bytes per launch from the committed counter runs (FETCH_SIZE and
     # WRITE_SIZE, calibrated on a 1 GiB copy) when they cover this kernel
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "qr2d_32768x4096_b128_update_traffic.json")
+    tf = os.path.join(ROOT, "profiles", f"qr2d_32768x4096_b128_{dom}_traffic.json")
     if os.path.exists(tf) and (m, n, b) == (32768, 4096, 128):
         with open(tf) as fh:
             tj = json.load(fh)
