@@ -201,8 +201,6 @@ int tree_level(cqr_ctx* c, double* Rleaf, int n, int node0, int b, double* V2, d
 int bgemm(cqr_ctx* c, const BgemmArgs& g, int batch, int max_m);
 int hr_core(cqr_ctx* c, const double* Q1top, int64_t ldq, const double* Rroot, double* Atop,
             int64_t lda, double* W, double* Uinv, double* Winv, double* LinvT, int64_t* signs, int b);
-int init_q1(cqr_ctx* c, double* Y, int64_t ldy, int64_t m, int L, int b, const double* C);
-int set_identity(cqr_ctx* c, double* C, int b);
 int launch_diag(cqr_ctx* c, const double* T, int64_t ldt, double* tau, int b);
 int launch_signed_nlu(cqr_ctx* c, double* A, int64_t lda, double* R, int64_t lda_R,
                       int64_t* signs, int b);
@@ -226,12 +224,6 @@ enum { SM_NONE = 0, SM_UNIT_LOWER = 1, SM_UPPER = 2, SM_LOWER = 3, SM_IDENTITY =
 int gemm_f64(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha,
              const double* A, int64_t lda, int smA, const double* B, int64_t ldb,
              double beta, double* C, int64_t ldc);
-// Batched MFMA GEMM: operands addressed per batch entry z as in BMat; m < 0
-// means "rows of leaf z" (max_m bounds them).  smC = SM_UPPER zeroes the
-// strictly lower part of each C.
-int gemm_f64_batched(cqr_ctx* c, int ta, int tb, int m, int n, int k, double alpha, const BMat& A, int smA,
-                     const BMat& B, int smB, double beta, const BMat& C, int smC, int batch, int64_t pm, int pL,
-                     int zoff, int max_m, int kind);
 // Same with an explicit split-K scratch (nullptr: no split) and a mask on B.
 int gemm_f64_ws(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha,
                 const double* A, int64_t lda, int smA, const double* B, int64_t ldb, int smB,

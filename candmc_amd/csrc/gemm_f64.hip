@@ -359,6 +359,65 @@ static int choose_splits(int64_t m, int64_t n, int64_t k, int64_t* kchunk) {
   return (int)s;
 }
 
+// Small products (m, n, k ≤ 256: the b×b triangles of T, W, L⁻ᵀ and the
+// b-wide Z' = Tᵀ·Z): one wave per 16×16 output tile, operands read straight
+// from memory (they sit in L2) with the structural masks applied per
+// element on the stored coordinates, FP64 MFMA with four accumulator
+// chains.  A 128×128×128 product is 64 single-wave workgroups instead of
+// one 256-thread tile walking eight LDS slabs.
+struct SmallArgs {
+  int m, n, k, ta, tb, smA, smB;
+  double alpha, beta;
+  const double* A;
+  int64_t lda;
+  const double* B;
+  int64_t ldb;
+  double* C;
+  int64_t ldc;
+};
+
+__global__ void __launch_bounds__(64) small_gemm_kernel(SmallArgs g) {
+  const int lane = threadIdx.x;
+  const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
+  auto x = [&](int r, int kk) {   // op(A)(i0 + r, kk)
+    const int i = i0 + r;
+    const bool ok = i < g.m && kk < g.k;
+    const int64_t sr = g.ta ? kk : i, sc = g.ta ? i : kk;
+    const double v = g.A[ok ? sr + sc * g.lda : 0];
+    return ok ? gmask(v, sr, sc, g.smA) : 0.0;
+  };
+  auto y = [&](int kk, int cc) {  // op(B)(kk, j0 + cc)
+    const int j = j0 + cc;
+    const bool ok = j < g.n && kk < g.k;
+    const int64_t sr = g.tb ? j : kk, sc = g.tb ? kk : j;
+    const double v = g.B[ok ? sr + sc * g.ldb : 0];
+    return ok ? gmask(v, sr, sc, g.smB) : 0.0;
+  };
+  const dbl4 acc = mfma_tile16_acc4(x, y, 0, g.k, lane);
+  const int i = i0 + (lane & 15);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int j = j0 + (lane >> 4) + 4 * v;
+    if (i < g.m && j < g.n) {
+      double* cp = g.C + i + (int64_t)j * g.ldc;
+      *cp = g.alpha * acc[v] + (g.beta != 0.0 ? g.beta * *cp : 0.0);
+    }
+  }
+}
+
+static int launch_small(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha,
+                        const double* A, int64_t lda, int smA, const double* B, int64_t ldb, int smB, double beta,
+                        double* C, int64_t ldc, int kind) {
+  SmallArgs g{};
+  g.m = (int)m; g.n = (int)n; g.k = (int)k; g.ta = ta; g.tb = tb; g.smA = smA; g.smB = smB;
+  g.alpha = alpha; g.beta = beta; g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  hipEvent_t ev = prof_begin(c);
+  hipLaunchKernelGGL(small_gemm_kernel, dim3((unsigned)((m + 15) / 16), (unsigned)((n + 15) / 16)), dim3(64), 0,
+                     c->stream, g);
+  prof_end(c, ev, kind, 2.0 * m * n * k, 8.0 * ((double)m * k + (double)k * n + 2.0 * m * n));
+  return check_launch(c, "small_gemm");
+}
+
 // Generic C := alpha·op(A)·op(B) + beta·C; split-K through the workspace
 // when the output is small and K long.  `scratch` must hold nsplit·m·n
 // doubles when splitting (callers size it with gemm_f64_scratch).
@@ -367,6 +426,8 @@ static int gemm_f64_impl(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64
                          double beta, double* C, int64_t ldc, double* scratch, int kind = PK_GEMM_SMALL,
                          int prec = CQR_PREC_F64) {
   if (m <= 0 || n <= 0) return CQR_OK;
+  if (prec == CQR_PREC_F64 && m <= 256 && n <= 256 && k <= 256)
+    return launch_small(c, ta, tb, m, n, k, alpha, A, lda, smA, B, ldb, smB, beta, C, ldc, kind);
   GemmArgs g{};
   g.m = m; g.n = n; g.k = k; g.alpha = alpha; g.beta = beta;
   g.A = A; g.lda = lda; g.smA = smA; g.B = B; g.ldb = ldb; g.smB = smB;
@@ -384,26 +445,6 @@ static int gemm_f64_impl(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64
                      C, ldc, alpha, beta);
   prof_end(c, ev, PK_REDUCE, (double)ns * m * n, 8.0 * (ns + (beta != 0.0 ? 2.0 : 1.0)) * m * n);
   return check_launch(c, "splitk_reduce");
-}
-
-int gemm_f64_batched(cqr_ctx* c, int ta, int tb, int m, int n, int k, double alpha, const BMat& A, int smA,
-                     const BMat& B, int smB, double beta, const BMat& C, int smC, int batch, int64_t pm, int pL,
-                     int zoff, int max_m, int kind) {
-  if (batch <= 0 || n <= 0 || max_m <= 0) return CQR_OK;
-  GemmArgs g{};
-  g.m = m < 0 ? max_m : m; g.n = n; g.k = k; g.alpha = alpha; g.beta = beta;
-  g.A = A.p; g.lda = A.ld; g.smA = smA; g.B = B.p; g.ldb = B.ld; g.smB = smB; g.C = const_cast<double*>(C.p);
-  g.ldc = C.ld; g.kchunk = k; g.split_stride = 0;
-  g.batched = 1;
-  g.a_mode = A.mode; g.b_mode = B.mode; g.c_mode = C.mode;
-  g.a_zs = A.zs; g.b_zs = B.zs; g.c_zs = C.zs;
-  g.pm = pm; g.pL = pL; g.zoff = zoff; g.m_leaf = m < 0; g.smC = smC;
-  dim3 grid((unsigned)((max_m + 127) / 128), (unsigned)((n + 127) / 128), (unsigned)batch);
-  hipEvent_t ev = prof_begin(c);
-  launch_kind(c, ta, tb, grid, g, kind);
-  prof_end(c, ev, kind, 2.0 * batch * (double)max_m * n * k,
-           8.0 * batch * ((double)max_m * k + (double)k * n + 2.0 * max_m * n));
-  return check_launch(c, "mfma_gemm_f64_batched");
 }
 
 // Batched over the problems of a group: operand modes BM_PROB_* and

@@ -8,7 +8,6 @@
 namespace cqr {
 
 int launch_tril_halve(cqr_ctx* c, const double* G, int b, double* T, int64_t ldt);
-int launch_baxpy(cqr_ctx* c, double* C, const double* X, int64_t bb, int batch, double alpha);
 int launch_copy2d(cqr_ctx* c, const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m,
                   int64_t n);
 
@@ -48,43 +47,6 @@ static BgemmArgs bg(int m, int n, int k, double alpha, double beta) {
   BgemmArgs g{};
   g.m = m; g.n = n; g.k = k; g.alpha = alpha; g.beta = beta;
   return g;
-}
-
-// Reverse tree (construct_tsqr_Q.cxx:147-217): C_0 = I at the root, then for
-// every node from the top level down, [C_r; C_s] = Q_node·[C_r; 0]:
-//   X = T·C_r,  C_s = triu(−V2·X),  C_r = C_r − X.
-static int reverse_tree(cqr_ctx* c, const TsqrPlan& p, const double* nodeV2, const double* nodeT,
-                        double* C, double* X) {
-  const int b = (int)p.b;
-  const int64_t bb = (int64_t)b * b;
-  CQR_TRY(set_identity(c, C, b));
-  for (int lv = (int)p.levels.size() - 1; lv >= 0; --lv) {
-    const int npairs = (int)p.levels[lv].size();
-    const int n = (int)(npairs == 0 ? 1 : p.levels[lv].back().second + 1);
-    const int64_t node0 = p.level_node0[lv];
-    CQR_TRY(gemm_f64_batched(c, 0, 0, b, b, b, 1.0, bmat(nodeT + node0 * bb, b, bb), SM_UPPER, bmat(C, b, bb),
-                             SM_NONE, 0.0, bmat(X, b, bb), SM_NONE, npairs, 0, 1, 0, b, PK_BGEMM));
-    CQR_TRY(gemm_f64_batched(c, 0, 0, b, b, b, -1.0, bmat(nodeV2 + node0 * bb, b, bb), SM_UPPER, bmat(X, b, bb),
-                             SM_NONE, 0.0, bmat(C, b, bb, BM_SHIFTED), SM_UPPER, npairs, 0, 1, (n + 1) / 2, b,
-                             PK_BGEMM));
-    CQR_TRY(launch_baxpy(c, C, X, bb, npairs, -1.0));
-  }
-  return CQR_OK;
-}
-
-// Per leaf: M = T_l·(V_topᵀ·C_l), then Q1_l = [C_l; 0] − V_l·M_l into Q (ld ldq).
-static int leaf_apply(cqr_ctx* c, const TsqrPlan& p, const double* A, int64_t lda, const double* leafT,
-                      const double* C, double* Z, double* M, double* Q, int64_t ldq) {
-  const int b = (int)p.b, L = (int)p.nleaves;
-  const int64_t bb = (int64_t)b * b;
-  CQR_TRY(gemm_f64_batched(c, 1, 0, b, b, b, 1.0, bmat(A, lda, 0, BM_LEAFROWS), SM_UNIT_LOWER, bmat(C, b, bb),
-                           SM_NONE, 0.0, bmat(Z, b, bb), SM_NONE, L, p.m, L, 0, b, PK_BGEMM));
-  CQR_TRY(gemm_f64_batched(c, 0, 0, b, b, b, 1.0, bmat(leafT, b, bb), SM_UPPER, bmat(Z, b, bb), SM_NONE, 0.0,
-                           bmat(M, b, bb), SM_NONE, L, p.m, L, 0, b, PK_BGEMM));
-  CQR_TRY(init_q1(c, Q, ldq, p.m, L, b, C));
-  return gemm_f64_batched(c, 0, 0, -1, b, b, -1.0, bmat(A, lda, 0, BM_LEAFROWS), SM_UNIT_LOWER, bmat(M, b, bb),
-                          SM_NONE, 1.0, bmat(Q, ldq, 0, BM_LEAFROWS), SM_NONE, L, p.m, L, 0,
-                          (int)p.max_leaf_rows, PK_BGEMM);
 }
 
 size_t panel_ws_bytes(int64_t m, int64_t b, int64_t nleaves, int64_t leaf_rows) {

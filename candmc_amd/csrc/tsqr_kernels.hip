@@ -573,25 +573,6 @@ tri_inverse_kernel(const double* __restrict__ X, int64_t ldx, double* __restrict
   }
 }
 
-// Yscr rows of leaf z := [C_z; 0]
-__global__ void init_q1_kernel(double* __restrict__ Y, int64_t ldy, int64_t m, int L, int b,
-                               const double* __restrict__ C) {
-  const int z = blockIdx.y;
-  const int64_t r0 = leaf_off(z, m, L), r1 = leaf_off(z + 1, m, L);
-  const int64_t rows = r1 - r0;
-  const double* Cz = C + (int64_t)z * b * b;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < rows * b;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t rr = e % rows, cc = e / rows;
-    Y[r0 + rr + cc * ldy] = rr < b ? Cz[rr + cc * b] : 0.0;
-  }
-}
-
-__global__ void set_identity_kernel(double* __restrict__ C, int b) {
-  for (int e = threadIdx.x + blockIdx.x * blockDim.x; e < b * b; e += blockDim.x * gridDim.x)
-    C[e] = (e % b) == (e / b) ? 1.0 : 0.0;
-}
-
 // tau_recon (alg/shared/util.cxx:437-465), one wave per column.
 __global__ void tau_recon_kernel(char s, int64_t ncol, int64_t nrow, int64_t lda,
                                  const double* __restrict__ Y, double* __restrict__ tau) {
@@ -678,11 +659,6 @@ __global__ void diag_kernel(const double* __restrict__ T, int64_t ldt, double* _
 // (SM_IDENTITY); no extra kernel needed.
 
 // C_z += alpha·X_z over z-batched b×b blocks (reverse-tree C_r −= X)
-__global__ void baxpy_kernel(double* __restrict__ C, const double* __restrict__ X, int64_t bb, double alpha) {
-  const int z = blockIdx.y;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < bb; e += (int64_t)gridDim.x * blockDim.x)
-    C[z * bb + e] += alpha * X[z * bb + e];
-}
 
 // T := tril(G) with the diagonal halved (qr_2d.cxx:42-51)
 __global__ void tril_halve_kernel(const double* __restrict__ G, int b, double* __restrict__ T, int64_t ldt) {
@@ -833,17 +809,7 @@ int launch_tri_inverse(cqr_ctx* c, const double* X, int64_t ldx, double* Xi, int
   return check_launch(c, "tri_inverse");
 }
 
-int init_q1(cqr_ctx* c, double* Y, int64_t ldy, int64_t m, int L, int b, const double* C) {
-  const int64_t rows = (m + L - 1) / L;
-  const int gx = (int)std::min<int64_t>(64, (rows * b + 255) / 256);
-  hipLaunchKernelGGL(init_q1_kernel, dim3(gx, L), dim3(256), 0, c->stream, Y, ldy, m, L, b, C);
-  return check_launch(c, "init_q1");
-}
 
-int set_identity(cqr_ctx* c, double* C, int b) {
-  hipLaunchKernelGGL(set_identity_kernel, dim3((b * b + 255) / 256), dim3(256), 0, c->stream, C, b);
-  return check_launch(c, "set_identity");
-}
 
 int launch_tau_recon(cqr_ctx* c, char s, int64_t ncol, int64_t nrow, int64_t lda, const double* Y,
                      double* tau) {
@@ -880,12 +846,6 @@ int launch_finite_scan(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int6
   return check_launch(c, "finite_scan");
 }
 
-int launch_baxpy(cqr_ctx* c, double* C, const double* X, int64_t bb, int batch, double alpha) {
-  if (batch <= 0) return CQR_OK;
-  const int gx = (int)std::min<int64_t>(64, (bb + 255) / 256);
-  hipLaunchKernelGGL(baxpy_kernel, dim3(gx, batch), dim3(256), 0, c->stream, C, X, bb, alpha);
-  return check_launch(c, "baxpy");
-}
 
 int launch_tril_halve(cqr_ctx* c, const double* G, int b, double* T, int64_t ldt) {
   hipLaunchKernelGGL(tril_halve_kernel, dim3((b * b + 255) / 256), dim3(256), 0, c->stream, G, b, T, ldt);
