@@ -1,14 +1,16 @@
-// The unmasked bulk of the WY trailing update, A −= Y·Z' (a plain FP64
-// GEMM: no triangle, no structure), through the vendor BLAS on the calling
-// stream, as an alternative to the hand-written lean kernel.
+// Plain FP64 products of the path (no triangle, no structure) through the
+// vendor BLAS on the calling stream: the unmasked bulk of the WY trailing
+// update A −= Y·Z' and Y = Q1·U⁻¹ of the Householder reconstruction, as an
+// alternative to the library's own MFMA kernels.
 #include <rocblas/rocblas.h>
 
 #include "cqr_internal.h"
 
 namespace cqr {
 
-int blas_update(cqr_ctx* c, int64_t m, int64_t n, int64_t k, const double* Y, int64_t ldy, const double* Z,
-                int64_t ldz, double* A, int64_t lda) {
+// C (m×n) = alpha·A·B + beta·C, all column-major, no transposes.
+int blas_gemm_nn(cqr_ctx* c, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
+                 const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int kind) {
   if (m <= 0 || n <= 0 || k <= 0) return CQR_OK;
   if (!c->blas) {
     rocblas_handle h = nullptr;
@@ -17,14 +19,20 @@ int blas_update(cqr_ctx* c, int64_t m, int64_t n, int64_t k, const double* Y, in
   }
   rocblas_handle h = static_cast<rocblas_handle>(c->blas);
   if (rocblas_set_stream(h, c->stream) != rocblas_status_success) return set_err(c, CQR_ERR_DEVICE, "rocblas_set_stream");
-  const double alpha = -1.0, beta = 1.0;
   hipEvent_t ev = prof_begin(c);
   const rocblas_status st = rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)m,
-                                          (rocblas_int)n, (rocblas_int)k, &alpha, Y, (rocblas_int)ldy, Z,
-                                          (rocblas_int)ldz, &beta, A, (rocblas_int)lda);
-  prof_end(c, ev, PK_GEMM_UPD, 2.0 * m * n * k, 8.0 * ((double)m * k + (double)k * n + 2.0 * m * n));
+                                          (rocblas_int)n, (rocblas_int)k, &alpha, A, (rocblas_int)lda, B,
+                                          (rocblas_int)ldb, &beta, C, (rocblas_int)ldc);
+  prof_end(c, ev, kind, 2.0 * m * n * k,
+           8.0 * ((double)m * k + (double)k * n + (beta != 0.0 ? 2.0 : 1.0) * m * n));
   if (st != rocblas_status_success) return set_err(c, CQR_ERR_DEVICE, "rocblas_dgemm: status %d", (int)st);
   return CQR_OK;
+}
+
+// A (m×n) −= Y·Z (Y m×k, Z k×n).
+int blas_update(cqr_ctx* c, int64_t m, int64_t n, int64_t k, const double* Y, int64_t ldy, const double* Z,
+                int64_t ldz, double* A, int64_t lda) {
+  return blas_gemm_nn(c, m, n, k, -1.0, Y, ldy, Z, ldz, 1.0, A, lda, PK_GEMM_UPD);
 }
 
 void blas_destroy(cqr_ctx* c) {

@@ -500,8 +500,7 @@ size_t wy_update_ws_doubles(int64_t m, int64_t n, int b) {
 int lean_gemm(cqr_ctx* c, int ak, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
               const double* B, int64_t ldb, double beta, double* C, int64_t ldc, double* scratch, int nsplit,
               int kind, int* used);
-int blas_update(cqr_ctx* c, int64_t m, int64_t n, int64_t k, const double* Y, int64_t ldy, const double* Z,
-                int64_t ldz, double* A, int64_t lda);
+
 
 // FP64 update with the unmasked bulk on the lean kernel: rows [0, bt) of Y
 // (its unit-lower top block) go through the masked general kernel, rows
@@ -554,7 +553,12 @@ int wy_update_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const
   // Z' = Tᵀ·Z (Qᵀ·A), or T·Z with t_trans (Q·A); Tt holds Tᵀ (lower), FP64
   CQR_TRY(gemm_f64_impl(c, t_trans ? 1 : 0, 0, b, n, b, 1.0, Tt, b, SM_LOWER, Z, b, SM_NONE, 0.0, Zp, b,
                         nullptr));
-  // A −= Y·Z'
+  // A −= Y·Z' (FP64 with the BLAS switch: the top b rows, where Y may be
+  // unit lower, on the masked kernel, the rest as a plain GEMM)
+  if (prec == CQR_PREC_F64 && c->blas_update && m > 2 * b) {
+    CQR_TRY(gemm_f64_impl(c, 0, 0, b, n, b, -1.0, Y, ldy, smY, Zp, b, SM_NONE, 1.0, A, lda, nullptr, PK_GEMM_UPD));
+    return blas_update(c, m - b, n, b, Y + b, ldy, Zp, b, A + b, lda);
+  }
   return gemm_f64_impl(c, 0, 0, m, n, b, -1.0, Y, ldy, smY, Zp, b, SM_NONE, 1.0, A, lda, nullptr,
                        PK_GEMM_UPD, prec);
 }

@@ -86,9 +86,14 @@ int hh_recon_panel(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, dou
   int64_t* sg = signs_out ? signs_out : reinterpret_cast<int64_t*>(base + w.signs);
   CQR_TRY(hr_core(c, Y, m, Rroot, A, lda, Wd, at(w.Uinv), at(w.Winv), at(w.LinvT), sg, bi));
   // Y = Q1·U⁻¹ below the top b rows (hh_recon.cxx:294-301)
-  if (m > b)
-    CQR_TRY(gemm_f64_ws(c, 0, 0, m - b, b, b, 1.0, Y + b, m, SM_NONE, at(w.Uinv), b, SM_UPPER, 0.0, A + b, lda,
-                        nullptr));
+  // (U⁻¹ is stored with explicit zeros below its diagonal: a plain GEMM)
+  if (m > b) {
+    if (c->blas_update)
+      CQR_TRY(blas_gemm_nn(c, m - b, b, b, 1.0, Y + b, m, at(w.Uinv), b, 0.0, A + b, lda, PK_GEMM_SMALL));
+    else
+      CQR_TRY(gemm_f64_ws(c, 0, 0, m - b, b, b, 1.0, Y + b, m, SM_NONE, at(w.Uinv), b, SM_UPPER, 0.0, A + b, lda,
+                          nullptr));
+  }
   // T = −W·L⁻ᵀ, Tᵀ = −L⁻¹·Wᵀ, T⁻ᵀ = −W⁻ᵀ·L (W = −T·Y1ᵀ)
   if (Tt)
     CQR_TRY(gemm_f64_ws(c, 1, 1, b, b, b, -1.0, at(w.LinvT), b, SM_UNIT_UPPER, Wd, b, SM_UPPER, 0.0, Tt, b,
