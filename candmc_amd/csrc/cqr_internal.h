@@ -49,8 +49,8 @@ struct cqr_ctx {
 
 namespace cqr {
 void blas_destroy(cqr_ctx* c);
-int blas_gemm_nn(cqr_ctx* c, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
-                 const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int kind);
+int blas_gemm(cqr_ctx* c, int ta, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
+              const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int kind);
 int blas_update(cqr_ctx* c, int64_t m, int64_t n, int64_t k, const double* Y, int64_t ldy, const double* Z,
                 int64_t ldz, double* A, int64_t lda);
 

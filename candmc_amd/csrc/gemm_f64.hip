@@ -550,12 +550,13 @@ int wy_update_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const
   // Z = Yᵀ·A  (b×n)
   CQR_TRY(gemm_f64_impl(c, 1, 0, b, n, m, 1.0, Y, ldy, smY, A, lda, SM_NONE, 0.0, Z, b,
                         np ? part : nullptr, PK_GEMM_YTA, prec));
+  const bool blas = prec == CQR_PREC_F64 && c->blas_update && m > 2 * b;
   // Z' = Tᵀ·Z (Qᵀ·A), or T·Z with t_trans (Q·A); Tt holds Tᵀ (lower), FP64
   CQR_TRY(gemm_f64_impl(c, t_trans ? 1 : 0, 0, b, n, b, 1.0, Tt, b, SM_LOWER, Z, b, SM_NONE, 0.0, Zp, b,
                         nullptr));
   // A −= Y·Z' (FP64 with the BLAS switch: the top b rows, where Y may be
   // unit lower, on the masked kernel, the rest as a plain GEMM)
-  if (prec == CQR_PREC_F64 && c->blas_update && m > 2 * b) {
+  if (blas) {
     CQR_TRY(gemm_f64_impl(c, 0, 0, b, n, b, -1.0, Y, ldy, smY, Zp, b, SM_NONE, 1.0, A, lda, nullptr, PK_GEMM_UPD));
     return blas_update(c, m - b, n, b, Y + b, ldy, Zp, b, A + b, lda);
   }

@@ -89,7 +89,7 @@ int hh_recon_panel(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, dou
   // (U⁻¹ is stored with explicit zeros below its diagonal: a plain GEMM)
   if (m > b) {
     if (c->blas_update)
-      CQR_TRY(blas_gemm_nn(c, m - b, b, b, 1.0, Y + b, m, at(w.Uinv), b, 0.0, A + b, lda, PK_GEMM_SMALL));
+      CQR_TRY(blas_gemm(c, 0, m - b, b, b, 1.0, Y + b, m, at(w.Uinv), b, 0.0, A + b, lda, PK_GEMM_SMALL));
     else
       CQR_TRY(gemm_f64_ws(c, 0, 0, m - b, b, b, 1.0, Y + b, m, SM_NONE, at(w.Uinv), b, SM_UPPER, 0.0, A + b, lda,
                           nullptr));
