@@ -151,10 +151,7 @@ int cqr_create(cqr_ctx** ctx, int device) {
       hipMemset(c->d_flag, 0, sizeof(int)) != hipSuccess ||
       hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->hrs, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_hr0, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_hr1, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
     (void)hipGetLastError();
     if (c->d_flag) (void)hipFree(c->d_flag);
     if (c->d_cnt) (void)hipFree(c->d_cnt);
@@ -163,9 +160,6 @@ int cqr_create(cqr_ctx** ctx, int device) {
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
-    if (c->hrs) (void)hipStreamDestroy(c->hrs);
-    if (c->ev_hr0) (void)hipEventDestroy(c->ev_hr0);
-    if (c->ev_hr1) (void)hipEventDestroy(c->ev_hr1);
     delete c;
     return CQR_ERR_NOMEM;
   }
@@ -178,7 +172,6 @@ int cqr_destroy(cqr_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   if (c->side) (void)hipStreamSynchronize(c->side);
-  if (c->hrs) (void)hipStreamSynchronize(c->hrs);
   for (const ProfRec& r : c->recs) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
@@ -193,9 +186,6 @@ int cqr_destroy(cqr_ctx* c) {
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
-  if (c->hrs) (void)hipStreamDestroy(c->hrs);
-  if (c->ev_hr0) (void)hipEventDestroy(c->ev_hr0);
-  if (c->ev_hr1) (void)hipEventDestroy(c->ev_hr1);
   delete c;
   return CQR_OK;
 }

@@ -43,8 +43,6 @@ struct cqr_ctx {
   int blas_update = 1;          // bulk of A −= Y·Z' through the vendor BLAS
   void* blas = nullptr;         // its handle, created on first use
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  hipStream_t hrs = nullptr;    // third stream: the reconstruction core beside construct_Q1
-  hipEvent_t ev_hr0 = nullptr, ev_hr1 = nullptr;
   int lookahead = 1;
   std::string err;
 };
@@ -182,13 +180,8 @@ int launch_ptree(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, dou
 
 // construct_Q1 over the whole tree and the leaves in one launch
 // (q1_kernel.hip): C is scratch of L×b×b; Q receives the m×b Q1 (ld ldq).
-// With publish_top, the items of leaf 0 also write its rows with
-// device-coherent stores and raise one flag per column block at
-// q1_top_flags(c, nnodes, b) (value: the launch epoch, c->q1_epoch).
 int launch_q1(cqr_ctx* c, const double* nodeV2, const double* nodeT, const double* leafT, const double* V,
-              int64_t ldv, int64_t m, double* C, double* Q, int64_t ldq, int L, int nnodes, int b,
-              bool publish_top = false);
-uint64_t* q1_top_flags(cqr_ctx* c, int nnodes, int b);
+              int64_t ldv, int64_t m, double* C, double* Q, int64_t ldq, int L, int nnodes, int b);
 
 // Batched launchers (one group of problems sharing b)
 int leaf_qr_batched(cqr_ctx* c, double* Abase, int64_t total_m, int L, int64_t rows, int b, double* R, double* T,
@@ -212,11 +205,6 @@ int tree_level(cqr_ctx* c, double* Rleaf, int n, int node0, int b, double* V2, d
 int bgemm(cqr_ctx* c, const BgemmArgs& g, int batch, int max_m);
 int hr_core(cqr_ctx* c, const double* Q1top, int64_t ldq, const double* Rroot, double* Atop,
             int64_t lda, double* W, double* Uinv, double* Winv, double* LinvT, int64_t* signs, int b);
-// Same, started before construct_Q1 ends: waits (bounded) until the ncb
-// flags at qflags reach epoch, then reads Q1's top rows device-coherently.
-int hr_core_after_q1(cqr_ctx* c, const double* Q1top, int64_t ldq, const double* Rroot, double* Atop, int64_t lda,
-                     double* W, double* Uinv, double* Winv, double* LinvT, int64_t* signs, int b,
-                     const uint64_t* qflags, uint64_t epoch);
 int launch_diag(cqr_ctx* c, const double* T, int64_t ldt, double* tau, int b);
 int launch_signed_nlu(cqr_ctx* c, double* A, int64_t lda, double* R, int64_t lda_R,
                       int64_t* signs, int b);

@@ -54,8 +54,7 @@ __global__ void __launch_bounds__(NT)
 q1_kernel(const double* __restrict__ nodeV2, const double* __restrict__ nodeT, const double* __restrict__ leafT,
           const double* __restrict__ V, int64_t ldv, int64_t m, double* __restrict__ C,
           double* __restrict__ Q, int64_t ldq, uint64_t* __restrict__ flags, uint64_t epoch, int L,
-          int nnodes, int b, int* __restrict__ err, const BatchProb* __restrict__ bp, int nprob,
-          uint64_t* __restrict__ top_flags) {
+          int nnodes, int b, int* __restrict__ err, const BatchProb* __restrict__ bp, int nprob) {
   __shared__ double S0[128][LDS_LD];
   __shared__ double S1[128][LDS_LD];
   __shared__ int s_ok;
@@ -233,38 +232,22 @@ q1_kernel(const double* __restrict__ nodeV2, const double* __restrict__ nodeT, c
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int jj = tc + 4 * v;
-        if (jj < nj) {
-          double* q = Q + r0 + row + (int64_t)(j0 + jj) * ldq;
-          const double val = (row < R ? S0[row][jj] : 0.0) - acc[v];
-          if (top_flags && z == 0 && row < b) store_coherent(q, val);   // read by a concurrent kernel
-          else *q = val;
-        }
+        if (jj < nj) Q[r0 + row + (int64_t)(j0 + jj) * ldq] = (row < R ? S0[row][jj] : 0.0) - acc[v];
       }
     }
   }
-  if (top_flags && z == 0) {   // leaf 0's rows of this column block are in memory
-    wait_memory();
-    __syncthreads();
-    if (tid == 0) flag_store(top_flags + cb, epoch);
-  }
-}
-
-uint64_t* q1_top_flags(cqr_ctx* c, int nnodes, int b) {
-  return c->d_qflags + (int64_t)nnodes * ((b + CW - 1) / CW);
 }
 
 int launch_q1(cqr_ctx* c, const double* nodeV2, const double* nodeT, const double* leafT, const double* V,
-              int64_t ldv, int64_t m, double* C, double* Q, int64_t ldq, int L, int nnodes, int b,
-              bool publish_top) {
+              int64_t ldv, int64_t m, double* C, double* Q, int64_t ldq, int L, int nnodes, int b) {
   if (b > 128 || b < 1) return set_err(c, CQR_ERR_UNSUPPORTED, "construct_Q1: b = %d", b);
   const int ncb = (b + CW - 1) / CW;
-  if ((int64_t)(nnodes + 1) * ncb > CQR_MAX_TREE_NODES * 8)
+  if ((int64_t)nnodes * ncb > CQR_MAX_TREE_NODES * 8)
     return set_err(c, CQR_ERR_UNSUPPORTED, "construct_Q1: tree of %d nodes", nnodes);
   const uint64_t epoch = ++c->q1_epoch;
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(q1_kernel, dim3((unsigned)((nnodes + L) * ncb)), dim3(NT), 0, c->stream, nodeV2, nodeT,
-                     leafT, V, ldv, m, C, Q, ldq, c->d_qflags, epoch, L, nnodes, b, c->d_terr, nullptr, 0,
-                     publish_top ? q1_top_flags(c, nnodes, b) : nullptr);
+                     leafT, V, ldv, m, C, Q, ldq, c->d_qflags, epoch, L, nnodes, b, c->d_terr, nullptr, 0);
   prof_end(c, ev, PK_BGEMM, 4.0 * nnodes * (double)b * b * b + 4.0 * (double)m * b * b,
            8.0 * (3.0 * nnodes * b * b + 2.0 * m * b + 2.0 * L * b * b));
   return check_launch(c, "q1_kernel");
@@ -277,7 +260,7 @@ int launch_q1_batched(cqr_ctx* c, const double* nodeV2, const double* nodeT, con
   if (total_items <= 0) return CQR_OK;
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(q1_kernel, dim3((unsigned)total_items), dim3(NT), 0, c->stream, nodeV2, nodeT, leafT, Vbase,
-                     (int64_t)0, (int64_t)0, C, Y, (int64_t)0, flags, (uint64_t)1, 0, 0, b, c->d_terr, bp, nprob, nullptr);
+                     (int64_t)0, (int64_t)0, C, Y, (int64_t)0, flags, (uint64_t)1, 0, 0, b, c->d_terr, bp, nprob);
   prof_end(c, ev, PK_BGEMM, 0.0, 0.0);
   return check_launch(c, "q1_kernel_batched");
 }

@@ -80,29 +80,11 @@ int hh_recon_panel(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, dou
   const double* Rroot = p.nnodes > 0 ? at(w.nodeR) + (p.nnodes - 1) * bb : leafR;
   // K3: construct_Q1 (construct_tsqr_Q.cxx:34-242) into the Y scratch
   double* Y = at(w.Y);
-  // K4: LU(Q1_top − S), R := S·R, W = U·S (hh_recon.cxx:282-308).  It only
-  // needs Q1's top b rows (leaf 0's items of construct_Q1), so it runs on
-  // the context's third stream as soon as those are flagged, beside the
-  // rest of construct_Q1; the main stream joins it before Y is formed.
+  CQR_TRY(launch_q1(c, nodeV2, nodeT, leafT, A, lda, m, at(w.C), Y, m, L, (int)p.nnodes, bi));
+  // K4: LU(Q1_top − S), R := S·R, W = U·S (hh_recon.cxx:282-308)
   double* Wd = W ? W : at(w.Wtmp);
   int64_t* sg = signs_out ? signs_out : reinterpret_cast<int64_t*>(base + w.signs);
-  const bool overlap = m >= 2 * b && (m / L) >= b;   // leaf 0 holds the top b rows
-  if (overlap) CQR_HIP(c, hipEventRecord(c->ev_hr0, c->stream));
-  CQR_TRY(launch_q1(c, nodeV2, nodeT, leafT, A, lda, m, at(w.C), Y, m, L, (int)p.nnodes, bi, overlap));
-  if (overlap) {
-    const uint64_t epoch = c->q1_epoch;
-    hipStream_t main = c->stream;
-    CQR_HIP(c, hipStreamWaitEvent(c->hrs, c->ev_hr0, 0));
-    c->stream = c->hrs;
-    const int st = hr_core_after_q1(c, Y, m, Rroot, A, lda, Wd, at(w.Uinv), at(w.Winv), at(w.LinvT), sg, bi,
-                                    q1_top_flags(c, (int)p.nnodes, bi), epoch);
-    c->stream = main;
-    CQR_TRY(st);
-    CQR_HIP(c, hipEventRecord(c->ev_hr1, c->hrs));
-    CQR_HIP(c, hipStreamWaitEvent(c->stream, c->ev_hr1, 0));
-  } else {
-    CQR_TRY(hr_core(c, Y, m, Rroot, A, lda, Wd, at(w.Uinv), at(w.Winv), at(w.LinvT), sg, bi));
-  }
+  CQR_TRY(hr_core(c, Y, m, Rroot, A, lda, Wd, at(w.Uinv), at(w.Winv), at(w.LinvT), sg, bi));
   // Y = Q1·U⁻¹ below the top b rows (hh_recon.cxx:294-301)
   // (U⁻¹ is stored with explicit zeros below its diagonal: a plain GEMM)
   if (m > b) {

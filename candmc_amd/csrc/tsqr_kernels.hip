@@ -325,7 +325,6 @@ __global__ void __launch_bounds__(256) bgemm_kernel(BgemmArgs g) {
 // the rank-8 update of the trailing matrix runs as FP64 MFMA tiles over all
 // waves.  On exit M holds L\U and sgn the signs.
 // ---------------------------------------------------------------------------
-template <bool COH = false>
 __device__ __forceinline__ void signed_lu_lds(const double* __restrict__ src, int64_t ld, double* M, int ldm, double* sgn, int b) {
   constexpr int NT = 512, NB = 8, RPL = 2;
   const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
@@ -337,8 +336,7 @@ __device__ __forceinline__ void signed_lu_lds(const double* __restrict__ src, in
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
         const int cc = w + 8 * j, rr = lane + 64 * s;
-        const int64_t e = (cc < b && rr < b) ? rr + (int64_t)cc * ld : 0;
-        v[j][s] = COH ? load_coherent(src + e) : src[e];
+        v[j][s] = src[(cc < b && rr < b) ? rr + (int64_t)cc * ld : 0];
       }
 #pragma unroll
     for (int j = 0; j < 16; ++j)
@@ -445,7 +443,6 @@ __device__ __forceinline__ void signed_lu_lds(const double* __restrict__ src, in
 
 // K4: Householder reconstruction core (one workgroup).  M is kept in LDS
 // with leading dimension b + 1.
-template <bool COH = false>
 __device__ __forceinline__ void hr_core_body(const double* __restrict__ Q1top, int64_t ldq, const double* __restrict__ Rroot,
                              double* __restrict__ Atop, int64_t lda, double* __restrict__ W,
                              double* __restrict__ Uinv, double* __restrict__ Winv, double* __restrict__ LinvT,
@@ -457,7 +454,7 @@ __device__ __forceinline__ void hr_core_body(const double* __restrict__ Q1top, i
   double* sgn = M + b * ldm;   // b
   double* d = sgn + b;         // b
   const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
-  signed_lu_lds<COH>(Q1top, ldq, M, ldm, sgn, b);
+  signed_lu_lds(Q1top, ldq, M, ldm, sgn, b);
   // outputs: L below the diagonal of Atop, S·R on and above it, W = U·S
   {
     double rv[16][2];
@@ -521,28 +518,6 @@ hr_core_kernel(const double* __restrict__ Q1top, int64_t ldq, const double* __re
                double* __restrict__ Uinv, double* __restrict__ Winv, double* __restrict__ LinvT,
                int64_t* __restrict__ signs, int b) {
   hr_core_body(Q1top, ldq, Rroot, Atop, lda, W, Uinv, Winv, LinvT, signs, b);
-}
-
-// Started beside construct_Q1: waits until its ncb top-row flags reach
-// epoch (bounded; a timeout raises err), then runs the core with
-// device-coherent loads of Q1's top rows.
-__global__ void __launch_bounds__(512)
-hr_core_wait_kernel(const double* __restrict__ Q1top, int64_t ldq, const double* __restrict__ Rroot,
-                    double* __restrict__ Atop, int64_t lda, double* __restrict__ W, double* __restrict__ Uinv,
-                    double* __restrict__ Winv, double* __restrict__ LinvT, int64_t* __restrict__ signs, int b,
-                    const uint64_t* __restrict__ qflags, uint64_t epoch, int ncb, int* __restrict__ err) {
-  __shared__ int s_ok;
-  if (threadIdx.x == 0) {
-    s_ok = 1;
-    for (int cb = 0; cb < ncb; ++cb)
-      if (!flag_wait(qflags + cb, epoch)) s_ok = 0;
-  }
-  __syncthreads();
-  if (!s_ok) {
-    if (threadIdx.x == 0) *err = 1;
-    return;
-  }
-  hr_core_body<true>(Q1top, ldq, Rroot, Atop, lda, W, Uinv, Winv, LinvT, signs, b);
 }
 
 // Batched: workgroup p handles problem p (its Q1 top in Y, its root R).
@@ -803,23 +778,6 @@ int hr_core(cqr_ctx* c, const double* Q1top, int64_t ldq, const double* Rroot, d
                      lda, W, Uinv, Winv, LinvT, signs, b);
   prof_end(c, ev, PK_HR, (2.0 / 3.0) * b * (double)b * b, 64.0 * b * b);
   return check_launch(c, "hr_core");
-}
-
-int hr_core_after_q1(cqr_ctx* c, const double* Q1top, int64_t ldq, const double* Rroot, double* Atop, int64_t lda,
-                     double* W, double* Uinv, double* Winv, double* LinvT, int64_t* signs, int b,
-                     const uint64_t* qflags, uint64_t epoch) {
-  static bool attr = false;
-  if (!attr) {   // the largest b (128); the kernel also holds a small static flag
-    CQR_HIP(c, hipFuncSetAttribute((const void*)hr_core_wait_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(sizeof(double) * (128 * 129 + 256))));
-    attr = true;
-  }
-  const size_t lds = sizeof(double) * ((size_t)b * (b + 1) + 2 * b);
-  hipEvent_t ev = prof_begin(c);
-  hipLaunchKernelGGL(hr_core_wait_kernel, dim3(1), dim3(512), lds, c->stream, Q1top, ldq, Rroot, Atop, lda, W, Uinv,
-                     Winv, LinvT, signs, b, qflags, epoch, (b + 15) / 16, c->d_terr);
-  prof_end(c, ev, PK_HR, (2.0 / 3.0) * b * (double)b * b, 64.0 * b * b);
-  return check_launch(c, "hr_core_wait");
 }
 
 int hr_core_batched(cqr_ctx* c, const BatchProb* bp, int nprob, const double* Y, const double* leafR,
