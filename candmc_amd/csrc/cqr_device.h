@@ -188,6 +188,29 @@ __device__ inline Reflector dlarfg_scalar(double alpha, double xnorm) {
   return r;
 }
 
+// 1/x from the hardware reciprocal refined by two Newton steps (within an
+// ulp or two of the correctly rounded quotient; x normal and finite).
+__device__ inline double recip_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// dlarfg when no rescaling can be needed (ss = ‖x‖² in (2^-960, 2^960),
+// |alpha| < 2^480): |beta| = sqrt(alpha² + ss) ≥ 2^-480, so knt = 0; the
+// two quotients use recip_nr.  Returns false (h untouched) otherwise.
+__device__ inline bool dlarfg_fast(double alpha, double ss, Reflector& h) {
+  if (!(ss > 0x1p-960 && ss < 0x1p960) || !(fabs(alpha) < 0x1p480)) return false;
+  const double beta = -copysign(sqrt(fma(alpha, alpha, ss)), alpha);
+  h.beta = beta;
+  h.tau = (beta - alpha) * recip_nr(beta);
+  h.scal = recip_nr(alpha - beta);
+  h.knt = 0;
+  return true;
+}
+
 // Robust norm finish: `ss` is the plain sum of squares; when it may have
 // under/overflowed the caller recomputes with scaling (rare, uniform path).
 __device__ inline bool ss_needs_rescale(double ss) {

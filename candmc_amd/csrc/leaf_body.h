@@ -82,6 +82,8 @@ __device__ __forceinline__ void leaf_qr_body(double* __restrict__ A, int64_t lda
         }
         ss = wave_sum(ss);
         const double alpha = readlane_d(piv, plane);
+        Reflector h;
+        if (!dlarfg_fast(alpha, ss, h)) {
         double xnorm;
         if (ss_needs_rescale(ss)) {
           double mx = 0.0;
@@ -101,7 +103,8 @@ __device__ __forceinline__ void leaf_qr_body(double* __restrict__ A, int64_t lda
         } else {
           xnorm = sqrt(ss);
         }
-        const Reflector h = dlarfg_scalar(alpha, xnorm);
+        h = dlarfg_scalar(alpha, xnorm);
+        }
         if (h.tau != 0.0) {
 #pragma unroll
           for (int s = 0; s < RPL; ++s)

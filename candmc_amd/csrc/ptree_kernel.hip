@@ -563,14 +563,7 @@ __device__ __forceinline__ void btree_body(const double* __restrict__ leafR, dou
         const double ss = readlane_d(red, jj);
         const double alpha = crow[jj];
         Reflector h;
-        if (!ss_needs_rescale(ss) && fabs(alpha) < 0x1p480) {
-          // ‖x‖ and |alpha| in range: |beta| = sqrt(alpha² + ‖x‖²) ≥ 2^-480, no safmin scaling
-          const double bm = sqrt(fma(alpha, alpha, ss));
-          h.beta = -copysign(bm, alpha);
-          h.tau = (h.beta - alpha) / h.beta;
-          h.scal = 1.0 / (alpha - h.beta);
-          h.knt = 0;
-        } else {
+        if (!dlarfg_fast(alpha, ss, h)) {
           double xnorm;
           if (ss_needs_rescale(ss)) {
             double mx = 0.0;
