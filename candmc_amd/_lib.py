@@ -41,6 +41,11 @@ SIGNATURES = {
     "cqr_tree_tsqr": (_I, [_P, _P, _I64, _P]),
     "cqr_bitree_tsqr": (_I, [_P, _P, _I64, _P, _P, _I64, _I64, _I64, _I, _P]),
     "cqr_construct_Q1": (_I, [_P, _P, _I64, _P, _P, _I64, _I64, _I64, _I64, _I64, _P]),
+    "cqr_construct_Q1_id": (_I, [_P, _P, _I64, _P, _P, _I64, _I64, _I64, _I64, _I64, _P, _P, _I64]),
+    "cqr_apply_tsqr_QT": (_I, [_P, _P, _I64, _P, _P, _I64, _I64, _I64, _I64, _I64, _P]),
+    "cqr_apply_tsqr_Q": (_I, [_P, _P, _I64, _P, _P, _I64, _I64, _I64, _I64, _I64, _P]),
+    "cqr_qr_tree_2d": (_I, [_P, _P, _I64, _I64, _I64, _I64, _I64, _I64]),
+    "cqr_tree_data_export": (_I, [_P, _P, _I64, _I64, _I64, _I64, _P, _P]),
     "cqr_signed_nlu": (_I, [_P, _P, _I64, _P, _I64, _I64, _P]),
     "cqr_recursive_nlu": (_I, [_P, _P, _I64, _P, _I64, _I64, _P]),
     "cqr_hh_recon_qr": (_I, [_P, _P, _I64, _I64, _I64, _P, _I64]),

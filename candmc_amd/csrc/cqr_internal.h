@@ -117,11 +117,14 @@ int make_plan(int64_t m, int64_t b, int64_t nleaves_hint, int64_t leaf_rows,
 //   BM_STRIDED  p + z·zs          (per-leaf / per-node b×b arrays)
 //   BM_LEAFROWS p + rowoff(z)     (leaf z's rows inside a column-major panel)
 //   BM_SHIFTED  p + (z+zoff)·zs   (the sender of tree pair z)
+//   BM_LEAFROWS_SHIFTED p + rowoff(z+zoff)  (the sender's rows of tree pair z)
+// m < 0 or k < 0 stands for the rows of leaf z.
 // ---------------------------------------------------------------------------
 //   BM_PROB_Y   p + bp[z].yoff + prow, ld bp[z].m   (problem z's Q1 block)
 //   BM_PROB_A   p + bp[z].aoff + prow, ld bp[z].m   (problem z's A block)
 //   BM_PROB_T   p + bp[z].toff                     (problem z's Tᵀ block)
-enum { BM_STRIDED = 0, BM_LEAFROWS = 1, BM_SHIFTED = 2, BM_PROB_Y = 3, BM_PROB_A = 4, BM_PROB_T = 5 };
+enum { BM_STRIDED = 0, BM_LEAFROWS = 1, BM_SHIFTED = 2, BM_PROB_Y = 3, BM_PROB_A = 4, BM_PROB_T = 5,
+       BM_LEAFROWS_SHIFTED = 6 };
 struct BMat {
   const double* p;
   int64_t ld;
@@ -130,7 +133,7 @@ struct BMat {
 };
 struct BgemmArgs {
   BMat A, B, C;
-  int m, n, k;            // m < 0: rows of leaf z
+  int m, n, k;            // m < 0 / k < 0: rows of leaf z
   int ta, tb, smA, smB, smC;
   double alpha, beta;
   int64_t pm;             // panel rows for BM_LEAFROWS / m < 0

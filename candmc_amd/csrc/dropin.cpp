@@ -163,16 +163,50 @@ void construct_Q1(double const* Y, int64_t const lda_Y, double const* tau, doubl
   (void)myRank; (void)root;
   const char* fn = "construct_Q1";
   need_single_rank(cdt, fn);
-  if (ID) die(fn, "ID != NULL is not served");
   if (!tree_data) die(fn, "tree_data from bitree_tsqr is required");
   cqr_ctx* c = candmc_qr_context();
   int64_t L = 0;
   check(cqr_plan_tsqr(m, b, numPes, 0, &L, nullptr, nullptr), fn);
   Stage y(Y, (size_t)lda_Y * b, false, fn), t(tau, (size_t)L * b, false, fn), q(Q1, (size_t)lda_Q * k, true, fn),
-      td(tree_data, (size_t)cqr_tree_data_doubles(m, b, numPes), false, fn);
-  check(cqr_construct_Q1(c, y.dev, lda_Y, t.dev, q.dev, lda_Q, m, b, k, numPes, td.dev), fn);
+      td(tree_data, (size_t)cqr_tree_data_doubles(m, b, numPes), false, fn), id(ID, (size_t)b * k, false, fn);
+  // ID is b×k with leading dimension b (construct_tsqr_Q.cxx:99,125)
+  check(cqr_construct_Q1_id(c, y.dev, lda_Y, t.dev, q.dev, lda_Q, m, b, k, numPes, td.dev, id.dev, b), fn);
   sync(fn);
   q.finish(fn);
+}
+
+void apply_tsqr_QT(double const* Y, int64_t const lda_Y, double const* tau, double* B, int64_t const lda_B,
+                   int64_t const m, int64_t const b, int64_t const k, int64_t const myRank, int64_t const numPes,
+                   int64_t const root, CommData_t cdt, double* tree_data) {
+  (void)myRank; (void)root;
+  const char* fn = "apply_tsqr_QT";
+  need_single_rank(cdt, fn);
+  if (!tree_data) die(fn, "tree_data from bitree_tsqr is required");
+  cqr_ctx* c = candmc_qr_context();
+  int64_t L = 0;
+  check(cqr_plan_tsqr(m, b, numPes, 0, &L, nullptr, nullptr), fn);
+  Stage y(Y, (size_t)lda_Y * b, false, fn), t(tau, (size_t)L * b, false, fn), bm(B, (size_t)lda_B * k, true, fn),
+      td(tree_data, (size_t)cqr_tree_data_doubles(m, b, numPes), false, fn);
+  check(cqr_apply_tsqr_QT(c, y.dev, lda_Y, t.dev, bm.dev, lda_B, m, b, k, numPes, td.dev), fn);
+  sync(fn);
+  bm.finish(fn);
+}
+
+// One rank per communicator: the one-rank branches of bitree_tsqr and
+// apply_tsqr_QT (dgeqrf of each panel, dormqr of the trailing columns).
+void QR_tree_2D(double* A, int64_t const lda_A, int64_t const m, int64_t const k, int64_t const b,
+                int64_t const myRank, int64_t const numPes, int64_t const root_row, int64_t const root_col,
+                CommData_t cdt_row, CommData_t cdt_col, CommData_t cdt_world, int64_t const _stop_at) {
+  (void)myRank; (void)numPes; (void)root_row; (void)root_col;
+  const char* fn = "QR_tree_2D";
+  need_single_rank(cdt_row, fn);
+  need_single_rank(cdt_col, fn);
+  need_single_rank(cdt_world, fn);
+  cqr_ctx* c = candmc_qr_context();
+  Stage a(A, (size_t)lda_A * k, true, fn);
+  check(cqr_qr_tree_2d(c, a.dev, lda_A, m, k, b, _stop_at, 1), fn);
+  sync(fn);
+  a.finish(fn);
 }
 
 // ---------------------------------------------------------------- hh_recon

@@ -226,6 +226,7 @@ __device__ __forceinline__ const double* bmat_ptr(const BMat& M, int z, const Bg
   switch (M.mode) {
     case BM_STRIDED: return M.p + (int64_t)z * M.zs;
     case BM_LEAFROWS: return M.p + leaf_off(z, g.pm, g.pL);
+    case BM_LEAFROWS_SHIFTED: return M.p + leaf_off(z + g.zoff, g.pm, g.pL);
     default: return M.p + (int64_t)(z + g.zoff) * M.zs;
   }
 }
@@ -245,7 +246,9 @@ __global__ void __launch_bounds__(256) bgemm_kernel(BgemmArgs g) {
   const int z = blockIdx.z;
   int m = g.m;
   if (m < 0) m = (int)(leaf_off(z + 1, g.pm, g.pL) - leaf_off(z, g.pm, g.pL));
-  const int n = g.n, k = g.k;
+  int k = g.k;
+  if (k < 0) k = (int)(leaf_off(z + 1, g.pm, g.pL) - leaf_off(z, g.pm, g.pL));
+  const int n = g.n;
   const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
   if (m0 >= m || n0 >= n) return;
   const double* A = bmat_ptr(g.A, z, g);
@@ -758,8 +761,9 @@ int bgemm(cqr_ctx* c, const BgemmArgs& g, int batch, int max_m) {
   dim3 grid((max_m + 63) / 64, (g.n + 63) / 64, batch);
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(bgemm_kernel, grid, dim3(256), 0, c->stream, g);
-  prof_end(c, ev, PK_BGEMM, 2.0 * batch * (double)max_m * g.n * g.k,
-           8.0 * batch * ((double)max_m * g.k + (double)g.k * g.n + 2.0 * max_m * g.n));
+  const double kk = g.k >= 0 ? (double)g.k : (double)g.pm / std::max(g.pL, 1);
+  prof_end(c, ev, PK_BGEMM, 2.0 * batch * (double)max_m * g.n * kk,
+           8.0 * batch * ((double)max_m * kk + kk * g.n + 2.0 * max_m * g.n));
   return check_launch(c, "bgemm");
 }
 

@@ -23,7 +23,7 @@ __all__ = [
     "local_tsqr", "tree_tsqr", "bitree_tsqr", "construct_Q1", "signed_NLU", "recursive_NLU",
     "hh_recon_qr", "compute_invT_from_W", "compute_invT_from_Y", "upd_A", "update_A",
     "QR_2D", "QR_2D_2D", "tau_recon", "pack_upper", "unpack_upper", "pack_lower", "unpack_lower",
-    "tree_data_size", "plan_tsqr",
+    "tree_data_size", "plan_tsqr", "apply_tsqr_QT", "QR_tree_2D", "tree_data_export",
 ]
 
 
@@ -187,12 +187,46 @@ def bitree_tsqr(A, lda_A, R, tau, m, b, myRank=0, numPes=1, root=0, req_id=0, cd
 
 def construct_Q1(Y, lda_Y, tau, Q1, lda_Q, m, b, k, myRank=0, numPes=1, root=0, cdt=None,
                  tree_data=None, ID=None):
-    """construct_tsqr_Q.cxx:34-242 (k = b, ID = NULL)."""
+    """construct_tsqr_Q.cxx:34-242: Q1 (m×k) := Q·[ID; 0] (ID b×k, ld b), or
+    the first k columns of the TSQR's Q when ID is None."""
     ctx = context()
-    if ID is not None:
-        raise CQRError(2, "ID != NULL is not supported")
-    ctx.check(ctx.lib.cqr_construct_Q1(ctx.h, _pd(Y), lda_Y, _pd(tau), _pd(Q1), lda_Q, m, b, k,
-                                       numPes, _pd(tree_data)))
+    if tree_data is None:
+        raise CQRError(1, "tree_data from bitree_tsqr is required")
+    if ID is None and k == b:
+        ctx.check(ctx.lib.cqr_construct_Q1(ctx.h, _pd(Y), lda_Y, _pd(tau), _pd(Q1), lda_Q, m, b, k,
+                                           numPes, _pd(tree_data)))
+    else:
+        ctx.check(ctx.lib.cqr_construct_Q1_id(ctx.h, _pd(Y), lda_Y, _pd(tau), _pd(Q1), lda_Q, m, b,
+                                              k, numPes, _pd(tree_data), _p(ID), b))
+
+
+def apply_tsqr_QT(Y, lda_Y, tau, B, lda_B, m, b, k, myRank=0, numPes=1, root=0, cdt=None,
+                  tree_data=None, trans=True):
+    """apply_tsqr_QT.cxx:31-165: B (m×k) := Qᵀ·B with the implicit Q of
+    bitree_tsqr (trans=False applies Q)."""
+    ctx = context()
+    if tree_data is None:
+        raise CQRError(1, "tree_data from bitree_tsqr is required")
+    fn = ctx.lib.cqr_apply_tsqr_QT if trans else ctx.lib.cqr_apply_tsqr_Q
+    ctx.check(fn(ctx.h, _pd(Y), lda_Y, _p(tau), _pd(B), lda_B, m, b, k, numPes, _pd(tree_data)))
+
+
+def QR_tree_2D(A, lda_A, m, k, b, numPes=0, stop_at=-1):
+    """qr_tree_2d.cxx:30-124 over numPes virtual row blocks per panel."""
+    ctx = context()
+    ctx.check(ctx.lib.cqr_qr_tree_2d(ctx.h, _pd(A), lda_A, m, k, b, stop_at, numPes))
+
+
+def tree_data_export(tree_data, m, b, numPes, rank):
+    """Virtual rank `rank`'s tree_data in the layout of bitree_tsqr.cxx:213-217
+    (per receiving level: dtpqrt T, min(b,16)×b, then packed upper V2)."""
+    ctx = context()
+    n = ctypes.c_int64()
+    ctx.check(ctx.lib.cqr_tree_data_export(ctx.h, None, m, b, numPes, rank, None, ctypes.byref(n)))
+    out = torch.zeros(max(n.value, 1), dtype=torch.float64, device=tree_data.device)
+    if n.value:
+        ctx.check(ctx.lib.cqr_tree_data_export(ctx.h, _pd(tree_data), m, b, numPes, rank, _pd(out), None))
+    return out[: n.value]
 
 
 # --- alg/QR/hh_recon -------------------------------------------------------

@@ -6,7 +6,7 @@
  * includes this header instead and links -lcandmc_qr.  The signatures are
  * those of the reference headers (paths relative to the CANDMC tree):
  *   alg/QR/tsqr/bitree_tsqr.h:18-138, alg/QR/hh_recon/hh_recon.h:13-75,
- *   alg/QR/qr_2d/qr_2d.h:25-188,293-302, alg/shared/util.h (tau_recon and
+ *   alg/QR/qr_2d/qr_2d.h:25-188,237-249,293-302, alg/shared/util.h (tau_recon and
  *   the packed-triangle helpers), alg/shared/comm.h:32-84 (CommData_t,
  *   pview).
  * Semantics: matrices are column-major doubles with explicit leading
@@ -62,6 +62,10 @@ void construct_Q1(double const* Y, int64_t const lda_Y, double const* tau, doubl
                   int64_t const lda_Q, int64_t const m, int64_t const b, int64_t const k,
                   int64_t const myRank, int64_t const numPes, int64_t const root, CommData_t cdt,
                   double* tree_data = 0, double* ID = 0);
+void apply_tsqr_QT(double const* Y, int64_t const lda_Y, double const* tau, double* B,
+                   int64_t const lda_B, int64_t const m, int64_t const b, int64_t const k,
+                   int64_t const myRank, int64_t const numPes, int64_t const root, CommData_t cdt,
+                   double* tree_data = 0);
 
 /* alg/QR/hh_recon/hh_recon.h */
 void compute_invT_from_W(double const* W, int64_t b, double* invT);
@@ -83,6 +87,10 @@ void QR_2D(double* A, int64_t lda_A, int64_t m, int64_t k, int64_t b, pview* pv,
 void QR_2D_2D(double* A, int64_t lda_A, int64_t m, int64_t k, int64_t b, int64_t b_sub, pview* pv,
               double* aggreg_Y, int64_t lda_aY, int* desc_A = 0, double* org_A = 0, int64_t IA = 1,
               int64_t JA = 1);
+void QR_tree_2D(double* A, int64_t const lda_A, int64_t const m, int64_t const k, int64_t const b,
+                int64_t const myRank, int64_t const numPes, int64_t const root_row,
+                int64_t const root_col, CommData_t cdt_row, CommData_t cdt_col, CommData_t cdt_world,
+                int64_t const _stop_at = -1);
 void QR_2D_pipe(double* A, int64_t lda_A, int64_t m, int64_t k, int64_t b, pview* pv,
                 double* last_Y = 0, int64_t lda_lY = 0, double* last_W = 0, double* my_last_W = 0);
 

@@ -121,6 +121,51 @@ int cqr_bitree_tsqr(cqr_ctx* ctx, double* A, int64_t lda, double* R, double* tau
 int cqr_construct_Q1(cqr_ctx* ctx, const double* Y, int64_t lda_Y, const double* tau,
                      double* Q1, int64_t lda_Q, int64_t m, int64_t b, int64_t k,
                      int64_t nleaves, const double* tree_data);
+/* construct_Q1 in full (alg/QR/tsqr/construct_tsqr_Q.cxx:34-242 with
+ * ID and any k): Q1 (m×k, k ≤ m) := Q·[ID; 0] with ID b×k (ld ld_ID), or,
+ * with ID == NULL, the first k columns of the TSQR's m×m Q.  Q is the
+ * product of the leaves' and tree nodes' reflectors from cqr_bitree_tsqr.
+ * (The reference's multi-rank branch seeds identity entries only in each
+ * rank's top b rows, so its columns b..k−1 are not Q's for k > b; this
+ * follows its one-rank dorgqr branch, :87-102, which returns Q's.)        */
+int cqr_construct_Q1_id(cqr_ctx* ctx, const double* Y, int64_t lda_Y, const double* tau,
+                        double* Q1, int64_t lda_Q, int64_t m, int64_t b, int64_t k,
+                        int64_t nleaves, const double* tree_data, const double* ID,
+                        int64_t ld_ID);
+
+/* ---- apply_tsqr_QT  (alg/QR/tsqr/apply_tsqr_QT.cxx:31-165) -------------
+ * B (m×k, ld lda_B) := Qᵀ·B with the implicit Q of cqr_bitree_tsqr (Y =
+ * its leaf [V\R] blocks, tree_data, same m, b, nleaves).  Row block l of
+ * the result is leaf l's rows; for B = the factored panel's input, the top
+ * b rows of leaf 0 receive R and every other row is zero.  tau is unused
+ * (tree_data holds the leaves' T).  cqr_apply_tsqr_Q applies Q instead.  */
+int cqr_apply_tsqr_QT(cqr_ctx* ctx, const double* Y, int64_t lda_Y, const double* tau,
+                      double* B, int64_t lda_B, int64_t m, int64_t b, int64_t k,
+                      int64_t nleaves, const double* tree_data);
+int cqr_apply_tsqr_Q(cqr_ctx* ctx, const double* Y, int64_t lda_Y, const double* tau,
+                     double* B, int64_t lda_B, int64_t m, int64_t b, int64_t k,
+                     int64_t nleaves, const double* tree_data);
+
+/* ---- QR_tree_2D  (alg/QR/qr_2d/qr_tree_2d.cxx:30-124) ------------------
+ * Tree-updated QR of the m×k matrix A over the first stop_at columns
+ * (stop_at < 0: min(m, k)), panels of b: each panel is factored by
+ * bitree TSQR over `nleaves` virtual row blocks (0 = automatic), its R is
+ * written over the top of the panel, and the trailing columns receive
+ * Qᵀ by cqr_apply_tsqr_QT; the next panel starts b rows and b columns
+ * further.  A := per-panel leaf [V\R] blocks with each panel's R on top
+ * and the transformed trailing columns; the per-panel tree factors are
+ * not kept (as in the reference).                                          */
+int cqr_qr_tree_2d(cqr_ctx* ctx, double* A, int64_t lda, int64_t m, int64_t k, int64_t b,
+                   int64_t stop_at, int64_t nleaves);
+
+/* ---- tree_data in the reference's layout --------------------------------
+ * Virtual rank `rank`'s share of tree_data as bitree_tsqr.cxx:213-217
+ * stores it: for every level at which the rank receives, the dtpqrt T
+ * (min(b,16)×b, ld min(b,16)) followed by the packed upper V2
+ * (b(b+1)/2 doubles, alg/shared/util.cxx pack_upper).  *ndoubles (nullable)
+ * receives the size; out == NULL only queries it.                          */
+int cqr_tree_data_export(cqr_ctx* ctx, const double* tree_data, int64_t m, int64_t b,
+                         int64_t nleaves, int64_t rank, double* out, int64_t* ndoubles);
 
 /* ---- a5  signed_NLU / recursive_NLU  (alg/QR/hh_recon/hh_recon.cxx:43-111)
  * LU without pivoting of the b×b block A − S, S = diag(s) chosen on the fly

@@ -10,6 +10,11 @@ What it restates (paths relative to the CANDMC source tree):
   tree_tsqr            alg/QR/tsqr/bitree_tsqr.cxx:20-34
   bitree_tsqr          alg/QR/tsqr/bitree_tsqr.cxx:91-248
   construct_Q1         alg/QR/tsqr/construct_tsqr_Q.cxx:34-242
+  construct_Q1_full    alg/QR/tsqr/construct_tsqr_Q.cxx:34-242 (any k, ID)
+  apply_tsqr_QT        alg/QR/tsqr/apply_tsqr_QT.cxx:31-165
+  apply_tsqr_Q         alg/QR/tsqr/construct_tsqr_Q.cxx:146-233 (general start)
+  QR_tree_2D           alg/QR/qr_2d/qr_tree_2d.cxx:30-124 (one process column)
+  tree_data_packed     alg/QR/tsqr/bitree_tsqr.cxx:213-217 (layout)
   signed_NLU           alg/QR/hh_recon/hh_recon.cxx:43-70
   recursive_NLU        alg/QR/hh_recon/hh_recon.cxx:80-111
   compute_invT_from_W  alg/QR/hh_recon/hh_recon.cxx:26-31
@@ -34,6 +39,15 @@ How each function is pinned (tests/test_oracle.py):
     fixtures in tests/golden.
   * construct_Q1: Q1 against dorgqr up to column signs, Q1ᵀQ1 = I,
     Q1·R = A.
+  * apply_tsqr_QT / apply_tsqr_Q: against a dense m×m Q assembled from
+    dorgqr of every leaf and the node transforms I − [I;V2]·T·[I;V2]ᵀ
+    embedded at the leaves' top rows; Qᵀ·A = [R; 0]; Q·(Qᵀ·B) = B.
+  * construct_Q1_full: equals construct_Q1 for k = b, dorgqr for one rank,
+    the dense Q's first k columns, and Q·ID for ID.
+  * QR_tree_2D: one rank equals LAPACK dgeqrf + dormqr; the [A, A]
+    invariant of test/QR/test_qr_tree_2d.cxx:83-143 (the right half becomes
+    the left half's R) for one and several ranks.
+  * tree_data_packed: element positions against pack_upper and dtpqrt's T.
   * signed_NLU / recursive_NLU: L·U = Q1_top − S with the sign rule, and
     a brute-force pure-Python LU on tiny inputs.
   * hh_recon_qr: [Y\\R] equals LAPACK dgeqrf's [V\\R] (the identity that
@@ -312,6 +326,123 @@ def construct_Q1(Ys, taus, tree_data, b, k=None):
             Q[j:, j:] = cq
         out.append(Q)
     return out
+
+
+def _dormqr(trans, Y, tau, C):
+    """dormqr('L', trans) of the reflectors in Y (columns with τ) on C."""
+    C = np.asfortranarray(np.array(C, dtype=np.float64))
+    if C.shape[1] == 0:
+        return C
+    c, work, info = _lp.dormqr("L", trans, np.asfortranarray(Y), tau, C, max(1, C.shape[1]) * 64)
+    assert info == 0
+    return np.asfortranarray(c)
+
+
+def apply_tsqr_QT(Ys, taus, tree_data, Bs, b):
+    """apply_tsqr_QT.cxx:31-165 over virtual ranks (root 0, tree_data branch).
+
+    Every rank applies its leaf reflectors transposed (dlarft + dlarfb
+    ('L','T'), :87-91, the same product as dormqr('L','T')); then the tree
+    levels run forward and each receiver r applies dtpmqrt('L','T') with its
+    stored (T, V2) to [top b rows of r; top b rows of its sender] (:131-140).
+    Bs: per-rank mb×k blocks; returns the transformed blocks.
+    """
+    P = len(Ys)
+    out = [_dormqr("T", Ys[r][:, :b], taus[r], Bs[r]) for r in range(P)]
+    if P == 1:                                # :73-80
+        return out
+    ptr = [0] * P
+    for n in bitree_levels(P):                # :104
+        for r, s in bitree_pairs(n):
+            T, V2 = tree_data[r][ptr[r]]
+            ptr[r] += 1
+            a, bb, info = _lp.dtpmqrt(b, V2, T, np.asfortranarray(out[r][:b]),
+                                      np.asfortranarray(out[s][:b]), side="L", trans="T")
+            assert info == 0
+            out[r][:b] = a
+            out[s][:b] = bb
+    return out
+
+
+def apply_tsqr_Q(Ys, taus, tree_data, Bs, b):
+    """Q·B for the same Q: the tree levels backward with dtpmqrt('L','N')
+    (the dtprfb loop of construct_tsqr_Q.cxx:195-208 on an arbitrary
+    [top; bottom]), then every rank's leaf reflectors (dormqr('L','N'),
+    :219-233)."""
+    P = len(Ys)
+    out = [np.asfortranarray(np.array(B, dtype=np.float64)) for B in Bs]
+    if P > 1:
+        ptr = [len(td) for td in tree_data]
+        for n in reversed(bitree_levels(P)):
+            for r, s in bitree_pairs(n):
+                ptr[r] -= 1
+                T, V2 = tree_data[r][ptr[r]]
+                a, bb, info = _lp.dtpmqrt(b, V2, T, np.asfortranarray(out[r][:b]),
+                                          np.asfortranarray(out[s][:b]), side="L", trans="N")
+                assert info == 0
+                out[r][:b] = a
+                out[s][:b] = bb
+    return [_dormqr("N", Ys[r][:, :b], taus[r], out[r]) for r in range(P)]
+
+
+def construct_Q1_full(Ys, taus, tree_data, b, k, ID=None):
+    """construct_tsqr_Q.cxx:34-242 for any k and ID: Q·[ID; 0] (ID b×k on
+    the root's top rows, :123-126), or the first k columns of Q (identity
+    columns at their global rows; for k ≤ b this is construct_Q1, for one
+    rank the dorgqr of :87-102)."""
+    sizes = [Y.shape[0] for Y in Ys]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    Es = [np.zeros((mb, k), order="F") for mb in sizes]
+    if ID is not None:
+        Es[0][:b, :] = ID
+    else:
+        for j in range(k):
+            r = int(np.searchsorted(offs, j, side="right") - 1)
+            Es[r][j - offs[r], j] = 1.0
+    return apply_tsqr_Q(Ys, taus, tree_data, Es, b)
+
+
+def QR_tree_2D(A, b, split=None, stop_at=-1):
+    """qr_tree_2d.cxx:30-124 with the panels' TSQR over virtual ranks.
+
+    split(m, b) gives the per-rank row counts of an m-row panel (None: one
+    rank, i.e. dgeqrf of the panel and dormqr('L','T') of the trailing
+    columns).  Per panel: bitree_tsqr with output_Y (A := leaf [V\\R]
+    blocks, :58-60), the root's R over the top of the panel (:61-62),
+    apply_tsqr_QT of the trailing columns (:96-97), then the next panel b
+    rows and b columns further (:116-120).
+    """
+    A = np.asfortranarray(np.array(A, dtype=np.float64))
+    m, k = A.shape
+    stop = min(m, k) if stop_at < 0 else min(m, k, stop_at)
+    j = 0
+    while j < stop:
+        mj, bw = m - j, min(b, stop - j)
+        sizes = [mj] if split is None else list(split(mj, bw))
+        offs = np.concatenate([[0], np.cumsum(sizes)]).astype(int)
+        blocks = [A[j + offs[r]:j + offs[r + 1], j:j + bw] for r in range(len(sizes))]
+        Ys, R, taus, td = bitree_tsqr(blocks, bw)
+        for r in range(len(sizes)):
+            A[j + offs[r]:j + offs[r + 1], j:j + bw] = Ys[r]
+        A[j:j + bw, j:j + bw] = np.triu(R) + np.tril(A[j:j + bw, j:j + bw], -1)
+        if k - j - bw > 0:
+            Bs = [A[j + offs[r]:j + offs[r + 1], j + bw:] for r in range(len(sizes))]
+            out = apply_tsqr_QT(Ys, taus, td, Bs, bw)
+            for r in range(len(sizes)):
+                A[j + offs[r]:j + offs[r + 1], j + bw:] = out[r]
+        j += b
+    return A
+
+
+def tree_data_packed(tree_data_r, b):
+    """One rank's tree_data as bitree_tsqr.cxx:213-217 stores it: per
+    receiving level the dtpqrt T (min(b,16)×b, column-major) and then
+    pack_upper(V2)."""
+    parts = []
+    for T, V2 in tree_data_r:
+        parts.append(np.asarray(T).reshape(-1, order="F"))
+        parts.append(pack_upper(V2, b))
+    return np.concatenate(parts) if parts else np.zeros(0)
 
 
 # --------------------------------------------------------------------------

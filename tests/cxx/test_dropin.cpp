@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "candmc/CANDMC_QR.h"
+#include "candmc_qr.h"
 
 static double colrel_residual(const std::vector<double>& A, const std::vector<double>& YR, int64_t m,
                               int64_t n, bool flip_last) {
@@ -89,6 +90,43 @@ int main() {
     const double norm = colrel_residual(A, YR, m, k, true);
     printf("QR_2D_pipe m=k=64 b=16: ||A-QR||_2 = %.2E\n", norm);
     if (!(norm <= 1e-9)) { printf("TEST FAILED!!!\n"); ++fails; } else printf("Test successful.\n");
+  }
+  {  // test_qr_tree_2d.cxx on one rank: QR of [A, A] gives [Y\R_1, R_2] with R_2 = R_1
+    const int64_t m = 96, k = 40, b = 8;
+    std::vector<double> A((size_t)m * 2 * k);
+    for (int64_t c = 0; c < k; ++c)
+      for (int64_t r = 0; r < m; ++r) A[r + c * m] = A[r + (k + c) * m] = seeded(r, c, m) - 0.5;
+    QR_tree_2D(A.data(), m, m, 2 * k, b, 0, 1, 0, 0, cdt, cdt, cdt, std::min(m, k));
+    double f = 0.0;
+    for (int64_t c = 0; c < k; ++c)
+      for (int64_t r = 0; r < m; ++r) {
+        const double l = A[r + c * m], x = A[r + (k + c) * m];
+        if (r > c) f += x * x;
+        else f += std::min((l - x) * (l - x), ((l - x) / l) * ((l - x) / l));
+      }
+    f = std::sqrt(f);
+    printf("QR of [A,A] gives [Y/R_1,0/R_2], checking norm ||0/R_2-R_1||_F=%E\n", f);
+    if (!(f < 1e-6)) { printf("TEST FAILED!!!\n"); ++fails; } else printf("Test passed.\n");
+  }
+  {  // bitree_tsqr over 4 virtual ranks, then apply_tsqr_QT of the input: [R; 0]
+    const int64_t m = 256, b = 16, P = 4;
+    std::vector<double> A((size_t)m * b), Y, R((size_t)b * b), tau((size_t)P * b);
+    srand48(11);
+    for (auto& x : A) x = drand48() - 0.5;
+    std::vector<double> td((size_t)cqr_tree_data_doubles(m, b, P));
+    Y = A;
+    bitree_tsqr(Y.data(), m, R.data(), tau.data(), m, b, 0, P, 0, 0, cdt, 1, td.data());
+    std::vector<double> B = A;
+    apply_tsqr_QT(Y.data(), m, tau.data(), B.data(), m, m, b, b, 0, P, 0, cdt, td.data());
+    double d = 0.0, rmax = 0.0;
+    for (int64_t c = 0; c < b; ++c)
+      for (int64_t r = 0; r < m; ++r) {
+        const double want = r <= c ? R[r + c * b] : 0.0;
+        d = std::max(d, std::fabs(B[r + c * m] - want));
+        rmax = std::max(rmax, std::fabs(want));
+      }
+    printf("apply_tsqr_QT of the factored panel: max |Q^T A - [R;0]| / max|R| = %.2E\n", d / rmax);
+    if (!(d / rmax < 1e-12)) { printf("TEST FAILED!!!\n"); ++fails; } else printf("Test passed.\n");
   }
   return fails ? 1 : 0;
 }

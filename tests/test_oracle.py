@@ -265,3 +265,152 @@ def test_conditioned_matrix_has_the_stated_spectrum():
     A = O.conditioned_matrix(300, 40, 1e6, 5)
     s = np.linalg.svd(A, compute_uv=False)
     assert abs(s[0] - 1.0) < 1e-12 and abs(s[-1] * 1e6 - 1.0) < 1e-6
+
+
+# ---- implicit TSQR Q (apply_tsqr_QT, construct_Q1 in full, QR_tree_2D) ----
+def _dense_tsqr_q(Ys, taus, td, b):
+    """Dense m×m Q = Q_leaves·Q_tree built without dormqr/dtpmqrt: each
+    leaf's full orthogonal factor from dorgqr, each node as the explicit
+    product of its reflectors u_i = [e_i at the receiver's top; V2(:, i) at
+    the sender's top] with τ_i = diag of the blocked T, nodes in level
+    order."""
+    sizes = [Y.shape[0] for Y in Ys]
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(int)
+    m = offs[-1]
+    Q = np.zeros((m, m))
+    for r, Y in enumerate(Ys):
+        mb = sizes[r]
+        a = np.zeros((mb, mb), order="F")
+        a[:, :b] = Y[:, :b]
+        q, _, info = lp.dorgqr(a, taus[r])
+        assert info == 0
+        Q[offs[r]:offs[r + 1], offs[r]:offs[r + 1]] = q
+    ptr = [0] * len(Ys)
+    nb = min(b, O.TAU_BLK)
+    for n in O.bitree_levels(len(Ys)):
+        for r, s in O.bitree_pairs(n):
+            T, V2 = td[r][ptr[r]]
+            ptr[r] += 1
+            G = np.eye(m)
+            for i in range(b):
+                u = np.zeros(m)
+                u[offs[r] + i] = 1.0
+                u[offs[s]:offs[s] + b] = V2[:, i]
+                tau = T[i % nb, i]
+                G = G - tau * np.outer(G @ u, u)
+            Q = Q @ G
+    return Q, offs
+
+
+def _leaf_split(m, b, L):
+    return [(l * m) // L for l in range(L + 1)]
+
+
+@pytest.mark.parametrize("m,b,L,k", [(160, 8, 5, 7), (300, 12, 4, 20), (96, 16, 1, 5), (260, 10, 7, 33)])
+def test_apply_tsqr_against_dense_q(m, b, L, k):
+    rng = np.random.default_rng(m + b + L)
+    A = rng.standard_normal((m, b))
+    B = rng.standard_normal((m, k))
+    o = _leaf_split(m, b, L)
+    Ys, R, taus, td = O.bitree_tsqr([A[o[r]:o[r + 1]].copy(order="F") for r in range(L)], b)
+    Q, offs = _dense_tsqr_q(Ys, taus, td, b)
+    assert np.abs(Q.T @ Q - np.eye(m)).max() < 1e-13
+    split = lambda X: [X[o[r]:o[r + 1]] for r in range(L)]
+    qtb = np.vstack(O.apply_tsqr_QT(Ys, taus, td, split(B), b))
+    assert np.abs(qtb - Q.T @ B).max() < 1e-12
+    qb = np.vstack(O.apply_tsqr_Q(Ys, taus, td, split(B), b))
+    assert np.abs(qb - Q @ B).max() < 1e-12
+    # Qᵀ·A = [R; 0] with R at the root's top rows
+    qta = np.vstack(O.apply_tsqr_QT(Ys, taus, td, split(A), b))
+    want = np.zeros((m, b))
+    want[:b] = R
+    assert np.abs(qta - want).max() / np.abs(R).max() < 1e-13
+
+
+@pytest.mark.parametrize("m,b,L", [(200, 8, 5), (120, 12, 1), (256, 16, 3)])
+def test_construct_q1_full(m, b, L):
+    rng = np.random.default_rng(7 * m + L)
+    A = rng.standard_normal((m, b))
+    o = _leaf_split(m, b, L)
+    Ys, R, taus, td = O.bitree_tsqr([A[o[r]:o[r + 1]].copy(order="F") for r in range(L)], b)
+    Q, _ = _dense_tsqr_q(Ys, taus, td, b)
+    for k in (b, b + 5, 2 * b + 3, 3):
+        q1 = np.vstack(O.construct_Q1_full(Ys, taus, td, b, k))
+        assert np.abs(q1 - Q[:, :k]).max() < 1e-12
+    assert np.abs(np.vstack(O.construct_Q1_full(Ys, taus, td, b, b))
+                  - np.vstack(O.construct_Q1(Ys, taus, td, b))).max() < 1e-13
+    ID = rng.standard_normal((b, b))
+    qid = np.vstack(O.construct_Q1_full(Ys, taus, td, b, b, ID=ID))
+    assert np.abs(qid - Q[:, :b] @ ID).max() < 1e-12
+    if L == 1:   # one rank: dorgqr of the leaf (construct_tsqr_Q.cxx:87-102)
+        k = b + 4
+        a = np.zeros((m, k), order="F")
+        a[:, :b] = Ys[0][:, :b]
+        q, _, info = lp.dorgqr(a, taus[0])
+        assert np.abs(np.vstack(O.construct_Q1_full(Ys, taus, td, b, k)) - q).max() < 1e-13
+
+
+def _aa_defect(A2, k):
+    """test_qr_tree_2d.cxx:83-143 metric for QR of [A, A] on one rank."""
+    m = A2.shape[0]
+    L, Rr = A2[:, :k], A2[:, k:2 * k]
+    f = 0.0
+    for c in range(k):
+        for r in range(m):
+            if r > c:
+                f += Rr[r, c] ** 2
+            else:
+                d = L[r, c] - Rr[r, c]
+                f += min(d * d, (d / L[r, c]) ** 2)
+    return np.sqrt(f)
+
+
+def test_qr_tree_2d_one_rank_is_dgeqrf_and_dormqr():
+    rng = np.random.default_rng(11)
+    m, k, b = 90, 40, 8
+    A = rng.standard_normal((m, k))
+    out = O.QR_tree_2D(A, b)
+    qr, tau, _, info = lp.dgeqrf(np.asfortranarray(A.copy()))
+    assert np.abs(out - qr).max() < 1e-12
+    # stop_at: factor 24 columns, Qᵀ on the rest
+    out = O.QR_tree_2D(A, b, stop_at=24)
+    qr, tau, _, _ = lp.dgeqrf(np.asfortranarray(A[:, :24].copy()))
+    rest, _, _ = lp.dormqr("L", "T", qr, tau, np.asfortranarray(A[:, 24:].copy()), 4096)
+    assert np.abs(out[:, :24] - qr).max() < 1e-12
+    assert np.abs(out[:, 24:] - rest).max() < 1e-12
+    # the HR-based QR_2D computes the same factorization for m > k
+    assert np.abs(O.QR_tree_2D(A, b) - O.QR_2D(A, b)).max() < 1e-11
+
+
+@pytest.mark.parametrize("ranks", [1, 3])
+def test_qr_tree_2d_aa_invariant(ranks):
+    m, k, b = 64, 24, 8
+    A = O.qr_2d_test_matrix(m, k)
+    split = None if ranks == 1 else (lambda mm, bw: np.diff(_leaf_split(mm, bw, min(ranks, mm // bw))))
+    out = O.QR_tree_2D(np.hstack([A, A]), b, split=split, stop_at=min(m, k))
+    assert _aa_defect(out, k) < 1e-10
+    # R (up to row signs) is LAPACK's
+    qr, _, _, _ = lp.dgeqrf(np.asfortranarray(A.copy()))
+    Rl, Rt = np.triu(qr[:k]), np.triu(out[:k, :k])
+    sg = np.sign(np.diag(Rt) / np.diag(Rl))
+    assert np.abs(Rt - sg[:, None] * Rl).max() < 1e-11
+
+
+def test_tree_data_packed_layout():
+    m, b, L = 96, 20, 4
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((m, b))
+    o = _leaf_split(m, b, L)
+    Ys, R, taus, td = O.bitree_tsqr([A[o[r]:o[r + 1]].copy(order="F") for r in range(L)], b)
+    nb = min(b, O.TAU_BLK)
+    for r in range(L):
+        p = O.tree_data_packed(td[r], b)
+        assert p.size == len(td[r]) * (nb * b + b * (b + 1) // 2)
+        for lv, (T, V2) in enumerate(td[r]):
+            base = lv * (nb * b + b * (b + 1) // 2)
+            Tb = p[base:base + nb * b].reshape(b, nb).T
+            assert np.array_equal(Tb, T)
+            assert np.array_equal(O.unpack_upper(p[base + nb * b:base + nb * b + b * (b + 1) // 2], b),
+                                  np.triu(V2))
+            # the blocked T's diagonal is the nodes' τ: the reflectors reproduce [R_r; R_s] → [R; 0]
+            assert np.all(np.abs(np.diag(T[np.arange(b) % nb, np.arange(b)])) <= 2.0)
