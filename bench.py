@@ -156,6 +156,42 @@ def lsq_leg(qr, torch, m=65536, n=2048, kappas=(1e2, 1e6, 1e8), seed=11):
     return out
 
 
+def tree_leg(qr, torch, m=32768, b=128, k=3968, seed=19, reps=3):
+    """Implicit TSQR Q (apply_tsqr_QT) of one 32768×128 panel's tree applied
+    to the 32768×3968 trailing block, and the tree-updated QR_tree_2D of a
+    16384×2048 matrix (one rank: dgeqrf + dormqr per panel; automatic
+    leaves: TSQR + apply_tsqr_QT per panel)."""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    L, _, _ = qr.plan_tsqr(m, b, 0)
+    P = torch.randn(m * b, generator=g, dtype=torch.float64, device="cuda")
+    R = torch.zeros(b * b, dtype=torch.float64, device="cuda")
+    tau = torch.zeros(L * b, dtype=torch.float64, device="cuda")
+    td = torch.zeros(qr.tree_data_size(m, b, 0), dtype=torch.float64, device="cuda")
+    qr.bitree_tsqr(P, m, R, tau, m, b, numPes=0, output_Y=1, tree_data=td)
+    B = torch.randn(m * k, generator=g, dtype=torch.float64, device="cuda")
+    run = lambda: qr.apply_tsqr_QT(P, m, tau, B, m, m, b, k, numPes=0, tree_data=td)
+    run()
+    ms = min(_timed(run, torch)[1] for _ in range(reps))
+    flops = 4.0 * m * b * k + 2.0 * L * b * b * k + 6.0 * (L - 1) * b * b * k
+    out = {"apply_tsqr_QT": {"m": m, "b": b, "k": k, "leaves": L, "ms": round(ms, 3),
+                             "tflops_alg": round(flops / ms / 1e9, 2)}}
+    mm, kk = 16384, 2048
+    A0 = torch.randn(mm * kk, generator=g, dtype=torch.float64, device="cuda")
+    A = torch.empty_like(A0)
+    for name, P_ in (("one_rank", 1), ("leaves", 0)):
+        def f():
+            qr.QR_tree_2D(A, mm, mm, kk, b, numPes=P_)
+        A.copy_(A0)
+        f()
+        best = 1e30
+        for _ in range(2):
+            A.copy_(A0)
+            best = min(best, _timed(f, torch)[1])
+        out[f"qr_tree_2d_{name}"] = {"m": mm, "k": kk, "b": b, "ms": round(best, 2),
+                                     "gflops": round(qr_flops(mm, kk) / best / 1e6, 1)}
+    return out
+
+
 def batch_leg(qr, torch, nprob=2048, seed=13, loop=True):
     """2048 tall-skinny QRs, m ~ U[256, 16384], n ∈ {16, 32, 64, 128}:
     batched launches (autotuned leaf heights, graph-replayed) against the
@@ -277,6 +313,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-lsq", action="store_true", help="skip the mixed-precision least-squares leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched tall-skinny QR leg")
+    ap.add_argument("--no-tree", action="store_true", help="skip the implicit-Q (apply_tsqr_QT) leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-loop", action="store_true", help="skip the per-problem loop baseline of the batch")
     args = ap.parse_args()
@@ -398,6 +435,9 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_lsq:
         out["lsq_mixed"] = lsq_leg(qr, torch)
+        torch.cuda.empty_cache()
+    if not args.no_tree:
+        out["tree_apply"] = tree_leg(qr, torch)
         torch.cuda.empty_cache()
     if not args.no_batch:
         out["batch"] = batch_leg(qr, torch, loop=not args.no_loop)

@@ -256,10 +256,18 @@ __global__ void __launch_bounds__(256) bgemm_kernel(BgemmArgs g) {
   double* C = const_cast<double*>(bmat_ptr(g.C, z, g));
   const int64_t lda = g.A.ld, ldb = g.B.ld, ldc = g.C.ld;
 
+  // 64×64 output tile, 16-deep k slabs in LDS ([k][m] and [k][n]); each of
+  // the four waves owns a 32×32 quarter as 2×2 FP64 MFMA 16×16 tiles
   __shared__ double As[16][65];
   __shared__ double Bs[16][65];
-  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  double acc[4][4] = {};
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int fr = lane >> 4, fc = lane & 15;
+  const int wm0 = (wave & 1) * 32, wn0 = (wave >> 1) * 32;
+  dbl4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
   for (int k0 = 0; k0 < k; k0 += 16) {
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {
@@ -287,33 +295,36 @@ __global__ void __launch_bounds__(256) bgemm_kernel(BgemmArgs g) {
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk) {
-      double av[4], bv[4];
+    for (int kk = 0; kk < 16; kk += 4) {
+      double af[2], bf[2];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) av[i] = As[kk][ty + 16 * i];
+      for (int i = 0; i < 2; ++i) af[i] = As[kk + fr][wm0 + 16 * i + fc];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][tx + 16 * j];
+      for (int j = 0; j < 2; ++j) bf[j] = Bs[kk + fr][wn0 + 16 * j + fc];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
+  // lane l, register v of acc[i][j] holds C[m0+wm0+16i+(l&15)][n0+wn0+16j+(l>>4)+4v]
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int gm = m0 + ty + 16 * i;
+  for (int i = 0; i < 2; ++i) {
+    const int gm = m0 + wm0 + 16 * i + fc;
     if (gm >= m) continue;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int gn = n0 + tx + 16 * j;
-      if (gn >= n) continue;
-      double* cp = C + gm + (int64_t)gn * ldc;
-      double v = g.alpha * acc[i][j];
-      if (g.beta != 0.0) v += g.beta * *cp;
-      if (g.smC == SM_UPPER && gm > gn) v = 0.0;
-      *cp = v;
-    }
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int gn = n0 + wn0 + 16 * j + fr + 4 * v;
+        if (gn >= n) continue;
+        double* cp = C + gm + (int64_t)gn * ldc;
+        double val = g.alpha * acc[i][j][v];
+        if (g.beta != 0.0) val += g.beta * *cp;
+        if (g.smC == SM_UPPER && gm > gn) val = 0.0;
+        *cp = val;
+      }
   }
 }
 
