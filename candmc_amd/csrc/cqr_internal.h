@@ -248,6 +248,10 @@ int wy_update_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const
 int wy_update_f64_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const double* Tt,
                      double* A, int64_t lda, int64_t m, int64_t n, int b, double* ws);
 size_t gemm_f64_scratch_doubles(int64_t m, int64_t n, int64_t k);
+// A (m×n) −= YT·(Yeᵀ·A) with an explicit unit-lower Ye and YT = Ye·Tt (both
+// m×b, ld m); workspace wy_update_ws_doubles(m, n, b) doubles.
+int wy_update_explicit(cqr_ctx* c, const double* Ye, const double* YT, int64_t m, int64_t n, int b, double* A,
+                       int64_t lda, double* ws);
 
 // Least-squares kernels (lsq_kernels.hip): C = alpha·op(A)·X + beta·C for
 // X with nr ≤ 16 columns; the transposed form needs
@@ -275,7 +279,7 @@ int finite_check(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n)
 // QR_2D workspace layout (byte offsets into one arena) and the driver; the
 // arena base is passed so callers can place their own buffers behind it.
 struct Qr2dWS {
-  size_t panel, upd_near, upd_far, W, Tt;
+  size_t panel, upd_near, upd_far, W, Tt, Ye;
 };
 size_t qr2d_ws(int64_t m, int64_t k, int64_t b, int64_t leaf_rows, Qr2dWS* o);
 int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b, double* aggreg_Y,

@@ -564,6 +564,34 @@ int wy_update_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const
                        PK_GEMM_UPD, prec);
 }
 
+// Trailing update with an explicit unit-lower Y (Ye, m×b, ld m) and
+// YT = Ye·Tᵀ-factor product (m×b, ld m, i.e. Ye·Tt): A (m×n) −= YT·(Yeᵀ·A).
+// No structural masks remain, so both products are plain: Z = Yeᵀ·A on the
+// lean kernel (split-K, from three column tiles on) or the general one, and
+// the update through the vendor BLAS.
+int wy_update_explicit(cqr_ctx* c, const double* Ye, const double* YT, int64_t m, int64_t n, int b, double* A,
+                       int64_t lda, double* ws) {
+  if (m <= 0 || n <= 0 || b <= 0) return CQR_OK;
+  double* part = ws;
+  const size_t np = gemm_f64_scratch_doubles(b, n, m);
+  double* Z = ws + ((np + 31) / 32) * 32;
+  const int ns = np ? (int)(np / ((size_t)b * n)) : 0;
+  if (n > 2 * 128 && ns >= 2) {
+    int used = 0;
+    CQR_TRY(lean_gemm(c, 1, b, n, m, 1.0, Ye, m, A, lda, 0.0, nullptr, 0, part, ns, PK_GEMM_YTA, &used));
+    const int gx = (int)std::min<int64_t>(2048, ((int64_t)b * n + 255) / 256);
+    hipEvent_t ev = prof_begin(c);
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(gx), dim3(256), 0, c->stream, part, (int64_t)b * n, used,
+                       (int64_t)b, n, Z, (int64_t)b, 1.0, 0.0);
+    prof_end(c, ev, PK_REDUCE, (double)used * b * n, 8.0 * (used + 1.0) * b * n);
+    CQR_TRY(check_launch(c, "splitk_reduce"));
+  } else {
+    CQR_TRY(gemm_f64_impl(c, 1, 0, b, n, m, 1.0, Ye, m, SM_NONE, A, lda, SM_NONE, 0.0, Z, b, np ? part : nullptr,
+                          PK_GEMM_YTA));
+  }
+  return blas_update(c, m, n, b, YT, m, Z, b, A, lda);
+}
+
 int wy_update_f64_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const double* Tt,
                      double* A, int64_t lda, int64_t m, int64_t n, int b, double* ws) {
   return wy_update_ws(c, Y, ldy, y_implicit, Tt, 0, A, lda, m, n, b, ws, CQR_PREC_F64);

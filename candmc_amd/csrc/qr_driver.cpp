@@ -386,6 +386,7 @@ size_t qr2d_ws(int64_t m, int64_t k, int64_t b, int64_t leaf_rows, Qr2dWS* o) {
   o->upd_far = s.add(8 * (max_update_ws(m, k, b, 2) + 64));
   o->W = s.add(8 * b0 * b0);
   o->Tt = s.add(8 * 2 * b0 * b0);
+  o->Ye = s.add(8 * 4 * (size_t)m * b0);   // explicit Y and Y·Tt, two panels each
   return s.bytes;
 }
 
@@ -404,6 +405,11 @@ int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b
   double* ws_near = reinterpret_cast<double*>(base + o.upd_near);
   double* ws_far = reinterpret_cast<double*>(base + o.upd_far);
   const bool la = c->lookahead && k > 2 * b && m > 2 * b;
+  // FP64 with the vendor BLAS: each panel's Y is made explicit (unit lower)
+  // and Y·Tt formed once, so both updates are two plain products
+  const bool expl = prec == CQR_PREC_F64 && c->blas_update;
+  double* Ye2[2] = {reinterpret_cast<double*>(base + o.Ye),
+                    reinterpret_cast<double*>(base + o.Ye) + 2 * m * std::min(k, b)};
   if (la) {  // the side stream starts after everything already queued
     CQR_HIP(c, hipEventRecord(c->ev_fork, c->stream));
     CQR_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
@@ -416,9 +422,18 @@ int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b
     if ((rc = hh_recon_panel(c, P, lda, mj, bj, W, nullptr, nullptr, Tt, 0, base + o.panel, nullptr))) break;
     if (aggreg_Y && (rc = launch_copy_unit_lower(c, P, lda, aggreg_Y + j + j * lda_aY, lda_aY, mj, bj))) break;
     if (!(kj - b > 0 && mj - b > 0)) break;
+    double* Ye = Ye2[p & 1];
+    double* YT = Ye + mj * bj;
+    if (expl) {
+      if ((rc = launch_copy_unit_lower(c, P, lda, Ye, mj, mj, bj))) break;
+      if ((rc = blas_gemm(c, 0, mj, bj, bj, 1.0, Ye, mj, Tt, bj, 0.0, YT, mj, PK_GEMM_SMALL))) break;
+    }
+    auto update = [&](int64_t col0, int64_t ncols, double* ws) -> int {
+      if (expl) return wy_update_explicit(c, Ye, YT, mj, ncols, (int)bj, P + col0 * lda, lda, ws);
+      return wy_update_ws(c, P, lda, 1, Tt, 0, P + col0 * lda, lda, mj, ncols, (int)bj, ws, prec);
+    };
     if (!la) {
-      if ((rc = wy_update_ws(c, P, lda, 1, Tt, 0, P + bj * lda, lda, mj, kj - bj, (int)bj, ws_near, prec)))
-        break;
+      if ((rc = update(bj, kj - bj, ws_near))) break;
       continue;
     }
     // Lookahead (qr_2d.cxx:378-635): the next panel's b columns first, on
@@ -431,7 +446,7 @@ int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b
       rc = set_err(c, CQR_ERR_DEVICE, "QR_2D: stream wait failed");
       break;
     }
-    if ((rc = wy_update_ws(c, P, lda, 1, Tt, 0, P + bj * lda, lda, mj, bn, (int)bj, ws_near, prec))) break;
+    if ((rc = update(bj, bn, ws_near))) break;
     if (rest > 0) {
       if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess ||
           hipStreamWaitEvent(c->side, c->ev_fork, 0) != hipSuccess) {
@@ -439,8 +454,7 @@ int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b
         break;
       }
       OnSide g(c);
-      if ((rc = wy_update_ws(c, P, lda, 1, Tt, 0, P + (bj + bn) * lda, lda, mj, rest, (int)bj, ws_far, prec)))
-        break;
+      if ((rc = update(bj + bn, rest, ws_far))) break;
       if (hipEventRecord(c->ev_join, c->side) != hipSuccess) {
         rc = set_err(c, CQR_ERR_DEVICE, "QR_2D: event record failed");
         break;
