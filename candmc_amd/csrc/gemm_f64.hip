@@ -304,6 +304,48 @@ __global__ void splitk_reduce_kernel(const double* __restrict__ P, int64_t strid
   }
 }
 
+// Few outputs over many partial slots (the next panel's b×b-wide Yᵀ·A): 64
+// outputs per workgroup, four groups of slots per output summed in a fixed
+// order, so the grid covers the chip even for a single 128×128 tile.
+__global__ void __launch_bounds__(256) splitk_reduce_grouped_kernel(const double* __restrict__ P, int64_t stride,
+                                                                    int nsplit, int64_t m, int64_t n,
+                                                                    double* __restrict__ Z, int64_t ldz,
+                                                                    double alpha, double beta) {
+  __shared__ double part[4][64];
+  const int t = threadIdx.x, el = t & 63, g = t >> 6;
+  const int64_t e = (int64_t)blockIdx.x * 64 + el;
+  const bool ok = e < m * n;
+  double s = 0.0;
+  if (ok)
+    for (int z = g; z < nsplit; z += 4) s += P[z * stride + e];
+  part[g][el] = s;
+  __syncthreads();
+  if (g == 0 && ok) {
+    const double tot = (part[0][el] + part[1][el]) + (part[2][el] + part[3][el]);
+    double* zp = Z + (e % m) + (e / m) * ldz;
+    double v = alpha * tot;
+    if (beta != 0.0) v += beta * *zp;
+    *zp = v;
+  }
+}
+
+// Z (m×n, ld ldz) = alpha·Σ_z P_z + beta·Z over ns partials (ld m, stride
+// apart), summed in a fixed order.
+static int launch_splitk_reduce(cqr_ctx* c, const double* P, int64_t stride, int ns, int64_t m, int64_t n, double* Z,
+                                int64_t ldz, double alpha, double beta) {
+  hipEvent_t ev = prof_begin(c);
+  const int64_t flat = (m * n + 255) / 256;
+  if (flat < 256 && ns >= 16) {
+    hipLaunchKernelGGL(splitk_reduce_grouped_kernel, dim3((unsigned)((m * n + 63) / 64)), dim3(256), 0, c->stream,
+                       P, stride, ns, m, n, Z, ldz, alpha, beta);
+  } else {
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)std::min<int64_t>(2048, flat)), dim3(256), 0, c->stream,
+                       P, stride, ns, m, n, Z, ldz, alpha, beta);
+  }
+  prof_end(c, ev, PK_REDUCE, (double)ns * m * n, 8.0 * (ns + (beta != 0.0 ? 2.0 : 1.0)) * m * n);
+  return check_launch(c, "splitk_reduce");
+}
+
 template <int ROLE, int PREC = CQR_PREC_F64>
 static void launch_role(hipStream_t st, int ta, int tb, dim3 grid, const GemmArgs& g) {
   if (!ta && !tb) hipLaunchKernelGGL((mfma_gemm_f64<false, false, ROLE, PREC>), grid, dim3(256), 0, st, g);
@@ -346,10 +388,13 @@ static int launch_mfma(cqr_ctx* c, int ta, int tb, const GemmArgs& g, int nsplit
 // Split-K choice: enough blocks to fill 256 CUs twice, chunks of ≥ 256 rows.
 static int choose_splits(int64_t m, int64_t n, int64_t k, int64_t* kchunk) {
   const int64_t tiles = ((m + 127) / 128) * ((n + 127) / 128);
+  // one or two output tiles (the next panel's Yᵀ·A): 128-deep chunks, so
+  // that the splits alone give every CU a workgroup
+  const int64_t kmin = tiles <= 2 ? 128 : 256;
   int64_t s = 1;
   if (tiles < 512 && k > 512) {
     s = (512 + tiles - 1) / tiles;
-    s = std::min<int64_t>(s, (k + 255) / 256);
+    s = std::min<int64_t>(s, (k + kmin - 1) / kmin);
     s = std::max<int64_t>(s, 1);
   }
   int64_t kc = (k + s - 1) / s;
@@ -439,12 +484,7 @@ static int gemm_f64_impl(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64
   }
   g.C = scratch; g.ldc = m; g.kchunk = kc; g.split_stride = m * n;
   CQR_TRY(launch_mfma(c, ta, tb, g, ns, kind, prec));
-  const int gx = (int)std::min<int64_t>(2048, (m * n + 255) / 256);
-  hipEvent_t ev = prof_begin(c);
-  hipLaunchKernelGGL(splitk_reduce_kernel, dim3(gx), dim3(256), 0, c->stream, scratch, m * n, ns, m, n,
-                     C, ldc, alpha, beta);
-  prof_end(c, ev, PK_REDUCE, (double)ns * m * n, 8.0 * (ns + (beta != 0.0 ? 2.0 : 1.0)) * m * n);
-  return check_launch(c, "splitk_reduce");
+  return launch_splitk_reduce(c, scratch, m * n, ns, m, n, C, ldc, alpha, beta);
 }
 
 // Batched over the problems of a group: operand modes BM_PROB_* and
@@ -521,12 +561,7 @@ static int wy_update_f64_split(cqr_ctx* c, const double* Y, int64_t ldy, int smY
   CQR_TRY(lean_gemm(c, 1, b, n, m - bt, 1.0, Y + bt, ldy, A + bt, lda, 0.0, nullptr, 0, part + (int64_t)b * n,
                     ns - 1, PK_GEMM_YTA, &used));
   const int nslots = 1 + used;
-  const int gx = (int)std::min<int64_t>(2048, ((int64_t)b * n + 255) / 256);
-  hipEvent_t ev = prof_begin(c);
-  hipLaunchKernelGGL(splitk_reduce_kernel, dim3(gx), dim3(256), 0, c->stream, part, (int64_t)b * n, nslots,
-                     (int64_t)b, n, Z, (int64_t)b, 1.0, 0.0);
-  prof_end(c, ev, PK_REDUCE, (double)nslots * b * n, 8.0 * (nslots + 1.0) * b * n);
-  CQR_TRY(check_launch(c, "splitk_reduce"));
+  CQR_TRY(launch_splitk_reduce(c, part, (int64_t)b * n, nslots, b, n, Z, b, 1.0, 0.0));
   CQR_TRY(gemm_f64_impl(c, t_trans ? 1 : 0, 0, b, n, b, 1.0, Tt, b, SM_LOWER, Z, b, SM_NONE, 0.0, Zp, b, nullptr));
   // A(0:bt) −= Y(0:bt)·Z' (masked), A(bt:m) −= Y(bt:m)·Z' (lean)
   CQR_TRY(gemm_f64_impl(c, 0, 0, bt, n, b, -1.0, Y, ldy, smY, Zp, b, SM_NONE, 1.0, A, lda, nullptr, PK_GEMM_UPD));
@@ -576,15 +611,10 @@ int wy_update_explicit(cqr_ctx* c, const double* Ye, const double* YT, int64_t m
   const size_t np = gemm_f64_scratch_doubles(b, n, m);
   double* Z = ws + ((np + 31) / 32) * 32;
   const int ns = np ? (int)(np / ((size_t)b * n)) : 0;
-  if (n > 2 * 128 && ns >= 2) {
+  if (ns >= 2) {
     int used = 0;
     CQR_TRY(lean_gemm(c, 1, b, n, m, 1.0, Ye, m, A, lda, 0.0, nullptr, 0, part, ns, PK_GEMM_YTA, &used));
-    const int gx = (int)std::min<int64_t>(2048, ((int64_t)b * n + 255) / 256);
-    hipEvent_t ev = prof_begin(c);
-    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(gx), dim3(256), 0, c->stream, part, (int64_t)b * n, used,
-                       (int64_t)b, n, Z, (int64_t)b, 1.0, 0.0);
-    prof_end(c, ev, PK_REDUCE, (double)used * b * n, 8.0 * (used + 1.0) * b * n);
-    CQR_TRY(check_launch(c, "splitk_reduce"));
+    CQR_TRY(launch_splitk_reduce(c, part, (int64_t)b * n, used, b, n, Z, b, 1.0, 0.0));
   } else {
     CQR_TRY(gemm_f64_impl(c, 1, 0, b, n, m, 1.0, Ye, m, SM_NONE, A, lda, SM_NONE, 0.0, Z, b, np ? part : nullptr,
                           PK_GEMM_YTA));
