@@ -172,3 +172,53 @@ def test_tree_data_export_layout(Q, m, b, P):
             Tg, Tw = g[:nb * b].reshape(b, nb).T, w[:nb * b].reshape(b, nb).T
             assert rel(Tg * keep, Tw * keep) < 1e-11
             assert rel(g[nb * b:], w[nb * b:]) < 1e-11
+
+
+def test_construct_Q1_full_square_Q(Q):
+    """k = m: the whole orthogonal Q of the TSQR (QᵀQ = I, Qᵀ·A = [R; 0])."""
+    m, b, P = 600, 40, 4
+    A = O.gaussian(m, b, 1234)
+    f = factor(Q, A, b, P)
+    dQ = torch.zeros(m * m, dtype=torch.float64, device="cuda")
+    Q.construct_Q1(f["dY"], m, f["dtau"], dQ, m, m, b, m, numPes=P, tree_data=f["td"])
+    Qm = Q.from_gpu(dQ, m, m)
+    assert np.abs(Qm.T @ Qm - np.eye(m)).max() < 1e-13
+    R = Q.from_gpu(f["dR"], b, b)
+    z = np.zeros((m, b))
+    z[:b] = R
+    assert rel(Qm.T @ A, z) < 1e-13
+    assert rel(Qm, np.vstack(O.construct_Q1_full(f["Ys"], f["taus"], f["otd"], b, m))) < 1e-12
+
+
+def test_apply_tsqr_strided_and_empty(Q):
+    """B with a leading dimension larger than m and k = 0 (a no-op)."""
+    m, b, P, k, ldb = 1000, 50, 3, 9, 1031
+    A = O.gaussian(m, b, 77)
+    B = O.gaussian(m, k, 78)
+    f = factor(Q, A, b, P)
+    Bp = np.zeros((ldb, k))
+    Bp[:m] = B
+    dB = Q.to_gpu(Bp)
+    Q.apply_tsqr_QT(f["dY"], m, f["dtau"], dB, ldb, m, b, k, numPes=P, tree_data=f["td"])
+    got = Q.from_gpu(dB, ldb, k)
+    want = np.vstack(O.apply_tsqr_QT(f["Ys"], f["taus"], f["otd"], blocks(B, f["o"]), b))
+    assert rel(got[:m], want) < 1e-12
+    assert np.array_equal(got[m:], Bp[m:])                 # rows beyond m untouched
+    Q.apply_tsqr_QT(f["dY"], m, f["dtau"], dB, ldb, m, b, 0, numPes=P, tree_data=f["td"])
+    assert np.array_equal(Q.from_gpu(dB, ldb, k), got)
+
+
+@pytest.mark.parametrize("P", [1, 0, 3])
+def test_qr_tree_2d_wide(Q, P):
+    """m < k: min(m, k) = m columns factored, the rest receive Qᵀ."""
+    m, k, b = 300, 520, 64
+    A = O.gaussian(m, k, 4321)
+    dA = Q.to_gpu(A)
+    Q.QR_tree_2D(dA, m, m, k, b, numPes=P)
+
+    def split(mm, bw):
+        L, _, _ = Q.plan_tsqr(mm, bw, P)
+        return np.diff(offsets(mm, L))
+
+    want = O.QR_tree_2D(A, b, split=None if P == 1 else split)
+    assert rel(Q.from_gpu(dA, m, k), want) < 1e-11
