@@ -248,6 +248,11 @@ int wy_update_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const
 int wy_update_f64_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const double* Tt,
                      double* A, int64_t lda, int64_t m, int64_t n, int b, double* ws);
 size_t gemm_f64_scratch_doubles(int64_t m, int64_t n, int64_t k);
+// C (m×n) := alpha·op(A)·op(B) + beta·C for n ≤ 16 and m, k ≤ 256 in one
+// workgroup; C may alias B (B is read completely first).
+int launch_small_thin(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                      int64_t lda, int smA, const double* B, int64_t ldb, int smB, double beta, double* C,
+                      int64_t ldc, int kind);
 // A (m×n) −= YT·(Yeᵀ·A) with an explicit unit-lower Ye and YT = Ye·Tt (both
 // m×b, ld m); workspace wy_update_ws_doubles(m, n, b) doubles.
 int wy_update_explicit(cqr_ctx* c, const double* Ye, const double* YT, int64_t m, int64_t n, int b, double* A,

@@ -463,6 +463,83 @@ static int launch_small(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64_
   return check_launch(c, "small_gemm");
 }
 
+// Products with few columns (n ≤ 16; m, k ≤ 256: the solves' R blocks and
+// the b×b×nrhs steps of the Q application): one workgroup, op(B) staged in
+// LDS, one thread per output row with NR accumulators, reading its row of
+// op(A) straight from memory 64 values at a time (all 64 loads in flight
+// before the first is used).  Every element of B is read before any element
+// of C is written, so C may alias B.
+template <int NR>
+__global__ void __launch_bounds__(256) small_thin_kernel(SmallArgs g) {
+  constexpr int KC = 64;
+  __shared__ double Bs[256][NR + 1];
+  const int t = threadIdx.x;
+  {  // Bs[kk][j] = op(B)(kk, j): thread t loads row kk = t
+    double v[NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      const bool ok = t < g.k && j < g.n;
+      const int64_t sr = g.tb ? j : t, sc = g.tb ? t : j;
+      v[j] = g.B[ok ? sr + sc * g.ldb : 0];
+    }
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      const bool ok = t < g.k && j < g.n;
+      const int64_t sr = g.tb ? j : t, sc = g.tb ? t : j;
+      if (t < g.k) Bs[t][j] = ok ? gmask(v[j], sr, sc, g.smB) : 0.0;
+    }
+  }
+  __syncthreads();
+  if (t >= g.m) return;
+  double acc[NR];
+#pragma unroll
+  for (int j = 0; j < NR; ++j) acc[j] = 0.0;
+  for (int k0 = 0; k0 < g.k; k0 += KC) {
+    double v[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) {
+      const int kk = k0 + q;
+      const bool ok = kk < g.k;
+      const int64_t sr = g.ta ? kk : t, sc = g.ta ? t : kk;
+      v[q] = g.A[ok ? sr + sc * g.lda : 0];
+    }
+#pragma unroll
+    for (int q = 0; q < KC; ++q) {
+      const int kk = k0 + q;
+      if (kk < g.k) {
+        const int64_t sr = g.ta ? kk : t, sc = g.ta ? t : kk;
+        const double a = gmask(v[q], sr, sc, g.smA);
+#pragma unroll
+        for (int j = 0; j < NR; ++j) acc[j] += a * Bs[kk][j];
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    if (j >= g.n) continue;
+    double* cp = g.C + t + (int64_t)j * g.ldc;
+    *cp = g.alpha * acc[j] + (g.beta != 0.0 ? g.beta * *cp : 0.0);
+  }
+}
+
+int launch_small_thin(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                      int64_t lda, int smA, const double* B, int64_t ldb, int smB, double beta, double* C,
+                      int64_t ldc, int kind) {
+  if (m <= 0 || n <= 0) return CQR_OK;
+  if (m > 256 || k > 256 || n > 16)
+    return set_err(c, CQR_ERR_INVALID, "small_thin: %lld×%lld×%lld outside 256×16×256", (long long)m, (long long)n,
+                   (long long)k);
+  SmallArgs g{};
+  g.m = (int)m; g.n = (int)n; g.k = (int)k; g.ta = ta; g.tb = tb; g.smA = smA; g.smB = smB;
+  g.alpha = alpha; g.beta = beta; g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  hipEvent_t ev = prof_begin(c);
+  if (n == 1) hipLaunchKernelGGL(small_thin_kernel<1>, dim3(1), dim3(256), 0, c->stream, g);
+  else if (n <= 4) hipLaunchKernelGGL(small_thin_kernel<4>, dim3(1), dim3(256), 0, c->stream, g);
+  else hipLaunchKernelGGL(small_thin_kernel<16>, dim3(1), dim3(256), 0, c->stream, g);
+  prof_end(c, ev, kind, 2.0 * m * n * k, 8.0 * ((double)m * k + (double)k * n + 2.0 * m * n));
+  return check_launch(c, "small_thin");
+}
+
 // Generic C := alpha·op(A)·op(B) + beta·C; split-K through the workspace
 // when the output is small and K long.  `scratch` must hold nsplit·m·n
 // doubles when splitting (callers size it with gemm_f64_scratch).
@@ -471,6 +548,8 @@ static int gemm_f64_impl(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64
                          double beta, double* C, int64_t ldc, double* scratch, int kind = PK_GEMM_SMALL,
                          int prec = CQR_PREC_F64) {
   if (m <= 0 || n <= 0) return CQR_OK;
+  if (prec == CQR_PREC_F64 && m <= 256 && n <= 16 && k <= 256)
+    return launch_small_thin(c, ta, tb, m, n, k, alpha, A, lda, smA, B, ldb, smB, beta, C, ldc, kind);
   if (prec == CQR_PREC_F64 && m <= 256 && n <= 256 && k <= 256)
     return launch_small(c, ta, tb, m, n, k, alpha, A, lda, smA, B, ldb, smB, beta, C, ldc, kind);
   GemmArgs g{};

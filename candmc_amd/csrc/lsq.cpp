@@ -27,6 +27,36 @@ namespace cqr {
 int apply_q_ws(cqr_ctx* c, const double* YR, int64_t ldy, const double* Tt, int64_t m, int64_t k, int64_t b,
                double* B, int64_t ldb, int64_t nr, int trans, double* ws) {
   const int64_t np = (k + b - 1) / b;
+  if (nr <= 16) {
+    // Few columns (solves, refinement): each panel is two HBM-bound thin
+    // products over Y's rows below its top block plus three b×b×nr
+    // products (the unit-lower top block of Y, and T)
+    double* scr = ws;
+    double* Z = ws + (thin_gemm_scratch_doubles(m, b, (int)nr) + 31) / 32 * 32;
+    double* Zp = Z + (b * nr + 31) / 32 * 32;
+    for (int64_t q = 0; q < np; ++q) {
+      const int64_t p = trans ? q : np - 1 - q;
+      const int64_t j = p * b, bj = std::min(b, k - j), mj = m - j;
+      if (mj <= 0) continue;
+      const double* Yp = YR + j + j * ldy;
+      double* Bp = B + j;
+      const int64_t bt = std::min(bj, mj);
+      // Z = Yᵀ·B: top block (unit lower) then the rows below it
+      CQR_TRY(gemm_f64_ws(c, 1, 0, bj, nr, bt, 1.0, Yp, ldy, SM_UNIT_LOWER, Bp, ldb, SM_NONE, 0.0, Z, bj,
+                          nullptr));
+      if (mj > bt)
+        CQR_TRY(thin_gemm(c, 1, mj - bt, bj, 1.0, Yp + bt, ldy, Bp + bt, ldb, (int)nr, 1.0, Z, bj, scr));
+      // Z' = Tᵀ·Z (Qᵀ) or T·Z (Q); Tt holds Tᵀ (lower, ld bj)
+      CQR_TRY(gemm_f64_ws(c, trans ? 0 : 1, 0, bj, nr, bj, 1.0, Tt + p * b * b, bj, SM_LOWER, Z, bj, SM_NONE, 0.0,
+                          Zp, bj, nullptr));
+      // B −= Y·Z'
+      CQR_TRY(gemm_f64_ws(c, 0, 0, bt, nr, bj, -1.0, Yp, ldy, SM_UNIT_LOWER, Zp, bj, SM_NONE, 1.0, Bp, ldb,
+                          nullptr));
+      if (mj > bt)
+        CQR_TRY(thin_gemm(c, 0, mj - bt, bj, -1.0, Yp + bt, ldy, Zp, bj, (int)nr, 1.0, Bp + bt, ldb, nullptr));
+    }
+    return CQR_OK;
+  }
   for (int64_t q = 0; q < np; ++q) {
     const int64_t p = trans ? q : np - 1 - q;
     const int64_t j = p * b, bj = std::min(b, k - j);
@@ -37,7 +67,11 @@ int apply_q_ws(cqr_ctx* c, const double* YR, int64_t ldy, const double* Tt, int6
   return CQR_OK;
 }
 
-size_t apply_q_ws_doubles(int64_t m, int64_t b, int64_t nr) { return wy_update_ws_doubles(m, nr, (int)b) + 64; }
+size_t apply_q_ws_doubles(int64_t m, int64_t b, int64_t nr) {
+  const size_t thin = nr <= 16 ? (thin_gemm_scratch_doubles(m, b, (int)nr) + 31) / 32 * 32 + 2 * ((b * nr + 31) / 32 * 32)
+                               : 0;
+  return std::max(wy_update_ws_doubles(m, nr, (int)b), thin) + 64;
+}
 
 // Inverses of the diagonal blocks of R (upper triangle of YR), 128 wide.
 int r_block_inverses(cqr_ctx* c, const double* YR, int64_t ldy, int64_t n, double* Rinv) {
@@ -58,13 +92,22 @@ int trsm_r_ws(cqr_ctx* c, const double* YR, int64_t ldy, int64_t n, const double
     const int64_t p = trans ? q : nb - 1 - q;
     const int64_t j = p * 128, bj = std::min<int64_t>(128, n - j);
     const double* Ri = Rinv + p * 128 * 128;
-    // x_p = R_pp⁻¹·y_p  (or R_pp⁻ᵀ·y_p)
-    CQR_TRY(gemm_f64_ws(c, trans ? 1 : 0, 0, bj, nr, bj, 1.0, Ri, bj, SM_UPPER, B + j, ldb, SM_NONE, 0.0, tmp,
-                        bj, nullptr));
-    CQR_TRY(launch_copy2d(c, tmp, bj, B + j, ldb, bj, nr));
-    if (!trans && j > 0)        // y(0:j) −= R(0:j, p)·x_p
-      CQR_TRY(gemm_f64_ws(c, 0, 0, j, nr, bj, -1.0, YR + j * ldy, ldy, SM_NONE, B + j, ldb, SM_NONE, 1.0, B,
-                          ldb, nullptr));
+    // x_p = R_pp⁻¹·y_p  (or R_pp⁻ᵀ·y_p); with few columns in place
+    if (nr <= 16) {
+      CQR_TRY(launch_small_thin(c, trans ? 1 : 0, 0, bj, nr, bj, 1.0, Ri, bj, SM_UPPER, B + j, ldb, SM_NONE, 0.0,
+                                B + j, ldb, PK_GEMM_SMALL));
+    } else {
+      CQR_TRY(gemm_f64_ws(c, trans ? 1 : 0, 0, bj, nr, bj, 1.0, Ri, bj, SM_UPPER, B + j, ldb, SM_NONE, 0.0, tmp,
+                          bj, nullptr));
+      CQR_TRY(launch_copy2d(c, tmp, bj, B + j, ldb, bj, nr));
+    }
+    if (!trans && j > 0) {      // y(0:j) −= R(0:j, p)·x_p
+      if (nr <= 16)
+        CQR_TRY(thin_gemm(c, 0, j, bj, -1.0, YR + j * ldy, ldy, B + j, ldb, (int)nr, 1.0, B, ldb, nullptr));
+      else
+        CQR_TRY(gemm_f64_ws(c, 0, 0, j, nr, bj, -1.0, YR + j * ldy, ldy, SM_NONE, B + j, ldb, SM_NONE, 1.0, B,
+                            ldb, nullptr));
+    }
     if (trans && j + bj < n)    // y(j+bj:n) −= R(p, j+bj:n)ᵀ·x_p
       CQR_TRY(gemm_f64_ws(c, 1, 0, n - j - bj, nr, bj, -1.0, YR + j + (j + bj) * ldy, ldy, SM_NONE, B + j, ldb,
                           SM_NONE, 1.0, B + j + bj, ldb, nullptr));

@@ -105,15 +105,25 @@ thin_gemm_t_kernel(const double* __restrict__ A, int64_t lda, int64_t m, int64_t
   }
 }
 
-// C = alpha·Σ_z P_z + beta·C (fixed order).
-__global__ void slab_reduce_kernel(const double* __restrict__ P, int nslab, int64_t k, int nr, double alpha,
-                                   double beta, double* __restrict__ C, int64_t ldc) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < k * nr; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = e % k, j = e / k;
-    double s = 0.0;
-    for (int z = 0; z < nslab; ++z) s += P[(int64_t)z * k * nr + e];
-    double* cp = C + r + j * ldc;
-    *cp = alpha * s + (beta != 0.0 ? beta * *cp : 0.0);
+// C = alpha·Σ_z P_z + beta·C (fixed order): 64 outputs per workgroup,
+// sixteen groups of slabs per output, then the sixteen group sums in order.
+__global__ void __launch_bounds__(1024) slab_reduce_kernel(const double* __restrict__ P, int nslab, int64_t k, int nr,
+                                                           double alpha, double beta, double* __restrict__ C,
+                                                           int64_t ldc) {
+  __shared__ double part[16][64];
+  const int t = threadIdx.x, el = t & 63, g = t >> 6;
+  const int64_t kn = k * nr, e = (int64_t)blockIdx.x * 64 + el;
+  double s = 0.0;
+  if (e < kn)
+    for (int z = g; z < nslab; z += 16) s += P[(int64_t)z * kn + e];
+  part[g][el] = s;
+  __syncthreads();
+  if (g == 0 && e < kn) {
+    double tot = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) tot += part[q][el];
+    double* cp = C + (e % k) + (e / k) * ldc;
+    *cp = alpha * tot + (beta != 0.0 ? beta * *cp : 0.0);
   }
 }
 
@@ -169,8 +179,8 @@ int thin_gemm(cqr_ctx* c, int ta, int64_t m, int64_t k, double alpha, const doub
   else if (nr <= 4) hipLaunchKernelGGL(thin_gemm_t_kernel<4>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, slab, scratch);
   else hipLaunchKernelGGL(thin_gemm_t_kernel<16>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, slab, scratch);
   CQR_TRY(check_launch(c, "thin_gemm_t"));
-  const int gx = (int)std::min<int64_t>(1024, (k * nr + 255) / 256);
-  hipLaunchKernelGGL(slab_reduce_kernel, dim3(gx), dim3(256), 0, c->stream, scratch, nslab, k, nr, alpha, beta, C, ldc);
+  hipLaunchKernelGGL(slab_reduce_kernel, dim3((unsigned)((k * nr + 63) / 64)), dim3(1024), 0, c->stream, scratch, nslab, k,
+                     nr, alpha, beta, C, ldc);
   return check_launch(c, "slab_reduce");
 }
 
