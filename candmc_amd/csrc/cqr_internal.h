@@ -254,17 +254,21 @@ int launch_small_thin(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64_t 
                       int64_t lda, int smA, const double* B, int64_t ldb, int smB, double beta, double* C,
                       int64_t ldc, int kind);
 // A (m×n) −= YT·(Yeᵀ·A) with an explicit unit-lower Ye and YT = Ye·Tt (both
-// m×b, ld m); workspace wy_update_ws_doubles(m, n, b) doubles.
+// m×b, ld m), the two products in `prec`; workspace wy_update_ws_doubles(m,
+// n, b) doubles.
 int wy_update_explicit(cqr_ctx* c, const double* Ye, const double* YT, int64_t m, int64_t n, int b, double* A,
-                       int64_t lda, double* ws);
+                       int64_t lda, double* ws, int prec);
 
 // Least-squares kernels (lsq_kernels.hip): C = alpha·op(A)·X + beta·C for
 // X with nr ≤ 16 columns; the transposed form needs
 // thin_gemm_scratch_doubles(m, k, nr) of scratch.
 int64_t thin_t_slab_rows(int64_t m, int64_t k);
 size_t thin_gemm_scratch_doubles(int64_t m, int64_t k, int nr);
+// unit_top > 0: A's first unit_top rows are unit lower (A(r, c) = 0 for
+// c > r and 1 at c = r, whatever is stored there).
 int thin_gemm(cqr_ctx* c, int ta, int64_t m, int64_t k, double alpha, const double* A, int64_t lda,
-              const double* X, int64_t ldx, int nr, double beta, double* C, int64_t ldc, double* scratch);
+              const double* X, int64_t ldx, int nr, double beta, double* C, int64_t ldc, double* scratch,
+              int unit_top = 0);
 int launch_axpby(cqr_ctx* c, int64_t m, int64_t n, double alpha, const double* X, int64_t ldx, double beta,
                  double* Y, int64_t ldy);
 int launch_colnorm(cqr_ctx* c, const double* X, int64_t ldx, int64_t m, int n, double* out);

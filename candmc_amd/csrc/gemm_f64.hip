@@ -684,11 +684,18 @@ int wy_update_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const
 // lean kernel (split-K, from three column tiles on) or the general one, and
 // the update through the vendor BLAS.
 int wy_update_explicit(cqr_ctx* c, const double* Ye, const double* YT, int64_t m, int64_t n, int b, double* A,
-                       int64_t lda, double* ws) {
+                       int64_t lda, double* ws, int prec) {
   if (m <= 0 || n <= 0 || b <= 0) return CQR_OK;
   double* part = ws;
   const size_t np = gemm_f64_scratch_doubles(b, n, m);
   double* Z = ws + ((np + 31) / 32) * 32;
+  if (prec != CQR_PREC_F64) {
+    // FP32/BF16 operands, FP32 accumulation; Z and the update land in FP64
+    CQR_TRY(gemm_f64_impl(c, 1, 0, b, n, m, 1.0, Ye, m, SM_NONE, A, lda, SM_NONE, 0.0, Z, b, np ? part : nullptr,
+                          PK_GEMM_YTA, prec));
+    return gemm_f64_impl(c, 0, 0, m, n, b, -1.0, YT, m, SM_NONE, Z, b, SM_NONE, 1.0, A, lda, nullptr, PK_GEMM_UPD,
+                         prec);
+  }
   const int ns = np ? (int)(np / ((size_t)b * n)) : 0;
   if (ns >= 2) {
     int used = 0;

@@ -29,8 +29,8 @@ int apply_q_ws(cqr_ctx* c, const double* YR, int64_t ldy, const double* Tt, int6
   const int64_t np = (k + b - 1) / b;
   if (nr <= 16) {
     // Few columns (solves, refinement): each panel is two HBM-bound thin
-    // products over Y's rows below its top block plus three b×b×nr
-    // products (the unit-lower top block of Y, and T)
+    // products over Y (its unit-lower top block masked inside them) and
+    // one b×b×nr product with T
     double* scr = ws;
     double* Z = ws + (thin_gemm_scratch_doubles(m, b, (int)nr) + 31) / 32 * 32;
     double* Zp = Z + (b * nr + 31) / 32 * 32;
@@ -40,20 +40,14 @@ int apply_q_ws(cqr_ctx* c, const double* YR, int64_t ldy, const double* Tt, int6
       if (mj <= 0) continue;
       const double* Yp = YR + j + j * ldy;
       double* Bp = B + j;
-      const int64_t bt = std::min(bj, mj);
-      // Z = Yᵀ·B: top block (unit lower) then the rows below it
-      CQR_TRY(gemm_f64_ws(c, 1, 0, bj, nr, bt, 1.0, Yp, ldy, SM_UNIT_LOWER, Bp, ldb, SM_NONE, 0.0, Z, bj,
-                          nullptr));
-      if (mj > bt)
-        CQR_TRY(thin_gemm(c, 1, mj - bt, bj, 1.0, Yp + bt, ldy, Bp + bt, ldb, (int)nr, 1.0, Z, bj, scr));
+      const int bt = (int)std::min(bj, mj);
+      // Z = Yᵀ·B over all the panel's rows (its top block unit lower)
+      CQR_TRY(thin_gemm(c, 1, mj, bj, 1.0, Yp, ldy, Bp, ldb, (int)nr, 0.0, Z, bj, scr, bt));
       // Z' = Tᵀ·Z (Qᵀ) or T·Z (Q); Tt holds Tᵀ (lower, ld bj)
       CQR_TRY(gemm_f64_ws(c, trans ? 0 : 1, 0, bj, nr, bj, 1.0, Tt + p * b * b, bj, SM_LOWER, Z, bj, SM_NONE, 0.0,
                           Zp, bj, nullptr));
       // B −= Y·Z'
-      CQR_TRY(gemm_f64_ws(c, 0, 0, bt, nr, bj, -1.0, Yp, ldy, SM_UNIT_LOWER, Zp, bj, SM_NONE, 1.0, Bp, ldb,
-                          nullptr));
-      if (mj > bt)
-        CQR_TRY(thin_gemm(c, 0, mj - bt, bj, -1.0, Yp + bt, ldy, Zp, bj, (int)nr, 1.0, Bp + bt, ldb, nullptr));
+      CQR_TRY(thin_gemm(c, 0, mj, bj, -1.0, Yp, ldy, Zp, bj, (int)nr, 1.0, Bp, ldb, nullptr, bt));
     }
     return CQR_OK;
   }

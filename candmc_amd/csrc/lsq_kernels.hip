@@ -16,7 +16,7 @@ template <int NR>
 __global__ void __launch_bounds__(256)
 thin_gemm_n_kernel(const double* __restrict__ A, int64_t lda, int64_t m, int64_t k,
                    const double* __restrict__ X, int64_t ldx, int nr, double alpha, double beta,
-                   double* __restrict__ C, int64_t ldc) {
+                   double* __restrict__ C, int64_t ldc, int unit_top) {
   __shared__ double Xs[TN_ROWS][NR];
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
   double acc[NR];
@@ -29,7 +29,15 @@ thin_gemm_n_kernel(const double* __restrict__ A, int64_t lda, int64_t m, int64_t
       Xs[kk][j] = (k0 + kk < k && j < nr) ? X[(k0 + kk) + (int64_t)j * ldx] : 0.0;
     }
     __syncthreads();
-    if (r < m) {
+    if (r < unit_top) {        // unit-lower top rows: A(r, c) = 0 for c > r, 1 at c = r
+      const int kend = (int)min<int64_t>(TN_ROWS, k - k0);
+      for (int kk = 0; kk < kend; ++kk) {
+        const int64_t cidx = k0 + kk;
+        const double a = cidx < r ? A[r + cidx * lda] : (cidx == r ? 1.0 : 0.0);
+#pragma unroll
+        for (int j = 0; j < NR; ++j) acc[j] += a * Xs[kk][j];
+      }
+    } else if (r < m) {
       const int kend = (int)min<int64_t>(TN_ROWS, k - k0);
       const double* ap = A + r + k0 * lda;
       int kk = 0;
@@ -64,7 +72,7 @@ template <int NR>
 __global__ void __launch_bounds__(256)
 thin_gemm_t_kernel(const double* __restrict__ A, int64_t lda, int64_t m, int64_t k,
                    const double* __restrict__ X, int64_t ldx, int nr, int64_t slab,
-                   double* __restrict__ P) {
+                   double* __restrict__ P, int unit_top) {
   constexpr int NH = NR > 1 ? NR / 2 : 1;
   __shared__ double As[TT_ROWS][128 + 1];
   __shared__ double Xs[TT_ROWS][NR];
@@ -79,7 +87,9 @@ thin_gemm_t_kernel(const double* __restrict__ A, int64_t lda, int64_t m, int64_t
     for (int e = tid; e < TT_ROWS * 128; e += 256) {
       const int rr = e % TT_ROWS, cc = e / TT_ROWS;
       const int64_t row = rb + rr, col = c0 + cc;
-      As[rr][cc] = (row < r1 && col < k) ? A[row + col * lda] : 0.0;
+      double v = (row < r1 && col < k) ? A[row + col * lda] : 0.0;
+      if (row < unit_top) v = row < col ? 0.0 : (row == col ? 1.0 : v);
+      As[rr][cc] = v;
     }
     for (int e = tid; e < TT_ROWS * NR; e += 256) {
       const int rr = e % TT_ROWS, j = e / TT_ROWS;
@@ -161,23 +171,24 @@ colnorm_kernel(const double* __restrict__ X, int64_t ldx, int64_t m, double* __r
 
 // ---------------------------------------------------------------------------
 int thin_gemm(cqr_ctx* c, int ta, int64_t m, int64_t k, double alpha, const double* A, int64_t lda,
-              const double* X, int64_t ldx, int nr, double beta, double* C, int64_t ldc, double* scratch) {
+              const double* X, int64_t ldx, int nr, double beta, double* C, int64_t ldc, double* scratch,
+              int unit_top) {
   if (nr < 1 || nr > 16) return set_err(c, CQR_ERR_UNSUPPORTED, "thin product with %d columns", nr);
   if (m <= 0 || k <= 0) return CQR_OK;
   if (!ta) {
     const dim3 grid((unsigned)((m + 255) / 256));
-    if (nr == 1) hipLaunchKernelGGL(thin_gemm_n_kernel<1>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, alpha, beta, C, ldc);
-    else if (nr <= 4) hipLaunchKernelGGL(thin_gemm_n_kernel<4>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, alpha, beta, C, ldc);
-    else hipLaunchKernelGGL(thin_gemm_n_kernel<16>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, alpha, beta, C, ldc);
+    if (nr == 1) hipLaunchKernelGGL(thin_gemm_n_kernel<1>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, alpha, beta, C, ldc, unit_top);
+    else if (nr <= 4) hipLaunchKernelGGL(thin_gemm_n_kernel<4>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, alpha, beta, C, ldc, unit_top);
+    else hipLaunchKernelGGL(thin_gemm_n_kernel<16>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, alpha, beta, C, ldc, unit_top);
     return check_launch(c, "thin_gemm_n");
   }
   // C (k×nr) = alpha·Aᵀ·X + beta·C, A m×k; slabs of rows, then a fixed-order sum
   const int64_t slab = thin_t_slab_rows(m, k);
   const int nslab = (int)((m + slab - 1) / slab);
   const dim3 grid((unsigned)((k + 127) / 128), (unsigned)nslab);
-  if (nr == 1) hipLaunchKernelGGL(thin_gemm_t_kernel<1>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, slab, scratch);
-  else if (nr <= 4) hipLaunchKernelGGL(thin_gemm_t_kernel<4>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, slab, scratch);
-  else hipLaunchKernelGGL(thin_gemm_t_kernel<16>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, slab, scratch);
+  if (nr == 1) hipLaunchKernelGGL(thin_gemm_t_kernel<1>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, slab, scratch, unit_top);
+  else if (nr <= 4) hipLaunchKernelGGL(thin_gemm_t_kernel<4>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, slab, scratch, unit_top);
+  else hipLaunchKernelGGL(thin_gemm_t_kernel<16>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, slab, scratch, unit_top);
   CQR_TRY(check_launch(c, "thin_gemm_t"));
   hipLaunchKernelGGL(slab_reduce_kernel, dim3((unsigned)((k * nr + 63) / 64)), dim3(1024), 0, c->stream, scratch, nslab, k,
                      nr, alpha, beta, C, ldc);

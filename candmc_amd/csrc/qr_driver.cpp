@@ -405,9 +405,11 @@ int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b
   double* ws_near = reinterpret_cast<double*>(base + o.upd_near);
   double* ws_far = reinterpret_cast<double*>(base + o.upd_far);
   const bool la = c->lookahead && k > 2 * b && m > 2 * b;
-  // FP64 with the vendor BLAS: each panel's Y is made explicit (unit lower)
-  // and Y·Tt formed once, so both updates are two plain products
-  const bool expl = prec == CQR_PREC_F64 && c->blas_update;
+  // Each panel's Y is made explicit (unit lower) and Y·Tt formed once (in
+  // FP64), so both updates are two plain products (the vendor BLAS for the
+  // FP64 update, the mixed-precision kernels otherwise); the own FP64
+  // kernels keep the masked form
+  const bool expl = prec != CQR_PREC_F64 || c->blas_update;
   double* Ye2[2] = {reinterpret_cast<double*>(base + o.Ye),
                     reinterpret_cast<double*>(base + o.Ye) + 2 * m * std::min(k, b)};
   if (la) {  // the side stream starts after everything already queued
@@ -429,7 +431,7 @@ int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b
       if ((rc = blas_gemm(c, 0, mj, bj, bj, 1.0, Ye, mj, Tt, bj, 0.0, YT, mj, PK_GEMM_SMALL))) break;
     }
     auto update = [&](int64_t col0, int64_t ncols, double* ws) -> int {
-      if (expl) return wy_update_explicit(c, Ye, YT, mj, ncols, (int)bj, P + col0 * lda, lda, ws);
+      if (expl) return wy_update_explicit(c, Ye, YT, mj, ncols, (int)bj, P + col0 * lda, lda, ws, prec);
       return wy_update_ws(c, P, lda, 1, Tt, 0, P + col0 * lda, lda, mj, ncols, (int)bj, ws, prec);
     };
     if (!la) {
