@@ -269,6 +269,10 @@ size_t thin_gemm_scratch_doubles(int64_t m, int64_t k, int nr);
 int thin_gemm(cqr_ctx* c, int ta, int64_t m, int64_t k, double alpha, const double* A, int64_t lda,
               const double* X, int64_t ldx, int nr, double beta, double* C, int64_t ldc, double* scratch,
               int unit_top = 0);
+// One step of the blocked R solve (lsq_kernels.hip): blocks [q0, q0+nq)
+// take block p's solution into account; block `solve` is then finished.
+int launch_trsm_step(cqr_ctx* c, const double* R, int64_t ldr, int64_t n, const double* Rinv, double* B, int64_t ldb,
+                     int nr, int trans, int p, int q0, int nq, int solve);
 int launch_axpby(cqr_ctx* c, int64_t m, int64_t n, double alpha, const double* X, int64_t ldx, double beta,
                  double* Y, int64_t ldy);
 int launch_colnorm(cqr_ctx* c, const double* X, int64_t ldx, int64_t m, int n, double* out);

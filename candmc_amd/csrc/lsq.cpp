@@ -82,6 +82,24 @@ int r_block_inverses(cqr_ctx* c, const double* YR, int64_t ldy, int64_t n, doubl
 int trsm_r_ws(cqr_ctx* c, const double* YR, int64_t ldy, int64_t n, const double* Rinv, double* B, int64_t ldb,
               int64_t nr, int trans, double* tmp) {
   const int64_t nb = (n + 127) / 128;
+  if (nr <= 16) {
+    // first diagonal block alone, then one launch per block step: the
+    // remaining blocks take the new block into account and the next one
+    // is finished by its own workgroup
+    const int64_t p0 = trans ? 0 : nb - 1, j0 = p0 * 128, b0 = std::min<int64_t>(128, n - j0);
+    CQR_TRY(launch_small_thin(c, trans ? 1 : 0, 0, b0, nr, b0, 1.0, Rinv + p0 * 128 * 128, b0, SM_UPPER, B + j0, ldb,
+                              SM_NONE, 0.0, B + j0, ldb, PK_GEMM_SMALL));
+    for (int64_t s = 1; s < nb; ++s) {
+      if (trans) {   // block p = s−1 known; blocks s.. take it in, block s is finished
+        CQR_TRY(launch_trsm_step(c, YR, ldy, n, Rinv, B, ldb, (int)nr, 1, (int)(s - 1), (int)s, (int)(nb - s),
+                                 (int)s));
+      } else {       // block p = nb−s known; blocks 0..p−1 take it in, block p−1 is finished
+        const int p = (int)(nb - s);
+        CQR_TRY(launch_trsm_step(c, YR, ldy, n, Rinv, B, ldb, (int)nr, 0, p, 0, p, p - 1));
+      }
+    }
+    return CQR_OK;
+  }
   for (int64_t q = 0; q < nb; ++q) {
     const int64_t p = trans ? q : nb - 1 - q;
     const int64_t j = p * 128, bj = std::min<int64_t>(128, n - j);

@@ -84,12 +84,24 @@ thin_gemm_t_kernel(const double* __restrict__ A, int64_t lda, int64_t m, int64_t
   for (int j = 0; j < NH; ++j) acc[j] = 0.0;
   for (int64_t rb = r0; rb < r1; rb += TT_ROWS) {
     __syncthreads();
-    for (int e = tid; e < TT_ROWS * 128; e += 256) {
-      const int rr = e % TT_ROWS, cc = e / TT_ROWS;
-      const int64_t row = rb + rr, col = c0 + cc;
-      double v = (row < r1 && col < k) ? A[row + col * lda] : 0.0;
-      if (row < unit_top) v = row < col ? 0.0 : (row == col ? 1.0 : v);
-      As[rr][cc] = v;
+    {  // the slab's 16 loads per thread are issued before any is stored
+      constexpr int PER = TT_ROWS * 128 / 256;
+      double v[PER];
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int e = tid + 256 * u, rr = e % TT_ROWS, cc = e / TT_ROWS;
+        const int64_t row = rb + rr, col = c0 + cc;
+        const bool ok = row < r1 && col < k;
+        v[u] = A[ok ? row + col * lda : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int e = tid + 256 * u, rr = e % TT_ROWS, cc = e / TT_ROWS;
+        const int64_t row = rb + rr, col = c0 + cc;
+        double x = (row < r1 && col < k) ? v[u] : 0.0;
+        if (row < unit_top) x = row < col ? 0.0 : (row == col ? 1.0 : x);
+        As[rr][cc] = x;
+      }
     }
     for (int e = tid; e < TT_ROWS * NR; e += 256) {
       const int rr = e % TT_ROWS, j = e / TT_ROWS;
@@ -135,6 +147,104 @@ __global__ void __launch_bounds__(1024) slab_reduce_kernel(const double* __restr
     double* cp = C + (e % k) + (e / k) * ldc;
     *cp = alpha * tot + (beta != 0.0 ? beta * *cp : 0.0);
   }
+}
+
+// One step of the blocked solve with R (upper triangle of a [Y\R] block, ld
+// ldr) on nr ≤ NR columns of B, in 128-row blocks.  Workgroup q of [q0,
+// q0 + grid) updates its block with block p of the solution,
+//   trans = 0 (R⁻¹, blocks backward): y_q −= R(q, p)·x_p
+//   trans = 1 (R⁻ᵀ, blocks forward):  y_q −= R(p, q)ᵀ·x_p,
+// and the workgroup of block `solve` then finishes that block:
+// x = op(Ri)·y with Ri its diagonal block's inverse (upper, ld its width).
+template <int NR>
+__global__ void __launch_bounds__(256)
+trsm_step_kernel(const double* __restrict__ R, int64_t ldr, int64_t n, const double* __restrict__ Rinv,
+                 double* __restrict__ B, int64_t ldb, int nr, int trans, int p, int q0, int solve) {
+  constexpr int KC = 32;
+  __shared__ double xs[128][NR + 1];
+  __shared__ double ys[128][NR + 1];
+  const int q = q0 + blockIdx.x, t = threadIdx.x;
+  const int64_t jp = (int64_t)p * 128, jq = (int64_t)q * 128;
+  const int bp = (int)min<int64_t>(128, n - jp), bq = (int)min<int64_t>(128, n - jq);
+  for (int e = t; e < 128 * NR; e += 256) {
+    const int r = e & 127, j = e >> 7;
+    xs[r][j] = (r < bp && j < nr) ? B[jp + r + (int64_t)j * ldb] : 0.0;
+  }
+  __syncthreads();
+  double acc[NR];
+#pragma unroll
+  for (int j = 0; j < NR; ++j) acc[j] = 0.0;
+  if (t < bq) {
+    for (int k0 = 0; k0 < bp; k0 += KC) {
+      double a[KC];
+#pragma unroll
+      for (int u = 0; u < KC; ++u) {
+        const int k = k0 + u;
+        const int64_t idx = trans ? (jp + k) + (jq + t) * ldr : (jq + t) + (jp + k) * ldr;
+        a[u] = R[k < bp ? idx : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < KC; ++u) {
+        const int k = k0 + u;
+        if (k < bp) {
+#pragma unroll
+          for (int j = 0; j < NR; ++j) acc[j] += a[u] * xs[k][j];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      if (j >= nr) continue;
+      double* yp = B + jq + t + (int64_t)j * ldb;
+      const double y = *yp - acc[j];
+      if (q == solve) ys[t][j] = y;
+      else *yp = y;
+    }
+  }
+  if (q != solve) return;                  // uniform across the workgroup
+  __syncthreads();
+  if (t >= bq) return;
+  const double* Ri = Rinv + (int64_t)q * 128 * 128;
+#pragma unroll
+  for (int j = 0; j < NR; ++j) acc[j] = 0.0;
+  for (int k0 = 0; k0 < bq; k0 += KC) {
+    // op(Ri)(t, k): Ri upper (t ≤ k), or Riᵀ (k ≤ t); a chunk's loads first
+    double a[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      const bool ok = k < bq && (trans ? k <= t : k >= t);
+      a[u] = Ri[ok ? (trans ? k + (int64_t)t * bq : t + (int64_t)k * bq) : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      const bool ok = k < bq && (trans ? k <= t : k >= t);
+      if (ok) {
+#pragma unroll
+        for (int j = 0; j < NR; ++j) acc[j] += a[u] * ys[k][j];
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NR; ++j)
+    if (j < nr) B[jq + t + (int64_t)j * ldb] = acc[j];
+}
+
+int launch_trsm_step(cqr_ctx* c, const double* R, int64_t ldr, int64_t n, const double* Rinv, double* B, int64_t ldb,
+                     int nr, int trans, int p, int q0, int nq, int solve) {
+  if (nq <= 0) return CQR_OK;
+  if (nr < 1 || nr > 16) return set_err(c, CQR_ERR_UNSUPPORTED, "trsm step with %d columns", nr);
+  if (nr == 1)
+    hipLaunchKernelGGL(trsm_step_kernel<1>, dim3(nq), dim3(256), 0, c->stream, R, ldr, n, Rinv, B, ldb, nr, trans, p,
+                       q0, solve);
+  else if (nr <= 4)
+    hipLaunchKernelGGL(trsm_step_kernel<4>, dim3(nq), dim3(256), 0, c->stream, R, ldr, n, Rinv, B, ldb, nr, trans, p,
+                       q0, solve);
+  else
+    hipLaunchKernelGGL(trsm_step_kernel<16>, dim3(nq), dim3(256), 0, c->stream, R, ldr, n, Rinv, B, ldb, nr, trans,
+                       p, q0, solve);
+  return check_launch(c, "trsm_step");
 }
 
 // Y = alpha·X + beta·Y on an m×n block.
