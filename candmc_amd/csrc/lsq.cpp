@@ -14,6 +14,9 @@
 // dr = Q·[h; d₂]) while f and g are formed in FP64.  Its fixed point is
 // the exact least-squares solution, so a factorization computed with an
 // FP32 trailing update converges to FP64 accuracy when κ(A)·u₃₂ < 1.
+// Cheaper corrected semi-normal-equation steps (dx = R⁻¹R⁻ᵀAᵀr, no Q;
+// Björck 1987) come first and are kept while they contract fast, which
+// they do when κ(A)²·u of the factorization is small.
 #include <cmath>
 #include <vector>
 
@@ -220,8 +223,46 @@ int cqr_lstsq(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, co
   CQR_TRY(launch_copy2d(c, B, ldb, r, m, m, nr));
   CQR_TRY(thin_gemm(c, 0, m, n, -1.0, A, lda, X, ldx, (int)nr, 1.0, r, m, nullptr));
   std::vector<double> hn(2 * nr);
+  // max over the columns of ‖d_j‖ / ‖x_j‖ (one host synchronisation)
+  auto rel_update = [&](const double* d, int64_t ldd, double* out) -> int {
+    CQR_TRY(launch_colnorm(c, d, ldd, n, (int)nr, nrm));
+    CQR_TRY(launch_colnorm(c, X, ldx, n, (int)nr, nrm + nr));
+    CQR_HIP(c, hipMemcpyAsync(hn.data(), nrm, sizeof(double) * 2 * nr, hipMemcpyDeviceToHost, c->stream));
+    CQR_HIP(c, hipStreamSynchronize(c->stream));
+    double rel = 0.0;
+    for (int64_t j = 0; j < nr; ++j) rel = std::max(rel, hn[j] / std::max(hn[nr + j], 1e-300));
+    if (std::isnan(hn[0])) rel = hn[0];
+    *out = rel;
+    return CQR_OK;
+  };
   double prev_rel = 0.0;
+  bool sne = true;   // semi-normal-equation corrections while they contract fast
+  int aug = 0;       // augmented-system steps taken
   for (int it = 1; it <= max_iter; ++it) {
+    if (sne) {
+      // dx = R⁻¹·R⁻ᵀ·(Aᵀ·r): the corrected semi-normal equations with the
+      // computed R (no Q application).  Their fixed point is also the exact
+      // least-squares solution; they contract by about κ²·u of the
+      // factorization, so they are used while each update shrinks by at
+      // least 8× (and the first is below half the solution).
+      CQR_TRY(thin_gemm(c, 1, m, n, 1.0, A, lda, r, m, (int)nr, 0.0, g, n, wsq));
+      CQR_TRY(trsm_r_ws(c, Af, m, n, Rinv, g, n, nr, 1, tmp));
+      CQR_TRY(trsm_r_ws(c, Af, m, n, Rinv, g, n, nr, 0, tmp));
+      double rel;
+      CQR_TRY(rel_update(g, n, &rel));
+      if (rel <= 0.5 && (it == 1 || rel <= prev_rel / 8)) {
+        CQR_TRY(launch_axpby(c, n, nr, 1.0, g, n, 1.0, X, ldx));
+        CQR_TRY(launch_copy2d(c, B, ldb, r, m, m, nr));
+        CQR_TRY(thin_gemm(c, 0, m, n, -1.0, A, lda, X, ldx, (int)nr, 1.0, r, m, nullptr));
+        if (iters) *iters = it;
+        if (last_update) *last_update = rel;
+        if (!(rel > tol)) break;
+        prev_rel = rel;
+        continue;
+      }
+      sne = false;   // not contracting fast enough: the augmented system from here
+    }
+    ++aug;
     // f = b − r − A·x,  g = −Aᵀ·r   (FP64)
     CQR_TRY(launch_copy2d(c, B, ldb, f, m, m, nr));
     CQR_TRY(launch_axpby(c, m, nr, -1.0, r, m, 1.0, f, m));
@@ -238,18 +279,14 @@ int cqr_lstsq(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, co
     CQR_TRY(launch_axpby(c, n, nr, 1.0, h, n, 1.0, X, ldx));
     CQR_TRY(launch_axpby(c, m, nr, 1.0, f, m, 1.0, r, m));
     // stop when every column's update is below tol relative to the solution
-    CQR_TRY(launch_colnorm(c, h, n, n, (int)nr, nrm));
-    CQR_TRY(launch_colnorm(c, X, ldx, n, (int)nr, nrm + nr));
-    CQR_HIP(c, hipMemcpyAsync(hn.data(), nrm, sizeof(double) * 2 * nr, hipMemcpyDeviceToHost, c->stream));
-    CQR_HIP(c, hipStreamSynchronize(c->stream));
-    double rel = 0.0;
-    for (int64_t j = 0; j < nr; ++j) rel = std::max(rel, hn[j] / std::max(hn[nr + j], 1e-300));
+    double rel;
+    CQR_TRY(rel_update(h, n, &rel));
     if (iters) *iters = it;
     if (last_update) *last_update = rel;
     if (!(rel > tol)) break;  // also stops on NaN
     // the updates stopped shrinking: the attainable accuracy (about κ·ε of
     // the FP64 residual) is reached, or κ·u of the factorization is ≥ 1
-    if (it > 1 && rel > 0.9 * prev_rel) break;
+    if (aug > 1 && rel > 0.9 * prev_rel) break;
     prev_rel = rel;
   }
   return CQR_OK;

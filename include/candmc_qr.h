@@ -249,9 +249,11 @@ int cqr_trsm_r(cqr_ctx* ctx, const double* YR, int64_t ldy, int64_t n, double* B
                int64_t nrhs, int trans);
 /* min ‖A·X − B‖ for m ≥ n, 1 ≤ nrhs ≤ 16: QR of a copy of A (A itself is
  * not modified) with the trailing update in `prec`, then up to max_iter
- * steps of iterative refinement with FP64 residuals (augmented system),
- * stopping when every column's update is ≤ tol relative to the solution,
- * or when the updates stop shrinking (by less than 10% per step).
+ * steps of iterative refinement with FP64 residuals: corrected
+ * semi-normal-equation steps (dx = R⁻¹R⁻ᵀAᵀr) while each shrinks the
+ * update at least 8×, then the augmented system; stopping when every
+ * column's update is ≤ tol relative to the solution, or when the
+ * augmented steps stop shrinking (by less than 10% per step).
  * iters / last_update (nullable) report the steps taken and the last
  * relative update.                                                       */
 int cqr_lstsq(cqr_ctx* ctx, const double* A, int64_t lda, int64_t m, int64_t n, const double* B,
