@@ -163,72 +163,87 @@ trsm_step_kernel(const double* __restrict__ R, int64_t ldr, int64_t n, const dou
   constexpr int KC = 32;
   __shared__ double xs[128][NR + 1];
   __shared__ double ys[128][NR + 1];
+  __shared__ double part[2][128][NR + 1];
   const int q = q0 + blockIdx.x, t = threadIdx.x;
+  const int r = t & 127, h = t >> 7;   // row of the block, half of the k range
   const int64_t jp = (int64_t)p * 128, jq = (int64_t)q * 128;
   const int bp = (int)min<int64_t>(128, n - jp), bq = (int)min<int64_t>(128, n - jq);
   for (int e = t; e < 128 * NR; e += 256) {
-    const int r = e & 127, j = e >> 7;
-    xs[r][j] = (r < bp && j < nr) ? B[jp + r + (int64_t)j * ldb] : 0.0;
+    const int rr = e & 127, j = e >> 7;
+    xs[rr][j] = (rr < bp && j < nr) ? B[jp + rr + (int64_t)j * ldb] : 0.0;
   }
   __syncthreads();
   double acc[NR];
 #pragma unroll
   for (int j = 0; j < NR; ++j) acc[j] = 0.0;
-  if (t < bq) {
-    for (int k0 = 0; k0 < bp; k0 += KC) {
+  if (r < bq) {
+    const int kb = h * 64, ke = min(bp, kb + 64);
+    for (int k0 = kb; k0 < ke; k0 += KC) {
       double a[KC];
 #pragma unroll
       for (int u = 0; u < KC; ++u) {
         const int k = k0 + u;
-        const int64_t idx = trans ? (jp + k) + (jq + t) * ldr : (jq + t) + (jp + k) * ldr;
-        a[u] = R[k < bp ? idx : 0];
+        const int64_t idx = trans ? (jp + k) + (jq + r) * ldr : (jq + r) + (jp + k) * ldr;
+        a[u] = R[k < ke ? idx : 0];
       }
 #pragma unroll
       for (int u = 0; u < KC; ++u) {
         const int k = k0 + u;
-        if (k < bp) {
+        if (k < ke) {
 #pragma unroll
           for (int j = 0; j < NR; ++j) acc[j] += a[u] * xs[k][j];
         }
       }
     }
+  }
+#pragma unroll
+  for (int j = 0; j < NR; ++j) part[h][r][j] = acc[j];
+  __syncthreads();
+  if (h == 0 && r < bq) {
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
       if (j >= nr) continue;
-      double* yp = B + jq + t + (int64_t)j * ldb;
-      const double y = *yp - acc[j];
-      if (q == solve) ys[t][j] = y;
+      double* yp = B + jq + r + (int64_t)j * ldb;
+      const double y = *yp - (part[0][r][j] + part[1][r][j]);
+      if (q == solve) ys[r][j] = y;
       else *yp = y;
     }
   }
   if (q != solve) return;                  // uniform across the workgroup
   __syncthreads();
-  if (t >= bq) return;
   const double* Ri = Rinv + (int64_t)q * 128 * 128;
 #pragma unroll
   for (int j = 0; j < NR; ++j) acc[j] = 0.0;
-  for (int k0 = 0; k0 < bq; k0 += KC) {
-    // op(Ri)(t, k): Ri upper (t ≤ k), or Riᵀ (k ≤ t); a chunk's loads first
-    double a[KC];
+  if (r < bq) {
+    const int kb = h * 64, ke = min(bq, kb + 64);
+    for (int k0 = kb; k0 < ke; k0 += KC) {
+      // op(Ri)(r, k): Ri upper (r ≤ k), or Riᵀ (k ≤ r); a chunk's loads first
+      double a[KC];
 #pragma unroll
-    for (int u = 0; u < KC; ++u) {
-      const int k = k0 + u;
-      const bool ok = k < bq && (trans ? k <= t : k >= t);
-      a[u] = Ri[ok ? (trans ? k + (int64_t)t * bq : t + (int64_t)k * bq) : 0];
-    }
+      for (int u = 0; u < KC; ++u) {
+        const int k = k0 + u;
+        const bool ok = k < ke && (trans ? k <= r : k >= r);
+        a[u] = Ri[ok ? (trans ? k + (int64_t)r * bq : r + (int64_t)k * bq) : 0];
+      }
 #pragma unroll
-    for (int u = 0; u < KC; ++u) {
-      const int k = k0 + u;
-      const bool ok = k < bq && (trans ? k <= t : k >= t);
-      if (ok) {
+      for (int u = 0; u < KC; ++u) {
+        const int k = k0 + u;
+        const bool ok = k < ke && (trans ? k <= r : k >= r);
+        if (ok) {
 #pragma unroll
-        for (int j = 0; j < NR; ++j) acc[j] += a[u] * ys[k][j];
+          for (int j = 0; j < NR; ++j) acc[j] += a[u] * ys[k][j];
+        }
       }
     }
   }
 #pragma unroll
-  for (int j = 0; j < NR; ++j)
-    if (j < nr) B[jq + t + (int64_t)j * ldb] = acc[j];
+  for (int j = 0; j < NR; ++j) part[h][r][j] = acc[j];
+  __syncthreads();
+  if (h == 0 && r < bq) {
+#pragma unroll
+    for (int j = 0; j < NR; ++j)
+      if (j < nr) B[jq + r + (int64_t)j * ldb] = part[0][r][j] + part[1][r][j];
+  }
 }
 
 int launch_trsm_step(cqr_ctx* c, const double* R, int64_t ldr, int64_t n, const double* Rinv, double* B, int64_t ldb,
