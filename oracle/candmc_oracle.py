@@ -1,8 +1,8 @@
 """ORACLE — CPU restatement of CANDMC's communication-avoiding QR hot path.
 
-TEST INFRASTRUCTURE ONLY.  This module is the checker for the HIP product
-path and the timed CPU baseline of ``bench.py``; nothing in ``candmc_amd``
-imports it, links it or calls it.
+TEST INFRASTRUCTURE ONLY.  The HIP product path's results are checked
+against this module, and ``bench.py`` times it as the CPU baseline; nothing
+in ``candmc_amd`` imports it, links it or calls it.
 
 What it restates (paths relative to the CANDMC source tree):
 
