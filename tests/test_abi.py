@@ -69,3 +69,8 @@ def test_compute_entry_points_reject_null_context():
     lib = _lib.load()
     assert lib.cqr_hh_recon_qr(None, None, 1, 1, 1, None, 0) == 1
     assert lib.cqr_qr_2d(None, None, 1, 1, 1, 1, None, 0) == 1
+    assert lib.cqr_apply_tsqr_QT(None, None, 1, None, None, 1, 1, 1, 1, 0, None) == 1
+    assert lib.cqr_apply_tsqr_Q(None, None, 1, None, None, 1, 1, 1, 1, 0, None) == 1
+    assert lib.cqr_construct_Q1_id(None, None, 1, None, None, 1, 1, 1, 1, 0, None, None, 0) == 1
+    assert lib.cqr_qr_tree_2d(None, None, 1, 1, 1, 1, -1, 0) == 1
+    assert lib.cqr_tree_data_export(None, None, 1, 1, 0, 0, None, None) == 1
