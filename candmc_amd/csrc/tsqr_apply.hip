@@ -259,9 +259,12 @@ int cqr_construct_Q1_id(cqr_ctx* c, const double* Y, int64_t lda_Y, const double
                         int64_t lda_Q, int64_t m, int64_t b, int64_t k, int64_t nleaves, const double* tree_data,
                         const double* ID, int64_t ld_ID) {
   CQR_CHECK_CTX(c);
-  if (!Y || !tree_data || !Q1 || b < 1 || m < b || k < 1 || k > m || lda_Q < m || (ID && ld_ID < b))
+  // every argument is checked before the start block is written into Q1
+  if (!Y || !tree_data || !Q1 || b < 1 || m < b || k < 1 || k > m || lda_Y < m || lda_Q < m || nleaves < 0 ||
+      (ID && ld_ID < b))
     return set_err(c, CQR_ERR_INVALID, "construct_Q1: bad arguments (m=%lld b=%lld k=%lld)", (long long)m,
                    (long long)b, (long long)k);
+  if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "construct_Q1: b=%lld > %d", (long long)b, CQR_MAX_B);
   if (!ID && k == b) return cqr_construct_Q1(c, Y, lda_Y, tau, Q1, lda_Q, m, b, k, nleaves, tree_data);
   const int gx = (int)std::min<int64_t>(4096, (m * k + 255) / 256);
   hipLaunchKernelGGL(start_block_kernel, dim3(gx), dim3(256), 0, c->stream, Q1, lda_Q, m, k, ID, ld_ID, (int)b);

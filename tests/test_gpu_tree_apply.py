@@ -222,3 +222,28 @@ def test_qr_tree_2d_wide(Q, P):
 
     want = O.QR_tree_2D(A, b, split=None if P == 1 else split)
     assert rel(Q.from_gpu(dA, m, k), want) < 1e-11
+
+
+def test_tree_entry_points_reject_bad_arguments(Q):
+    """Status 1 for bad leading dimensions or ranks, 2 for b > 128; a
+    rejected construct_Q1 leaves Q1 as it was."""
+    from candmc_amd._lib import CQRError
+    m, b, P, k = 1000, 50, 3, 60
+    print("tree entry points: bad arguments", flush=True)
+    f = factor(Q, O.gaussian(m, b, 31), b, P)
+    dQ1 = torch.full((m * k,), 7.0, dtype=torch.float64, device="cuda")
+    with pytest.raises(CQRError) as e:
+        Q.construct_Q1(f["dY"], m - 1, f["dtau"], dQ1, m, m, b, k, numPes=P, tree_data=f["td"])
+    assert e.value.status == 1
+    assert bool((dQ1 == 7.0).all())
+    dB = Q.to_gpu(O.gaussian(m, 5, 32))
+    with pytest.raises(CQRError) as e:
+        Q.apply_tsqr_QT(f["dY"], m, f["dtau"], dB, m - 1, m, b, 5, numPes=P, tree_data=f["td"])
+    assert e.value.status == 1
+    with pytest.raises(CQRError) as e:
+        Q.tree_data_export(f["td"], m, b, P, f["L"])
+    assert e.value.status == 1
+    dA = Q.to_gpu(O.gaussian(400, 300, 33))
+    with pytest.raises(CQRError) as e:
+        Q.QR_tree_2D(dA, 400, 400, 300, 129, numPes=P)
+    assert e.value.status == 2
