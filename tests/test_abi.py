@@ -74,3 +74,34 @@ def test_compute_entry_points_reject_null_context():
     assert lib.cqr_construct_Q1_id(None, None, 1, None, None, 1, 1, 1, 1, 0, None, None, 0) == 1
     assert lib.cqr_qr_tree_2d(None, None, 1, 1, 1, 1, -1, 0) == 1
     assert lib.cqr_tree_data_export(None, None, 1, 1, 0, 0, None, None) == 1
+
+
+def _c_kind(decl):
+    """'ptr' for any pointer, else the base C type of one declaration."""
+    if "*" in decl:
+        return "ptr"
+    words = [w for w in decl.split() if w not in ("const", "unsigned")]
+    return words[0] if len(words) <= 2 else "?"
+
+
+def _ctypes_kind(t):
+    if t in (ctypes.c_void_p, ctypes.c_char_p) or (isinstance(t, type) and issubclass(t, ctypes._Pointer)):
+        return "ptr"
+    return {ctypes.c_int64: "int64_t", ctypes.c_int: "int", ctypes.c_size_t: "size_t",
+            ctypes.c_double: "double", ctypes.c_char: "char"}.get(t, repr(t))
+
+
+def test_ctypes_signatures_match_header_prototypes():
+    """Every prototype's argument count, argument kinds and return kind
+    agree with the ctypes table, so no call passes a 32-bit int where the
+    library reads 64 bits."""
+    src = open(os.path.join(ROOT, "include", "candmc_qr.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    src = re.sub(r"//[^\n]*", "", src)
+    protos = re.findall(r"([A-Za-z_][A-Za-z0-9_ \*]*?)\b(cqr_[A-Za-z0-9_]+)\s*\(([^;{]*?)\)\s*;", src)
+    assert {name for _, name, _ in protos} == set(header_symbols())
+    for ret, name, args in protos:
+        decls = [a.strip() for a in args.split(",") if a.strip() and a.strip() != "void"]
+        res, argtypes = _lib.SIGNATURES[name]
+        assert [_c_kind(d) for d in decls] == [_ctypes_kind(t) for t in argtypes], name
+        assert _c_kind(ret.strip()) == _ctypes_kind(res), name
