@@ -48,7 +48,10 @@ hipEvent_t prof_begin(cqr_ctx* c) {
 void prof_end(cqr_ctx* c, hipEvent_t a, int kind, double flops, double bytes) {
   if (!c->prof || !a) return;
   hipEvent_t b = pool_get(c);
-  if (!b) return;
+  if (!b) {
+    c->pool.push_back(a);  // no pair: the start event goes back unused
+    return;
+  }
   (void)hipEventRecord(b, c->stream);
   c->recs.push_back({a, b, kind, flops, bytes});
 }
