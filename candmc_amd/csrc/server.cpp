@@ -19,6 +19,7 @@
 #include <mutex>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "cqr_internal.h"
@@ -55,6 +56,7 @@ struct cqr_server {
   std::condition_variable cv_work, cv_done;
   std::deque<Job> queue;
   std::unordered_map<int64_t, cqr_result> done;   // finished, not yet collected
+  std::unordered_set<int64_t> live;               // submitted, not yet collected
   std::vector<int64_t> done_order;
   std::vector<cqr_ctx*> ctxs;
   std::vector<hipStream_t> streams;
@@ -169,6 +171,7 @@ int64_t cqr_server_submit(cqr_server* s, const cqr_request* r) {
     j.id = s->next_id++;
     j.res.id = j.id;
     s->queue.push_back(j);
+    s->live.insert(j.id);
   }
   s->cv_work.notify_all();
   return j.id;
@@ -177,10 +180,14 @@ int64_t cqr_server_submit(cqr_server* s, const cqr_request* r) {
 int cqr_server_wait(cqr_server* s, int64_t id, cqr_result* out) {
   if (!s || id < 0) return CQR_ERR_INVALID;
   std::unique_lock<std::mutex> lk(s->mu);
-  if (id >= s->next_id) return CQR_ERR_INVALID;
-  s->cv_done.wait(lk, [&] { return s->done.count(id) > 0; });
+  // an id never submitted, or already collected by a wait or a drain, has
+  // nothing left to wait for
+  if (!s->live.count(id)) return CQR_ERR_INVALID;
+  s->cv_done.wait(lk, [&] { return s->done.count(id) > 0 || !s->live.count(id); });
+  if (!s->done.count(id)) return CQR_ERR_INVALID;  // a concurrent drain took it
   if (out) *out = s->done[id];
   s->done.erase(id);
+  s->live.erase(id);
   for (size_t k = 0; k < s->done_order.size(); ++k)
     if (s->done_order[k] == id) {
       s->done_order.erase(s->done_order.begin() + (long)k);
@@ -191,18 +198,22 @@ int cqr_server_wait(cqr_server* s, int64_t id, cqr_result* out) {
 
 int cqr_server_drain(cqr_server* s, cqr_result* out, int max, int* count) {
   if (!s || (max > 0 && !out) || !count) return CQR_ERR_INVALID;
-  std::lock_guard<std::mutex> lk(s->mu);
   int k = 0;
-  std::vector<int64_t> keep;
-  for (int64_t id : s->done_order) {
-    if (k < max) {
-      out[k++] = s->done[id];
-      s->done.erase(id);
-    } else {
-      keep.push_back(id);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::vector<int64_t> keep;
+    for (int64_t id : s->done_order) {
+      if (k < max) {
+        out[k++] = s->done[id];
+        s->done.erase(id);
+        s->live.erase(id);
+      } else {
+        keep.push_back(id);
+      }
     }
+    s->done_order.swap(keep);
   }
-  s->done_order.swap(keep);
+  if (k) s->cv_done.notify_all();  // waiters on a drained id return
   *count = k;
   return CQR_OK;
 }

@@ -319,7 +319,10 @@ int cqr_server_destroy(cqr_server* server);      /* waits for the slots     */
 /* Returns the request id (≥ 0) or −status (e.g. −CQR_ERR_NOMEM when the
  * footprint exceeds the whole budget).                                   */
 int64_t cqr_server_submit(cqr_server* server, const cqr_request* request);
-int cqr_server_wait(cqr_server* server, int64_t id, cqr_result* result);   /* consumes */
+/* Blocks until request id finishes and consumes its result; returns
+ * CQR_ERR_INVALID at once for an id never submitted or already collected
+ * (by a wait or a drain).                                                */
+int cqr_server_wait(cqr_server* server, int64_t id, cqr_result* result);
 /* Moves up to max finished results (in completion order) into out.      */
 int cqr_server_drain(cqr_server* server, cqr_result* out, int max, int* count);
 int cqr_server_pending(cqr_server* server, int64_t* queued, size_t* bytes_in_use);
