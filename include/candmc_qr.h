@@ -315,7 +315,9 @@ typedef struct cqr_server cqr_server;
 /* Workspace bytes one least-squares solve needs (its admission footprint). */
 size_t cqr_lstsq_workspace_bytes(int64_t m, int64_t n, int64_t nrhs, int64_t b);
 int cqr_server_create(cqr_server** server, int device, int nslots, size_t hbm_budget);
-int cqr_server_destroy(cqr_server* server);      /* waits for the slots     */
+/* Waits for the requests already running; queued requests that have not
+ * started are dropped (their results never appear).                     */
+int cqr_server_destroy(cqr_server* server);
 /* Returns the request id (≥ 0) or −status (e.g. −CQR_ERR_NOMEM when the
  * footprint exceeds the whole budget).                                   */
 int64_t cqr_server_submit(cqr_server* server, const cqr_request* request);
