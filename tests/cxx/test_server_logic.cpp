@@ -5,7 +5,8 @@
 // collected by a wait or a drain, returns CQR_ERR_INVALID instead of
 // blocking; two waiters on one id, or a waiter racing a drain, end with
 // exactly one collector; every request of a burst through two slots is
-// served.  Built with g++ and run by tests/test_server_logic.py.
+// served; destroying a server with requests queued drops them and returns
+// once the running one ends.  Built with g++ and run by tests/test_server_logic.py.
 #include <hip/hip_runtime_api.h>
 
 #include <atomic>
@@ -129,6 +130,15 @@ int main() {
   size_t in_use = 1;
   CHECK(cqr_server_pending(s, &queued, &in_use) == CQR_OK && queued == 0 && in_use == 0);
   CHECK(cqr_server_destroy(s) == CQR_OK);
+
+  // destroying with requests queued returns once the running one ends
+  cqr_server* s1slot = nullptr;
+  CHECK(cqr_server_create(&s1slot, 0, 1, 1 << 20) == CQR_OK);
+  solve_ms = 50;
+  for (int i = 0; i < 20; ++i) CHECK(cqr_server_submit(s1slot, &r) >= 0);
+  const auto t0 = std::chrono::steady_clock::now();
+  CHECK(cqr_server_destroy(s1slot) == CQR_OK);
+  CHECK(std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(500));
   printf("server logic ok\n");
   return 0;
 }
