@@ -319,8 +319,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU "
-              "runs are not part of this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the library "
+              "factors each matrix on a single MI355X and has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
     import torch
