@@ -5,6 +5,8 @@ import ctypes
 import os
 import re
 
+import numpy as np
+
 import pytest
 
 from candmc_amd import _lib
@@ -105,3 +107,46 @@ def test_ctypes_signatures_match_header_prototypes():
         res, argtypes = _lib.SIGNATURES[name]
         assert [_c_kind(d) for d in decls] == [_ctypes_kind(t) for t in argtypes], name
         assert _c_kind(ret.strip()) == _ctypes_kind(res), name
+
+
+def test_plan_sweep_against_reference_tree():
+    """Seeded sweep over shapes, width and leaf hints: every accepted plan
+    keeps ≥ b rows per leaf, honours the hint where the rows allow it, and
+    its level count and tree_data size follow the reference's tree
+    (bitree_tsqr.cxx:172-208)."""
+    lib = _lib.load()
+    rng = np.random.default_rng(1234)
+    accepted = 0
+    for _ in range(300):
+        b = int(rng.integers(1, 129))
+        m = int(rng.integers(b, 200 * b + 1))
+        hint = int(rng.integers(0, 40))
+        s, L, rows, levels = _plan(m, b, hint)
+        assert s in (0, 2)
+        if s:
+            continue
+        accepted += 1
+        assert 1 <= L <= max(1, m // b)
+        assert L >= min(max(hint, 1), m // b)
+        assert rows == -(-m // L) and m // L >= b
+        assert levels == len(O.bitree_levels(L))
+        nodes = sum(len(O.bitree_pairs(n)) for n in O.bitree_levels(L))
+        assert lib.cqr_tree_data_doubles(m, b, hint) == L * b * b + nodes * (2 * b * b + b)
+    assert accepted > 250
+
+
+@pytest.mark.parametrize("L", list(range(1, 70)))
+def test_reference_tree_absorbs_every_leaf_once(L):
+    """The oracle's tree schedule: each level pairs receiver r with sender
+    r + ceil(n/2); over all levels every leaf but 0 is sent exactly once,
+    always to a lower rank still active, and leaf 0 ends holding R."""
+    sent = []
+    active = set(range(L))
+    for n in O.bitree_levels(L):
+        assert active == set(range(n))
+        for r, src in O.bitree_pairs(n):
+            assert r < src < n and r in active and src in active
+            sent.append(src)
+        active -= {src for _, src in O.bitree_pairs(n)}
+    assert sorted(sent) == list(range(1, L))
+    assert active == {0}
