@@ -4,6 +4,8 @@ schedule (alg/QR/tsqr/bitree_tsqr.cxx:172-208).  No compute call is made."""
 import ctypes
 import os
 import re
+import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -149,3 +151,38 @@ def test_reference_tree_absorbs_every_leaf_once(L):
         active -= {src for _, src in O.bitree_pairs(n)}
     assert sorted(sent) == list(range(1, L))
     assert active == {0}
+
+
+C_CLIENT = r"""
+#include <stdio.h>
+#include "candmc_qr.h"
+int main(void) {
+  int64_t L = 0, rows = 0, levels = 0;
+  if (cqr_plan_tsqr(8192, 64, 4, 0, &L, &rows, &levels) != CQR_OK) return 1;
+  printf("%lld %lld %lld %s\n", (long long)L, (long long)rows, (long long)levels, cqr_status_string(CQR_OK));
+  return cqr_hh_recon_qr(NULL, NULL, 1, 1, 1, NULL, 0) == CQR_ERR_INVALID ? 0 : 2;
+}
+"""
+
+
+def test_headers_compile_standalone_and_link_from_c(tmp_path):
+    """candmc_qr.h is plain C99 (no HIP or torch types) and a C program
+    links the library and calls its host-only entry points; the drop-in
+    header is plain C++11."""
+    if shutil.which("gcc") is None or shutil.which("g++") is None:
+        pytest.skip("needs gcc and g++")
+    inc = os.path.join(ROOT, "include")
+    src = tmp_path / "client.c"
+    src.write_text(C_CLIENT)
+    exe = str(tmp_path / "client")
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-I" + inc, str(src), "-o", exe,
+                    "-L" + libdir, "-lcandmc_qr", "-Wl,-rpath," + libdir], check=True, timeout=120)
+    run = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert run.returncode == 0, run.stdout + run.stderr
+    s, L, rows, levels = _plan(8192, 64, 4)
+    assert run.stdout.split() == [str(L), str(rows), str(levels), "ok"]
+    cxx = tmp_path / "dropin.cpp"
+    cxx.write_text('#include "candmc/CANDMC_QR.h"\nint main() { return 0; }\n')
+    subprocess.run(["g++", "-std=c++11", "-Wall", "-Wextra", "-Werror", "-fsyntax-only", "-I" + inc, str(cxx)],
+                   check=True, timeout=120)
