@@ -1,6 +1,9 @@
 // Context, errors, workspace arena and TSQR planning (host only).
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
+#include <set>
+#include <tuple>
 
 #include "cqr_internal.h"
 
@@ -21,6 +24,17 @@ int set_err(cqr_ctx* c, int code, const char* fmt, ...) {
 int check_launch(cqr_ctx* c, const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(c, CQR_ERR_DEVICE, "%s launch: %s", what, hipGetErrorString(e));
+  return CQR_OK;
+}
+
+int set_max_lds(cqr_ctx* c, const void* fn, int bytes) {
+  static std::mutex mu;
+  static std::set<std::tuple<const void*, int, int>> done;
+  std::lock_guard<std::mutex> lk(mu);
+  const auto key = std::make_tuple(fn, c->device, bytes);
+  if (done.count(key)) return CQR_OK;
+  CQR_HIP(c, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  done.insert(key);
   return CQR_OK;
 }
 

@@ -71,6 +71,9 @@ int set_err(cqr_ctx* c, int code, const char* fmt, ...);
     if (s_ != CQR_OK) return s_;                                              \
   } while (0)
 int check_launch(cqr_ctx* c, const char* what);
+// Raises the dynamic-LDS limit of kernel `fn` to `bytes` on the context's
+// device, once per (kernel, device, bytes); safe from several host threads.
+int set_max_lds(cqr_ctx* c, const void* fn, int bytes);
 
 // Event bracketing of a launch when profiling is on (no-ops otherwise).
 hipEvent_t prof_begin(cqr_ctx* c);

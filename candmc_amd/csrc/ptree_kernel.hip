@@ -714,11 +714,7 @@ static int btree_launch(cqr_ctx* c, const double* leafR, double* nodeR, double* 
                         uint64_t* cnt, int L, int nnodes, int b, uint64_t epoch, const BatchProb* bp, int nprob) {
   const size_t lds = sizeof(double) * btree_smem_doubles<BMAX>();
   auto k = btree_kernel<BMAX>;
-  static bool attr = false;
-  if (!attr) {
-    CQR_HIP(c, hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr = true;
-  }
+  CQR_TRY(set_max_lds(c, (const void*)k, (int)lds));
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(k, dim3(nnodes), dim3(4 * BMAX), lds, c->stream, leafR, nodeR, V2, T, tau, cnt, epoch, L, b,
                      c->d_terr, bp, nprob);
@@ -756,11 +752,7 @@ static int leaf_tree_launch(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L
   const size_t lds = sizeof(double) * std::max<size_t>(btree_smem_doubles<128>(),
                                                       (size_t)b * b + 2 * 64 * RPL + 128);
   auto k = leaf_tree_kernel<RPL>;
-  static bool attr = false;
-  if (!attr) {
-    CQR_HIP(c, hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr = true;
-  }
+  CQR_TRY(set_max_lds(c, (const void*)k, (int)lds));
   const uint64_t epoch = ++c->tree_epoch;
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(k, dim3(L + nnodes), dim3(512), lds, c->stream, A, lda, m, L, b, leafR, leafT, leafTau, nodeR,

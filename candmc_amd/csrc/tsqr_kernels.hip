@@ -699,11 +699,7 @@ static int leaf_launch(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int
                        double* R, double* T, double* tau, const BatchProb* bp = nullptr, int nprob = 0) {
   const size_t lds = sizeof(double) * ((size_t)b * b + 2 * 64 * RPL + 128);
   auto k = leaf_qr_kernel<WAVES, CPW, RPL>;
-  static bool attr = false;
-  if (!attr) {
-    CQR_HIP(c, hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, 163840));
-    attr = true;
-  }
+  CQR_TRY(set_max_lds(c, (const void*)k, 163840));
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(k, dim3(L), dim3(64 * WAVES), lds, c->stream, A, lda, m, L, b, R, T, tau, bp, nprob);
   prof_end(c, ev, PK_LEAF, 2.0 * m * b * b - (2.0 / 3.0) * L * (double)b * b * b, 16.0 * m * b);
@@ -749,11 +745,7 @@ static int node_launch(cqr_ctx* c, double* Rleaf, int n, int node0, int b, doubl
   if (npairs == 0) return CQR_OK;
   const size_t lds = sizeof(double) * ((size_t)b * b + 2 * 64 * RPL + 128);
   auto k = tree_node_kernel<WAVES, CPW, RPL>;
-  static bool attr = false;
-  if (!attr) {
-    CQR_HIP(c, hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, 163840));
-    attr = true;
-  }
+  CQR_TRY(set_max_lds(c, (const void*)k, 163840));
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(k, dim3(npairs), dim3(64 * WAVES), lds, c->stream, Rleaf, n, node0, b, V2, T, tau);
   prof_end(c, ev, PK_TREE, npairs * (2.0 / 3.0) * b * (double)b * b, npairs * 40.0 * b * b);
@@ -778,15 +770,11 @@ int bgemm(cqr_ctx* c, const BgemmArgs& g, int batch, int max_m) {
   return check_launch(c, "bgemm");
 }
 
-static int big_lds_attr(cqr_ctx* c, const void* k) {
-  CQR_HIP(c, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 163840));
-  return CQR_OK;
-}
+static int big_lds_attr(cqr_ctx* c, const void* k) { return set_max_lds(c, k, 163840); }
 
 int hr_core(cqr_ctx* c, const double* Q1top, int64_t ldq, const double* Rroot, double* Atop,
             int64_t lda, double* W, double* Uinv, double* Winv, double* LinvT, int64_t* signs, int b) {
-  static bool attr = false;
-  if (!attr) { CQR_TRY(big_lds_attr(c, (const void*)hr_core_kernel)); attr = true; }
+  CQR_TRY(big_lds_attr(c, (const void*)hr_core_kernel));
   const size_t lds = sizeof(double) * ((size_t)b * (b + 1) + 2 * b);
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(hr_core_kernel, dim3(1), dim3(512), lds, c->stream, Q1top, ldq, Rroot, Atop,
@@ -797,8 +785,7 @@ int hr_core(cqr_ctx* c, const double* Q1top, int64_t ldq, const double* Rroot, d
 
 int hr_core_batched(cqr_ctx* c, const BatchProb* bp, int nprob, const double* Y, const double* leafR,
                     const double* nodeR, double* Abase, double* W, double* Uinv, double* Winv, double* LinvT, int b) {
-  static bool attr = false;
-  if (!attr) { CQR_TRY(big_lds_attr(c, (const void*)hr_core_batched_kernel)); attr = true; }
+  CQR_TRY(big_lds_attr(c, (const void*)hr_core_batched_kernel));
   const size_t lds = sizeof(double) * ((size_t)b * (b + 1) + 2 * b);
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(hr_core_batched_kernel, dim3(nprob), dim3(512), lds, c->stream, bp, Y, leafR, nodeR, Abase, W,
@@ -808,8 +795,7 @@ int hr_core_batched(cqr_ctx* c, const BatchProb* bp, int nprob, const double* Y,
 }
 
 int launch_signed_nlu(cqr_ctx* c, double* A, int64_t lda, double* R, int64_t lda_R, int64_t* signs, int b) {
-  static bool attr = false;
-  if (!attr) { CQR_TRY(big_lds_attr(c, (const void*)signed_nlu_kernel)); attr = true; }
+  CQR_TRY(big_lds_attr(c, (const void*)signed_nlu_kernel));
   const size_t lds = sizeof(double) * ((size_t)b * (b + 1) + b);
   hipLaunchKernelGGL(signed_nlu_kernel, dim3(1), dim3(512), lds, c->stream, A, lda, R, lda_R, signs, b);
   return check_launch(c, "signed_nlu");
@@ -817,8 +803,7 @@ int launch_signed_nlu(cqr_ctx* c, double* A, int64_t lda, double* R, int64_t lda
 
 int launch_tri_inverse(cqr_ctx* c, const double* X, int64_t ldx, double* Xi, int64_t ldi, int b,
                        int upper, int unit) {
-  static bool attr = false;
-  if (!attr) { CQR_TRY(big_lds_attr(c, (const void*)tri_inverse_kernel)); attr = true; }
+  CQR_TRY(big_lds_attr(c, (const void*)tri_inverse_kernel));
   const size_t lds = sizeof(double) * ((size_t)b * b + b);
   hipLaunchKernelGGL(tri_inverse_kernel, dim3(1), dim3(512), lds, c->stream, X, ldx, Xi, ldi, b, upper, unit);
   return check_launch(c, "tri_inverse");
