@@ -51,6 +51,57 @@ def test_pack_unpack_roundtrip():
         assert np.array_equal(O.unpack_lower(O.pack_lower(L, b, 1), b, 1), L)
 
 
+@pytest.mark.parametrize("b", [1, 2, 5, 16, 33])
+def test_packed_triangles_match_lapack_dtrttp(b):
+    """util.cxx:322-406 store triangles column by column, LAPACK's packed
+    storage: pack_upper / unpack_upper are dtrttp / dtpttr with 'U',
+    pack_lower / unpack_lower(has_diag=1) with 'L'; has_diag=0 is the 'L'
+    pack with each column's diagonal entry removed."""
+    rng = np.random.default_rng(100 + b)
+    A = np.asfortranarray(rng.standard_normal((b, b)))
+    ap_u, info = lp.dtrttp(A, uplo="U")
+    assert info == 0
+    assert np.array_equal(O.pack_upper(A, b), ap_u)
+    full_u, info = lp.dtpttr(b, ap_u, uplo="U")
+    assert info == 0
+    assert np.array_equal(O.unpack_upper(ap_u, b), np.triu(full_u))
+    ap_l, info = lp.dtrttp(A, uplo="L")
+    assert info == 0
+    assert np.array_equal(O.pack_lower(A, b, 1), ap_l)
+    full_l, info = lp.dtpttr(b, ap_l, uplo="L")
+    assert np.array_equal(O.unpack_lower(ap_l, b, 1), np.tril(full_l))
+    # strictly lower: drop entry j·b − j(j−1)/2 (the diagonal of column j)
+    diag_pos = [j * b - j * (j - 1) // 2 for j in range(b)]
+    strict = np.delete(ap_l, diag_pos)
+    assert np.array_equal(O.pack_lower(A, b, 0), strict)
+    assert np.array_equal(O.unpack_lower(strict, b, 0), np.tril(A, -1))
+    # a row-major pack (the transpose's order) differs once b > 2
+    if b > 2:
+        assert not np.array_equal(O.pack_upper(A, b), lp.dtrttp(np.asfortranarray(A.T), uplo="L")[0])
+
+
+def test_tree_data_packed_offsets_follow_index_formula():
+    """bitree_tsqr.cxx:213-217: per receiving level, the dtpqrt T (nb×b,
+    ld nb, nb = min(b, 16)) at offset i + j·nb, then pack_upper(V2) with
+    V2(i, j), i ≤ j, at nb·b + j(j+1)/2 + i; levels follow one another."""
+    for b in (5, 20):
+        nb = min(b, O.TAU_BLK)
+        per = nb * b + b * (b + 1) // 2
+        levels = []
+        for lv in range(3):
+            T = np.array([[1e6 * lv + 1e3 * j + i for j in range(b)] for i in range(nb)])
+            V2 = np.array([[-(1e6 * lv + 1e3 * j + i) for j in range(b)] for i in range(b)])
+            levels.append((T, V2))
+        p = O.tree_data_packed(levels, b)
+        assert p.size == 3 * per
+        for lv, (T, V2) in enumerate(levels):
+            for j in range(b):
+                for i in range(nb):
+                    assert p[lv * per + i + j * nb] == T[i, j]
+                for i in range(j + 1):
+                    assert p[lv * per + nb * b + j * (j + 1) // 2 + i] == V2[i, j]
+
+
 def test_dlarft_equals_reflector_product_and_dgeqrt():
     A = O.gaussian(40, 9, 3)
     qr, tau = O.local_tsqr(A)
