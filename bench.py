@@ -140,15 +140,19 @@ def lsq_leg(qr, torch, m=65536, n=2048, kappas=(1e2, 1e6, 1e8), seed=11):
         s = torch.logspace(0, -np.log10(kappa), n, dtype=torch.float64, device="cuda")
         A = ((U * s) @ V.T).T.contiguous().reshape(-1)        # column-major m×n
         B = ((U * s) @ (V.T @ xt) + 1e-6 * noise).reshape(-1)
-        (X64, it64, _), t64 = _timed(lambda: qr.lstsq(A, m, m, n, B, m, 1, prec=qr.PREC_F64, max_iter=2,
-                                                      tol=1e-14), torch)
+        (X64, it64, u64, st64), t64 = _timed(lambda: qr.lstsq(A, m, m, n, B, m, 1, prec=qr.PREC_F64, max_iter=6,
+                                                               tol=1e-12, raise_noconverge=False), torch)
         x64 = X64.double()
-        case = {"kappa": kappa, "fp64_ms": round(t64, 2), "fp64_iters": it64}
+        case = {"kappa": kappa, "fp64_ms": round(t64, 2), "fp64_iters": it64, "fp64_status": st64,
+                "fp64_last_update": float(f"{u64:.3g}")}
+        ctx = qr.context()
         for name, prec in (("fp32", qr.PREC_F32), ("bf16", qr.PREC_BF16)):
-            (Xm, itm, upd), tm = _timed(lambda: qr.lstsq(A, m, m, n, B, m, 1, prec=prec, max_iter=40,
-                                                          tol=1e-11), torch)
+            (Xm, itm, upd, st), tm = _timed(lambda: qr.lstsq(A, m, m, n, B, m, 1, prec=prec, max_iter=40,
+                                                              tol=1e-10, raise_noconverge=False), torch)
             err = float((Xm - x64).abs().max() / x64.abs().max())
-            case[name] = {"ms": round(tm, 2), "iters": itm, "rel_err_vs_fp64": float(f"{err:.3g}"),
+            used = {0: "fp64", 1: "fp32", 2: "bf16"}.get(ctx.last_prec(), "none")
+            case[name] = {"ms": round(tm, 2), "iters": itm, "status": st, "factorized_in": used,
+                          "last_update": float(f"{upd:.3g}"), "rel_err_vs_fp64": float(f"{err:.3g}"),
                           "met_1e-10": bool(err <= 1e-10), "speedup_vs_fp64": round(t64 / tm, 3),
                           "gflops_factor_equiv": round(flops / (tm * 1e-3) / 1e9, 1)}
         out["cases"].append(case)
@@ -337,8 +341,8 @@ def main():
 
     for _ in range(args.warmup):
         A.copy_(A0)
-        qr.QR_2D(A, m, m, n, b)
-    torch.cuda.synchronize()
+        qr.QR_2D(A, m, m, n, b, sync=False)
+    ctx.sync()
 
     def run_steps(profiled):
         """K factorizations, each bracketed by events on the launch stream;
@@ -354,10 +358,11 @@ def main():
         for s in range(args.steps):
             A.copy_(A0)
             ev[s][0].record(stream)
-            qr.QR_2D(A, m, m, n, b)
+            qr.QR_2D(A, m, m, n, b, sync=False)
             ev[s][1].record(stream)
         torch.cuda.synchronize()
         w = time.perf_counter() - w0
+        ctx.sync()   # raises if any tree wait of these steps gave up
         pr = None
         if profiled:
             pr = ctx.profile_read()

@@ -29,7 +29,6 @@ SIGNATURES = {
     "cqr_set_check_finite": (_I, [_P, _I]),
     "cqr_set_leaf_rows": (_I, [_P, _I64]),
     "cqr_set_lookahead": (_I, [_P, _I]),
-    "cqr_set_blas_update": (_I, [_P, _I]),
     "cqr_status_string": (_c.c_char_p, [_I]),
     "cqr_profile_enable": (_I, [_P, _I]),
     "cqr_profile_read": (_I, [_P, _I, _c.POINTER(_D), _c.POINTER(_D), _c.POINTER(_D), _c.POINTER(_I64)]),
@@ -52,6 +51,10 @@ SIGNATURES = {
     "cqr_hh_recon_qr_ex": (_I, [_P, _P, _I64, _I64, _I64, _P, _P, _P, _I64]),
     "cqr_compute_invT_from_W": (_I, [_P, _P, _I64, _P]),
     "cqr_compute_invT_from_Y": (_I, [_P, _P, _I64, _P, _I64, _I64, _I64]),
+    "cqr_compute_invT_from_Y2": (_I, [_P, _P, _I64, _P, _I64, _P, _I64, _I64, _I64, _I64]),
+    "cqr_trim": (_I, [_P]),
+    "cqr_set_lstsq_escalate": (_I, [_P, _I]),
+    "cqr_lstsq_last_prec": (_I, [_P]),
     "cqr_upd_A": (_I, [_P, _P, _I64, _P, _I64, _I64, _I64, _I64, _P, _I]),
     "cqr_update_A": (_I, [_P, _P, _I64, _P, _I64, _I64, _I64, _I64, _P, _P, _I64, _I]),
     "cqr_qr_2d": (_I, [_P, _P, _I64, _I64, _I64, _I64, _P, _I64]),
@@ -86,7 +89,7 @@ SIGNATURES = {
 }
 
 STATUS = {0: "CQR_OK", 1: "CQR_ERR_INVALID", 2: "CQR_ERR_UNSUPPORTED", 3: "CQR_ERR_NOMEM",
-          4: "CQR_ERR_DEVICE", 5: "CQR_ERR_NONFINITE"}
+          4: "CQR_ERR_DEVICE", 5: "CQR_ERR_NONFINITE", 6: "CQR_ERR_NOCONVERGE", 7: "CQR_ERR_CANCELLED"}
 
 _lib = None
 

@@ -194,7 +194,6 @@ int cqr_destroy(cqr_ctx* c) {
     (void)hipEventDestroy(r.b);
   }
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
-  cqr::blas_destroy(c);
   if (c->ws) (void)hipFree(c->ws);
   if (c->d_flag) (void)hipFree(c->d_flag);
   if (c->d_cnt) (void)hipFree(c->d_cnt);
@@ -234,16 +233,22 @@ int cqr_set_lookahead(cqr_ctx* c, int on) {
   return CQR_OK;
 }
 
-int cqr_set_blas_update(cqr_ctx* c, int on) {
-  if (!c) return CQR_ERR_INVALID;
-  c->blas_update = on ? 1 : 0;
-  return CQR_OK;
-}
-
 int cqr_reserve(cqr_ctx* c, size_t bytes) {
   if (!c) return CQR_ERR_INVALID;
   (void)hipSetDevice(c->device);
   return ensure_ws(c, bytes);
+}
+
+int cqr_trim(cqr_ctx* c) {
+  if (!c) return CQR_ERR_INVALID;
+  (void)hipSetDevice(c->device);
+  if (!c->ws) return CQR_OK;
+  CQR_HIP(c, hipStreamSynchronize(c->stream));
+  CQR_HIP(c, hipStreamSynchronize(c->side));
+  CQR_HIP(c, hipFree(c->ws));
+  c->ws = nullptr;
+  c->ws_bytes = 0;
+  return CQR_OK;
 }
 
 size_t cqr_workspace_bytes(const cqr_ctx* c) { return c ? c->ws_bytes : 0; }
@@ -293,7 +298,9 @@ const char* cqr_status_string(int s) {
     case CQR_ERR_UNSUPPORTED: return "unsupported shape";
     case CQR_ERR_NOMEM: return "out of device memory";
     case CQR_ERR_DEVICE: return "HIP runtime error";
-    case CQR_ERR_NONFINITE: return "non-finite input";
+    case CQR_ERR_NONFINITE: return "non-finite input or result";
+    case CQR_ERR_NOCONVERGE: return "refinement did not reach the tolerance";
+    case CQR_ERR_CANCELLED: return "request cancelled";
     default: return "unknown status";
   }
 }

@@ -40,19 +40,19 @@ struct cqr_ctx {
   uint64_t q1_epoch = 0;
   uint64_t tree_epoch = 0;      // launch epoch folded into the counters
   hipStream_t side = nullptr;   // second stream of the QR_2D lookahead
-  int blas_update = 1;          // bulk of A −= Y·Z' through the vendor BLAS
-  void* blas = nullptr;         // its handle, created on first use
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int lookahead = 1;
+  int lsq_escalate = 1;         // refactor a stalled mixed solve in FP64
+  int lsq_last_prec = -1;       // factorization precision of the last solve's X
   std::string err;
 };
 
 namespace cqr {
-void blas_destroy(cqr_ctx* c);
-int blas_gemm(cqr_ctx* c, int ta, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
-              const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int kind);
-int blas_update(cqr_ctx* c, int64_t m, int64_t n, int64_t k, const double* Y, int64_t ldy, const double* Z,
-                int64_t ldz, double* A, int64_t lda);
+// C (M×N) := beta·C + alpha·P·Z for K ≤ 256, beta ∈ {0, 1}: P column-major
+// (ld ldp), Z column-major K×N (ld ldz), on FP64 MFMA (gemm_update.hip).
+// shape 0: 128×128 tiles, 1: 256×128, 2: 128×256.
+int update_gemm(cqr_ctx* c, int64_t M, int64_t N, int64_t K, double alpha, const double* P, int64_t ldp,
+                const double* Z, int64_t ldz, double beta, double* C, int64_t ldc, int kind, int shape);
 
 // ---------------------------------------------------------------------------
 // Errors
@@ -279,8 +279,12 @@ int launch_trsm_step(cqr_ctx* c, const double* R, int64_t ldr, int64_t n, const 
 int launch_axpby(cqr_ctx* c, int64_t m, int64_t n, double alpha, const double* X, int64_t ldx, double beta,
                  double* Y, int64_t ldy);
 int launch_colnorm(cqr_ctx* c, const double* X, int64_t ldx, int64_t m, int n, double* out);
+// out[0..1] := max and min |R(i,i)| over i < n (NaN if any entry is NaN).
+int launch_diag_range(cqr_ctx* c, const double* R, int64_t ldr, int64_t n, double* out);
 int launch_copy2d(cqr_ctx* c, const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m,
                   int64_t n);
+// dst (m×n, ld ldd) := v everywhere (a kernel, so graph capture records no memset)
+int launch_fill2d(cqr_ctx* c, double* dst, int64_t ldd, int64_t m, int64_t n, double v);
 
 // Drivers (qr_driver.cpp)
 size_t panel_ws_bytes(int64_t m, int64_t b, int64_t nleaves, int64_t leaf_rows);
@@ -306,5 +310,16 @@ struct LsqWS {
 size_t lstsq_layout(int64_t m, int64_t n, int64_t nr, int64_t b, int64_t leaf_rows, Qr2dWS* o, LsqWS* w);
 int hh_recon_core(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, double* W,
                   double* T, double* invT_T, double* Tt, int64_t nleaves);
+// dgeqrf of one m×b block on caller scratch of local_tsqr_ws_bytes bytes.
+size_t local_tsqr_ws_bytes(int64_t m, int64_t b, int64_t leaf_rows);
+int local_tsqr_ws(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, double* tau, double* T, char* base);
+// bitree_tsqr without tree_data over P ranks (tsqr_apply.hip): each rank's
+// row block gets its dgeqrf [V\R] in place, the tree reduces the ranks' R
+// with the reference's schedule, and with output_Y the senders' top
+// triangles receive the node reflectors V2 and the receivers' the node R.
+int bitree_ranks(cqr_ctx* c, double* A, int64_t lda, double* R, double* tau, int64_t m, int64_t b, int64_t P,
+                 int output_Y);
+// Ranks doing work for P ranks over m rows of width b (bitree_tsqr.cxx:123-126).
+inline int64_t ranks_working(int64_t m, int64_t b, int64_t P) { return P * b > m ? m / b : P; }
 
 }  // namespace cqr

@@ -137,24 +137,112 @@ void local_tsqr(double* A, double* R, int64_t const m, int64_t const b, int64_t 
   t.finish(fn);
 }
 
+namespace {
+
+// ---- tree factors of the several-rank path ------------------------------
+// The library's TSQR tree over numPes virtual ranks keeps per-leaf and
+// per-node factors (cqr_tree_data_doubles doubles) and every leaf's taus,
+// more than the reference sizes its tree_data (2b²(ln P + 2) doubles,
+// alg/QR/hh_recon/hh_recon.cxx:151) and tau (b doubles) for.  So the
+// factors stay on the device, held here, and the caller's tree_data
+// receives only a five-double tag {magic, id, m, b, numPes} that names
+// them; construct_Q1 and apply_tsqr_QT find them by that tag.  The last
+// kTreeCache factorizations are held (the oldest is dropped first).
+constexpr double kTreeMagic = 7.0e300;
+constexpr size_t kTreeCache = 8;
+constexpr int kTagDoubles = 5;
+struct TreeEntry {
+  int64_t id, m, b, P;
+  double* td = nullptr;   // cqr_tree_data_doubles(m, b, P) doubles
+  double* tau = nullptr;  // every leaf's taus
+};
+std::vector<TreeEntry> g_trees;
+int64_t g_tree_next = 1;
+
+void put_tag(double* dst, const double* tag, const char* fn) {
+  if (on_device(dst)) {
+    if (hipMemcpy(dst, tag, kTagDoubles * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      die(fn, "cannot write the tree_data tag");
+  } else {
+    for (int i = 0; i < kTagDoubles; ++i) dst[i] = tag[i];
+  }
+}
+
+const TreeEntry& find_tree(const double* tree_data, int64_t m, int64_t b, int64_t P, const char* fn) {
+  double tag[kTagDoubles];
+  if (on_device(tree_data)) {
+    if (hipMemcpy(tag, tree_data, sizeof tag, hipMemcpyDeviceToHost) != hipSuccess) die(fn, "cannot read tree_data");
+  } else {
+    for (int i = 0; i < kTagDoubles; ++i) tag[i] = tree_data[i];
+  }
+  if (tag[0] != kTreeMagic) die(fn, "tree_data does not come from bitree_tsqr in this process");
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (const TreeEntry& e : g_trees)
+    if ((double)e.id == tag[1]) {
+      if (e.m != m || e.b != b || e.P != P) die(fn, "tree_data belongs to a TSQR of another shape");
+      return e;
+    }
+  die(fn, "the TSQR factors of this tree_data were released (more than 8 newer factorizations)");
+}
+
+}  // namespace
+
+// numPes virtual ranks.  tree_data == NULL, or a single working rank: the
+// reference's rank semantics (cqr_bitree_tsqr without tree_data: dgeqrf per
+// rank, V sent back with output_Y, tree_data untouched).  Otherwise the
+// library's leaves: A := leaf [V\R] blocks, the factors held on the device
+// and tree_data := their tag.  tau always receives b doubles (rank 0's, or
+// the first leaf's).
 void bitree_tsqr(double* A, int64_t const lda_A, double* R, double* tau, int64_t const m, int64_t const b,
                  int64_t const myRank, int64_t const numPes, int64_t const root, int64_t const req_id,
                  CommData_t cdt, int64_t const output_Y, double* tree_data) {
   (void)myRank; (void)root; (void)req_id;
   const char* fn = "bitree_tsqr";
   need_single_rank(cdt, fn);
+  if (numPes < 1 || b < 1 || m < b || lda_A < m) die(fn, "bad arguments (needs numPes >= 1, m >= b, lda_A >= m)");
   cqr_ctx* c = candmc_qr_context();
+  std::vector<double> tmp;
+  if (!tau) {  // the reference's local_tsqr then uses a buffer of its own
+    tmp.resize(b);
+    tau = tmp.data();
+  }
+  Stage a(A, (size_t)lda_A * b, true, fn), r(R, (size_t)b * b, true, fn), t(tau, (size_t)b, true, fn);
+  if (!tree_data || cqr::ranks_working(m, b, numPes) == 1) {
+    check(cqr_bitree_tsqr(c, a.dev, lda_A, r.dev, t.dev, m, b, numPes, (int)output_Y, nullptr), fn);
+    sync(fn);
+    a.finish(fn);
+    r.finish(fn);
+    t.finish(fn);
+    return;
+  }
   int64_t L = 0;
   check(cqr_plan_tsqr(m, b, numPes, 0, &L, nullptr, nullptr), fn);
-  if (!tree_data) die(fn, "tree_data is required (cqr_tree_data_doubles(m, b, numPes) doubles)");
-  Stage a(A, (size_t)lda_A * b, true, fn), r(R, (size_t)b * b, true, fn), t(tau, (size_t)L * b, true, fn),
-      td(tree_data, (size_t)cqr_tree_data_doubles(m, b, numPes), true, fn);
-  check(cqr_bitree_tsqr(c, a.dev, lda_A, r.dev, t.dev, m, b, numPes, (int)output_Y, td.dev), fn);
+  TreeEntry e;
+  e.m = m; e.b = b; e.P = numPes;
+  const size_t ntd = (size_t)cqr_tree_data_doubles(m, b, numPes);
+  if (hipMalloc(&e.td, ntd * sizeof(double)) != hipSuccess || hipMalloc(&e.tau, (size_t)L * b * sizeof(double)) != hipSuccess)
+    die(fn, "cannot hold the tree factors on the device");
+  check(cqr_bitree_tsqr(c, a.dev, lda_A, r.dev, e.tau, m, b, numPes, (int)output_Y, e.td), fn);
+  if (hipMemcpyAsync(t.dev, e.tau, (size_t)b * sizeof(double), hipMemcpyDeviceToDevice,
+                     (hipStream_t)cqr_get_stream(c)) != hipSuccess)
+    die(fn, "tau copy failed");
   sync(fn);
   a.finish(fn);
   r.finish(fn);
   t.finish(fn);
-  td.finish(fn);
+  double tag[kTagDoubles];
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    e.id = g_tree_next++;
+    if (g_trees.size() >= kTreeCache) {
+      (void)hipFree(g_trees.front().td);
+      (void)hipFree(g_trees.front().tau);
+      g_trees.erase(g_trees.begin());
+    }
+    g_trees.push_back(e);
+  }
+  tag[0] = kTreeMagic; tag[1] = (double)e.id; tag[2] = (double)m; tag[3] = (double)b; tag[4] = (double)numPes;
+  put_tag(tree_data, tag, fn);
 }
 
 void construct_Q1(double const* Y, int64_t const lda_Y, double const* tau, double* Q1, int64_t const lda_Q,
@@ -163,14 +251,19 @@ void construct_Q1(double const* Y, int64_t const lda_Y, double const* tau, doubl
   (void)myRank; (void)root;
   const char* fn = "construct_Q1";
   need_single_rank(cdt, fn);
-  if (!tree_data) die(fn, "tree_data from bitree_tsqr is required");
+  if (numPes < 1 || b < 1 || m < b || k < 1 || k > m) die(fn, "bad arguments");
   cqr_ctx* c = candmc_qr_context();
-  int64_t L = 0;
-  check(cqr_plan_tsqr(m, b, numPes, 0, &L, nullptr, nullptr), fn);
-  Stage y(Y, (size_t)lda_Y * b, false, fn), t(tau, (size_t)L * b, false, fn), q(Q1, (size_t)lda_Q * k, true, fn),
-      td(tree_data, (size_t)cqr_tree_data_doubles(m, b, numPes), false, fn), id(ID, (size_t)b * k, false, fn);
+  const bool one = cqr::ranks_working(m, b, numPes) == 1;
+  if (!one && !tree_data) die(fn, "tree_data from bitree_tsqr is required for several ranks");
+  Stage y(Y, (size_t)lda_Y * b, false, fn), q(Q1, (size_t)lda_Q * k, true, fn), id(ID, (size_t)b * k, false, fn);
   // ID is b×k with leading dimension b (construct_tsqr_Q.cxx:99,125)
-  check(cqr_construct_Q1_id(c, y.dev, lda_Y, t.dev, q.dev, lda_Q, m, b, k, numPes, td.dev, id.dev, b), fn);
+  if (one) {  // the one-rank dorgqr / dormqr branch (construct_tsqr_Q.cxx:87-102)
+    Stage t(tau, (size_t)b, false, fn);
+    check(cqr_construct_Q1_id(c, y.dev, lda_Y, t.dev, q.dev, lda_Q, m, b, k, 1, nullptr, id.dev, b), fn);
+  } else {
+    const TreeEntry& e = find_tree(tree_data, m, b, numPes, fn);
+    check(cqr_construct_Q1_id(c, y.dev, lda_Y, e.tau, q.dev, lda_Q, m, b, k, numPes, e.td, id.dev, b), fn);
+  }
   sync(fn);
   q.finish(fn);
 }
@@ -181,13 +274,18 @@ void apply_tsqr_QT(double const* Y, int64_t const lda_Y, double const* tau, doub
   (void)myRank; (void)root;
   const char* fn = "apply_tsqr_QT";
   need_single_rank(cdt, fn);
-  if (!tree_data) die(fn, "tree_data from bitree_tsqr is required");
+  if (numPes < 1 || b < 1 || m < b || k < 0) die(fn, "bad arguments");
   cqr_ctx* c = candmc_qr_context();
-  int64_t L = 0;
-  check(cqr_plan_tsqr(m, b, numPes, 0, &L, nullptr, nullptr), fn);
-  Stage y(Y, (size_t)lda_Y * b, false, fn), t(tau, (size_t)L * b, false, fn), bm(B, (size_t)lda_B * k, true, fn),
-      td(tree_data, (size_t)cqr_tree_data_doubles(m, b, numPes), false, fn);
-  check(cqr_apply_tsqr_QT(c, y.dev, lda_Y, t.dev, bm.dev, lda_B, m, b, k, numPes, td.dev), fn);
+  const bool one = cqr::ranks_working(m, b, numPes) == 1;
+  if (!one && !tree_data) die(fn, "tree_data from bitree_tsqr is required for several ranks");
+  Stage y(Y, (size_t)lda_Y * b, false, fn), bm(B, (size_t)lda_B * k, true, fn);
+  if (one) {  // dormqr('L','T') of the one-rank branch (apply_tsqr_QT.cxx:73-80)
+    Stage t(tau, (size_t)b, false, fn);
+    check(cqr_apply_tsqr_QT(c, y.dev, lda_Y, t.dev, bm.dev, lda_B, m, b, k, 1, nullptr), fn);
+  } else {
+    const TreeEntry& e = find_tree(tree_data, m, b, numPes, fn);
+    check(cqr_apply_tsqr_QT(c, y.dev, lda_Y, e.tau, bm.dev, lda_B, m, b, k, numPes, e.td), fn);
+  }
   sync(fn);
   bm.finish(fn);
 }
@@ -264,6 +362,19 @@ void compute_invT_from_Y(double const* Y, int64_t lda_Y, double* T, int64_t lda_
   cqr_ctx* c = candmc_qr_context();
   Stage y(Y, (size_t)lda_Y * b, false, fn), t(T, (size_t)lda_T * b, true, fn);
   check(cqr_compute_invT_from_Y(c, y.dev, lda_Y, t.dev, lda_T, mb, b), fn);
+  sync(fn);
+  t.finish(fn);
+}
+
+// The two-Y form (qr_2d.cxx:76-106): T (b1×b2) := Y1ᵀ·Y2.
+void compute_invT_from_Y(double const* Y1, int64_t lda_Y1, double const* Y2, int64_t lda_Y2, double* T,
+                         int64_t lda_T, int64_t mb, int64_t b1, int64_t b2, pview* pv) {
+  const char* fn = "compute_invT_from_Y";
+  need_grid(pv, fn);
+  cqr_ctx* c = candmc_qr_context();
+  Stage y1(Y1, (size_t)lda_Y1 * b1, false, fn), y2(Y2, (size_t)lda_Y2 * b2, false, fn),
+      t(T, (size_t)lda_T * b2, true, fn);
+  check(cqr_compute_invT_from_Y2(c, y1.dev, lda_Y1, y2.dev, lda_Y2, t.dev, lda_T, mb, b1, b2), fn);
   sync(fn);
   t.finish(fn);
 }

@@ -644,8 +644,7 @@ static int wy_update_f64_split(cqr_ctx* c, const double* Y, int64_t ldy, int smY
   CQR_TRY(gemm_f64_impl(c, t_trans ? 1 : 0, 0, b, n, b, 1.0, Tt, b, SM_LOWER, Z, b, SM_NONE, 0.0, Zp, b, nullptr));
   // A(0:bt) −= Y(0:bt)·Z' (masked), A(bt:m) −= Y(bt:m)·Z' (lean)
   CQR_TRY(gemm_f64_impl(c, 0, 0, bt, n, b, -1.0, Y, ldy, smY, Zp, b, SM_NONE, 1.0, A, lda, nullptr, PK_GEMM_UPD));
-  if (c->blas_update) return blas_update(c, m - bt, n, b, Y + bt, ldy, Zp, b, A + bt, lda);
-  return lean_gemm(c, 0, m - bt, n, b, -1.0, Y + bt, ldy, Zp, b, 1.0, A + bt, lda, nullptr, 1, PK_GEMM_UPD, nullptr);
+  return update_gemm(c, m - bt, n, b, -1.0, Y + bt, ldy, Zp, b, 1.0, A + bt, lda, PK_GEMM_UPD, 0);
 }
 
 int wy_update_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const double* Tt, int t_trans,
@@ -664,15 +663,15 @@ int wy_update_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const
   // Z = Yᵀ·A  (b×n)
   CQR_TRY(gemm_f64_impl(c, 1, 0, b, n, m, 1.0, Y, ldy, smY, A, lda, SM_NONE, 0.0, Z, b,
                         np ? part : nullptr, PK_GEMM_YTA, prec));
-  const bool blas = prec == CQR_PREC_F64 && c->blas_update && m > 2 * b;
+  const bool plain = prec == CQR_PREC_F64 && m > 2 * b;
   // Z' = Tᵀ·Z (Qᵀ·A), or T·Z with t_trans (Q·A); Tt holds Tᵀ (lower), FP64
   CQR_TRY(gemm_f64_impl(c, t_trans ? 1 : 0, 0, b, n, b, 1.0, Tt, b, SM_LOWER, Z, b, SM_NONE, 0.0, Zp, b,
                         nullptr));
-  // A −= Y·Z' (FP64 with the BLAS switch: the top b rows, where Y may be
-  // unit lower, on the masked kernel, the rest as a plain GEMM)
-  if (blas) {
+  // A −= Y·Z' (FP64: the top b rows, where Y may be unit lower, on the
+  // masked kernel, the rest on the rank-b update kernel)
+  if (plain) {
     CQR_TRY(gemm_f64_impl(c, 0, 0, b, n, b, -1.0, Y, ldy, smY, Zp, b, SM_NONE, 1.0, A, lda, nullptr, PK_GEMM_UPD));
-    return blas_update(c, m - b, n, b, Y + b, ldy, Zp, b, A + b, lda);
+    return update_gemm(c, m - b, n, b, -1.0, Y + b, ldy, Zp, b, 1.0, A + b, lda, PK_GEMM_UPD, 0);
   }
   return gemm_f64_impl(c, 0, 0, m, n, b, -1.0, Y, ldy, smY, Zp, b, SM_NONE, 1.0, A, lda, nullptr,
                        PK_GEMM_UPD, prec);
@@ -705,7 +704,7 @@ int wy_update_explicit(cqr_ctx* c, const double* Ye, const double* YT, int64_t m
     CQR_TRY(gemm_f64_impl(c, 1, 0, b, n, m, 1.0, Ye, m, SM_NONE, A, lda, SM_NONE, 0.0, Z, b, np ? part : nullptr,
                           PK_GEMM_YTA));
   }
-  return blas_update(c, m, n, b, YT, m, Z, b, A, lda);
+  return update_gemm(c, m, n, b, -1.0, YT, m, Z, b, 1.0, A, lda, PK_GEMM_UPD, 0);
 }
 
 int wy_update_f64_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const double* Tt,

@@ -1,0 +1,215 @@
+// FP64 MFMA kernel for the rank-b trailing update of the compact-WY QR,
+//   C (M×N) −= P (M×K) · Z (K×N),   K = b ≤ 128 (or an aggregate of two),
+// the second dgemm of upd_A (alg/QR/qr_2d/qr_2d.cxx:275) with P = Y·Tᵀ
+// (or Y·T, or Q1 for Y = Q1·U⁻¹) formed once per panel.  P is column-major
+// (m-contiguous), Z column-major (k-contiguous), C column-major.
+//
+// K is small, so a workgroup's C tile is read once at the start straight
+// into the MFMA accumulators and written once at the end: no epilogue
+// read-modify-write, no second C buffer.  P is negated as it is staged, so
+// every MFMA accumulates −P·Z onto C.  The tile is (64·WM)×(64·WN) with WM·WN
+// waves of 64×64 (16 v_mfma_f64_16x16x4f64 tiles each); K runs in 16-deep
+// slabs double-buffered in LDS, the next slab fetched into registers with
+// 16-byte loads during the current one.  Items are dealt to the XCDs in
+// contiguous ranges, column tiles fastest, so the workgroups of one XCD
+// share a row block of P in its L2.
+#include "cqr_device.h"
+#include "cqr_internal.h"
+
+namespace cqr {
+
+namespace {
+
+typedef double d4u __attribute__((ext_vector_type(4)));
+typedef double d2u __attribute__((ext_vector_type(2)));
+
+struct UpdArgs {
+  int64_t M, N, K;
+  const double* P;
+  int64_t ldp;
+  const double* Z;
+  int64_t ldz;
+  double* C;
+  int64_t ldc;
+  double alpha;        // C := beta·C + alpha·P·Z (alpha = −1 for the update)
+  int beta1;           // 1: C is read (beta = 1); 0: C is overwritten (beta = 0)
+  int vec;             // 16-byte loads allowed (even lds, aligned bases)
+  int ntm, ntn;        // tile grid
+  int per_xcd;         // items per XCD
+};
+
+constexpr int UBK = 16;
+
+}  // namespace
+
+// FULL: the tile lies inside M×N, K is a multiple of 16 and 16-byte loads
+// are allowed, so every access is unconditional (straight-line code lets
+// all of a slab's loads be in flight together).
+template <int WM, int WN, bool FULL>
+__device__ __forceinline__ void upd_tile(const UpdArgs& g, int64_t m0, int64_t n0, double* __restrict__ Psm,
+                                         double* __restrict__ Zsm) {
+  constexpr int NT = 64 * WM * WN;
+  constexpr int BM = 64 * WM, BN = 64 * WN;
+  constexpr int PMP = BM + 16;          // pitch of a P slab row [k][m]
+  constexpr int PZ = UBK + 2;           // pitch of a Z slab row [n][k]
+  constexpr int NPA = UBK * BM / 2 / NT;  // 16-byte P loads per thread per slab
+  constexpr int NZB = UBK * BN / 2 / NT;  // 16-byte Z loads per thread per slab
+  static_assert(NPA >= 1 && NZB >= 1, "tile too small for the thread count");
+  double(*Ps)[UBK * PMP] = reinterpret_cast<double(*)[UBK * PMP]>(Psm);
+  double(*Zs)[BN * PZ] = reinterpret_cast<double(*)[BN * PZ]>(Zsm);
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr bool full = FULL;
+  const int wm0 = (wave % WM) * 64, wn0 = (wave / WM) * 64;
+  const int fr = lane >> 4, fc = lane & 15;
+
+  // accumulators start as C: lane l, register v of acc[i][j] holds
+  // C[m0+wm0+16i+(l&15)][n0+wn0+16j+(l>>4)+4v]
+  d4u acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int64_t gm = m0 + wm0 + 16 * i + fc;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t gn = n0 + wn0 + 16 * j + fr + 4 * v;
+        acc[i][j][v] = (g.beta1 && (full || (gm < g.M && gn < g.N))) ? g.C[gm + gn * g.ldc] : 0.0;
+      }
+  }
+
+  // slab element ownership: P as pairs along m, Z as pairs along k
+  d2u rp[NPA], rz[NZB];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int s = 0; s < NPA; ++s) {
+      const int e = tid + NT * s;              // pair index in the BM×16 slab
+      const int kk = e / (BM / 2), mp = (e % (BM / 2)) * 2;
+      const int64_t k = k0 + kk, m = m0 + mp;
+      if (full) {
+        rp[s] = *reinterpret_cast<const d2u*>(g.P + m + k * g.ldp);
+      } else {
+        rp[s].x = (k < g.K && m < g.M) ? g.P[m + k * g.ldp] : 0.0;
+        rp[s].y = (k < g.K && m + 1 < g.M) ? g.P[m + 1 + k * g.ldp] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NZB; ++s) {
+      const int e = tid + NT * s;              // pair index in the 16×BN slab
+      const int nn = e / (UBK / 2), kp = (e % (UBK / 2)) * 2;
+      const int64_t k = k0 + kp, n = n0 + nn;
+      if (full) {
+        rz[s] = *reinterpret_cast<const d2u*>(g.Z + k + n * g.ldz);
+      } else {
+        rz[s].x = (k < g.K && n < g.N) ? g.Z[k + n * g.ldz] : 0.0;
+        rz[s].y = (k + 1 < g.K && n < g.N) ? g.Z[k + 1 + n * g.ldz] : 0.0;
+      }
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int s = 0; s < NPA; ++s) {
+      const int e = tid + NT * s;
+      const int kk = e / (BM / 2), mp = (e % (BM / 2)) * 2;
+      d2u x = rp[s];
+      x.x *= g.alpha;
+      x.y *= g.alpha;
+      *reinterpret_cast<d2u*>(&Ps[buf][kk * PMP + mp]) = x;
+    }
+#pragma unroll
+    for (int s = 0; s < NZB; ++s) {
+      const int e = tid + NT * s;
+      const int nn = e / (UBK / 2), kp = (e % (UBK / 2)) * 2;
+      *reinterpret_cast<d2u*>(&Zs[buf][nn * PZ + kp]) = rz[s];
+    }
+  };
+  auto compute = [&](int buf) {
+#pragma unroll
+    for (int kk = 0; kk < UBK; kk += 4) {
+      double af[4], bf[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = Ps[buf][(kk + fr) * PMP + wm0 + 16 * i + fc];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bf[j] = Zs[buf][(wn0 + 16 * j + fc) * PZ + kk + fr];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+    }
+  };
+
+  const int nslab = (int)((g.K + UBK - 1) / UBK);
+  load(0);
+  store(0);
+  __syncthreads();
+  // the first slab on its own: its MFMAs wait once for the C loads, so the
+  // loop below only ever waits for its own slab loads
+  if (nslab > 1) load(UBK);
+  compute(0);
+  if (nslab > 1) store(1);
+  __syncthreads();
+  for (int sl = 1; sl < nslab; ++sl) {
+    const int buf = sl & 1;
+    if (sl + 1 < nslab) load((int64_t)(sl + 1) * UBK);
+    compute(buf);
+    if (sl + 1 < nslab) store(buf ^ 1);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int64_t gm = m0 + wm0 + 16 * i + fc;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t gn = n0 + wn0 + 16 * j + fr + 4 * v;
+        if (full || (gm < g.M && gn < g.N)) g.C[gm + gn * g.ldc] = acc[i][j][v];
+      }
+  }
+}
+
+template <int WM, int WN>
+__global__ void __launch_bounds__(64 * WM * WN, (WM * WN <= 4) ? 2 : 1) upd_gemm_f64(UpdArgs g) {
+  constexpr int BM = 64 * WM, BN = 64 * WN;
+  __shared__ __align__(16) double Ps[2 * UBK * (BM + 16)];
+  __shared__ __align__(16) double Zs[2 * BN * (UBK + 2)];
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int tiles = g.ntm * g.ntn;
+  const int u = xcd * g.per_xcd + slot;
+  if (slot >= g.per_xcd || u >= tiles) return;
+  const int tm = u / g.ntn, tn = u - tm * g.ntn;
+  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+  if (g.vec && (m0 + BM <= g.M) && (n0 + BN <= g.N) && (g.K % UBK == 0)) upd_tile<WM, WN, true>(g, m0, n0, Ps, Zs);
+  else upd_tile<WM, WN, false>(g, m0, n0, Ps, Zs);
+}
+
+// C (M×N) := beta·C + alpha·P·Z with K ≤ 256 and beta ∈ {0, 1} (alpha = −1,
+// beta = 1: the trailing update).  shape: 0 = 128×128 tiles (4 waves, two
+// workgroups per CU), 1 = 256×128 (8 waves), 2 = 128×256 (8 waves).
+int update_gemm(cqr_ctx* c, int64_t M, int64_t N, int64_t K, double alpha, const double* P, int64_t ldp,
+                const double* Z, int64_t ldz, double beta, double* C, int64_t ldc, int kind, int shape) {
+  if (M <= 0 || N <= 0) return CQR_OK;
+  if (K > 256 || K < 1) return set_err(c, CQR_ERR_UNSUPPORTED, "update_gemm: K = %lld outside [1, 256]", (long long)K);
+  if (beta != 0.0 && beta != 1.0) return set_err(c, CQR_ERR_UNSUPPORTED, "update_gemm: beta must be 0 or 1");
+  UpdArgs g{};
+  g.M = M; g.N = N; g.K = K; g.P = P; g.ldp = ldp; g.Z = Z; g.ldz = ldz; g.C = C; g.ldc = ldc; g.alpha = alpha;
+  g.beta1 = beta != 0.0;
+  g.vec = (ldp % 2 == 0) && (ldz % 2 == 0) && ((uintptr_t)P % 16 == 0) && ((uintptr_t)Z % 16 == 0);
+  const int bm = shape == 1 ? 256 : 128, bn = shape == 2 ? 256 : 128;
+  g.ntm = (int)((M + bm - 1) / bm);
+  g.ntn = (int)((N + bn - 1) / bn);
+  const int64_t tiles = (int64_t)g.ntm * g.ntn;
+  if (tiles >= (int64_t)1 << 30) return set_err(c, CQR_ERR_UNSUPPORTED, "update_gemm: too many tiles");
+  g.per_xcd = (int)((tiles + 7) / 8);
+  const dim3 grid((unsigned)(8 * g.per_xcd));
+  hipEvent_t ev = prof_begin(c);
+  if (shape == 1) hipLaunchKernelGGL((upd_gemm_f64<4, 2>), grid, dim3(512), 0, c->stream, g);
+  else if (shape == 2) hipLaunchKernelGGL((upd_gemm_f64<2, 4>), grid, dim3(512), 0, c->stream, g);
+  else hipLaunchKernelGGL((upd_gemm_f64<2, 2>), grid, dim3(256), 0, c->stream, g);
+  prof_end(c, ev, kind, 2.0 * M * N * K, 8.0 * ((double)M * K + (double)K * N + (g.beta1 ? 2.0 : 1.0) * (double)M * N));
+  return check_launch(c, "upd_gemm_f64");
+}
+
+}  // namespace cqr

@@ -188,30 +188,51 @@ int cqr_trsm_r(cqr_ctx* c, const double* YR, int64_t ldy, int64_t n, double* B, 
   return trsm_r_ws(c, YR, ldy, n, Rinv, B, ldb, nrhs, trans ? 1 : 0, reinterpret_cast<double*>(base + oT));
 }
 
-int cqr_lstsq(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, const double* B, int64_t ldb,
-              int64_t nrhs, double* X, int64_t ldx, int64_t b, int prec, int max_iter, double tol, int* iters,
-              double* last_update) {
-  CQR_CHECK_CTX(c);
-  if (iters) *iters = 0;
-  if (last_update) *last_update = 0.0;
-  if (!A || !B || !X || m < 1 || n < 1 || m < n || lda < m || ldb < m || ldx < n || nrhs < 1 || b < 1 ||
-      max_iter < 0)
-    return set_err(c, CQR_ERR_INVALID, "lstsq: bad arguments (needs m >= n >= 1, nrhs >= 1)");
-  if (nrhs > 16) return set_err(c, CQR_ERR_UNSUPPORTED, "lstsq: at most 16 right-hand sides");
-  if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "lstsq: b > %d", CQR_MAX_B);
-  if (prec != CQR_PREC_F64 && prec != CQR_PREC_F32 && prec != CQR_PREC_BF16) return set_err(c, CQR_ERR_UNSUPPORTED, "lstsq: precision %d", prec);
-  CQR_TRY(finite_check(c, A, lda, m, n));
-  const int64_t nr = nrhs;
-  Qr2dWS o;
-  LsqWS w;
-  CQR_TRY(ensure_ws(c, lstsq_layout(m, n, nr, b, c->leaf_rows, &o, &w)));
-  char* base = static_cast<char*>(c->ws);
+}  // extern "C"
+
+namespace cqr {
+
+namespace {
+
+// Unit roundoff of the trailing update's arithmetic.
+double prec_unit(int prec) {
+  return prec == CQR_PREC_F32 ? 0x1p-24 : prec == CQR_PREC_BF16 ? 0x1p-8 : 0x1p-53;
+}
+
+// Beyond this κ̂·u the mixed factorization is not worth refining: the
+// stationary refinement contracts by about κ·u per step (and not at all
+// from κ·u ≈ 1), so the FP64 factorization is taken at once.
+constexpr double kEscalateKappaU = 1e-2;
+enum { ATTEMPT_ESCALATE = -1 };
+
+struct LsqRun {
+  int iters = 0;
+  double rel = 0.0;
+};
+
+// One factorization in `prec` and its refinement.  Returns CQR_OK when the
+// last relative update is ≤ tol (or max_iter is 0), CQR_ERR_NOCONVERGE when
+// the refinement ended above tol, CQR_ERR_NONFINITE when the iterate or R
+// holds a NaN or Inf, and ATTEMPT_ESCALATE (only with `escalate`, for a
+// mixed `prec`) when R's diagonal range already rules the mixed factors out.
+int lstsq_attempt(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, const double* B, int64_t ldb,
+                  int64_t nr, double* X, int64_t ldx, int64_t b, int prec, int max_iter, double tol, bool escalate,
+                  char* base, const Qr2dWS& o, const LsqWS& w, LsqRun& out) {
   auto at = [&](size_t off) { return reinterpret_cast<double*>(base + off); };
   double *Af = at(w.Af), *Tt = at(w.Tt), *Rinv = at(w.Ri), *r = at(w.R), *f = at(w.F), *g = at(w.G),
          *h = at(w.H), *tmp = at(w.Tmp), *wsq = at(w.Q), *nrm = at(w.N);
   // factor a copy of A (the caller's A stays the FP64 reference operator)
   CQR_TRY(launch_copy2d(c, A, lda, Af, m, m, n));
   CQR_TRY(qr2d_run(c, Af, m, m, n, b, nullptr, 0, base, o, Tt, prec));
+  std::vector<double> hn(std::max<int64_t>(2 * nr, 2));
+  {  // κ̂ = max|r_ii| / min|r_ii|, a lower bound on κ(A)
+    CQR_TRY(launch_diag_range(c, Af, m, n, nrm));
+    CQR_HIP(c, hipMemcpyAsync(hn.data(), nrm, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CQR_HIP(c, hipStreamSynchronize(c->stream));
+    if (!std::isfinite(hn[0]) || !std::isfinite(hn[1])) return set_err(c, CQR_ERR_NONFINITE, "lstsq: R is not finite");
+    if (escalate && prec != CQR_PREC_F64 && !(hn[1] > 0.0 && hn[0] / hn[1] * prec_unit(prec) <= kEscalateKappaU))
+      return ATTEMPT_ESCALATE;
+  }
   CQR_TRY(r_block_inverses(c, Af, m, n, Rinv));
   // x0 = R⁻¹·(Qᵀb)(0:n)
   CQR_TRY(launch_copy2d(c, B, ldb, f, m, m, nr));
@@ -222,17 +243,22 @@ int cqr_lstsq(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, co
   // r = b − A·x
   CQR_TRY(launch_copy2d(c, B, ldb, r, m, m, nr));
   CQR_TRY(thin_gemm(c, 0, m, n, -1.0, A, lda, X, ldx, (int)nr, 1.0, r, m, nullptr));
-  std::vector<double> hn(2 * nr);
-  // max over the columns of ‖d_j‖ / ‖x_j‖ (one host synchronisation)
-  auto rel_update = [&](const double* d, int64_t ldd, double* out) -> int {
+  // max over the columns of ‖d_j‖ / ‖x_j‖ (one host synchronisation); NaN
+  // if any column's update or solution norm is NaN
+  auto rel_update = [&](const double* d, int64_t ldd, double* res) -> int {
     CQR_TRY(launch_colnorm(c, d, ldd, n, (int)nr, nrm));
     CQR_TRY(launch_colnorm(c, X, ldx, n, (int)nr, nrm + nr));
     CQR_HIP(c, hipMemcpyAsync(hn.data(), nrm, sizeof(double) * 2 * nr, hipMemcpyDeviceToHost, c->stream));
     CQR_HIP(c, hipStreamSynchronize(c->stream));
     double rel = 0.0;
-    for (int64_t j = 0; j < nr; ++j) rel = std::max(rel, hn[j] / std::max(hn[nr + j], 1e-300));
-    if (std::isnan(hn[0])) rel = hn[0];
-    *out = rel;
+    for (int64_t j = 0; j < nr; ++j) {
+      if (std::isnan(hn[j]) || std::isnan(hn[nr + j])) {
+        rel = NAN;
+        break;
+      }
+      rel = std::max(rel, hn[j] / std::max(hn[nr + j], 1e-300));
+    }
+    *res = rel;
     return CQR_OK;
   };
   double prev_rel = 0.0;
@@ -254,11 +280,15 @@ int cqr_lstsq(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, co
         CQR_TRY(launch_axpby(c, n, nr, 1.0, g, n, 1.0, X, ldx));
         CQR_TRY(launch_copy2d(c, B, ldb, r, m, m, nr));
         CQR_TRY(thin_gemm(c, 0, m, n, -1.0, A, lda, X, ldx, (int)nr, 1.0, r, m, nullptr));
-        if (iters) *iters = it;
-        if (last_update) *last_update = rel;
-        if (!(rel > tol)) break;
+        out.iters += 1;
+        out.rel = rel;
+        if (rel <= tol) return CQR_OK;
         prev_rel = rel;
         continue;
+      }
+      if (std::isnan(rel)) {
+        out.rel = rel;
+        return set_err(c, CQR_ERR_NONFINITE, "lstsq: the refinement produced NaN");
       }
       sne = false;   // not contracting fast enough: the augmented system from here
     }
@@ -281,15 +311,66 @@ int cqr_lstsq(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, co
     // stop when every column's update is below tol relative to the solution
     double rel;
     CQR_TRY(rel_update(h, n, &rel));
-    if (iters) *iters = it;
-    if (last_update) *last_update = rel;
-    if (!(rel > tol)) break;  // also stops on NaN
+    out.iters += 1;
+    out.rel = rel;
+    if (std::isnan(rel)) return set_err(c, CQR_ERR_NONFINITE, "lstsq: the refinement produced NaN");
+    if (rel <= tol) return CQR_OK;
     // the updates stopped shrinking: the attainable accuracy (about κ·ε of
-    // the FP64 residual) is reached, or κ·u of the factorization is ≥ 1
+    // the FP64 residual) is reached above tol, or κ·u of the factorization
+    // is ≥ 1
     if (aug > 1 && rel > 0.9 * prev_rel) break;
     prev_rel = rel;
   }
+  return set_err(c, CQR_ERR_NOCONVERGE, "lstsq: the last relative update %.3g is above tol %.3g after %d steps",
+                 out.rel, tol, out.iters);
+}
+
+}  // namespace
+
+}  // namespace cqr
+
+extern "C" {
+
+int cqr_set_lstsq_escalate(cqr_ctx* c, int on) {
+  CQR_CHECK_CTX(c);
+  c->lsq_escalate = on ? 1 : 0;
   return CQR_OK;
+}
+
+int cqr_lstsq_last_prec(const cqr_ctx* c) { return c ? c->lsq_last_prec : -1; }
+
+int cqr_lstsq(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, const double* B, int64_t ldb,
+              int64_t nrhs, double* X, int64_t ldx, int64_t b, int prec, int max_iter, double tol, int* iters,
+              double* last_update) {
+  CQR_CHECK_CTX(c);
+  if (iters) *iters = 0;
+  if (last_update) *last_update = 0.0;
+  c->lsq_last_prec = -1;
+  if (!A || !B || !X || m < 1 || n < 1 || m < n || lda < m || ldb < m || ldx < n || nrhs < 1 || b < 1 ||
+      max_iter < 0 || !(tol >= 0.0))
+    return set_err(c, CQR_ERR_INVALID, "lstsq: bad arguments (needs m >= n >= 1, nrhs >= 1, tol >= 0)");
+  if (nrhs > 16) return set_err(c, CQR_ERR_UNSUPPORTED, "lstsq: at most 16 right-hand sides");
+  if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "lstsq: b > %d", CQR_MAX_B);
+  if (prec != CQR_PREC_F64 && prec != CQR_PREC_F32 && prec != CQR_PREC_BF16) return set_err(c, CQR_ERR_UNSUPPORTED, "lstsq: precision %d", prec);
+  CQR_TRY(finite_check(c, A, lda, m, n));
+  Qr2dWS o;
+  LsqWS w;
+  CQR_TRY(ensure_ws(c, lstsq_layout(m, n, nrhs, b, c->leaf_rows, &o, &w)));
+  char* base = static_cast<char*>(c->ws);
+  const bool esc = c->lsq_escalate && prec != CQR_PREC_F64;
+  LsqRun run;
+  int st = lstsq_attempt(c, A, lda, m, n, B, ldb, nrhs, X, ldx, b, prec, max_iter, tol, esc, base, o, w, run);
+  int used = prec;
+  // a mixed factorization ruled out by R's diagonal, or whose refinement
+  // ended above tol: refactor in FP64 and refine again
+  if (esc && (st == ATTEMPT_ESCALATE || st == CQR_ERR_NOCONVERGE)) {
+    used = CQR_PREC_F64;
+    st = lstsq_attempt(c, A, lda, m, n, B, ldb, nrhs, X, ldx, b, CQR_PREC_F64, max_iter, tol, false, base, o, w, run);
+  }
+  if (iters) *iters = run.iters;
+  if (last_update) *last_update = run.rel;
+  c->lsq_last_prec = used;
+  return st;
 }
 
 }  // extern "C"

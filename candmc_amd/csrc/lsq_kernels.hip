@@ -276,14 +276,28 @@ __global__ void axpby_kernel(int64_t m, int64_t n, double alpha, const double* _
 __global__ void __launch_bounds__(256)
 colnorm_kernel(const double* __restrict__ X, int64_t ldx, int64_t m, double* __restrict__ out) {
   __shared__ double part[4];
+  __shared__ int pnan[4];
   const int j = blockIdx.x;
   const double* x = X + (int64_t)j * ldx;
   double mx = 0.0;
-  for (int64_t r = threadIdx.x; r < m; r += 256) mx = fmax(mx, fabs(x[r]));
+  int nan = 0;   // fmax drops NaN, so a NaN entry is tracked on its own
+  for (int64_t r = threadIdx.x; r < m; r += 256) {
+    const double a = fabs(x[r]);
+    nan |= a != a;
+    mx = fmax(mx, a);
+  }
   mx = wave_max(mx);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = mx;
+  for (int o = 32; o >= 1; o >>= 1) nan |= __shfl_xor(nan, o);
+  if ((threadIdx.x & 63) == 0) {
+    part[threadIdx.x >> 6] = mx;
+    pnan[threadIdx.x >> 6] = nan;
+  }
   __syncthreads();
   mx = fmax(fmax(part[0], part[1]), fmax(part[2], part[3]));
+  if (pnan[0] | pnan[1] | pnan[2] | pnan[3]) {
+    if (threadIdx.x == 0) out[j] = NAN;
+    return;
+  }
   __syncthreads();
   double ss = 0.0;
   if (mx > 0.0)
@@ -340,6 +354,41 @@ int launch_axpby(cqr_ctx* c, int64_t m, int64_t n, double alpha, const double* X
   const int gx = (int)std::min<int64_t>(2048, (m * n + 255) / 256);
   hipLaunchKernelGGL(axpby_kernel, dim3(gx), dim3(256), 0, c->stream, m, n, alpha, X, ldx, beta, Y, ldy);
   return check_launch(c, "axpby");
+}
+
+// out[0] = max_i |R(i,i)|, out[1] = min_i |R(i,i)| over the n diagonal
+// entries (a NaN in any propagates to both).  One workgroup.
+__global__ void __launch_bounds__(256)
+diag_range_kernel(const double* __restrict__ R, int64_t ldr, int64_t n, double* __restrict__ out) {
+  __shared__ double pmax[4], pmin[4];
+  __shared__ int pnan[4];
+  double mx = 0.0, mn = INFINITY;
+  int nan = 0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) {
+    const double v = fabs(R[i + i * ldr]);
+    nan |= v != v;
+    mx = fmax(mx, v);
+    mn = fmin(mn, v);
+  }
+  mx = wave_max(mx);
+  mn = -wave_max(-mn);
+  for (int o = 32; o >= 1; o >>= 1) nan |= __shfl_xor(nan, o);
+  if ((threadIdx.x & 63) == 0) {
+    pmax[threadIdx.x >> 6] = mx;
+    pmin[threadIdx.x >> 6] = mn;
+    pnan[threadIdx.x >> 6] = nan;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const bool any_nan = pnan[0] | pnan[1] | pnan[2] | pnan[3];
+    out[0] = any_nan ? NAN : fmax(fmax(pmax[0], pmax[1]), fmax(pmax[2], pmax[3]));
+    out[1] = any_nan ? NAN : fmin(fmin(pmin[0], pmin[1]), fmin(pmin[2], pmin[3]));
+  }
+}
+
+int launch_diag_range(cqr_ctx* c, const double* R, int64_t ldr, int64_t n, double* out) {
+  hipLaunchKernelGGL(diag_range_kernel, dim3(1), dim3(256), 0, c->stream, R, ldr, n, out);
+  return check_launch(c, "diag_range");
 }
 
 int launch_colnorm(cqr_ctx* c, const double* X, int64_t ldx, int64_t m, int n, double* out) {

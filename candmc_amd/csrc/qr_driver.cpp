@@ -87,13 +87,7 @@ int hh_recon_panel(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, dou
   CQR_TRY(hr_core(c, Y, m, Rroot, A, lda, Wd, at(w.Uinv), at(w.Winv), at(w.LinvT), sg, bi));
   // Y = Q1·U⁻¹ below the top b rows (hh_recon.cxx:294-301)
   // (U⁻¹ is stored with explicit zeros below its diagonal: a plain GEMM)
-  if (m > b) {
-    if (c->blas_update)
-      CQR_TRY(blas_gemm(c, 0, m - b, b, b, 1.0, Y + b, m, at(w.Uinv), b, 0.0, A + b, lda, PK_GEMM_SMALL));
-    else
-      CQR_TRY(gemm_f64_ws(c, 0, 0, m - b, b, b, 1.0, Y + b, m, SM_NONE, at(w.Uinv), b, SM_UPPER, 0.0, A + b, lda,
-                          nullptr));
-  }
+  if (m > b) CQR_TRY(update_gemm(c, m - b, b, b, 1.0, Y + b, m, at(w.Uinv), b, 0.0, A + b, lda, PK_GEMM_SMALL, 0));
   // T = −W·L⁻ᵀ, Tᵀ = −L⁻¹·Wᵀ, T⁻ᵀ = −W⁻ᵀ·L (W = −T·Y1ᵀ)
   if (Tt)
     CQR_TRY(gemm_f64_ws(c, 1, 1, b, b, b, -1.0, at(w.LinvT), b, SM_UNIT_UPPER, Wd, b, SM_UPPER, 0.0, Tt, b,
@@ -164,28 +158,50 @@ int cqr_hh_recon_qr(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, do
   return cqr_hh_recon_qr_ex(c, A, lda, m, b, W, nullptr, nullptr, nleaves);
 }
 
+}  // extern "C"
+
+namespace cqr {
+
+size_t local_tsqr_ws_bytes(int64_t m, int64_t b, int64_t leaf_rows) {
+  const int64_t bb = b * b;
+  if (m <= leaf_capacity_rows(b)) return 8 * 2 * bb + 256;
+  Sizer s;
+  s.add(panel_ws_bytes(m, b, 0, leaf_rows));
+  s.add(8 * bb);
+  s.add(8 * bb);
+  return s.bytes;
+}
+
+// dgeqrf of one m×b block (bitree_tsqr.cxx:48-71) on the scratch at base
+// (local_tsqr_ws_bytes): one leaf in registers, or, for taller blocks,
+// TSQR + HR, which gives the same [V\R]; tau = diag(T).
+int local_tsqr_ws(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, double* tau, double* T, char* base) {
+  const int64_t bb = b * b;
+  if (m <= leaf_capacity_rows(b)) {
+    double* R = reinterpret_cast<double*>(base);
+    double* Tw = T ? T : R + bb;
+    return leaf_qr(c, A, lda, m, 1, (int)b, R, Tw, tau);
+  }
+  Sizer s;
+  const size_t pw = s.add(panel_ws_bytes(m, b, 0, c->leaf_rows));
+  const size_t tw = s.add(8 * bb), ww = s.add(8 * bb);
+  double* Tw = T ? T : reinterpret_cast<double*>(base + tw);
+  CQR_TRY(hh_recon_panel(c, A, lda, m, b, reinterpret_cast<double*>(base + ww), Tw, nullptr, nullptr, 0,
+                         base + pw, nullptr));
+  return launch_diag(c, Tw, b, tau, (int)b);
+}
+
+}  // namespace cqr
+
+extern "C" {
+
 int cqr_local_tsqr(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, double* tau, double* T) {
   CQR_CHECK_CTX(c);
   if (!A || !tau || b < 1 || m < b || lda < m) return set_err(c, CQR_ERR_INVALID, "local_tsqr: bad arguments");
   if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "local_tsqr: b > %d", CQR_MAX_B);
   CQR_TRY(finite_check(c, A, lda, m, b));
-  const int64_t bb = b * b;
-  if (m <= leaf_capacity_rows(b)) {
-    CQR_TRY(ensure_ws(c, 8 * 2 * bb + 256));
-    double* R = static_cast<double*>(c->ws);
-    double* Tw = T ? T : R + bb;
-    return leaf_qr(c, A, lda, m, 1, (int)b, R, Tw, tau);
-  }
-  // Larger blocks: TSQR + HR gives the same [V\R]; tau = diag(T).
-  Sizer s;
-  const size_t pw = s.add(panel_ws_bytes(m, b, 0, c->leaf_rows));
-  const size_t tw = s.add(8 * bb), ww = s.add(8 * bb);
-  CQR_TRY(ensure_ws(c, s.bytes));
-  char* base = (char*)c->ws;
-  double* Tw = T ? T : reinterpret_cast<double*>(base + tw);
-  CQR_TRY(hh_recon_panel(c, A, lda, m, b, reinterpret_cast<double*>(base + ww), Tw, nullptr, nullptr, 0,
-                         base + pw, nullptr));
-  return launch_diag(c, Tw, b, tau, (int)b);
+  CQR_TRY(ensure_ws(c, local_tsqr_ws_bytes(m, b, c->leaf_rows)));
+  return local_tsqr_ws(c, A, lda, m, b, tau, T, (char*)c->ws);
 }
 
 int cqr_tree_tsqr(cqr_ctx* c, double* A2b, int64_t b, double* T) {
@@ -215,10 +231,11 @@ int cqr_tree_tsqr(cqr_ctx* c, double* A2b, int64_t b, double* T) {
 int cqr_bitree_tsqr(cqr_ctx* c, double* A, int64_t lda, double* R, double* tau, int64_t m, int64_t b,
                     int64_t nleaves, int output_Y, double* tree_data) {
   CQR_CHECK_CTX(c);
-  (void)output_Y;
-  if (!A || !R || !tau || !tree_data || b < 1 || m < b || lda < m || nleaves < 0)
+  if (!A || !R || !tau || b < 1 || m < b || lda < m || nleaves < 0)
     return set_err(c, CQR_ERR_INVALID, "bitree_tsqr: bad arguments");
   if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "bitree_tsqr: b > %d", CQR_MAX_B);
+  // no tree_data: the reference's per-rank semantics over nleaves ranks
+  if (!tree_data) return bitree_ranks(c, A, lda, R, tau, m, b, std::max<int64_t>(nleaves, 1), output_Y);
   TsqrPlan p;
   int s = make_plan(m, b, nleaves, 0, p);
   if (s != CQR_OK) return set_err(c, s, "bitree_tsqr: cannot plan");
@@ -239,8 +256,9 @@ int cqr_construct_Q1(cqr_ctx* c, const double* Y, int64_t lda_Y, const double* t
                      int64_t lda_Q, int64_t m, int64_t b, int64_t k, int64_t nleaves,
                      const double* tree_data) {
   CQR_CHECK_CTX(c);
-  (void)tau;
-  if (!Y || !Q1 || !tree_data || b < 1 || m < b || lda_Y < m || lda_Q < m)
+  // no tree_data: the one-rank dorgqr branch (construct_tsqr_Q.cxx:87-102)
+  if (!tree_data) return cqr_construct_Q1_id(c, Y, lda_Y, tau, Q1, lda_Q, m, b, k, nleaves, nullptr, nullptr, b);
+  if (!Y || !Q1 || b < 1 || m < b || lda_Y < m || lda_Q < m)
     return set_err(c, CQR_ERR_INVALID, "construct_Q1: bad arguments");
   if (k != b) return set_err(c, CQR_ERR_UNSUPPORTED, "construct_Q1: only k == b is supported");
   TsqrPlan p;
@@ -301,9 +319,23 @@ int cqr_compute_invT_from_Y(cqr_ctx* c, const double* Y, int64_t lda_Y, double* 
   if (mb > 0) {
     CQR_TRY(gemm_f64_ws(c, 1, 0, b, b, mb, 1.0, Y, lda_Y, SM_NONE, Y, lda_Y, SM_NONE, 0.0, G, b, part));
   } else {
-    CQR_HIP(c, hipMemsetAsync(G, 0, 8 * bb, c->stream));
+    CQR_TRY(launch_fill2d(c, G, b, b, b, 0.0));
   }
   return launch_tril_halve(c, G, (int)b, T, lda_T);
+}
+
+int cqr_compute_invT_from_Y2(cqr_ctx* c, const double* Y1, int64_t lda_Y1, const double* Y2, int64_t lda_Y2,
+                             double* T, int64_t lda_T, int64_t mb, int64_t b1, int64_t b2) {
+  CQR_CHECK_CTX(c);
+  if (!Y1 || !Y2 || !T || b1 < 1 || b2 < 1 || mb < 0 || lda_Y1 < mb || lda_Y2 < mb || lda_T < b1)
+    return set_err(c, CQR_ERR_INVALID, "compute_invT_from_Y (two Y): bad arguments");
+  if (mb == 0) {  // no rows: the product is zero
+    return launch_fill2d(c, T, lda_T, b1, b2, 0.0);
+  }
+  const size_t sd = gemm_f64_scratch_doubles(b1, b2, mb);
+  CQR_TRY(ensure_ws(c, 8 * (sd + 64)));
+  return gemm_f64_ws(c, 1, 0, b1, b2, mb, 1.0, Y1, lda_Y1, SM_NONE, Y2, lda_Y2, SM_NONE, 0.0, T, lda_T,
+                     sd ? static_cast<double*>(c->ws) : nullptr);
 }
 
 static int upd_core(cqr_ctx* c, const double* Y, int64_t lda_Y, int implicitY, double* A, int64_t lda_A,
@@ -406,10 +438,9 @@ int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b
   double* ws_far = reinterpret_cast<double*>(base + o.upd_far);
   const bool la = c->lookahead && k > 2 * b && m > 2 * b;
   // Each panel's Y is made explicit (unit lower) and Y·Tt formed once (in
-  // FP64), so both updates are two plain products (the vendor BLAS for the
-  // FP64 update, the mixed-precision kernels otherwise); the own FP64
-  // kernels keep the masked form
-  const bool expl = prec != CQR_PREC_F64 || c->blas_update;
+  // FP64), so both updates are two plain products: Z = Yᵀ·A (split-K) and
+  // A −= (Y·Tt)·Z (the rank-b update kernel, or the mixed-precision kernels)
+  const bool expl = true;
   double* Ye2[2] = {reinterpret_cast<double*>(base + o.Ye),
                     reinterpret_cast<double*>(base + o.Ye) + 2 * m * std::min(k, b)};
   if (la) {  // the side stream starts after everything already queued
@@ -428,7 +459,7 @@ int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b
     double* YT = Ye + mj * bj;
     if (expl) {
       if ((rc = launch_copy_unit_lower(c, P, lda, Ye, mj, mj, bj))) break;
-      if ((rc = blas_gemm(c, 0, mj, bj, bj, 1.0, Ye, mj, Tt, bj, 0.0, YT, mj, PK_GEMM_SMALL))) break;
+      if ((rc = update_gemm(c, mj, bj, bj, 1.0, Ye, mj, Tt, bj, 0.0, YT, mj, PK_GEMM_SMALL, 0))) break;
     }
     auto update = [&](int64_t col0, int64_t ncols, double* ws) -> int {
       if (expl) return wy_update_explicit(c, Ye, YT, mj, ncols, (int)bj, P + col0 * lda, lda, ws, prec);

@@ -22,14 +22,9 @@
 #include <unordered_set>
 #include <vector>
 
-#include "cqr_internal.h"
+#include <hip/hip_runtime_api.h>
 
-extern "C" {
-int cqr_lstsq(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, const double* B, int64_t ldb,
-              int64_t nrhs, double* X, int64_t ldx, int64_t b, int prec, int max_iter, double tol, int* iters,
-              double* last_update);
-size_t cqr_lstsq_workspace_bytes(int64_t m, int64_t n, int64_t nrhs, int64_t b);
-}
+#include "../../include/candmc_qr.h"
 
 namespace {
 
@@ -49,7 +44,9 @@ struct Job {
 
 struct cqr_server {
   int device = 0;
-  size_t budget = 0, in_use = 0;
+  size_t budget = 0, in_use = 0;   // in_use: Σ the slots' resident arenas
+  std::vector<size_t> resident;    // per slot: its context's arena bytes
+  int waiters = 0;                 // threads inside cqr_server_wait
   int64_t next_id = 0;
   bool stop = false;
   std::mutex mu;
@@ -65,6 +62,13 @@ struct cqr_server {
 
 namespace {
 
+// Bytes a context's arena holds after it has grown for a request of
+// `bytes` (the arena keeps an eighth of headroom when it grows).
+size_t arena_after(const cqr_ctx* c, size_t bytes) {
+  const size_t have = cqr_workspace_bytes(c);
+  return have >= bytes ? have : bytes + bytes / 8;
+}
+
 void serve(cqr_server* s, int slot) {
   cqr_ctx* c = s->ctxs[slot];
   (void)hipSetDevice(s->device);
@@ -72,29 +76,61 @@ void serve(cqr_server* s, int slot) {
     Job job;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      // FIFO: the head waits until its footprint fits (no overtaking)
-      s->cv_work.wait(lk, [&] { return s->stop || (!s->queue.empty() && s->in_use + s->queue.front().bytes <= s->budget); });
-      if (s->stop) return;
-      job = s->queue.front();
-      s->queue.pop_front();
-      s->in_use += job.bytes;
+      for (;;) {
+        if (s->stop) return;
+        if (!s->queue.empty()) {
+          // FIFO: the head is admitted when this slot's arena, grown for it,
+          // keeps the sum of all resident arenas within the budget
+          const size_t after = arena_after(c, s->queue.front().bytes);
+          if (s->in_use - s->resident[slot] + after <= s->budget) {
+            job = s->queue.front();
+            s->queue.pop_front();
+            s->in_use += after - s->resident[slot];
+            s->resident[slot] = after;
+            break;
+          }
+          if (s->resident[slot] > 0) {
+            // idle with an arena the head cannot share: give it back
+            const size_t mine = s->resident[slot];
+            lk.unlock();
+            (void)cqr_trim(c);
+            lk.lock();
+            s->in_use -= mine;
+            s->resident[slot] = 0;
+            s->cv_work.notify_all();
+            continue;
+          }
+        }
+        s->cv_work.wait(lk);
+      }
     }
     const cqr_request& r = job.req;
     job.res.t_start_ns = now_ns();
     const bool mixed = r.tier == CQR_TIER_MIXED;
     int iters = 0;
     double upd = 0.0;
-    int st = cqr_lstsq(c, r.A, r.lda, r.m, r.n, r.B, r.ldb, r.nrhs, r.X, r.ldx, 128,
-                       mixed ? CQR_PREC_F32 : CQR_PREC_F64, mixed ? 30 : 4, mixed ? 1e-10 : 1e-12, &iters, &upd);
-    if (st == CQR_OK) st = cqr_sync(c);
+    // the arena grows here, to what admission charged, not inside the solve
+    int st = cqr_reserve(c, job.bytes);
+    if (st == CQR_OK)
+      st = cqr_lstsq(c, r.A, r.lda, r.m, r.n, r.B, r.ldb, r.nrhs, r.X, r.ldx, 128,
+                     mixed ? CQR_PREC_F32 : CQR_PREC_F64, mixed ? 30 : 4, mixed ? 1e-10 : 1e-12, &iters, &upd);
+    if (st == CQR_OK || st == CQR_ERR_NOCONVERGE) {
+      const int ss = cqr_sync(c);
+      if (ss != CQR_OK) st = ss;
+    }
     job.res.t_end_ns = now_ns();
     job.res.status = st;
     job.res.iters = iters;
     job.res.last_update = upd;
+    job.res.prec = cqr_lstsq_last_prec(c);
     job.res.slot = slot;
+    const size_t held = cqr_workspace_bytes(c);
+    job.res.bytes = (int64_t)held;
     {
       std::lock_guard<std::mutex> lk(s->mu);
-      s->in_use -= job.bytes;
+      // the arena stays resident with the slot (charged until released)
+      s->in_use += held - s->resident[slot];
+      s->resident[slot] = held;
       s->done[job.id] = job.res;
       s->done_order.push_back(job.id);
     }
@@ -132,6 +168,7 @@ int cqr_server_create(cqr_server** out, int device, int nslots, size_t hbm_budge
     s->ctxs.push_back(c);
     s->streams.push_back(stream);
   }
+  s->resident.assign(nslots, 0);
   for (int i = 0; i < nslots; ++i) s->workers.emplace_back(serve, s, i);
   *out = s;
   return CQR_OK;
@@ -145,6 +182,21 @@ int cqr_server_destroy(cqr_server* s) {
   }
   s->cv_work.notify_all();
   for (std::thread& t : s->workers) t.join();
+  {
+    // requests that never started finish as cancelled, so a thread waiting
+    // for one returns; then wait for every waiter to leave
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (Job& j : s->queue) {
+      j.res.status = CQR_ERR_CANCELLED;
+      j.res.prec = -1;
+      j.res.slot = -1;
+      s->done[j.id] = j.res;
+      s->done_order.push_back(j.id);
+    }
+    s->queue.clear();
+    s->cv_done.notify_all();
+    s->cv_done.wait(lk, [&] { return s->waiters == 0; });
+  }
   for (size_t k = 0; k < s->ctxs.size(); ++k) {
     cqr_destroy(s->ctxs[k]);
     (void)hipStreamDestroy(s->streams[k]);
@@ -159,7 +211,8 @@ int64_t cqr_server_submit(cqr_server* s, const cqr_request* r) {
       r->ldb < r->m || r->ldx < r->n || (r->tier != CQR_TIER_F64 && r->tier != CQR_TIER_MIXED))
     return -CQR_ERR_INVALID;
   const size_t bytes = cqr_lstsq_workspace_bytes(r->m, r->n, r->nrhs, 128);
-  if (bytes == 0 || bytes > s->budget) return -CQR_ERR_NOMEM;  // can never be admitted
+  // can never be admitted: its arena (with headroom) exceeds the budget
+  if (bytes == 0 || bytes + bytes / 8 > s->budget) return -CQR_ERR_NOMEM;
   Job j;
   j.req = *r;
   j.bytes = bytes;
@@ -183,7 +236,10 @@ int cqr_server_wait(cqr_server* s, int64_t id, cqr_result* out) {
   // an id never submitted, or already collected by a wait or a drain, has
   // nothing left to wait for
   if (!s->live.count(id)) return CQR_ERR_INVALID;
+  ++s->waiters;
   s->cv_done.wait(lk, [&] { return s->done.count(id) > 0 || !s->live.count(id); });
+  --s->waiters;
+  if (s->waiters == 0) s->cv_done.notify_all();   // a destroy may wait for this
   if (!s->done.count(id)) return CQR_ERR_INVALID;  // a concurrent drain took it
   if (out) *out = s->done[id];
   s->done.erase(id);
@@ -222,7 +278,7 @@ int cqr_server_pending(cqr_server* s, int64_t* queued, size_t* bytes_in_use) {
   if (!s) return CQR_ERR_INVALID;
   std::lock_guard<std::mutex> lk(s->mu);
   if (queued) *queued = (int64_t)s->queue.size();
-  if (bytes_in_use) *bytes_in_use = s->in_use;
+  if (bytes_in_use) *bytes_in_use = s->in_use;   // resident arenas
   return CQR_OK;
 }
 

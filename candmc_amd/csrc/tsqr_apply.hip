@@ -36,6 +36,44 @@ __global__ void copy_upper_kernel(const double* __restrict__ R, int64_t ldr, dou
   if (r <= c) A[r + (int64_t)c * lda] = R[r + (int64_t)c * ldr];
 }
 
+// R (b×b, ld ldr) := the upper triangle of A's top b×b block, zeros below
+// (pack_upper + unpack_upper into a zero buffer, bitree_tsqr.cxx:143-145).
+__global__ void upper_zero_kernel(const double* __restrict__ A, int64_t lda, double* __restrict__ R, int64_t ldr,
+                                  int b) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= b * b) return;
+  const int r = e % b, c = e / b;
+  R[r + (int64_t)c * ldr] = r <= c ? A[r + (int64_t)c * lda] : 0.0;
+}
+
+// Tt := Tᵀ of the compact-WY factor of b reflectors from their Gram matrix
+// G = VᵀV (b×b, ld b) and tau (LAPACK dlarft('F','C')): T(i,i) = τ_i and
+// T(0:i, i) = −τ_i·T(0:i, 0:i)·G(0:i, i), column by column.  One workgroup;
+// thread r owns row r of T, kept in LDS (b·b doubles).  A zero τ leaves its
+// row and column of T zero (the reflector is the identity).
+__global__ void __launch_bounds__(512) larft_kernel(const double* __restrict__ G, const double* __restrict__ tau,
+                                                    double* __restrict__ Tt, int b) {
+  extern __shared__ double Ts[];
+  const int r = threadIdx.x;
+  for (int e = r; e < b * b; e += blockDim.x) Ts[e] = 0.0;
+  __syncthreads();
+  for (int i = 0; i < b; ++i) {
+    const double ti = tau[i];
+    if (r < i) {
+      double s = 0.0;
+      for (int k = r; k < i; ++k) s += Ts[r + k * b] * G[k + (int64_t)i * b];
+      Ts[r + i * b] = -ti * s;
+    } else if (r == i) {
+      Ts[i + i * b] = ti;
+    }
+    __syncthreads();
+  }
+  for (int e = r; e < b * b; e += blockDim.x) {
+    const int rr = e % b, cc = e / b;
+    Tt[cc + rr * b] = Ts[e];
+  }
+}
+
 // Q (m×k) := [ID; 0] (ID b×k, ld ldid) or the first k columns of I_m.
 __global__ void start_block_kernel(double* __restrict__ Q, int64_t ldq, int64_t m, int64_t k,
                                    const double* __restrict__ ID, int64_t ldid, int b) {
@@ -163,6 +201,106 @@ int tsqr_apply(cqr_ctx* c, int trans, const TsqrPlan& p, const double* Y, int64_
   return CQR_OK;
 }
 
+// One rank (np_work == 1): Y (m×b, ld ldy) is the block's dgeqrf [V\R] and
+// tau its b taus, so Q = H_1···H_b = I − V·T·Vᵀ with T = dlarft(V, tau).
+// B (m×k) := Qᵀ·B (trans = 1, dormqr('L','T'), apply_tsqr_QT.cxx:73-80) or
+// Q·B (trans = 0, the dorgqr / dormqr('L','N') of construct_tsqr_Q.cxx:87-102).
+size_t local_q_ws_bytes(int64_t m, int64_t b, int64_t k) {
+  Sizer s;
+  s.add(8 * b * b);
+  s.add(8 * b * b);
+  s.add(8 * (gemm_f64_scratch_doubles(b, b, m) + 64));
+  s.add(8 * (wy_update_ws_doubles(m, k, (int)b) + 64));
+  return s.bytes;
+}
+
+int local_q_apply(cqr_ctx* c, int trans, const double* Y, int64_t ldy, const double* tau, double* B, int64_t ldb,
+                  int64_t m, int64_t b, int64_t k, char* base) {
+  Sizer s;
+  const size_t oG = s.add(8 * b * b), oT = s.add(8 * b * b), oP = s.add(8 * (gemm_f64_scratch_doubles(b, b, m) + 64)),
+               oW = s.add(8 * (wy_update_ws_doubles(m, k, (int)b) + 64));
+  auto at = [&](size_t o) { return reinterpret_cast<double*>(base + o); };
+  CQR_TRY(gemm_f64_ws(c, 1, 0, b, b, m, 1.0, Y, ldy, SM_UNIT_LOWER, Y, ldy, SM_UNIT_LOWER, 0.0, at(oG), b, at(oP)));
+  const int bi = (int)b;
+  CQR_TRY(set_max_lds(c, (const void*)larft_kernel, 163840));
+  hipLaunchKernelGGL(larft_kernel, dim3(1), dim3(std::max(64, std::min(512, (bi + 63) / 64 * 64))),
+                     sizeof(double) * (size_t)b * b, c->stream, at(oG), tau, at(oT), bi);
+  CQR_TRY(check_launch(c, "larft"));
+  return wy_update_ws(c, Y, ldy, 1, at(oT), trans ? 0 : 1, B, ldb, m, k, bi, at(oW), CQR_PREC_F64);
+}
+
+}  // namespace
+
+// bitree_tsqr without tree_data (bitree_tsqr.cxx:91-248) over P virtual
+// ranks.  Rank r holds rows [off_r, off_r + mb_r) with mb_r from the
+// reference's distribution (:110-111, root 0); rows past the last full b
+// block go to the last working rank.  One working rank: dgeqrf of the whole
+// block (:129-146; A keeps its input unless output_Y).  Several: every rank's
+// block gets its dgeqrf [V\R] in place (:156); the tree over the ranks' R
+// (pairs (r, r + ⌈n/2⌉), :172-232) runs as one launch per level, and with
+// output_Y each level writes the node's R into the receiver's top triangle
+// and the node's reflector block V2 into the sender's (:184-189, :223-229).
+// tau receives rank 0's b taus; R the final R, zero below the diagonal.
+int bitree_ranks(cqr_ctx* c, double* A, int64_t lda, double* R, double* tau, int64_t m, int64_t b, int64_t P,
+                 int output_Y) {
+  const int64_t nw = ranks_working(m, b, P), bb = b * b;
+  std::vector<int64_t> off(nw + 1, 0);
+  for (int64_t r = 0; r < nw; ++r) {
+    int64_t mb = ((m / b) / P) * b;
+    if ((m / b) % P > r) mb += b;
+    off[r + 1] = off[r] + mb;
+  }
+  off[nw] = m;  // the rows past the last full block
+  int64_t maxrows = 0;
+  for (int64_t r = 0; r < nw; ++r) maxrows = std::max(maxrows, off[r + 1] - off[r]);
+  Sizer s;
+  const size_t oL = s.add(local_tsqr_ws_bytes(maxrows, b, c->leaf_rows));
+  const size_t oR = s.add(8 * nw * bb), oV = s.add(8 * nw * bb), oT = s.add(8 * nw * bb), oNt = s.add(8 * nw * b),
+               oTau = s.add(8 * b), oC = s.add(nw == 1 && !output_Y ? 8 * (size_t)m * b : 0);
+  CQR_TRY(ensure_ws(c, s.bytes));
+  char* base = (char*)c->ws;
+  auto at = [&](size_t o) { return reinterpret_cast<double*>(base + o); };
+  const int bi = (int)b;
+  const unsigned gb = (unsigned)((bb + 255) / 256);
+  if (nw == 1) {
+    double* W = A;
+    int64_t ldw = lda;
+    if (!output_Y) {  // the reference factors a copy (R_buf)
+      W = at(oC);
+      ldw = m;
+      CQR_TRY(launch_copy2d(c, A, lda, W, m, m, b));
+    }
+    CQR_TRY(local_tsqr_ws(c, W, ldw, m, b, tau, nullptr, base + oL));
+    hipLaunchKernelGGL(upper_zero_kernel, dim3(gb), dim3(256), 0, c->stream, W, ldw, R, b, bi);
+    return check_launch(c, "upper_zero");
+  }
+  double* Rl = at(oR);
+  for (int64_t r = 0; r < nw; ++r) {
+    double* Ar = A + off[r];
+    CQR_TRY(local_tsqr_ws(c, Ar, lda, off[r + 1] - off[r], b, r == 0 ? tau : at(oTau), nullptr, base + oL));
+    hipLaunchKernelGGL(upper_zero_kernel, dim3(gb), dim3(256), 0, c->stream, Ar, lda, Rl + r * bb, b, bi);
+    CQR_TRY(check_launch(c, "upper_zero"));
+  }
+  int node = 0;
+  for (int64_t n = nw; n > 1; n = (n + 1) / 2) {
+    const int64_t half = (n + 1) / 2, npairs = n / 2;
+    CQR_TRY(tree_level(c, Rl, (int)n, node, bi, at(oV), at(oT), at(oNt)));
+    if (output_Y) {
+      for (int64_t z = 0; z < npairs; ++z) {
+        hipLaunchKernelGGL(copy_upper_kernel, dim3(gb), dim3(256), 0, c->stream, Rl + z * bb, b, A + off[z], lda, bi);
+        hipLaunchKernelGGL(copy_upper_kernel, dim3(gb), dim3(256), 0, c->stream, at(oV) + (node + z) * bb, b,
+                           A + off[z + half], lda, bi);
+      }
+      CQR_TRY(check_launch(c, "copy_upper"));
+    }
+    node += (int)npairs;
+  }
+  hipLaunchKernelGGL(upper_zero_kernel, dim3(gb), dim3(256), 0, c->stream, Rl, b, R, b, bi);
+  return check_launch(c, "upper_zero");
+}
+
+namespace {
+
 struct TreeView {
   const double *leafT, *nodeV2, *nodeT;
 };
@@ -175,13 +313,20 @@ TreeView tree_view(const double* tree_data, const TsqrPlan& p) {
   return v;
 }
 
-int apply_entry(cqr_ctx* c, const char* fn, int trans, const double* Y, int64_t lda_Y, double* B, int64_t lda_B,
-                int64_t m, int64_t b, int64_t k, int64_t nleaves, const double* tree_data) {
+int apply_entry(cqr_ctx* c, const char* fn, int trans, const double* Y, int64_t lda_Y, const double* tau, double* B,
+                int64_t lda_B, int64_t m, int64_t b, int64_t k, int64_t nleaves, const double* tree_data) {
   CQR_CHECK_CTX(c);
-  if (!Y || !B || !tree_data || b < 1 || m < b || lda_Y < m || lda_B < m || k < 0 || nleaves < 0)
+  if (!Y || !B || b < 1 || m < b || lda_Y < m || lda_B < m || k < 0 || nleaves < 0)
     return set_err(c, CQR_ERR_INVALID, "%s: bad arguments (m=%lld b=%lld k=%lld)", fn, (long long)m,
                    (long long)b, (long long)k);
   if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "%s: b=%lld > %d", fn, (long long)b, CQR_MAX_B);
+  if (!tree_data) {  // the one-rank branch: Y, tau of one dgeqrf
+    if (ranks_working(m, b, std::max<int64_t>(nleaves, 1)) != 1 || !tau)
+      return set_err(c, CQR_ERR_INVALID, "%s: without tree_data only one rank (nleaves <= 1) and tau are allowed", fn);
+    if (k == 0) return CQR_OK;
+    CQR_TRY(ensure_ws(c, local_q_ws_bytes(m, b, k)));
+    return local_q_apply(c, trans, Y, lda_Y, tau, B, lda_B, m, b, k, (char*)c->ws);
+  }
   if (k == 0) return CQR_OK;
   TsqrPlan p;
   const int s = make_plan(m, b, nleaves, 0, p);
@@ -245,14 +390,13 @@ extern "C" {
 
 int cqr_apply_tsqr_QT(cqr_ctx* c, const double* Y, int64_t lda_Y, const double* tau, double* B, int64_t lda_B,
                       int64_t m, int64_t b, int64_t k, int64_t nleaves, const double* tree_data) {
-  (void)tau;  // the leaves' compact-WY T in tree_data carries their tau
-  return apply_entry(c, "apply_tsqr_QT", 1, Y, lda_Y, B, lda_B, m, b, k, nleaves, tree_data);
+  // with tree_data the leaves' compact-WY T carries their tau
+  return apply_entry(c, "apply_tsqr_QT", 1, Y, lda_Y, tau, B, lda_B, m, b, k, nleaves, tree_data);
 }
 
 int cqr_apply_tsqr_Q(cqr_ctx* c, const double* Y, int64_t lda_Y, const double* tau, double* B, int64_t lda_B,
                      int64_t m, int64_t b, int64_t k, int64_t nleaves, const double* tree_data) {
-  (void)tau;
-  return apply_entry(c, "apply_tsqr_Q", 0, Y, lda_Y, B, lda_B, m, b, k, nleaves, tree_data);
+  return apply_entry(c, "apply_tsqr_Q", 0, Y, lda_Y, tau, B, lda_B, m, b, k, nleaves, tree_data);
 }
 
 int cqr_construct_Q1_id(cqr_ctx* c, const double* Y, int64_t lda_Y, const double* tau, double* Q1,
@@ -260,16 +404,16 @@ int cqr_construct_Q1_id(cqr_ctx* c, const double* Y, int64_t lda_Y, const double
                         const double* ID, int64_t ld_ID) {
   CQR_CHECK_CTX(c);
   // every argument is checked before the start block is written into Q1
-  if (!Y || !tree_data || !Q1 || b < 1 || m < b || k < 1 || k > m || lda_Y < m || lda_Q < m || nleaves < 0 ||
-      (ID && ld_ID < b))
+  if (!Y || !Q1 || b < 1 || m < b || k < 1 || k > m || lda_Y < m || lda_Q < m || nleaves < 0 ||
+      (ID && ld_ID < b) || (!tree_data && (!tau || ranks_working(m, b, std::max<int64_t>(nleaves, 1)) != 1)))
     return set_err(c, CQR_ERR_INVALID, "construct_Q1: bad arguments (m=%lld b=%lld k=%lld)", (long long)m,
                    (long long)b, (long long)k);
   if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "construct_Q1: b=%lld > %d", (long long)b, CQR_MAX_B);
-  if (!ID && k == b) return cqr_construct_Q1(c, Y, lda_Y, tau, Q1, lda_Q, m, b, k, nleaves, tree_data);
+  if (!ID && k == b && tree_data) return cqr_construct_Q1(c, Y, lda_Y, tau, Q1, lda_Q, m, b, k, nleaves, tree_data);
   const int gx = (int)std::min<int64_t>(4096, (m * k + 255) / 256);
   hipLaunchKernelGGL(start_block_kernel, dim3(gx), dim3(256), 0, c->stream, Q1, lda_Q, m, k, ID, ld_ID, (int)b);
   CQR_TRY(check_launch(c, "construct_Q1 start block"));
-  return apply_entry(c, "construct_Q1", 0, Y, lda_Y, Q1, lda_Q, m, b, k, nleaves, tree_data);
+  return apply_entry(c, "construct_Q1", 0, Y, lda_Y, tau, Q1, lda_Q, m, b, k, nleaves, tree_data);
 }
 
 int cqr_qr_tree_2d(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b, int64_t stop_at,

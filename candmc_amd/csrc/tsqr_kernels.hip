@@ -682,6 +682,12 @@ __global__ void tril_halve_kernel(const double* __restrict__ G, int b, double* _
   T[rr + cc * ldt] = rr < cc ? 0.0 : (rr == cc ? 0.5 * G[e] : G[e]);
 }
 
+__global__ void fill2d_kernel(double* __restrict__ D, int64_t ldd, int64_t m, int64_t n, double v) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m * n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    D[(e % m) + (e / m) * ldd] = v;
+}
+
 __global__ void copy2d_kernel(const double* __restrict__ S, int64_t lds, double* __restrict__ D, int64_t ldd,
                               int64_t m, int64_t n) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m * n;
@@ -857,6 +863,13 @@ int launch_copy2d(cqr_ctx* c, const double* src, int64_t lds, double* dst, int64
   const int gx = (int)std::min<int64_t>(2048, (m * n + 255) / 256);
   hipLaunchKernelGGL(copy2d_kernel, dim3(gx), dim3(256), 0, c->stream, src, lds, dst, ldd, m, n);
   return check_launch(c, "copy2d");
+}
+
+int launch_fill2d(cqr_ctx* c, double* dst, int64_t ldd, int64_t m, int64_t n, double v) {
+  if (m <= 0 || n <= 0) return CQR_OK;
+  const int gx = (int)std::min<int64_t>(2048, (m * n + 255) / 256);
+  hipLaunchKernelGGL(fill2d_kernel, dim3(gx), dim3(256), 0, c->stream, dst, ldd, m, n, v);
+  return check_launch(c, "fill2d");
 }
 
 int launch_diag(cqr_ctx* c, const double* T, int64_t ldt, double* tau, int b) {

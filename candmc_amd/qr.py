@@ -23,7 +23,7 @@ __all__ = [
     "local_tsqr", "tree_tsqr", "bitree_tsqr", "construct_Q1", "signed_NLU", "recursive_NLU",
     "hh_recon_qr", "compute_invT_from_W", "compute_invT_from_Y", "upd_A", "update_A",
     "QR_2D", "QR_2D_2D", "tau_recon", "pack_upper", "unpack_upper", "pack_lower", "unpack_lower",
-    "tree_data_size", "plan_tsqr", "apply_tsqr_QT", "QR_tree_2D", "tree_data_export",
+    "tree_data_size", "plan_tsqr", "apply_tsqr_QT", "QR_tree_2D", "tree_data_export", "compute_invT_from_Y2",
 ]
 
 
@@ -51,16 +51,25 @@ class Context:
     def sync(self):
         self.check(self.lib.cqr_sync(self.h))
 
+    def done(self, status, sync=True):
+        """Raise on a failed launch; with sync, wait for the stream and raise
+        if a TSQR tree or construct_Q1 wait gave up (cqr_sync reports it)."""
+        self.check(status)
+        if sync:
+            self.sync()
+
+    def last_prec(self):
+        """Factorization precision of the last lstsq's X (PREC_*)."""
+        return int(self.lib.cqr_lstsq_last_prec(self.h))
+
+    def set_lstsq_escalate(self, on: bool):
+        self.check(self.lib.cqr_set_lstsq_escalate(self.h, 1 if on else 0))
+
     def set_leaf_rows(self, rows: int):
         self.check(self.lib.cqr_set_leaf_rows(self.h, int(rows)))
 
     def set_lookahead(self, on: bool):
         self.check(self.lib.cqr_set_lookahead(self.h, 1 if on else 0))
-
-    def set_blas_update(self, on):
-        """Bulk of the FP64 trailing update through rocBLAS dgemm (True, the
-        default) or the library's own MFMA kernel (False)."""
-        self.check(self.lib.cqr_set_blas_update(self.h, 1 if on else 0))
 
     def set_check_finite(self, on: bool):
         self.check(self.lib.cqr_set_check_finite(self.h, 1 if on else 0))
@@ -156,65 +165,62 @@ def tree_data_size(m, b, numPes=0):
 
 
 # --- alg/QR/tsqr ----------------------------------------------------------
-def local_tsqr(A, R, m, b, lda_A, tau=None):
+def local_tsqr(A, R, m, b, lda_A, tau=None, sync=True):
     """bitree_tsqr.cxx:48-71: dgeqrf of one m×b block.  R is the caller's
     scratch in the reference; here it receives the compact-WY T (b×b) when
     given.  Returns tau (b)."""
     ctx = context()
     if tau is None:
         tau = torch.empty(b, dtype=torch.float64, device=A.device)
-    ctx.check(ctx.lib.cqr_local_tsqr(ctx.h, _pd(A), lda_A, m, b, _pd(tau), _pd(R)))
+    ctx.done(sync=sync, status=ctx.lib.cqr_local_tsqr(ctx.h, _pd(A), lda_A, m, b, _pd(tau), _pd(R)))
     return tau
 
 
-def tree_tsqr(A, buf, b, tau=None):
+def tree_tsqr(A, buf, b, tau=None, sync=True):
     """bitree_tsqr.cxx:20-34: dtpqrt of [R1; R2] in a 2b×b buffer; tau
     receives the nb×b blocked T (nb = min(b, 16)).  buf is unused."""
     ctx = context()
-    ctx.check(ctx.lib.cqr_tree_tsqr(ctx.h, _pd(A), b, _pd(tau)))
+    ctx.done(sync=sync, status=ctx.lib.cqr_tree_tsqr(ctx.h, _pd(A), b, _pd(tau)))
 
 
 def bitree_tsqr(A, lda_A, R, tau, m, b, myRank=0, numPes=1, root=0, req_id=0, cdt=None,
-                output_Y=0, tree_data=None):
-    """bitree_tsqr.cxx:91-248 over numPes virtual leaves (0 = automatic).
-    tau must hold nleaves×b doubles and tree_data tree_data_size(m,b,numPes)."""
+                output_Y=0, tree_data=None, sync=True):
+    """bitree_tsqr.cxx:91-248.  With tree_data: the library's leaves (at
+    least numPes; 0 = automatic), tau holding nleaves×b doubles and tree_data
+    tree_data_size(m, b, numPes).  Without: the reference's per-rank
+    semantics over numPes ranks (tau: rank 0's b doubles; with output_Y the
+    senders' top triangles receive the tree's V2)."""
     ctx = context()
-    if tree_data is None:
-        raise CQRError(1, "tree_data is required (size tree_data_size(m, b, numPes))")
-    ctx.check(ctx.lib.cqr_bitree_tsqr(ctx.h, _pd(A), lda_A, _pd(R), _pd(tau), m, b, numPes,
+    ctx.done(sync=sync, status=ctx.lib.cqr_bitree_tsqr(ctx.h, _pd(A), lda_A, _pd(R), _pd(tau), m, b, numPes,
                                       int(output_Y), _pd(tree_data)))
 
 
 def construct_Q1(Y, lda_Y, tau, Q1, lda_Q, m, b, k, myRank=0, numPes=1, root=0, cdt=None,
-                 tree_data=None, ID=None):
+                 tree_data=None, ID=None, sync=True):
     """construct_tsqr_Q.cxx:34-242: Q1 (m×k) := Q·[ID; 0] (ID b×k, ld b), or
     the first k columns of the TSQR's Q when ID is None."""
     ctx = context()
-    if tree_data is None:
-        raise CQRError(1, "tree_data from bitree_tsqr is required")
     if ID is None and k == b:
-        ctx.check(ctx.lib.cqr_construct_Q1(ctx.h, _pd(Y), lda_Y, _pd(tau), _pd(Q1), lda_Q, m, b, k,
+        ctx.done(sync=sync, status=ctx.lib.cqr_construct_Q1(ctx.h, _pd(Y), lda_Y, _pd(tau), _pd(Q1), lda_Q, m, b, k,
                                            numPes, _pd(tree_data)))
     else:
-        ctx.check(ctx.lib.cqr_construct_Q1_id(ctx.h, _pd(Y), lda_Y, _pd(tau), _pd(Q1), lda_Q, m, b,
+        ctx.done(sync=sync, status=ctx.lib.cqr_construct_Q1_id(ctx.h, _pd(Y), lda_Y, _pd(tau), _pd(Q1), lda_Q, m, b,
                                               k, numPes, _pd(tree_data), _p(ID), b))
 
 
 def apply_tsqr_QT(Y, lda_Y, tau, B, lda_B, m, b, k, myRank=0, numPes=1, root=0, cdt=None,
-                  tree_data=None, trans=True):
+                  tree_data=None, trans=True, sync=True):
     """apply_tsqr_QT.cxx:31-165: B (m×k) := Qᵀ·B with the implicit Q of
     bitree_tsqr (trans=False applies Q)."""
     ctx = context()
-    if tree_data is None:
-        raise CQRError(1, "tree_data from bitree_tsqr is required")
     fn = ctx.lib.cqr_apply_tsqr_QT if trans else ctx.lib.cqr_apply_tsqr_Q
     ctx.check(fn(ctx.h, _pd(Y), lda_Y, _p(tau), _pd(B), lda_B, m, b, k, numPes, _pd(tree_data)))
 
 
-def QR_tree_2D(A, lda_A, m, k, b, numPes=0, stop_at=-1):
+def QR_tree_2D(A, lda_A, m, k, b, numPes=0, stop_at=-1, sync=True):
     """qr_tree_2d.cxx:30-124 over numPes virtual row blocks per panel."""
     ctx = context()
-    ctx.check(ctx.lib.cqr_qr_tree_2d(ctx.h, _pd(A), lda_A, m, k, b, stop_at, numPes))
+    ctx.done(sync=sync, status=ctx.lib.cqr_qr_tree_2d(ctx.h, _pd(A), lda_A, m, k, b, stop_at, numPes))
 
 
 def tree_data_export(tree_data, m, b, numPes, rank):
@@ -230,102 +236,118 @@ def tree_data_export(tree_data, m, b, numPes, rank):
 
 
 # --- alg/QR/hh_recon -------------------------------------------------------
-def signed_NLU(A, R, b, lda_A, lda_R, signs=None):
+def signed_NLU(A, R, b, lda_A, lda_R, signs=None, sync=True):
     """hh_recon.cxx:43-70; signs is an int64 GPU tensor or None."""
     ctx = context()
-    ctx.check(ctx.lib.cqr_signed_nlu(ctx.h, _pd(A), lda_A, _pd(R), lda_R, b, _p(signs)))
+    ctx.done(sync=sync, status=ctx.lib.cqr_signed_nlu(ctx.h, _pd(A), lda_A, _pd(R), lda_R, b, _p(signs)))
 
 
-def recursive_NLU(A, R, b, lda_A, lda_R, signs=None):
+def recursive_NLU(A, R, b, lda_A, lda_R, signs=None, sync=True):
     """hh_recon.cxx:80-111."""
     ctx = context()
-    ctx.check(ctx.lib.cqr_recursive_nlu(ctx.h, _pd(A), lda_A, _pd(R), lda_R, b, _p(signs)))
+    ctx.done(sync=sync, status=ctx.lib.cqr_recursive_nlu(ctx.h, _pd(A), lda_A, _pd(R), lda_R, b, _p(signs)))
 
 
 def hh_recon_qr(A, lda_A, m, b, W, myRank=0, numPes=1, root=0, req_id=0, cdt=None,
-                T=None, invT_T=None):
+                T=None, invT_T=None, sync=True):
     """hh_recon.cxx:129-321: A := [Y\\R], W := −T·Y1ᵀ; numPes virtual leaves."""
     ctx = context()
-    ctx.check(ctx.lib.cqr_hh_recon_qr_ex(ctx.h, _pd(A), lda_A, m, b, _pd(W), _pd(T), _pd(invT_T),
+    ctx.done(sync=sync, status=ctx.lib.cqr_hh_recon_qr_ex(ctx.h, _pd(A), lda_A, m, b, _pd(W), _pd(T), _pd(invT_T),
                                          numPes))
 
 
-def compute_invT_from_W(W, b, invT):
+def compute_invT_from_W(W, b, invT, sync=True):
     """hh_recon.cxx:26-31: invT holds Y1 on entry, T⁻ᵀ on exit."""
     ctx = context()
-    ctx.check(ctx.lib.cqr_compute_invT_from_W(ctx.h, _pd(W), b, _pd(invT)))
+    ctx.done(sync=sync, status=ctx.lib.cqr_compute_invT_from_W(ctx.h, _pd(W), b, _pd(invT)))
 
 
 # --- alg/QR/qr_2d ------------------------------------------------------------
-def compute_invT_from_Y(Y, lda_Y, T, lda_T, mb, b, pv=None):
+def compute_invT_from_Y(Y, lda_Y, T, lda_T, mb, b, pv=None, sync=True):
     """qr_2d.cxx:22-60."""
     ctx = context()
-    ctx.check(ctx.lib.cqr_compute_invT_from_Y(ctx.h, _pd(Y), lda_Y, _pd(T), lda_T, mb, b))
+    ctx.done(sync=sync, status=ctx.lib.cqr_compute_invT_from_Y(ctx.h, _pd(Y), lda_Y, _pd(T), lda_T, mb, b))
 
 
-def upd_A(Ybuf, lda_Y, A, lda_A, mb, kb, b, W, pv=None, W_is_T=False):
+def compute_invT_from_Y2(Y1, lda_Y1, Y2, lda_Y2, T, lda_T, mb, b1, b2, pv=None, sync=True):
+    """qr_2d.cxx:76-106 (the two-Y form): T (b1×b2) := Y1ᵀ·Y2."""
+    ctx = context()
+    ctx.done(sync=sync, status=ctx.lib.cqr_compute_invT_from_Y2(ctx.h, _pd(Y1), lda_Y1, _pd(Y2), lda_Y2, _pd(T),
+                                                                lda_T, mb, b1, b2))
+
+
+def upd_A(Ybuf, lda_Y, A, lda_A, mb, kb, b, W, pv=None, W_is_T=False, sync=True):
     """qr_2d.cxx:224-282 (Ybuf explicit)."""
     ctx = context()
-    ctx.check(ctx.lib.cqr_upd_A(ctx.h, _pd(Ybuf), lda_Y, _pd(A), lda_A, mb, kb, b, _pd(W),
+    ctx.done(sync=sync, status=ctx.lib.cqr_upd_A(ctx.h, _pd(Ybuf), lda_Y, _pd(A), lda_A, mb, kb, b, _pd(W),
                                 1 if W_is_T else 0))
 
 
-def update_A(Y, lda_Y, A, lda_A, m, k, b, W, pv=None, aggreg_Y=None, lda_aY=0, W_is_T=False):
+def update_A(Y, lda_Y, A, lda_A, m, k, b, W, pv=None, aggreg_Y=None, lda_aY=0, W_is_T=False, sync=True):
     """qr_2d.cxx:124-177 (Y is the panel [Y\\R])."""
     ctx = context()
-    ctx.check(ctx.lib.cqr_update_A(ctx.h, _pd(Y), lda_Y, _pd(A), lda_A, m, k, b, _pd(W),
+    ctx.done(sync=sync, status=ctx.lib.cqr_update_A(ctx.h, _pd(Y), lda_Y, _pd(A), lda_A, m, k, b, _pd(W),
                                    _pd(aggreg_Y), lda_aY, 1 if W_is_T else 0))
 
 
-def QR_2D(A, lda_A, m, k, b, pv=None, aggreg_Y=None, lda_aY=0):
+def QR_2D(A, lda_A, m, k, b, pv=None, aggreg_Y=None, lda_aY=0, sync=True):
     """qr_2d.cxx:299-363 on one GPU."""
     ctx = context()
-    ctx.check(ctx.lib.cqr_qr_2d(ctx.h, _pd(A), lda_A, m, k, b, _pd(aggreg_Y), lda_aY))
+    ctx.done(sync=sync, status=ctx.lib.cqr_qr_2d(ctx.h, _pd(A), lda_A, m, k, b, _pd(aggreg_Y), lda_aY))
 
 
-def QR_2D_2D(A, lda_A, m, k, b, b_sub, pv=None, aggreg_Y=None, lda_aY=0):
+def QR_2D_2D(A, lda_A, m, k, b, b_sub, pv=None, aggreg_Y=None, lda_aY=0, sync=True):
     """qr_2d.cxx:772-897 on one GPU."""
     ctx = context()
-    ctx.check(ctx.lib.cqr_qr_2d_2d(ctx.h, _pd(A), lda_A, m, k, b, b_sub, _pd(aggreg_Y), lda_aY))
+    ctx.done(sync=sync, status=ctx.lib.cqr_qr_2d_2d(ctx.h, _pd(A), lda_A, m, k, b, b_sub, _pd(aggreg_Y), lda_aY))
 
 
 # --- least-squares path ------------------------------------------------------
 PREC_F64, PREC_F32, PREC_BF16 = 0, 1, 2
 
 
-def geqrf2d(A, lda, m, k, b, Tt=None, prec=PREC_F64):
+def geqrf2d(A, lda, m, k, b, Tt=None, prec=PREC_F64, sync=True):
     """QR_2D that also returns the panel factors Tᵀ (⌈k/b⌉ blocks of b·b)
     and runs the trailing update in `prec`.  Returns Tt."""
     ctx = context()
     if Tt is None:
         Tt = torch.zeros(((k + b - 1) // b) * b * b, dtype=torch.float64, device=A.device)
-    ctx.check(ctx.lib.cqr_geqrf2d(ctx.h, _pd(A), lda, m, k, b, _pd(Tt), int(prec)))
+    ctx.done(sync=sync, status=ctx.lib.cqr_geqrf2d(ctx.h, _pd(A), lda, m, k, b, _pd(Tt), int(prec)))
     return Tt
 
 
-def apply_qt(YR, ldy, Tt, m, k, b, B, ldb, nrhs, trans=True):
+def apply_qt(YR, ldy, Tt, m, k, b, B, ldb, nrhs, trans=True, sync=True):
     """B := Qᵀ·B (trans) or Q·B, Q from geqrf2d (apply_tsqr_QT.cxx:31-165 analogue)."""
     ctx = context()
-    ctx.check(ctx.lib.cqr_apply_qt(ctx.h, _pd(YR), ldy, _pd(Tt), m, k, b, _pd(B), ldb, nrhs,
+    ctx.done(sync=sync, status=ctx.lib.cqr_apply_qt(ctx.h, _pd(YR), ldy, _pd(Tt), m, k, b, _pd(B), ldb, nrhs,
                                    1 if trans else 0))
 
 
-def trsm_r(YR, ldy, n, B, ldb, nrhs, trans=False):
+def trsm_r(YR, ldy, n, B, ldb, nrhs, trans=False, sync=True):
     """B(0:n) := R⁻¹·B (or R⁻ᵀ·B), R the upper triangle of YR."""
     ctx = context()
-    ctx.check(ctx.lib.cqr_trsm_r(ctx.h, _pd(YR), ldy, n, _pd(B), ldb, nrhs, 1 if trans else 0))
+    ctx.done(sync=sync, status=ctx.lib.cqr_trsm_r(ctx.h, _pd(YR), ldy, n, _pd(B), ldb, nrhs, 1 if trans else 0))
 
 
-def lstsq(A, lda, m, n, B, ldb, nrhs, X=None, ldx=None, b=128, prec=PREC_F64, max_iter=10, tol=1e-14):
-    """min ‖A·X − B‖ with refinement on FP64 residuals; returns (X, iters, last_update)."""
+def lstsq(A, lda, m, n, B, ldb, nrhs, X=None, ldx=None, b=128, prec=PREC_F64, max_iter=10, tol=1e-14,
+          raise_noconverge=True):
+    """min ‖A·X − B‖ with refinement on FP64 residuals; returns (X, iters,
+    last_update).  A refinement that ends above tol raises CQRError with
+    status 6 (CQR_ERR_NOCONVERGE) unless raise_noconverge is False; then
+    the result is (X, iters, last_update, status).  context().last_prec()
+    tells whether a mixed solve was refactored in FP64."""
     ctx = context()
     ldx = n if ldx is None else ldx
     if X is None:
         X = torch.zeros(ldx * nrhs, dtype=torch.float64, device=A.device)
     it, upd = ctypes.c_int(), ctypes.c_double()
-    ctx.check(ctx.lib.cqr_lstsq(ctx.h, _pd(A), lda, m, n, _pd(B), ldb, nrhs, _pd(X), ldx, b, int(prec),
-                                int(max_iter), float(tol), ctypes.byref(it), ctypes.byref(upd)))
-    return X, it.value, upd.value
+    st = ctx.lib.cqr_lstsq(ctx.h, _pd(A), lda, m, n, _pd(B), ldb, nrhs, _pd(X), ldx, b, int(prec),
+                           int(max_iter), float(tol), ctypes.byref(it), ctypes.byref(upd))
+    if st == 6 and not raise_noconverge:
+        ctx.sync()
+        return X, it.value, upd.value, st
+    ctx.done(st)
+    return (X, it.value, upd.value) if raise_noconverge else (X, it.value, upd.value, st)
 
 
 # --- batched tall-skinny QR ----------------------------------------------------
@@ -393,13 +415,15 @@ class Batch:
     def new_t(self, A):
         return torch.zeros(self.total_t, dtype=A.dtype, device=A.device)
 
-    def qr(self, A, Tt, use_graph=False):
+    def qr(self, A, Tt, use_graph=False, sync=True):
         dt = {torch.float64: 0, torch.float32: 1}[A.dtype]
         assert Tt.dtype == A.dtype
         self.ctx.check(self.ctx.lib.cqr_batch_qr(self.ctx.h, self.h, _p(A), _p(Tt), dt, 1 if use_graph else 0))
+        if sync:
+            self.ctx.sync()
 
-    def qr_loop(self, A, Tt):
-        self.ctx.check(self.ctx.lib.cqr_batch_qr_loop(self.ctx.h, self.h, _pd(A), _pd(Tt)))
+    def qr_loop(self, A, Tt, sync=True):
+        self.ctx.done(self.ctx.lib.cqr_batch_qr_loop(self.ctx.h, self.h, _pd(A), _pd(Tt)), sync)
 
     def tune(self, A, reps=2):
         self.ctx.check(self.ctx.lib.cqr_batch_tune(self.ctx.h, self.h, _pd(A), reps))
@@ -418,7 +442,7 @@ class _Request(ctypes.Structure):
 
 class _Result(ctypes.Structure):
     _fields_ = [("id", ctypes.c_int64), ("status", ctypes.c_int), ("iters", ctypes.c_int), ("slot", ctypes.c_int),
-                ("last_update", ctypes.c_double), ("t_submit_ns", ctypes.c_int64), ("t_start_ns", ctypes.c_int64),
+                ("prec", ctypes.c_int), ("last_update", ctypes.c_double), ("t_submit_ns", ctypes.c_int64), ("t_start_ns", ctypes.c_int64),
                 ("t_end_ns", ctypes.c_int64), ("bytes", ctypes.c_int64)]
 
 
@@ -488,26 +512,26 @@ class Server:
 
 
 # --- alg/shared/util ---------------------------------------------------------
-def tau_recon(structure, ncol, nrow, lda, Y, tau):
+def tau_recon(structure, ncol, nrow, lda, Y, tau, sync=True):
     ctx = context()
-    ctx.check(ctx.lib.cqr_tau_recon(ctx.h, structure.encode()[:1], ncol, nrow, lda, _pd(Y), _pd(tau)))
+    ctx.done(sync=sync, status=ctx.lib.cqr_tau_recon(ctx.h, structure.encode()[:1], ncol, nrow, lda, _pd(Y), _pd(tau)))
 
 
-def pack_upper(R_in, R_out, b, lda_in):
+def pack_upper(R_in, R_out, b, lda_in, sync=True):
     ctx = context()
-    ctx.check(ctx.lib.cqr_pack_upper(ctx.h, _pd(R_in), _pd(R_out), b, lda_in))
+    ctx.done(sync=sync, status=ctx.lib.cqr_pack_upper(ctx.h, _pd(R_in), _pd(R_out), b, lda_in))
 
 
-def unpack_upper(R_in, R_out, b, lda_out):
+def unpack_upper(R_in, R_out, b, lda_out, sync=True):
     ctx = context()
-    ctx.check(ctx.lib.cqr_unpack_upper(ctx.h, _pd(R_in), _pd(R_out), b, lda_out))
+    ctx.done(sync=sync, status=ctx.lib.cqr_unpack_upper(ctx.h, _pd(R_in), _pd(R_out), b, lda_out))
 
 
-def pack_lower(R_in, R_out, b, lda_in, has_diag):
+def pack_lower(R_in, R_out, b, lda_in, has_diag, sync=True):
     ctx = context()
-    ctx.check(ctx.lib.cqr_pack_lower(ctx.h, _pd(R_in), _pd(R_out), b, lda_in, int(has_diag)))
+    ctx.done(sync=sync, status=ctx.lib.cqr_pack_lower(ctx.h, _pd(R_in), _pd(R_out), b, lda_in, int(has_diag)))
 
 
-def unpack_lower(R_in, R_out, b, lda_out, has_diag):
+def unpack_lower(R_in, R_out, b, lda_out, has_diag, sync=True):
     ctx = context()
-    ctx.check(ctx.lib.cqr_unpack_lower(ctx.h, _pd(R_in), _pd(R_out), b, lda_out, int(has_diag)))
+    ctx.done(sync=sync, status=ctx.lib.cqr_unpack_lower(ctx.h, _pd(R_in), _pd(R_out), b, lda_out, int(has_diag)))

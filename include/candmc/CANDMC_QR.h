@@ -77,6 +77,8 @@ void hh_recon_qr(double* A, int64_t lda_A, int64_t m, int64_t b, double* W, int6
 /* alg/QR/qr_2d/qr_2d.h */
 void compute_invT_from_Y(double const* Y, int64_t lda_Y, double* T, int64_t lda_T, int64_t mb,
                          int64_t b, pview* pv);
+void compute_invT_from_Y(double const* Y1, int64_t lda_Y1, double const* Y2, int64_t lda_Y2, double* T,
+                         int64_t lda_T, int64_t mb, int64_t b1, int64_t b2, pview* pv);
 void update_A(double const* Y, int64_t lda_Y, double* A, int64_t lda_A, int64_t m, int64_t k,
               int64_t b, double const* W, pview* pv, double* aggreg_Y, int64_t lda_aY,
               bool W_is_T = false);

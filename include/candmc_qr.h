@@ -28,7 +28,12 @@ enum cqr_status {
   CQR_ERR_UNSUPPORTED = 2, /* outside the kernels' limits (e.g. b > 128)    */
   CQR_ERR_NOMEM = 3,       /* workspace could not be allocated              */
   CQR_ERR_DEVICE = 4,      /* a HIP runtime call or kernel launch failed    */
-  CQR_ERR_NONFINITE = 5    /* input holds NaN/Inf (only with check_finite)  */
+  CQR_ERR_NONFINITE = 5,   /* input holds NaN/Inf (only with check_finite),
+                              or a least-squares R / iterate is not finite  */
+  CQR_ERR_NOCONVERGE = 6,  /* least-squares refinement ended above tol: X
+                              holds the last iterate                       */
+  CQR_ERR_CANCELLED = 7    /* a served request dropped before it started
+                              (the server was destroyed)                   */
 };
 
 /* Arithmetic of the trailing (compact-WY) update; panels are always FP64. */
@@ -53,6 +58,9 @@ void* cqr_get_stream(const cqr_ctx* ctx);
  * results are then invalid). */
 int cqr_sync(cqr_ctx* ctx);
 int cqr_reserve(cqr_ctx* ctx, size_t bytes);          /* pre-grow the arena   */
+/* Waits for the context's streams and frees its workspace arena (it grows
+ * again on the next call that needs it).                                 */
+int cqr_trim(cqr_ctx* ctx);
 size_t cqr_workspace_bytes(const cqr_ctx* ctx);
 int cqr_set_check_finite(cqr_ctx* ctx, int on);       /* scan inputs first    */
 int cqr_set_leaf_rows(cqr_ctx* ctx, int64_t rows);    /* 0 = automatic        */
@@ -62,12 +70,6 @@ int cqr_set_leaf_rows(cqr_ctx* ctx, int64_t rows);    /* 0 = automatic        */
  * alg/QR/qr_2d/qr_2d.cxx:378-635, QR_2D_pipe).  Results differ from the
  * non-lookahead order only in the summation order of Yᵀ·A.               */
 int cqr_set_lookahead(cqr_ctx* ctx, int on);
-/* The unmasked bulk of the FP64 trailing update A −= Y·(T·Yᵀ·A), a plain
- * GEMM once Z' = T·Yᵀ·A is formed, runs through the vendor BLAS (rocBLAS
- * dgemm) when on (the default), else through the library's own MFMA kernel.
- * The structured parts (unit-lower Y block, triangular T) always use the
- * library's kernels.  No counterpart in the reference (a tuning switch).  */
-int cqr_set_blas_update(cqr_ctx* ctx, int on);
 /* Kernel timing: while enabled, the library records a HIP event pair on
  * its stream around every kernel it launches; cqr_profile_read waits for
  * the stream and returns, for one kernel class, the summed event time,
@@ -195,6 +197,15 @@ int cqr_compute_invT_from_W(cqr_ctx* ctx, const double* W, int64_t b, double* in
 int cqr_compute_invT_from_Y(cqr_ctx* ctx, const double* Y, int64_t lda_Y, double* T,
                             int64_t lda_T, int64_t mb, int64_t b);
 
+/* ---- a8  compute_invT_from_Y, two-Y form  (alg/QR/qr_2d/qr_2d.cxx:76-106,
+ *      decl alg/QR/qr_2d/qr_2d.h:47) ----------------------------------------
+ * T (b1×b2, ld lda_T) := Y1ᵀ·Y2 with Y1 mb×b1 (ld lda_Y1) and Y2 mb×b2
+ * (ld lda_Y2): the off-diagonal block of an aggregated T⁻ᵀ.  mb = 0 gives
+ * a zero T.                                                                */
+int cqr_compute_invT_from_Y2(cqr_ctx* ctx, const double* Y1, int64_t lda_Y1, const double* Y2,
+                             int64_t lda_Y2, double* T, int64_t lda_T, int64_t mb, int64_t b1,
+                             int64_t b2);
+
 /* ---- a9  upd_A  (alg/QR/qr_2d/qr_2d.cxx:224-282) ------------------------
  * A (mb×kb) := A − Y·(T⁻ᵀ)⁻¹·(Yᵀ·A) with Y explicit (mb×b).  W selects T⁻ᵀ
  * as in the reference: NULL → compute_invT_from_Y(Y); W_is_T → W is T⁻ᵀ;
@@ -254,11 +265,23 @@ int cqr_trsm_r(cqr_ctx* ctx, const double* YR, int64_t ldy, int64_t n, double* B
  * update at least 8×, then the augmented system; stopping when every
  * column's update is ≤ tol relative to the solution, or when the
  * augmented steps stop shrinking (by less than 10% per step).
- * iters / last_update (nullable) report the steps taken and the last
- * relative update.                                                       */
+ * Returns CQR_OK only when the last relative update is ≤ tol (or
+ * max_iter = 0); CQR_ERR_NOCONVERGE when the refinement ended above tol
+ * (X holds the last iterate); CQR_ERR_NONFINITE when R or an iterate is
+ * not finite.  With a mixed `prec` and escalation on (the default), a
+ * solve whose R has max|r_ii|/min|r_ii|·u > 1e-2 (u the unit roundoff of
+ * `prec`), or whose refinement ends above tol, is refactored in FP64 and
+ * refined again; cqr_lstsq_last_prec then reports CQR_PREC_F64.
+ * iters / last_update (nullable) report the steps taken (over both
+ * factorizations) and the last relative update.                          */
 int cqr_lstsq(cqr_ctx* ctx, const double* A, int64_t lda, int64_t m, int64_t n, const double* B,
               int64_t ldb, int64_t nrhs, double* X, int64_t ldx, int64_t b, int prec, int max_iter,
               double tol, int* iters, double* last_update);
+/* Escalation of stalled mixed-precision solves to FP64 (on by default).   */
+int cqr_set_lstsq_escalate(cqr_ctx* ctx, int on);
+/* Precision of the factorization that produced the last cqr_lstsq's X
+ * (CQR_PREC_*), or −1 when that call failed before factoring.            */
+int cqr_lstsq_last_prec(const cqr_ctx* ctx);
 
 /* ---- batched tall-skinny QR (no reference counterpart: a batch of
  *      independent hh_recon_qr panels, alg/QR/hh_recon/hh_recon.cxx:129-321,
@@ -307,6 +330,8 @@ typedef struct cqr_request {
 typedef struct cqr_result {
   int64_t id;
   int status, iters, slot;
+  int prec;          /* factorization precision of X (CQR_PREC_*): a mixed
+                        request escalated to FP64 reports CQR_PREC_F64     */
   double last_update;
   int64_t t_submit_ns, t_start_ns, t_end_ns;   /* steady clock            */
   int64_t bytes;                               /* admitted footprint      */

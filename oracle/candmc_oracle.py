@@ -281,6 +281,41 @@ def bitree_tsqr(blocks, b):
     return Ys, Rs[0].copy(order="F"), taus, tree_data
 
 
+def bitree_tsqr_no_tree_data(blocks, b, output_Y=1):
+    """bitree_tsqr.cxx:91-248 without tree_data (the branch test_bitree_tsqr
+    exercises, :319).  One working rank: dgeqrf of the whole block, A holds
+    [V\\R] only with output_Y (:129-146).  Several: each rank's block gets
+    its dgeqrf [V\\R] (:156); a receiver factors the stacked [R_mine; R_recv]
+    with dgeqrf (local_tsqr, :211); with output_Y it writes the new R into its
+    own top triangle (:223-224) and sends the packed upper lower half (the
+    node's V2) back, which the sender unpacks into its top triangle
+    (:184-189).  Returns (blocks after the call, R, rank 0's tau)."""
+    P = len(blocks)
+    out = [np.asfortranarray(B.copy()) for B in blocks]
+    if P == 1:
+        y, t = local_tsqr(blocks[0])
+        if output_Y:
+            out[0] = y
+        return out, np.triu(y[:b, :b]), t
+    taus, Rs = [], []
+    for r in range(P):
+        y, t = local_tsqr(blocks[r])
+        out[r] = y
+        taus.append(t)
+        Rs.append(np.triu(y[:b, :b]))
+    for n in bitree_levels(P):
+        for r, s in bitree_pairs(n):
+            y, _ = local_tsqr(np.vstack([Rs[r], Rs[s]]))
+            Rs[r] = np.triu(y[:b, :b])
+            if output_Y:
+                top = out[r][:b, :b]
+                top[np.triu_indices(b)] = Rs[r][np.triu_indices(b)]
+                V2 = np.triu(y[b:, :b])
+                top = out[s][:b, :b]
+                top[np.triu_indices(b)] = V2[np.triu_indices(b)]
+    return out, Rs[0].copy(order="F"), taus[0]
+
+
 def construct_Q1(Ys, taus, tree_data, b, k=None):
     """construct_tsqr_Q.cxx:34-242 (tree_data branch, ID = NULL).
 

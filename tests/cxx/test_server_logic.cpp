@@ -5,8 +5,9 @@
 // collected by a wait or a drain, returns CQR_ERR_INVALID instead of
 // blocking; two waiters on one id, or a waiter racing a drain, end with
 // exactly one collector; every request of a burst through two slots is
-// served; destroying a server with requests queued drops them and returns
-// once the running one ends.  Built with g++ and run by tests/test_server_logic.py.
+// served; the slots' resident arenas are what admission charges and never
+// exceed the budget; destroying a server with requests queued cancels them
+// (a waiter returns) and returns once the running one ends.  Built with g++ and run by tests/test_server_logic.py.
 #include <hip/hip_runtime_api.h>
 
 #include <atomic>
@@ -27,11 +28,31 @@ hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned int) {
   return hipSuccess;
 }
 hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+// a stand-in context is one size_t: its arena bytes
 int cqr_create(cqr_ctx** c, int) {
-  *c = static_cast<cqr_ctx*>(malloc(8));
+  *c = static_cast<cqr_ctx*>(calloc(1, sizeof(size_t)));
   return CQR_OK;
 }
+static std::atomic<size_t> arena_now{0}, arena_peak{0};  // Σ of the stand-in arenas
+static void arena_set(cqr_ctx* c, size_t v) {
+  size_t* p = reinterpret_cast<size_t*>(c);
+  arena_now += v - *p;
+  *p = v;
+  size_t cur = arena_now.load(), pk = arena_peak.load();
+  while (cur > pk && !arena_peak.compare_exchange_weak(pk, cur)) {}
+}
+int cqr_reserve(cqr_ctx* c, size_t bytes) {
+  if (*reinterpret_cast<size_t*>(c) < bytes) arena_set(c, bytes + bytes / 8);
+  return CQR_OK;
+}
+int cqr_trim(cqr_ctx* c) {
+  arena_set(c, 0);
+  return CQR_OK;
+}
+size_t cqr_workspace_bytes(const cqr_ctx* c) { return *reinterpret_cast<const size_t*>(c); }
+int cqr_lstsq_last_prec(const cqr_ctx*) { return CQR_PREC_F64; }
 int cqr_destroy(cqr_ctx* c) {
+  arena_set(c, 0);
   free(c);
   return CQR_OK;
 }
@@ -43,7 +64,7 @@ int cqr_lstsq(cqr_ctx*, const double*, int64_t, int64_t, int64_t, const double*,
   *iters = 1;
   return CQR_OK;
 }
-size_t cqr_lstsq_workspace_bytes(int64_t, int64_t, int64_t, int64_t) { return 1000; }
+size_t cqr_lstsq_workspace_bytes(int64_t m, int64_t, int64_t, int64_t) { return (size_t)m * 100; }
 }
 
 #define CHECK(x)                                         \
@@ -128,8 +149,51 @@ int main() {
   for (int64_t id : ids) CHECK(cqr_server_wait(s, id, &res) == CQR_OK && res.id == id && res.status == CQR_OK);
   int64_t queued = -1;
   size_t in_use = 1;
-  CHECK(cqr_server_pending(s, &queued, &in_use) == CQR_OK && queued == 0 && in_use == 0);
+  // the slots' arenas stay resident and are what the server charges
+  CHECK(cqr_server_pending(s, &queued, &in_use) == CQR_OK && queued == 0 && in_use == arena_now.load());
+  CHECK(in_use > 0 && in_use <= (1u << 20));
   CHECK(cqr_server_destroy(s) == CQR_OK);
+  CHECK(arena_now.load() == 0);
+
+  // resident arenas never exceed the budget: two slots, a 3000-byte budget,
+  // requests of 1000 and 2000 bytes (arenas 1125 and 2250): an idle slot
+  // gives its arena back when the head cannot fit beside it
+  {
+    arena_peak = 0;
+    cqr_server* sb = nullptr;
+    CHECK(cqr_server_create(&sb, 0, 2, 3000) == CQR_OK);
+    cqr_request small = r, big = r;
+    small.m = 10; small.lda = small.ldb = 10;
+    big.m = 20; big.lda = big.ldb = 20;
+    solve_ms = 2;
+    std::vector<int64_t> q;
+    for (int i = 0; i < 40; ++i) q.push_back(cqr_server_submit(sb, (i % 3) ? &small : &big));
+    for (int64_t id : q) CHECK(cqr_server_wait(sb, id, &res) == CQR_OK && res.status == CQR_OK);
+    CHECK(arena_peak.load() <= 3000);
+    big.m = 30; big.lda = big.ldb = 30;    // 3000 bytes: its arena (3375) exceeds the budget
+    CHECK(cqr_server_submit(sb, &big) == -CQR_ERR_NOMEM);
+    CHECK(cqr_server_destroy(sb) == CQR_OK);
+  }
+
+  // a thread waiting for a request that is still queued when the server is
+  // destroyed returns with CQR_ERR_CANCELLED, and the destroy returns
+  {
+    cqr_server* sc = nullptr;
+    CHECK(cqr_server_create(&sc, 0, 1, 1 << 20) == CQR_OK);
+    solve_ms = 100;
+    const int64_t run = cqr_server_submit(sc, &r), queued_id = cqr_server_submit(sc, &r);
+    (void)run;
+    std::atomic<int> wst2{-1}, rst{-1};
+    std::thread wq([&] {
+      cqr_result rr{};
+      wst2 = cqr_server_wait(sc, queued_id, &rr);
+      rst = rr.status;
+    });
+    std::this_thread::sleep_for(std::chrono::milliseconds(20));
+    CHECK(cqr_server_destroy(sc) == CQR_OK);
+    wq.join();
+    CHECK(wst2.load() == CQR_OK && rst.load() == CQR_ERR_CANCELLED);
+  }
 
   // destroying with requests queued returns once the running one ends
   cqr_server* s1slot = nullptr;

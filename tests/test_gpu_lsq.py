@@ -82,32 +82,85 @@ def test_trsm_r(Q, n, trans):
 def test_lstsq_fp64(Q, m, n, nrhs, b):
     A = O.gaussian(m, n, 11 + n)
     B = O.gaussian(m, nrhs, 12 + n)
-    X, iters, upd = Q.lstsq(Q.to_gpu(A), m, m, n, Q.to_gpu(B), m, nrhs, b=b, max_iter=3)
+    X, iters, upd = Q.lstsq(Q.to_gpu(A), m, m, n, Q.to_gpu(B), m, nrhs, b=b, max_iter=3, tol=1e-12)
     x = Q.from_gpu(X, n, nrhs)
     want = sl.lstsq(A, B)[0]
-    assert rel(x, want) < 1e-12
+    assert rel(x, want) < 1e-12 and upd <= 1e-12
     assert np.abs(A.T @ (B - A @ x)).max() < 1e-10 * np.abs(A).max() * np.abs(B).max() * m
     assert 1 <= iters <= 3
 
 
-@pytest.mark.parametrize("kappa", [1e2, 1e6])
-def test_lstsq_fp32_update_refined_to_fp64(Q, kappa):
-    """FP32 trailing update + FP64-residual refinement reaches the FP64
-    least-squares solution (the mixed-precision workload, small size)."""
+def _conditioned_problem(m, n, nrhs, kappa, seed):
+    A = O.conditioned_matrix(m, n, kappa, seed)
+    rng = np.random.default_rng(seed + 1)
+    B = np.asfortranarray(A @ rng.standard_normal((n, nrhs)) + 1e-6 * rng.standard_normal((m, nrhs)))
+    return A, B, sl.lstsq(A, B)[0]
+
+
+def test_lstsq_fp32_update_refined_to_fp64(Q):
+    """κ = 1e2: the FP32 trailing update + FP64-residual refinement reaches
+    the FP64 least-squares solution without refactoring."""
     m, n, nrhs = 4096, 256, 2
-    A = O.conditioned_matrix(m, n, kappa, 21)
-    rng = np.random.default_rng(22)
-    xt = rng.standard_normal((n, nrhs))
-    B = np.asfortranarray(A @ xt + 1e-6 * rng.standard_normal((m, nrhs)))
-    want = sl.lstsq(A, B)[0]
+    A, B, want = _conditioned_problem(m, n, nrhs, 1e2, 21)
     X, iters, upd = Q.lstsq(Q.to_gpu(A), m, m, n, Q.to_gpu(B), m, nrhs, b=128, prec=Q.PREC_F32,
-                            max_iter=20, tol=1e-15)
-    x = Q.from_gpu(X, n, nrhs)
-    tol = max(1e-10, 20 * kappa * EPS)
-    assert rel(x, want) < tol, (rel(x, want), iters, upd)
-    assert iters >= 2          # the FP32 factorization alone is not FP64-accurate
+                            max_iter=20, tol=1e-12)
+    assert Q.context().last_prec() == Q.PREC_F32
+    assert rel(Q.from_gpu(X, n, nrhs), want) < 1e-10, (iters, upd)
+    assert iters >= 2 and upd <= 1e-12    # the FP32 factorization alone is not FP64-accurate
     X0, _, _ = Q.lstsq(Q.to_gpu(A), m, m, n, Q.to_gpu(B), m, nrhs, b=128, prec=Q.PREC_F32, max_iter=0)
     assert rel(Q.from_gpu(X0, n, nrhs), want) > 1e-12
+
+
+def test_lstsq_fp32_at_kappa_1e6_with_and_without_escalation(Q):
+    """κ = 1e6 (κ·u₃₂ ≈ 0.06 < 1): the FP32 factors refine to the FP64
+    solution, with escalation on (it may refactor in FP64 when R's diagonal
+    range flags the factors) and off (always the FP32 factors).  Either way
+    the answer is within max(1e-10, 20κε) and the status is CQR_OK."""
+    m, n, nrhs, kappa = 4096, 256, 2, 1e6
+    A, B, want = _conditioned_problem(m, n, nrhs, kappa, 23)
+    ctx = Q.context()
+    X, iters, upd = Q.lstsq(Q.to_gpu(A), m, m, n, Q.to_gpu(B), m, nrhs, b=128, prec=Q.PREC_F32,
+                            max_iter=30, tol=1e-10)
+    assert ctx.last_prec() in (Q.PREC_F32, Q.PREC_F64) and upd <= 1e-10
+    assert rel(Q.from_gpu(X, n, nrhs), want) < max(1e-10, 20 * kappa * EPS), (iters, upd)
+    ctx.set_lstsq_escalate(False)
+    try:
+        X, it2, upd = Q.lstsq(Q.to_gpu(A), m, m, n, Q.to_gpu(B), m, nrhs, b=128, prec=Q.PREC_F32,
+                              max_iter=30, tol=1e-10)
+        assert ctx.last_prec() == Q.PREC_F32 and upd <= 1e-10
+    finally:
+        ctx.set_lstsq_escalate(True)
+    assert rel(Q.from_gpu(X, n, nrhs), want) < max(1e-10, 20 * kappa * EPS), (it2, upd)
+    assert it2 >= 2
+
+
+@pytest.mark.parametrize("prec,kappa", [(1, 1e8), (2, 1e8), (2, 1e6)])
+def test_lstsq_mixed_never_fails_silently(Q, prec, kappa):
+    """The mixed configurations whose refinement cannot reach 1e-10 (κ·u of
+    the factorization ≳ 1): with escalation on, the solve is refactored in
+    FP64 and either meets 1e-10 against scipy's FP64 solution or returns
+    CQR_ERR_NOCONVERGE with FP64-quality X; with escalation off it either
+    meets 1e-10 or returns CQR_ERR_NOCONVERGE.  CQR_OK with an error above
+    1e-10 fails the test."""
+    m, n, nrhs = 4096, 256, 1
+    A, B, want = _conditioned_problem(m, n, nrhs, kappa, 41 + prec)
+    ctx = Q.context()
+    dA, dB = Q.to_gpu(A), Q.to_gpu(B)
+    X, iters, upd, st = Q.lstsq(dA, m, m, n, dB, m, nrhs, b=128, prec=prec, max_iter=30, tol=1e-10,
+                                raise_noconverge=False)
+    err = rel(Q.from_gpu(X, n, nrhs), want)
+    assert ctx.last_prec() == Q.PREC_F64
+    assert (st == 0 and err <= 1e-10 and upd <= 1e-10) or (st == 6 and upd > 1e-10), (st, err, upd)
+    assert err < 100 * kappa * EPS       # FP64-quality either way
+    ctx.set_lstsq_escalate(False)
+    try:
+        X, iters, upd, st = Q.lstsq(dA, m, m, n, dB, m, nrhs, b=128, prec=prec, max_iter=30, tol=1e-10,
+                                    raise_noconverge=False)
+        assert ctx.last_prec() == prec
+    finally:
+        ctx.set_lstsq_escalate(True)
+    err = rel(Q.from_gpu(X, n, nrhs), want)
+    assert (st == 0 and err <= 1e-10) or st == 6, (st, err, upd)
 
 
 def test_lstsq_errors(Q):
@@ -121,15 +174,36 @@ def test_lstsq_errors(Q):
 
 def test_lstsq_bf16_update_refined_to_fp64(Q):
     """BF16 trailing update (u = 2⁻⁸) converges under refinement when
-    κ·u < 1: κ = 1e2 reaches the FP64 solution, in more steps than FP32."""
+    κ·u < 1: κ = 1e2 reaches the FP64 solution, in more steps than FP32
+    (escalation off: with it on, κ̂·u > 1e-2 refactors in FP64 at once)."""
     m, n, nrhs = 4096, 256, 1
-    A = O.conditioned_matrix(m, n, 1e2, 31)
-    rng = np.random.default_rng(32)
-    B = np.asfortranarray(A @ rng.standard_normal((n, nrhs)) + 1e-6 * rng.standard_normal((m, nrhs)))
-    want = sl.lstsq(A, B)[0]
+    A, B, want = _conditioned_problem(m, n, nrhs, 1e2, 31)
+    ctx = Q.context()
+    ctx.set_lstsq_escalate(False)
+    try:
+        X, iters, upd = Q.lstsq(Q.to_gpu(A), m, m, n, Q.to_gpu(B), m, nrhs, b=128, prec=Q.PREC_BF16,
+                                max_iter=80, tol=1e-12)
+        assert ctx.last_prec() == Q.PREC_BF16
+        _, it32, _ = Q.lstsq(Q.to_gpu(A), m, m, n, Q.to_gpu(B), m, nrhs, b=128, prec=Q.PREC_F32,
+                             max_iter=80, tol=1e-12)
+    finally:
+        ctx.set_lstsq_escalate(True)
+    assert rel(Q.from_gpu(X, n, nrhs), want) < 1e-10, (iters, upd)
+    assert it32 < iters < 80
     X, iters, upd = Q.lstsq(Q.to_gpu(A), m, m, n, Q.to_gpu(B), m, nrhs, b=128, prec=Q.PREC_BF16,
                             max_iter=80, tol=1e-12)
-    assert rel(Q.from_gpu(X, n, nrhs), want) < 1e-10, (iters, upd)
-    _, it32, _ = Q.lstsq(Q.to_gpu(A), m, m, n, Q.to_gpu(B), m, nrhs, b=128, prec=Q.PREC_F32,
-                         max_iter=80, tol=1e-12)
-    assert it32 < iters < 80
+    assert ctx.last_prec() == Q.PREC_F64
+    assert rel(Q.from_gpu(X, n, nrhs), want) < 1e-10
+
+
+def test_lstsq_nan_in_any_column_is_reported(Q):
+    """A NaN in the second right-hand side (not only the first) ends the
+    solve with CQR_ERR_NONFINITE instead of CQR_OK."""
+    from candmc_amd._lib import CQRError
+    m, n = 600, 40
+    A = O.gaussian(m, n, 51)
+    B = O.gaussian(m, 2, 52)
+    B[7, 1] = np.nan
+    with pytest.raises(CQRError) as e:
+        Q.lstsq(Q.to_gpu(A), m, m, n, Q.to_gpu(np.asfortranarray(B)), m, 2, b=128, max_iter=4, tol=1e-12)
+    assert e.value.status == 5

@@ -156,7 +156,8 @@ def test_signed_nlu(Q, b):
 
 
 # ---------------------------------------------------------------- a6/a7 hh_recon
-@pytest.mark.parametrize("m,b,P,kind", [(136, 17, 4, "golden"), (8192, 64, 4, "drand"), (32768, 128, 0, "gauss"),
+@pytest.mark.parametrize("m,b,P,kind", [(136, 17, 4, "golden"), (8192, 64, 4, "drand"), (8192, 64, 4, "gauss"),
+                                        (32768, 128, 0, "gauss"),
                                         (4096, 128, 0, "gauss"), (2000, 40, 3, "gauss"), (128, 128, 0, "gauss"),
                                         (16384, 16, 0, "gauss"), (777, 31, 2, "gauss")])
 def test_hh_recon_qr(Q, m, b, P, kind):
@@ -277,25 +278,20 @@ def test_qr_2d_lookahead_on_and_off(Q, m, k, b):
     assert rel(out[0], out[1]) < 1e-12
 
 
-@pytest.mark.parametrize("m,k,b", [(2048, 1024, 128), (1500, 900, 64)])
-def test_qr_2d_update_through_blas_or_own_kernel(Q, m, k, b):
-    """The bulk of the trailing update through rocBLAS dgemm or through the
-    library's MFMA kernel: the same factorization up to round-off."""
-    A = O.gaussian(m, k, 3 * m + k)
-    want = O.QR_2D(A, b)
-    ctx = Q.context()
-    out = []
-    for on in (False, True):
-        ctx.set_blas_update(on)
-        try:
-            dA = Q.to_gpu(A)
-            Q.QR_2D(dA, m, m, k, b)
-            out.append(Q.from_gpu(dA, m, k))
-        finally:
-            ctx.set_blas_update(True)
-    for YR in out:
-        assert rel(YR, want) < 1e-11
-    assert rel(out[0], out[1]) < 1e-12
+@pytest.mark.parametrize("m,n,k", [(4099, 700, 128), (32768, 520, 128), (777, 333, 64), (300, 129, 100),
+                                   (8193, 4000, 128)])
+def test_rank_b_update_kernel(Q, m, n, k):
+    """upd_A with T⁻ᵀ = I (so Tᵀ = I): C −= Y·(Yᵀ·C), on full and ragged
+    tiles of the rank-b update kernel, against an FP64 product on the host.
+    (The kernel's overwrite form, Y = Q1·U⁻¹ and Y·Tᵀ, runs in every panel
+    of the hh_recon_qr and QR_2D tests.)"""
+    rng = np.random.default_rng(m + n + k)
+    Y = np.asfortranarray(rng.standard_normal((m, k)) / np.sqrt(m))
+    C = np.asfortranarray(rng.standard_normal((m, n)))
+    dC = Q.to_gpu(C)
+    Q.upd_A(Q.to_gpu(Y), m, dC, m, m, n, k, Q.to_gpu(np.eye(k)), W_is_T=True)
+    want = C - Y @ (Y.T @ C)
+    assert rel(Q.from_gpu(dC, m, n), want) < 1e-12
 
 
 def test_qr_2d_reference_default_input(Q):

@@ -247,3 +247,76 @@ def test_tree_entry_points_reject_bad_arguments(Q):
     with pytest.raises(CQRError) as e:
         Q.QR_tree_2D(dA, 400, 400, 300, 129, numPes=P)
     assert e.value.status == 2
+
+
+# ---------------------------------------------------------------- no tree_data
+@pytest.mark.parametrize("m,b,P,output_Y", [(8192, 64, 4, 1), (8192, 64, 4, 0), (3000, 20, 3, 1), (1024, 128, 1, 1),
+                                            (1024, 128, 1, 0), (640, 64, 5, 1)])
+def test_bitree_tsqr_without_tree_data(Q, m, b, P, output_Y):
+    """bitree_tsqr(..., tree_data = NULL), the call of test_bitree_tsqr.cxx:319:
+    per-rank dgeqrf blocks, with output_Y the senders' top triangles hold the
+    nodes' V2 and rank 0's the final R; tau is rank 0's (b doubles)."""
+    print(f"bitree no tree_data m={m} b={b} P={P} output_Y={output_Y}", flush=True)
+    A = O.gaussian(m, b, 900 + m + b + P)
+    ranges = O.split_rows(m, b, P)
+    want, Rw, tauw = O.bitree_tsqr_no_tree_data([A[s:e].copy(order="F") for s, e in ranges], b, output_Y)
+    dA = Q.to_gpu(A)
+    dR = torch.full((b * b,), 3.0, dtype=torch.float64, device="cuda")
+    dtau = torch.full((b + 1,), 5.0, dtype=torch.float64, device="cuda")
+    Q.bitree_tsqr(dA, m, dR, dtau, m, b, numPes=P, output_Y=output_Y)
+    got = Q.from_gpu(dA, m, b)
+    R = Q.from_gpu(dR, b, b)
+    sg = np.sign(np.diag(R) / np.diag(Rw))
+    assert np.all(sg == 1.0)                      # same dlarfg sign rule at every node
+    assert rel(R, Rw) < 1e-12 and np.all(np.tril(R, -1) == 0.0)
+    assert rel(dtau.cpu().numpy()[:b], tauw) < 1e-12 and float(dtau[b]) == 5.0
+    for (s, e), W in zip(ranges, want):
+        assert rel(got[s:e], W) < 1e-11, (s, e)
+
+
+@pytest.mark.parametrize("m,b,k,use_id", [(1000, 40, 1000, False), (777, 31, 31, False), (512, 64, 64, True)])
+def test_one_rank_construct_Q1_and_apply(Q, m, b, k, use_id):
+    """One rank (construct_tsqr_Q.cxx:87-102, apply_tsqr_QT.cxx:73-80):
+    Q1 from [V\\R] and tau alone, as LAPACK's dorgqr / dormqr('L','N') of
+    [ID; 0]; apply_tsqr_QT as dormqr('L','T')."""
+    from scipy.linalg import lapack as lp
+    print(f"one-rank Q m={m} b={b} k={k} ID={use_id}", flush=True)
+    A = O.gaussian(m, b, 70 + m)
+    dA = Q.to_gpu(A)
+    dR = torch.zeros(b * b, dtype=torch.float64, device="cuda")
+    dtau = torch.zeros(b, dtype=torch.float64, device="cuda")
+    Q.bitree_tsqr(dA, m, dR, dtau, m, b, numPes=1, output_Y=1)
+    YR, tau = Q.from_gpu(dA, m, b), dtau.cpu().numpy()
+    qr, tl = O.local_tsqr(A)
+    assert rel(YR, qr) < 1e-12 and rel(tau, tl) < 1e-12
+    dQ1 = torch.zeros(m * k, dtype=torch.float64, device="cuda")
+    if use_id:
+        ID = O.gaussian(b, k, 5)
+        Q.construct_Q1(dA, m, dtau, dQ1, m, m, b, k, numPes=1, ID=Q.to_gpu(ID))
+        C = np.zeros((m, k), order="F")
+        C[:b] = ID
+        want, _, info = lp.dormqr("L", "N", qr, tl, C, max(1, k) * 64)
+    else:
+        Q.construct_Q1(dA, m, dtau, dQ1, m, m, b, k, numPes=1)
+        Qf, _, info = lp.dorgqr(np.asfortranarray(np.hstack([qr, np.zeros((m, k - b))])), tl)
+        want = Qf[:, :k]
+    assert info == 0
+    assert rel(Q.from_gpu(dQ1, m, k), want) < 1e-12
+    B = O.gaussian(m, 9, 6)
+    dB = Q.to_gpu(B)
+    Q.apply_tsqr_QT(dA, m, dtau, dB, m, m, b, 9, numPes=1)
+    want, _, info = lp.dormqr("L", "T", qr, tl, np.asfortranarray(B), 9 * 64)
+    assert rel(Q.from_gpu(dB, m, 9), want) < 1e-12
+
+
+def test_compute_invT_from_Y_two_Y(Q):
+    """qr_2d.cxx:76-106: T (b1×b2, ld lda_T) := Y1ᵀ·Y2; rows past b1 untouched."""
+    mb, b1, b2, ldt = 3001, 70, 130, 80
+    Y1, Y2 = O.gaussian(mb, b1, 1), O.gaussian(mb, b2, 2)
+    dT = torch.full((ldt * b2,), -1.0, dtype=torch.float64, device="cuda")
+    Q.compute_invT_from_Y2(Q.to_gpu(Y1), mb, Q.to_gpu(Y2), mb, dT, ldt, mb, b1, b2)
+    T = dT.cpu().numpy().reshape(b2, ldt).T
+    assert rel(T[:b1], Y1.T @ Y2) < 1e-13
+    assert np.all(T[b1:] == -1.0)
+    Q.compute_invT_from_Y2(Q.to_gpu(Y1), mb, Q.to_gpu(Y2), mb, dT, ldt, 0, b1, b2)
+    assert np.all(dT.cpu().numpy().reshape(b2, ldt).T[:b1] == 0.0)

@@ -47,3 +47,14 @@ def test_package_never_imports_the_oracle():
             if f.endswith((".py", ".cpp", ".hip", ".h")):
                 src = open(os.path.join(d, f), encoding="utf-8").read()
                 assert not pat.search(src), os.path.join(d, f)
+
+
+def test_library_links_no_vendor_blas():
+    """Every product of the path runs on the library's own MFMA kernels:
+    the shared library depends on no BLAS (rocBLAS, hipBLAS, hipBLASLt)."""
+    import subprocess
+    from candmc_amd import _lib
+    out = subprocess.run(["readelf", "-d", _lib.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    needed = [l for l in out.splitlines() if "NEEDED" in l]
+    assert needed
+    assert not any(("blas" in l.lower()) for l in needed), needed

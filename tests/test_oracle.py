@@ -157,6 +157,49 @@ def test_bitree_R_and_construct_Q1(m, b, P):
     assert np.abs(Q1 - Ql * sg[None, :]).max() < 1e-12
 
 
+@pytest.mark.parametrize("m,b,P", [(256, 16, 4), (300, 12, 3), (96, 16, 1)])
+def test_bitree_no_tree_data_rebuilds_A(m, b, P):
+    """The no-tree_data branch leaves enough in A to rebuild it: the leaf
+    reflectors below each rank's diagonal, the node reflectors [e_i; V2(:,i)]
+    in the senders' top triangles, R in rank 0's.  Q·[R; 0] = A by brute
+    force (explicit reflector products), R is LAPACK's up to row signs, and
+    each V2 is dtpqrt's for the same node (same dlarfg sign rule)."""
+    rng = np.random.default_rng(m + P)
+    A = rng.standard_normal((m, b))
+    ranges = O.split_rows(m, b, P)
+    blocks = [A[s:e].copy(order="F") for s, e in ranges]
+    out, R, tau0 = O.bitree_tsqr_no_tree_data(blocks, b, 1)
+    X = np.zeros((m, b))
+    X[:b] = R
+    def H(rows, v):                       # I − τ·u·uᵀ on the given rows
+        tau = 2.0 / (v @ v)
+        E = np.eye(m)
+        E[np.ix_(rows, rows)] -= tau * np.outer(v, v)
+        return E
+    Qm = np.eye(m)
+    for r, (s0, e0) in enumerate(ranges):        # leaves: H_0 ⋯ H_{b−1}
+        for i in range(b):
+            v = np.concatenate([[1.0], out[r][i + 1:, i]])
+            Qm = Qm @ H(list(range(s0 + i, e0)), v)
+    node_q = np.eye(m)
+    for n in O.bitree_levels(P):                 # levels forward: Q_tree = Q_1 Q_2 ⋯
+        for z, snd in O.bitree_pairs(n):
+            for i in range(b):
+                v = np.concatenate([[1.0], out[snd][:i + 1, i]])
+                rows = [ranges[z][0] + i] + [ranges[snd][0] + q for q in range(i + 1)]
+                node_q = node_q @ H(rows, v)
+    assert np.abs(Qm @ node_q @ X - A).max() < 1e-13
+    qr, t = O.local_tsqr(A)
+    Rl = np.triu(qr[:b])
+    sg = np.sign(np.diag(R) / np.diag(Rl))
+    assert np.abs(R - sg[:, None] * Rl).max() < 1e-13
+    assert np.array_equal(tau0, O.local_tsqr(blocks[0])[1])
+    if P > 1:
+        _, _, _, td = O.bitree_tsqr(blocks, b)
+        z, snd = O.bitree_pairs(P)[0]
+        assert np.abs(np.triu(out[snd][:b, :b]) - td[z][0][1]).max() < 1e-13
+
+
 def _brute_signed_lu(M):
     b = M.shape[0]
     A = [[float(M[i, j]) for j in range(b)] for i in range(b)]
