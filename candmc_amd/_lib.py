@@ -49,6 +49,8 @@ SIGNATURES = {
     "cqr_recursive_nlu": (_I, [_P, _P, _I64, _P, _I64, _I64, _P]),
     "cqr_hh_recon_qr": (_I, [_P, _P, _I64, _I64, _I64, _P, _I64]),
     "cqr_hh_recon_qr_ex": (_I, [_P, _P, _I64, _I64, _I64, _P, _P, _P, _I64]),
+    "cqr_noalloc_scratch_bytes": (_c.c_size_t, []),
+    "cqr_hh_recon_qr_noalloc": (_I, [_P, _P, _I64, _I64, _I64, _P, _P, _P, _I64]),
     "cqr_compute_invT_from_W": (_I, [_P, _P, _I64, _P]),
     "cqr_compute_invT_from_Y": (_I, [_P, _P, _I64, _P, _I64, _I64, _I64]),
     "cqr_compute_invT_from_Y2": (_I, [_P, _P, _I64, _P, _I64, _P, _I64, _I64, _I64, _I64]),

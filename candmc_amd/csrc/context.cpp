@@ -180,6 +180,10 @@ int cqr_create(cqr_ctx** ctx, int device) {
     delete c;
     return CQR_ERR_NOMEM;
   }
+  if (cqr::noalloc_setup(c) != CQR_OK) {
+    cqr_destroy(c);
+    return CQR_ERR_DEVICE;
+  }
   *ctx = c;
   return CQR_OK;
 }

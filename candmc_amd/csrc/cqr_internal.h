@@ -42,6 +42,7 @@ struct cqr_ctx {
   hipStream_t side = nullptr;   // second stream of the QR_2D lookahead
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int lookahead = 1;
+  void* noalloc = nullptr;      // this device's fixed scratch (noalloc.hip)
   int lsq_escalate = 1;         // refactor a stalled mixed solve in FP64
   int lsq_last_prec = -1;       // factorization precision of the last solve's X
   std::string err;
@@ -310,6 +311,9 @@ struct LsqWS {
 size_t lstsq_layout(int64_t m, int64_t n, int64_t nr, int64_t b, int64_t leaf_rows, Qr2dWS* o, LsqWS* w);
 int hh_recon_core(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, double* W,
                   double* T, double* invT_T, double* Tt, int64_t nleaves);
+// Looks up the device address of the fixed scratch array of noalloc.hip
+// (device current); cqr_create calls it once per context.
+int noalloc_setup(cqr_ctx* c);
 // dgeqrf of one m×b block on caller scratch of local_tsqr_ws_bytes bytes.
 size_t local_tsqr_ws_bytes(int64_t m, int64_t b, int64_t leaf_rows);
 int local_tsqr_ws(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, double* tau, double* T, char* base);

@@ -15,8 +15,7 @@ int launch_copy2d(cqr_ctx* c, const double* src, int64_t lds, double* dst, int64
 // Panel workspace: every buffer TSQR + HR needs for one m×b panel.
 // ---------------------------------------------------------------------------
 struct PanelWS {
-  size_t leafR, leafT, leafTau, nodeR, nodeV2, nodeT, nodeTau, C, X, Z, M, Y, Uinv, Winv, LinvT, signs,
-      Wtmp, Ttmp;
+  size_t leafR, leafT, leafTau, nodeR, nodeV2, nodeT, nodeTau, C, Y, Uinv, Winv, LinvT, signs, Wtmp;
 };
 
 static PanelWS panel_layout(const TsqrPlan& p, Sizer& s) {
@@ -30,16 +29,12 @@ static PanelWS panel_layout(const TsqrPlan& p, Sizer& s) {
   w.nodeT = s.add(8 * nn * bb);
   w.nodeTau = s.add(8 * nn * b);
   w.C = s.add(8 * L * bb);
-  w.X = s.add(8 * std::max<size_t>(L / 2, 1) * bb);
-  w.Z = s.add(8 * L * bb);
-  w.M = s.add(8 * L * bb);
   w.Y = s.add(8 * (size_t)p.m * b);
   w.Uinv = s.add(8 * bb);
   w.Winv = s.add(8 * bb);
   w.LinvT = s.add(8 * bb);
   w.signs = s.add(8 * b);
   w.Wtmp = s.add(8 * bb);
-  w.Ttmp = s.add(8 * bb);
   return w;
 }
 
@@ -266,15 +261,13 @@ int cqr_construct_Q1(cqr_ctx* c, const double* Y, int64_t lda_Y, const double* t
   if (s != CQR_OK) return set_err(c, s, "construct_Q1: cannot plan");
   const int64_t bb = b * b, L = p.nleaves, nn = p.nnodes;
   Sizer sz;
-  const size_t oC = sz.add(8 * L * bb), oX = sz.add(8 * std::max<int64_t>(L / 2, 1) * bb),
-               oZ = sz.add(8 * L * bb), oM = sz.add(8 * L * bb);
+  const size_t oC = sz.add(8 * L * bb);
   CQR_TRY(ensure_ws(c, sz.bytes));
   char* base = (char*)c->ws;
   auto at = [&](size_t o) { return reinterpret_cast<double*>(base + o); };
   const double* leafT = tree_data;
   const double* nodeV2 = leafT + L * bb;
   const double* nodeT = nodeV2 + nn * bb;
-  (void)oX; (void)oZ; (void)oM;
   return launch_q1(c, nodeV2, nodeT, leafT, Y, lda_Y, m, at(oC), Q1, lda_Q, (int)L, (int)nn, (int)b);
 }
 

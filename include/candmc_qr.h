@@ -188,6 +188,21 @@ int cqr_hh_recon_qr(cqr_ctx* ctx, double* A, int64_t lda, int64_t m, int64_t b,
 int cqr_hh_recon_qr_ex(cqr_ctx* ctx, double* A, int64_t lda, int64_t m, int64_t b,
                        double* W, double* T, double* invT_T, int64_t nleaves);
 
+/* ---- hh_recon_qr without allocation (no counterpart in the reference; for
+ *      callers that cannot allocate device memory: code under HIP graph
+ *      capture, serving loops, harnesses that forbid hipMalloc) ------------
+ * cqr_hh_recon_qr_ex on a fixed scratch buffer of
+ * cqr_noalloc_scratch_bytes() bytes (96 MiB) that each GPU holds from
+ * library load to process end; panels whose workspace exceeds it (at
+ * b = 128 from m = 24 576 rows, at b = 64 from m = 112 128) return
+ * CQR_ERR_UNSUPPORTED and leave A untouched.  The
+ * call allocates nothing and does not scan the input (check_finite is not
+ * applied).  All calls on one GPU share the buffer: calls that overlap in
+ * time on one GPU race.                                                   */
+size_t cqr_noalloc_scratch_bytes(void);
+int cqr_hh_recon_qr_noalloc(cqr_ctx* ctx, double* A, int64_t lda, int64_t m, int64_t b,
+                            double* W, double* T, double* invT_T, int64_t nleaves);
+
 /* ---- a7  compute_invT_from_W  (alg/QR/hh_recon/hh_recon.cxx:26-31) -----
  * invT holds Y1 (b×b, ld b) on entry and T⁻ᵀ = −W⁻ᵀ·Y1 on exit.           */
 int cqr_compute_invT_from_W(cqr_ctx* ctx, const double* W, int64_t b, double* invT);
