@@ -54,6 +54,11 @@ namespace cqr {
 // shape 0: 128×128 tiles, 1: 256×128, 2: 128×256.
 int update_gemm(cqr_ctx* c, int64_t M, int64_t N, int64_t K, double alpha, const double* P, int64_t ldp,
                 const double* Z, int64_t ldz, double beta, double* C, int64_t ldc, int kind, int shape);
+// P·Z with Z = [Z1 | Z2] split at column n1: C1 := P·Z1 (and a copy in C1b
+// when non-null), C2 := P·Z2, in one pass over P.
+int update_gemm_split(cqr_ctx* c, int64_t M, int64_t N, int64_t n1, int64_t K, const double* P, int64_t ldp,
+                      const double* Z, int64_t ldz, double* C1, int64_t ldc1, double* C1b, int64_t ldc1b, double* C2,
+                      int64_t ldc2, int kind);
 
 // ---------------------------------------------------------------------------
 // Errors
@@ -290,7 +295,8 @@ int launch_fill2d(cqr_ctx* c, double* dst, int64_t ldd, int64_t m, int64_t n, do
 // Drivers (qr_driver.cpp)
 size_t panel_ws_bytes(int64_t m, int64_t b, int64_t nleaves, int64_t leaf_rows);
 int hh_recon_panel(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, double* W, double* T,
-                   double* invT_T, double* Tt, int64_t nleaves, char* base, int64_t* signs_out);
+                   double* invT_T, double* Tt, int64_t nleaves, char* base, int64_t* signs_out,
+                   double* Ye = nullptr, double* YT = nullptr);
 #define CQR_CHECK_CTX(c)                     \
   do {                                       \
     if (!(c)) return CQR_ERR_INVALID;        \

@@ -33,6 +33,13 @@ struct UpdArgs {
   int64_t ldc;
   double alpha;        // C := beta·C + alpha·P·Z (alpha = −1 for the update)
   int beta1;           // 1: C is read (beta = 1); 0: C is overwritten (beta = 0)
+  // with beta = 0 only: columns [n1, N) go to C3 (column gn − n1, ld ldc3)
+  // instead of C, and columns [0, n1) also to C2 (ld ldc2) when non-null
+  int64_t n1;
+  double* C2;
+  int64_t ldc2;
+  double* C3;
+  int64_t ldc3;
   int vec;             // 16-byte loads allowed (even lds, aligned bases)
   int ntm, ntn;        // tile grid
   int per_xcd;         // items per XCD
@@ -165,7 +172,15 @@ __device__ __forceinline__ void upd_tile(const UpdArgs& g, int64_t m0, int64_t n
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int64_t gn = n0 + wn0 + 16 * j + fr + 4 * v;
-        if (full || (gm < g.M && gn < g.N)) g.C[gm + gn * g.ldc] = acc[i][j][v];
+        if (full || (gm < g.M && gn < g.N)) {
+          const double val = acc[i][j][v];
+          if (gn < g.n1) {
+            g.C[gm + gn * g.ldc] = val;
+            if (g.C2) g.C2[gm + gn * g.ldc2] = val;
+          } else {
+            g.C3[gm + (gn - g.n1) * g.ldc3] = val;
+          }
+        }
       }
   }
 }
@@ -185,6 +200,24 @@ __global__ void __launch_bounds__(64 * WM * WN, (WM * WN <= 4) ? 2 : 1) upd_gemm
   else upd_tile<WM, WN, false>(g, m0, n0, Ps, Zs);
 }
 
+static int launch_update(cqr_ctx* c, UpdArgs& g, int kind, int shape) {
+  g.vec = (g.ldp % 2 == 0) && (g.ldz % 2 == 0) && ((uintptr_t)g.P % 16 == 0) && ((uintptr_t)g.Z % 16 == 0);
+  const int bm = shape == 1 ? 256 : 128, bn = shape == 2 ? 256 : 128;
+  g.ntm = (int)((g.M + bm - 1) / bm);
+  g.ntn = (int)((g.N + bn - 1) / bn);
+  const int64_t tiles = (int64_t)g.ntm * g.ntn;
+  if (tiles >= (int64_t)1 << 30) return set_err(c, CQR_ERR_UNSUPPORTED, "update_gemm: too many tiles");
+  g.per_xcd = (int)((tiles + 7) / 8);
+  const dim3 grid((unsigned)(8 * g.per_xcd));
+  hipEvent_t ev = prof_begin(c);
+  if (shape == 1) hipLaunchKernelGGL((upd_gemm_f64<4, 2>), grid, dim3(512), 0, c->stream, g);
+  else if (shape == 2) hipLaunchKernelGGL((upd_gemm_f64<2, 4>), grid, dim3(512), 0, c->stream, g);
+  else hipLaunchKernelGGL((upd_gemm_f64<2, 2>), grid, dim3(256), 0, c->stream, g);
+  const double outs = (g.beta1 ? 2.0 : 1.0) + (g.C2 ? (double)g.n1 / g.N : 0.0);
+  prof_end(c, ev, kind, 2.0 * g.M * g.N * g.K, 8.0 * ((double)g.M * g.K + (double)g.K * g.N + outs * g.M * g.N));
+  return check_launch(c, "upd_gemm_f64");
+}
+
 // C (M×N) := beta·C + alpha·P·Z with K ≤ 256 and beta ∈ {0, 1} (alpha = −1,
 // beta = 1: the trailing update).  shape: 0 = 128×128 tiles (4 waves, two
 // workgroups per CU), 1 = 256×128 (8 waves), 2 = 128×256 (8 waves).
@@ -196,20 +229,26 @@ int update_gemm(cqr_ctx* c, int64_t M, int64_t N, int64_t K, double alpha, const
   UpdArgs g{};
   g.M = M; g.N = N; g.K = K; g.P = P; g.ldp = ldp; g.Z = Z; g.ldz = ldz; g.C = C; g.ldc = ldc; g.alpha = alpha;
   g.beta1 = beta != 0.0;
-  g.vec = (ldp % 2 == 0) && (ldz % 2 == 0) && ((uintptr_t)P % 16 == 0) && ((uintptr_t)Z % 16 == 0);
-  const int bm = shape == 1 ? 256 : 128, bn = shape == 2 ? 256 : 128;
-  g.ntm = (int)((M + bm - 1) / bm);
-  g.ntn = (int)((N + bn - 1) / bn);
-  const int64_t tiles = (int64_t)g.ntm * g.ntn;
-  if (tiles >= (int64_t)1 << 30) return set_err(c, CQR_ERR_UNSUPPORTED, "update_gemm: too many tiles");
-  g.per_xcd = (int)((tiles + 7) / 8);
-  const dim3 grid((unsigned)(8 * g.per_xcd));
-  hipEvent_t ev = prof_begin(c);
-  if (shape == 1) hipLaunchKernelGGL((upd_gemm_f64<4, 2>), grid, dim3(512), 0, c->stream, g);
-  else if (shape == 2) hipLaunchKernelGGL((upd_gemm_f64<2, 4>), grid, dim3(512), 0, c->stream, g);
-  else hipLaunchKernelGGL((upd_gemm_f64<2, 2>), grid, dim3(256), 0, c->stream, g);
-  prof_end(c, ev, kind, 2.0 * M * N * K, 8.0 * ((double)M * K + (double)K * N + (g.beta1 ? 2.0 : 1.0) * (double)M * N));
-  return check_launch(c, "upd_gemm_f64");
+  g.n1 = N;
+  return launch_update(c, g, kind, shape);
+}
+
+// P (M×K) times [Z1 | Z2] (K×N, ld ldz, Z2 = its columns [n1, N)) into
+// three destinations at once: C1 := P·Z1 (ld ldc1) and, if C1b is
+// non-null, a copy in C1b (ld ldc1b); C2 := P·Z2 (ld ldc2).  One pass over
+// P for two products that share it (Y = Q1·U⁻¹ with the explicit Y, and
+// Y·Tᵀ = Q1·(U⁻¹·Tᵀ)).
+int update_gemm_split(cqr_ctx* c, int64_t M, int64_t N, int64_t n1, int64_t K, const double* P, int64_t ldp,
+                      const double* Z, int64_t ldz, double* C1, int64_t ldc1, double* C1b, int64_t ldc1b, double* C2,
+                      int64_t ldc2, int kind) {
+  if (M <= 0 || N <= 0) return CQR_OK;
+  if (K > 256 || K < 1 || n1 < 0 || n1 > N)
+    return set_err(c, CQR_ERR_UNSUPPORTED, "update_gemm_split: K = %lld, n1 = %lld", (long long)K, (long long)n1);
+  UpdArgs g{};
+  g.M = M; g.N = N; g.K = K; g.P = P; g.ldp = ldp; g.Z = Z; g.ldz = ldz; g.C = C1; g.ldc = ldc1; g.alpha = 1.0;
+  g.beta1 = 0;
+  g.n1 = n1; g.C2 = C1b; g.ldc2 = ldc1b; g.C3 = C2; g.ldc3 = ldc2;
+  return launch_update(c, g, kind, 0);
 }
 
 }  // namespace cqr

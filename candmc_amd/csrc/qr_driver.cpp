@@ -30,7 +30,7 @@ static PanelWS panel_layout(const TsqrPlan& p, Sizer& s) {
   w.nodeTau = s.add(8 * nn * b);
   w.C = s.add(8 * L * bb);
   w.Y = s.add(8 * (size_t)p.m * b);
-  w.Uinv = s.add(8 * bb);
+  w.Uinv = s.add(8 * 2 * bb);   // [U⁻¹ | U⁻¹·Tᵀ], b×2b
   w.Winv = s.add(8 * bb);
   w.LinvT = s.add(8 * bb);
   w.signs = s.add(8 * b);
@@ -54,8 +54,12 @@ size_t panel_ws_bytes(int64_t m, int64_t b, int64_t nleaves, int64_t leaf_rows) 
 
 // TSQR + Householder reconstruction of one panel (hh_recon.cxx:129-321).
 // Outputs: A := [Y\R], W (b×b), and optionally T, T⁻ᵀ and Tᵀ (all ld b).
+// With Tt, Ye and YT all given (m > b), also the explicit unit-lower Y in Ye
+// and Y·Tᵀ in YT (both m×b, ld m), the lower parts from the same pass over
+// Q1 that forms Y: [Y | Y·Tᵀ] = Q1·[U⁻¹ | U⁻¹·Tᵀ].
 int hh_recon_panel(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, double* W, double* T,
-                   double* invT_T, double* Tt, int64_t nleaves, char* base, int64_t* signs_out) {
+                   double* invT_T, double* Tt, int64_t nleaves, char* base, int64_t* signs_out, double* Ye,
+                   double* YT) {
   TsqrPlan p;
   int s = make_plan(m, b, nleaves, c->leaf_rows, p);
   if (s != CQR_OK) return set_err(c, s, "hh_recon_qr: cannot plan m=%lld b=%lld", (long long)m, (long long)b);
@@ -80,13 +84,22 @@ int hh_recon_panel(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, dou
   double* Wd = W ? W : at(w.Wtmp);
   int64_t* sg = signs_out ? signs_out : reinterpret_cast<int64_t*>(base + w.signs);
   CQR_TRY(hr_core(c, Y, m, Rroot, A, lda, Wd, at(w.Uinv), at(w.Winv), at(w.LinvT), sg, bi));
-  // Y = Q1·U⁻¹ below the top b rows (hh_recon.cxx:294-301)
-  // (U⁻¹ is stored with explicit zeros below its diagonal: a plain GEMM)
-  if (m > b) CQR_TRY(update_gemm(c, m - b, b, b, 1.0, Y + b, m, at(w.Uinv), b, 0.0, A + b, lda, PK_GEMM_SMALL, 0));
   // T = −W·L⁻ᵀ, Tᵀ = −L⁻¹·Wᵀ, T⁻ᵀ = −W⁻ᵀ·L (W = −T·Y1ᵀ)
   if (Tt)
     CQR_TRY(gemm_f64_ws(c, 1, 1, b, b, b, -1.0, at(w.LinvT), b, SM_UNIT_UPPER, Wd, b, SM_UPPER, 0.0, Tt, b,
                         nullptr));
+  // Y = Q1·U⁻¹ below the top b rows (hh_recon.cxx:294-301)
+  // (U⁻¹ is stored with explicit zeros below its diagonal: a plain GEMM)
+  double* Uinv = at(w.Uinv);
+  if (m > b && Tt && Ye && YT) {
+    CQR_TRY(gemm_f64_ws(c, 0, 0, b, b, b, 1.0, Uinv, b, SM_NONE, Tt, b, SM_NONE, 0.0, Uinv + bb, b, nullptr));
+    CQR_TRY(update_gemm_split(c, m - b, 2 * b, b, b, Y + b, m, Uinv, b, A + b, lda, Ye + b, m, YT + b, m,
+                              PK_GEMM_SMALL));
+    CQR_TRY(launch_copy_unit_lower(c, A, lda, Ye, m, b, b));
+    CQR_TRY(gemm_f64_ws(c, 0, 0, b, b, b, 1.0, A, lda, SM_UNIT_LOWER, Tt, b, SM_NONE, 0.0, YT, m, nullptr));
+  } else if (m > b) {
+    CQR_TRY(update_gemm(c, m - b, b, b, 1.0, Y + b, m, Uinv, b, 0.0, A + b, lda, PK_GEMM_SMALL, 0));
+  }
   if (T)
     CQR_TRY(gemm_f64_ws(c, 0, 0, b, b, b, -1.0, Wd, b, SM_UPPER, at(w.LinvT), b, SM_UNIT_UPPER, 0.0, T, b,
                         nullptr));
@@ -445,15 +458,16 @@ int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b
     const int64_t mj = m - j, kj = k - j, bj = std::min(kj, b);
     double* P = A + j + j * lda;
     double* Tt = Tt_out ? Tt_out + p * b * b : Tt2[p & 1];
-    if ((rc = hh_recon_panel(c, P, lda, mj, bj, W, nullptr, nullptr, Tt, 0, base + o.panel, nullptr))) break;
-    if (aggreg_Y && (rc = launch_copy_unit_lower(c, P, lda, aggreg_Y + j + j * lda_aY, lda_aY, mj, bj))) break;
-    if (!(kj - b > 0 && mj - b > 0)) break;
+    // the explicit Y and Y·Tt come out of the panel's own Y product
+    // whenever a trailing update follows
+    const bool more = kj - b > 0 && mj - b > 0;
     double* Ye = Ye2[p & 1];
     double* YT = Ye + mj * bj;
-    if (expl) {
-      if ((rc = launch_copy_unit_lower(c, P, lda, Ye, mj, mj, bj))) break;
-      if ((rc = update_gemm(c, mj, bj, bj, 1.0, Ye, mj, Tt, bj, 0.0, YT, mj, PK_GEMM_SMALL, 0))) break;
-    }
+    if ((rc = hh_recon_panel(c, P, lda, mj, bj, W, nullptr, nullptr, Tt, 0, base + o.panel, nullptr,
+                             more && expl ? Ye : nullptr, more && expl ? YT : nullptr)))
+      break;
+    if (aggreg_Y && (rc = launch_copy_unit_lower(c, P, lda, aggreg_Y + j + j * lda_aY, lda_aY, mj, bj))) break;
+    if (!more) break;
     auto update = [&](int64_t col0, int64_t ncols, double* ws) -> int {
       if (expl) return wy_update_explicit(c, Ye, YT, mj, ncols, (int)bj, P + col0 * lda, lda, ws, prec);
       return wy_update_ws(c, P, lda, 1, Tt, 0, P + col0 * lda, lda, mj, ncols, (int)bj, ws, prec);

@@ -194,7 +194,7 @@ int cqr_hh_recon_qr_ex(cqr_ctx* ctx, double* A, int64_t lda, int64_t m, int64_t 
  * cqr_hh_recon_qr_ex on a fixed scratch buffer of
  * cqr_noalloc_scratch_bytes() bytes (96 MiB) that each GPU holds from
  * library load to process end; panels whose workspace exceeds it (at
- * b = 128 from m = 24 576 rows, at b = 64 from m = 112 128) return
+ * b = 128 from m = 24 321 rows, at b = 64 from m = 111 947) return
  * CQR_ERR_UNSUPPORTED and leave A untouched.  The
  * call allocates nothing and does not scan the input (check_finite is not
  * applied).  All calls on one GPU share the buffer: calls that overlap in
