@@ -215,8 +215,12 @@ int leaf_qr(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int b, double*
 int tree_level(cqr_ctx* c, double* Rleaf, int n, int node0, int b, double* V2, double* T,
                double* tau);
 int bgemm(cqr_ctx* c, const BgemmArgs& g, int batch, int max_m);
+// With Tt (b = 128 only): also Tᵀ, and U⁻¹·Tᵀ into M2 when non-null, and the
+// top b rows of the explicit Y and of Y·Tᵀ into Ye / YT (ld ldy) when non-null.
 int hr_core(cqr_ctx* c, const double* Q1top, int64_t ldq, const double* Rroot, double* Atop,
-            int64_t lda, double* W, double* Uinv, double* Winv, double* LinvT, int64_t* signs, int b);
+            int64_t lda, double* W, double* Uinv, double* Winv, double* LinvT, int64_t* signs, int b,
+            double* Tt = nullptr, double* M2 = nullptr, double* Ye = nullptr, double* YT = nullptr,
+            int64_t ldy = 0);
 int launch_diag(cqr_ctx* c, const double* T, int64_t ldt, double* tau, int b);
 int launch_signed_nlu(cqr_ctx* c, double* A, int64_t lda, double* R, int64_t lda_R,
                       int64_t* signs, int b);

@@ -83,20 +83,29 @@ int hh_recon_panel(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, dou
   // K4: LU(Q1_top − S), R := S·R, W = U·S (hh_recon.cxx:282-308)
   double* Wd = W ? W : at(w.Wtmp);
   int64_t* sg = signs_out ? signs_out : reinterpret_cast<int64_t*>(base + w.signs);
-  CQR_TRY(hr_core(c, Y, m, Rroot, A, lda, Wd, at(w.Uinv), at(w.Winv), at(w.LinvT), sg, bi));
+  double* Uinv = at(w.Uinv);
+  const bool expl = m > b && Tt && Ye && YT;
+  // at b = 128 the HR core also forms Tᵀ (and, for the explicit Y, U⁻¹·Tᵀ
+  // and the top b rows of Y and Y·Tᵀ) in the same launch
+  const bool fused = Tt && b == 128;
+  CQR_TRY(hr_core(c, Y, m, Rroot, A, lda, Wd, Uinv, at(w.Winv), at(w.LinvT), sg, bi, fused ? Tt : nullptr,
+                  fused && expl ? Uinv + bb : nullptr, fused && expl ? Ye : nullptr, fused && expl ? YT : nullptr,
+                  m));
   // T = −W·L⁻ᵀ, Tᵀ = −L⁻¹·Wᵀ, T⁻ᵀ = −W⁻ᵀ·L (W = −T·Y1ᵀ)
-  if (Tt)
+  if (Tt && !fused)
     CQR_TRY(gemm_f64_ws(c, 1, 1, b, b, b, -1.0, at(w.LinvT), b, SM_UNIT_UPPER, Wd, b, SM_UPPER, 0.0, Tt, b,
                         nullptr));
   // Y = Q1·U⁻¹ below the top b rows (hh_recon.cxx:294-301)
   // (U⁻¹ is stored with explicit zeros below its diagonal: a plain GEMM)
-  double* Uinv = at(w.Uinv);
-  if (m > b && Tt && Ye && YT) {
-    CQR_TRY(gemm_f64_ws(c, 0, 0, b, b, b, 1.0, Uinv, b, SM_NONE, Tt, b, SM_NONE, 0.0, Uinv + bb, b, nullptr));
+  if (expl) {
+    if (!fused) {
+      CQR_TRY(gemm_f64_ws(c, 0, 0, b, b, b, 1.0, Uinv, b, SM_NONE, Tt, b, SM_NONE, 0.0, Uinv + bb, b, nullptr));
+      CQR_TRY(launch_copy_unit_lower(c, A, lda, Ye, m, b, b));
+      // Y1·Tᵀ = −Wᵀ exactly (W = −T·Y1ᵀ); formed as a product like the rest
+      CQR_TRY(gemm_f64_ws(c, 0, 0, b, b, b, 1.0, A, lda, SM_UNIT_LOWER, Tt, b, SM_NONE, 0.0, YT, m, nullptr));
+    }
     CQR_TRY(update_gemm_split(c, m - b, 2 * b, b, b, Y + b, m, Uinv, b, A + b, lda, Ye + b, m, YT + b, m,
                               PK_GEMM_SMALL));
-    CQR_TRY(launch_copy_unit_lower(c, A, lda, Ye, m, b, b));
-    CQR_TRY(gemm_f64_ws(c, 0, 0, b, b, b, 1.0, A, lda, SM_UNIT_LOWER, Tt, b, SM_NONE, 0.0, YT, m, nullptr));
   } else if (m > b) {
     CQR_TRY(update_gemm(c, m - b, b, b, 1.0, Y + b, m, Uinv, b, 0.0, A + b, lda, PK_GEMM_SMALL, 0));
   }

@@ -526,12 +526,95 @@ __device__ __forceinline__ void hr_core_body(const double* __restrict__ Q1top, i
   FOR_RC(rr, cc, b, NT) LinvT[rr + cc * b] = rr <= cc ? M[rr + cc * ldm] : 0.0;
 }
 
+// The products that follow the HR core in a QR_2D panel, in the same
+// workgroup (b = 128, 8 waves; wave w owns the 16-row block w of each
+// result).  On entry L⁻ᵀ is in M (upper part, ld b+1) and W, U⁻¹, L are in
+// global memory, stored by this workgroup.
+//   Tt = Tᵀ = −L⁻¹·Wᵀ  (lower; T = −W·L⁻ᵀ, hh_recon.cxx:282-308)
+//   M2 = U⁻¹·Tᵀ        (so that Q1·[U⁻¹ | M2] gives Y and Y·Tᵀ at once)
+//   with Ye: Ye top = [unit lower L],  YT top = Y1·Tᵀ = −Wᵀ  (W = −T·Y1ᵀ)
+__device__ __forceinline__ void hr_tail_body(double* __restrict__ M, int ldm, const double* __restrict__ Atop,
+                                             int64_t lda, const double* __restrict__ W,
+                                             const double* __restrict__ Uinv, double* __restrict__ Tt,
+                                             double* __restrict__ M2, double* __restrict__ Ye,
+                                             double* __restrict__ YT, int64_t ldy) {
+  constexpr int b = 128;
+  const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
+  const int r16 = lane & 15, q4 = lane >> 4;
+  const int i0 = 16 * w;
+  __syncthreads();   // W, U⁻¹ and L stored by every wave are visible
+  dbl4 tt[8];
+#pragma unroll
+  for (int tn = 0; tn < 8; ++tn) {
+    tt[tn] = dbl4{0.0, 0.0, 0.0, 0.0};
+    if (tn > w) continue;   // Tᵀ is lower: tiles right of the diagonal are zero
+    // L⁻¹[i][k] = L⁻ᵀ[k][i], nonzero for k ≤ i;  Wᵀ[k][j] = W[j][k], nonzero for j ≤ k
+    auto x = [&](int m, int k) {
+      const int i = i0 + m;
+      const bool ok = k <= i;
+      const double v = M[ok ? k + i * ldm : 0];   // clamped, loaded unconditionally
+      return ok ? v : 0.0;
+    };
+    auto y = [&](int k, int n) {
+      const int j = 16 * tn + n;
+      const bool ok = j <= k && k < b;
+      const double v = W[ok ? j + k * b : 0];
+      return ok ? v : 0.0;
+    };
+    tt[tn] = mfma_tile16_acc4(x, y, 16 * tn, i0 + 16, lane);
+  }
+  __syncthreads();   // every wave is done reading L⁻ᵀ from M
+  // Tᵀ (negated) into M, column-major ld ldm, and out
+#pragma unroll
+  for (int tn = 0; tn < 8; ++tn)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = i0 + r16, j = 16 * tn + q4 + 4 * v;
+      const double val = -tt[tn][v];
+      M[i + j * ldm] = val;
+      Tt[i + j * b] = val;
+    }
+  __syncthreads();
+  if (M2) {
+#pragma unroll
+    for (int tn = 0; tn < 8; ++tn) {
+      // U⁻¹[i][k] nonzero for k ≥ i; Tᵀ[k][j] nonzero for k ≥ j
+      auto x = [&](int m, int k) {
+        const int i = i0 + m;
+        const bool ok = k >= i && k < b;
+        const double v = Uinv[ok ? i + k * b : 0];
+        return ok ? v : 0.0;
+      };
+      auto y = [&](int k, int n) {
+        const int j = 16 * tn + n;
+        const bool ok = k >= j && k < b;
+        const double v = M[ok ? k + j * ldm : 0];
+        return ok ? v : 0.0;
+      };
+      const dbl4 acc = mfma_tile16_acc4(x, y, max(i0, 16 * tn), b, lane);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) M2[(i0 + r16) + (16 * tn + q4 + 4 * v) * b] = acc[v];
+    }
+  }
+  if (Ye) {
+    FOR_RC(rr, cc, b, 512) {
+      Ye[rr + cc * ldy] = rr > cc ? Atop[rr + cc * lda] : (rr == cc ? 1.0 : 0.0);
+      YT[rr + cc * ldy] = rr >= cc ? -W[cc + rr * b] : 0.0;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(512)
 hr_core_kernel(const double* __restrict__ Q1top, int64_t ldq, const double* __restrict__ Rroot,
                double* __restrict__ Atop, int64_t lda, double* __restrict__ W,
                double* __restrict__ Uinv, double* __restrict__ Winv, double* __restrict__ LinvT,
-               int64_t* __restrict__ signs, int b) {
+               int64_t* __restrict__ signs, int b, double* __restrict__ Tt, double* __restrict__ M2,
+               double* __restrict__ Ye, double* __restrict__ YT, int64_t ldy) {
   hr_core_body(Q1top, ldq, Rroot, Atop, lda, W, Uinv, Winv, LinvT, signs, b);
+  if (Tt) {
+    extern __shared__ double smem[];
+    hr_tail_body(smem, b + 1, Atop, lda, W, Uinv, Tt, M2, Ye, YT, ldy);
+  }
 }
 
 // Batched: workgroup p handles problem p (its Q1 top in Y, its root R).
@@ -779,13 +862,17 @@ int bgemm(cqr_ctx* c, const BgemmArgs& g, int batch, int max_m) {
 static int big_lds_attr(cqr_ctx* c, const void* k) { return set_max_lds(c, k, 163840); }
 
 int hr_core(cqr_ctx* c, const double* Q1top, int64_t ldq, const double* Rroot, double* Atop,
-            int64_t lda, double* W, double* Uinv, double* Winv, double* LinvT, int64_t* signs, int b) {
+            int64_t lda, double* W, double* Uinv, double* Winv, double* LinvT, int64_t* signs, int b,
+            double* Tt, double* M2, double* Ye, double* YT, int64_t ldy) {
+  if (Tt && b != 128) return set_err(c, CQR_ERR_UNSUPPORTED, "hr_core: fused Tt needs b = 128");
+  if (Ye && !(Tt && YT)) return set_err(c, CQR_ERR_INVALID, "hr_core: Ye needs Tt and YT");
   CQR_TRY(big_lds_attr(c, (const void*)hr_core_kernel));
   const size_t lds = sizeof(double) * ((size_t)b * (b + 1) + 2 * b);
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(hr_core_kernel, dim3(1), dim3(512), lds, c->stream, Q1top, ldq, Rroot, Atop,
-                     lda, W, Uinv, Winv, LinvT, signs, b);
-  prof_end(c, ev, PK_HR, (2.0 / 3.0) * b * (double)b * b, 64.0 * b * b);
+                     lda, W, Uinv, Winv, LinvT, signs, b, Tt, M2, Ye, YT, ldy);
+  const double bbb = (double)b * b * b;
+  prof_end(c, ev, PK_HR, (2.0 / 3.0) * bbb + (Tt ? bbb / 3.0 : 0.0) + (M2 ? bbb : 0.0), 64.0 * b * b);
   return check_launch(c, "hr_core");
 }
 
