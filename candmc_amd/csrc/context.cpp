@@ -168,7 +168,8 @@ int cqr_create(cqr_ctx** ctx, int device) {
       hipMemset(c->d_flag, 0, sizeof(int)) != hipSuccess ||
       hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_gate, hipEventDisableTiming) != hipSuccess) {
     (void)hipGetLastError();
     if (c->d_flag) (void)hipFree(c->d_flag);
     if (c->d_cnt) (void)hipFree(c->d_cnt);
@@ -177,6 +178,7 @@ int cqr_create(cqr_ctx** ctx, int device) {
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->ev_gate) (void)hipEventDestroy(c->ev_gate);
     delete c;
     return CQR_ERR_NOMEM;
   }
@@ -206,6 +208,7 @@ int cqr_destroy(cqr_ctx* c) {
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->ev_gate) (void)hipEventDestroy(c->ev_gate);
   delete c;
   return CQR_OK;
 }

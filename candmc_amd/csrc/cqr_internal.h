@@ -40,7 +40,8 @@ struct cqr_ctx {
   uint64_t q1_epoch = 0;
   uint64_t tree_epoch = 0;      // launch epoch folded into the counters
   hipStream_t side = nullptr;   // second stream of the QR_2D lookahead
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_gate = nullptr;
+  int gate_at = 0;              // QR_2D lookahead: each panel records ev_gate after construct_Q1
   int lookahead = 1;
   void* noalloc = nullptr;      // this device's fixed scratch (noalloc.hip)
   int lsq_escalate = 1;         // refactor a stalled mixed solve in FP64
@@ -271,6 +272,11 @@ int launch_small_thin(cqr_ctx* c, int ta, int tb, int64_t m, int64_t n, int64_t 
 // n, b) doubles.
 int wy_update_explicit(cqr_ctx* c, const double* Ye, const double* YT, int64_t m, int64_t n, int b, double* A,
                        int64_t lda, double* ws, int prec);
+// Its two halves: Z = Yeᵀ·A into ws (*Zout points at it), then A −= YT·Z.
+int wy_explicit_yta(cqr_ctx* c, const double* Ye, int64_t m, int64_t n, int b, const double* A, int64_t lda,
+                    double* ws, int prec, double** Zout);
+int wy_explicit_apply(cqr_ctx* c, const double* YT, int64_t m, int64_t n, int b, const double* Z, double* A,
+                      int64_t lda, int prec);
 
 // Least-squares kernels (lsq_kernels.hip): C = alpha·op(A)·X + beta·C for
 // X with nr ≤ 16 columns; the transposed form needs

@@ -682,29 +682,43 @@ int wy_update_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const
 // No structural masks remain, so both products are plain: Z = Yeᵀ·A on the
 // lean kernel (split-K, from three column tiles on) or the general one, and
 // the update through the vendor BLAS.
-int wy_update_explicit(cqr_ctx* c, const double* Ye, const double* YT, int64_t m, int64_t n, int b, double* A,
-                       int64_t lda, double* ws, int prec) {
-  if (m <= 0 || n <= 0 || b <= 0) return CQR_OK;
+// Z (b×n, in ws) = Yᵀ·A: the first product of the explicit update.
+int wy_explicit_yta(cqr_ctx* c, const double* Ye, int64_t m, int64_t n, int b, const double* A, int64_t lda,
+                    double* ws, int prec, double** Zout) {
   double* part = ws;
   const size_t np = gemm_f64_scratch_doubles(b, n, m);
   double* Z = ws + ((np + 31) / 32) * 32;
-  if (prec != CQR_PREC_F64) {
-    // FP32/BF16 operands, FP32 accumulation; Z and the update land in FP64
-    CQR_TRY(gemm_f64_impl(c, 1, 0, b, n, m, 1.0, Ye, m, SM_NONE, A, lda, SM_NONE, 0.0, Z, b, np ? part : nullptr,
-                          PK_GEMM_YTA, prec));
-    return gemm_f64_impl(c, 0, 0, m, n, b, -1.0, YT, m, SM_NONE, Z, b, SM_NONE, 1.0, A, lda, nullptr, PK_GEMM_UPD,
-                         prec);
-  }
+  *Zout = Z;
+  if (m <= 0 || n <= 0 || b <= 0) return CQR_OK;
+  if (prec != CQR_PREC_F64)   // FP32/BF16 operands, FP32 accumulation; Z lands in FP64
+    return gemm_f64_impl(c, 1, 0, b, n, m, 1.0, Ye, m, SM_NONE, A, lda, SM_NONE, 0.0, Z, b, np ? part : nullptr,
+                         PK_GEMM_YTA, prec);
   const int ns = np ? (int)(np / ((size_t)b * n)) : 0;
   if (ns >= 2) {
     int used = 0;
     CQR_TRY(lean_gemm(c, 1, b, n, m, 1.0, Ye, m, A, lda, 0.0, nullptr, 0, part, ns, PK_GEMM_YTA, &used));
-    CQR_TRY(launch_splitk_reduce(c, part, (int64_t)b * n, used, b, n, Z, b, 1.0, 0.0));
-  } else {
-    CQR_TRY(gemm_f64_impl(c, 1, 0, b, n, m, 1.0, Ye, m, SM_NONE, A, lda, SM_NONE, 0.0, Z, b, np ? part : nullptr,
-                          PK_GEMM_YTA));
+    return launch_splitk_reduce(c, part, (int64_t)b * n, used, b, n, Z, b, 1.0, 0.0);
   }
+  return gemm_f64_impl(c, 1, 0, b, n, m, 1.0, Ye, m, SM_NONE, A, lda, SM_NONE, 0.0, Z, b, np ? part : nullptr,
+                       PK_GEMM_YTA);
+}
+
+// A −= (Y·Tᵀ)·Z: the second product of the explicit update.
+int wy_explicit_apply(cqr_ctx* c, const double* YT, int64_t m, int64_t n, int b, const double* Z, double* A,
+                      int64_t lda, int prec) {
+  if (m <= 0 || n <= 0 || b <= 0) return CQR_OK;
+  if (prec != CQR_PREC_F64)
+    return gemm_f64_impl(c, 0, 0, m, n, b, -1.0, YT, m, SM_NONE, Z, b, SM_NONE, 1.0, A, lda, nullptr, PK_GEMM_UPD,
+                         prec);
   return update_gemm(c, m, n, b, -1.0, YT, m, Z, b, 1.0, A, lda, PK_GEMM_UPD, 0);
+}
+
+int wy_update_explicit(cqr_ctx* c, const double* Ye, const double* YT, int64_t m, int64_t n, int b, double* A,
+                       int64_t lda, double* ws, int prec) {
+  if (m <= 0 || n <= 0 || b <= 0) return CQR_OK;
+  double* Z = nullptr;
+  CQR_TRY(wy_explicit_yta(c, Ye, m, n, b, A, lda, ws, prec, &Z));
+  return wy_explicit_apply(c, YT, m, n, b, Z, A, lda, prec);
 }
 
 int wy_update_f64_ws(cqr_ctx* c, const double* Y, int64_t ldy, int y_implicit, const double* Tt,
