@@ -73,13 +73,49 @@ def accuracy(A, YR, nprobe=2, seed=7):
     return res / (n * EPS), orth
 
 
+def host_info():
+    """The host the CPU lines ran on: usable cores, CPU model, LAPACK build."""
+    from threadpoolctl import threadpool_info
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    blas = [d for d in threadpool_info() if d.get("user_api") == "blas"]
+    lapack = ", ".join(f"{d.get('internal_api')} {d.get('version')} ({os.path.basename(d.get('filepath', ''))}, "
+                       f"{d.get('num_threads')} threads)" for d in blas) or "unknown"
+    return {"nproc": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count(), "cpu_model": model,
+            "lapack": lapack}
+
+
+def cpu_dgeqrf(A):
+    """Multithreaded LAPACK dgeqrf of the whole matrix (the reference's
+    sequential QR benchmark, bench/QR/bench_qr_seq.cxx:66, calls dgeqrf)."""
+    from scipy.linalg import lapack
+    from threadpoolctl import threadpool_info
+    m, n = A.shape
+    threads = max([d.get("num_threads", 1) for d in threadpool_info() if d.get("user_api") == "blas"] or [1])
+    W = np.asfortranarray(A.copy())
+    t = time.perf_counter()
+    _, _, work, info = lapack.dgeqrf(W, lwork=-1)
+    qr_, tau, _, info = lapack.dgeqrf(W, lwork=int(work[0]), overwrite_a=True)
+    dt = time.perf_counter() - t
+    assert info == 0
+    return {"value": round(qr_flops(m, n) / dt / 1e9, 2), "unit": "GFLOP/s", "cores": int(threads),
+            "kind": "lapack", "sample": f"dgeqrf of the same {m}x{n} matrix", "seconds": round(dt, 3)}
+
+
 def cpu_baseline(A, b, max_seconds):
     """The oracle's restatement of QR_2D (numpy + LAPACK), timed on host cores."""
     from threadpoolctl import threadpool_info
 
     from oracle import candmc_oracle as O
     m, n = A.shape
-    threads = max([d.get("num_threads", 1) for d in threadpool_info()] or [1])
+    threads = max([d.get("num_threads", 1) for d in threadpool_info() if d.get("user_api") == "blas"] or [1])
     # estimate: the full factorization when it fits the time budget, else the
     # first panels (flops = F(m,n) − F(m−pb, n−pb))
     W = np.asfortranarray(A.copy())
@@ -99,7 +135,8 @@ def cpu_baseline(A, b, max_seconds):
     sample = (f"QR_2D of the same {m}x{n} matrix, b={b}, "
               + ("full factorization" if panels >= n // b else f"first {panels} panels"))
     return {"value": round(flops / dt / 1e9, 2), "unit": "GFLOP/s", "cores": int(threads),
-            "kind": "port", "sample": sample, "seconds": round(dt, 3)}
+            "kind": "port", "sample": sample, "seconds": round(dt, 3), "host": host_info(),
+            "dgeqrf": cpu_dgeqrf(A)}
 
 
 # --------------------------------------------------------------------------

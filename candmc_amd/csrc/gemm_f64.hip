@@ -88,20 +88,24 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 // rounded to FP32 while they are staged in LDS and multiplied on FP32 MFMA
 // (v_mfma_f32_16x16x4f32) with FP32 accumulation; the result is added to
 // the FP64 output in FP64.
-typedef short s4v __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ unsigned short to_bf16_bits(double x) {
   return __builtin_bit_cast(unsigned short, (__bf16)(float)x);
 }
 
-// CQR_PREC_BF16: operands rounded to BF16 (k-contiguous LDS rows so that a
-// lane's four consecutive k values are one 8-byte read), multiplied on
-// v_mfma_f32_16x16x16bf16_1k with FP32 accumulation.
+// CQR_PREC_BF16: operands rounded to BF16 into k-contiguous LDS rows (a
+// lane's eight consecutive k values are one 16-byte read), 32-deep slabs
+// multiplied on gfx950's v_mfma_f32_16x16x32_bf16 with FP32 accumulation.
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+
 template <bool TA, bool TB, int ROLE, int PREC = CQR_PREC_F64>
 __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
-  constexpr int BM = 128, BN = 128, BK = 16, PAD = 4, KP = BK + 4;
-  constexpr int LDS_A = BM + PAD, LDS_B = BN + PAD;
   constexpr bool LP = PREC != CQR_PREC_F64, BF = PREC == CQR_PREC_BF16;
+  // a slab is NH halves of 16 k each; every thread stages 8 values of A and
+  // 8 of B per half
+  constexpr int NH = BF ? 2 : 1;
+  constexpr int BM = 128, BN = 128, BK = 16 * NH, PAD = 4, KP = BK + 8;
+  constexpr int LDS_A = BM + PAD, LDS_B = BN + PAD;
   typedef typename std::conditional<LP, float, double>::type ST;
   typedef typename std::conditional<LP, f4v, d4>::type AT;
   constexpr size_t SZ_STD = 2 * BK * (LDS_A + LDS_B) * sizeof(ST);
@@ -159,20 +163,20 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = AT{0, 0, 0, 0};
 
-  double ra[8], rb[8];
-  auto load = [&](int64_t k0) {
-    const bool kfull = k0 + BK <= kend;
-    const bool mA = TA ? mask_active(g.smA, k0, k0 + BK - 1, m0, m0 + BM - 1)
-                       : mask_active(g.smA, m0, m0 + BM - 1, k0, k0 + BK - 1);
-    const bool mB = TB ? mask_active(g.smB, n0, n0 + BN - 1, k0, k0 + BK - 1)
-                       : mask_active(g.smB, k0, k0 + BK - 1, n0, n0 + BN - 1);
+  double ra[8 * NH], rb[8 * NH];
+  auto load_half = [&](int64_t k0, int h) {
+    const bool kfull = k0 + 16 <= kend;
+    const bool mA = TA ? mask_active(g.smA, k0, k0 + 15, m0, m0 + BM - 1)
+                       : mask_active(g.smA, m0, m0 + BM - 1, k0, k0 + 15);
+    const bool mB = TB ? mask_active(g.smB, n0, n0 + BN - 1, k0, k0 + 15)
+                       : mask_active(g.smB, k0, k0 + 15, n0, n0 + BN - 1);
     const double* qa = pa + k0 * a_kstep;
     const double* qb = pb + k0 * b_kstep;
     if (full_mn && kfull && !mA && !mB) {
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
-        ra[r] = qa[r * a_rstep];
-        rb[r] = qb[r * b_rstep];
+        ra[8 * h + r] = qa[r * a_rstep];
+        rb[8 * h + r] = qb[r * b_rstep];
       }
       return;
     }
@@ -184,27 +188,33 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
         v = qa[r * a_rstep];
         if (mA) v = gmask(v, TA ? gk : gm, TA ? gm : gk, g.smA);
       }
-      ra[r] = v;
+      ra[8 * h + r] = v;
       const int64_t gn = n0 + bn + (TB ? 0 : 16 * r), gk2 = k0 + bk + (TB ? 2 * r : 0);
       double u = 0.0;
       if (gn < g.n && gk2 < kend) {
         u = qb[r * b_rstep];
         if (mB) u = gmask(u, TB ? gn : gk2, TB ? gk2 : gn, g.smB);
       }
-      rb[r] = u;
+      rb[8 * h + r] = u;
     }
+  };
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int h = 0; h < NH; ++h) load_half(k0 + 16 * h, h);
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      if constexpr (BF) {
-        As16[buf][am + (TA ? 16 * r : 0)][ak + (TA ? 0 : 2 * r)] = to_bf16_bits(ra[r]);
-        Bs16[buf][bn + (TB ? 0 : 16 * r)][bk + (TB ? 2 * r : 0)] = to_bf16_bits(rb[r]);
-      } else {
-        As[buf][ak + (TA ? 0 : 2 * r)][am + (TA ? 16 * r : 0)] = (ST)ra[r];
-        Bs[buf][bk + (TB ? 2 * r : 0)][bn + (TB ? 0 : 16 * r)] = (ST)rb[r];
+    for (int h = 0; h < NH; ++h)
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        if constexpr (BF) {
+          As16[buf][am + (TA ? 16 * r : 0)][16 * h + ak + (TA ? 0 : 2 * r)] = to_bf16_bits(ra[8 * h + r]);
+          Bs16[buf][bn + (TB ? 0 : 16 * r)][16 * h + bk + (TB ? 2 * r : 0)] = to_bf16_bits(rb[8 * h + r]);
+        } else {
+          As[buf][ak + (TA ? 0 : 2 * r)][am + (TA ? 16 * r : 0)] = (ST)ra[r];
+          Bs[buf][bk + (TB ? 2 * r : 0)][bn + (TB ? 0 : 16 * r)] = (ST)rb[r];
+        }
       }
-    }
   };
 
   const int64_t ntiles = (kend - kbeg + BK - 1) / BK;
@@ -218,17 +228,17 @@ __global__ void __launch_bounds__(256) mfma_gemm_f64(GemmArgs g) {
     const int buf = (int)(t & 1);
     if (t + 1 < ntiles) load(kbeg + (t + 1) * BK);
     if constexpr (BF) {
-      // one K = 16 product per tile pair: lane l supplies k = 4(l>>4)..+3
-      s4v af[4], bf[4];
+      // one K = 32 product per tile pair: lane l supplies k = 8(l>>4)..+7
+      bf8v af[4], bf[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = *reinterpret_cast<const s4v*>(&As16[buf][wm0 + i * 16 + fc][4 * fr]);
+      for (int i = 0; i < 4; ++i) af[i] = *reinterpret_cast<const bf8v*>(&As16[buf][wm0 + i * 16 + fc][8 * fr]);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bf[j] = *reinterpret_cast<const s4v*>(&Bs16[buf][wn0 + j * 16 + fc][4 * fr]);
+      for (int j = 0; j < 4; ++j) bf[j] = *reinterpret_cast<const bf8v*>(&Bs16[buf][wn0 + j * 16 + fc][8 * fr]);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf[j], af[i], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[j], af[i], acc[i][j], 0, 0, 0);
     } else
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
