@@ -70,6 +70,12 @@ SIGNATURES = {
     "cqr_server_wait": (_I, [_P, _I64, _P]),
     "cqr_server_drain": (_I, [_P, _P, _I, _c.POINTER(_I)]),
     "cqr_server_pending": (_I, [_P, _c.POINTER(_I64), _c.POINTER(_c.c_size_t)]),
+    "cqr_server_set_batching": (_I, [_P, _I]),
+    "cqr_lstsq_batched": (_I, [_P, _I64, _P, _I, _c.c_double, _P, _P, _P]),
+    "cqr_lstsq_batched_workspace_bytes": (_c.c_size_t, [_I64, _P]),
+    "cqr_lstsq_factored": (_I, [_P, _P, _I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64, _I64, _P, _I64, _I,
+                                _c.c_double, _P, _P]),
+    "cqr_lstsq_factored_workspace_bytes": (_c.c_size_t, [_I64, _I64, _I64, _I64]),
     "cqr_batch_create": (_I, [_P, _c.POINTER(_P), _I64, _P, _P]),
     "cqr_batch_destroy": (_I, [_P, _P]),
     "cqr_batch_offsets": (_I, [_P, _P, _P, _c.POINTER(_I64), _c.POINTER(_I64)]),

@@ -57,7 +57,9 @@ struct cqr_batch {
 
 namespace {
 
-int plan_group(cqr_ctx* c, cqr_batch* bt, Group& g) {
+// Plans a group's problems; with `alloc` its device table is allocated and
+// filled here, without it the caller places and fills g.d_bp.
+int plan_group(cqr_ctx* c, cqr_batch* bt, Group& g, bool alloc = true) {
   const int b = g.b;
   g.hp.clear();
   g.total_leaves = g.total_nodes = g.total_items = 0;
@@ -101,6 +103,7 @@ int plan_group(cqr_ctx* c, cqr_batch* bt, Group& g) {
   g.cnt = s.add(8 * nn);
   g.qflags = s.add(8 * nn * (size_t)((b + 15) / 16));
   g.bytes = s.bytes;
+  if (!alloc) return CQR_OK;
   if (g.d_bp) (void)hipFree(g.d_bp);
   g.d_bp = nullptr;
   CQR_HIP(c, hipMalloc(&g.d_bp, sizeof(BatchProb) * np));
@@ -339,6 +342,130 @@ int cqr_batch_qr_loop(cqr_ctx* c, cqr_batch* bt, double* A, double* Tt) {
     CQR_TRY(hh_recon_panel(c, A + bt->aoff[p], mp, mp, np, W, nullptr, nullptr, Tt + bt->toff[p], 0, base, nullptr));
   }
   return CQR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Least squares for many requests at once: one batched factorization of all
+// of them (the launches above, per width group), then each request's solve
+// and FP64 refinement (lstsq_refine) from its factors.  Everything lives in
+// the context's arena; no other allocation.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct LsqBatchPlan {
+  cqr_batch bt;
+  size_t oA = 0, oT = 0, oBp = 0, oG = 0, oR = 0, bytes = 0;
+  std::vector<size_t> bp_off;     // per group, inside oBp
+  size_t refine = 0;
+};
+
+int plan_lsq_batch(cqr_ctx* c, int64_t np, const cqr_request* rq, LsqBatchPlan& P) {
+  cqr_batch& bt = P.bt;
+  bt.nprob = np;
+  bt.m.resize(np);
+  bt.n.resize(np);
+  bt.aoff.resize(np);
+  bt.toff.resize(np);
+  size_t refine = 0;
+  for (int64_t p = 0; p < np; ++p) {
+    const cqr_request& r = rq[p];
+    if (!r.A || !r.B || !r.X || r.n < 1 || r.m < r.n || r.lda < r.m || r.ldb < r.m || r.ldx < r.n || r.nrhs < 1)
+      return set_err(c, CQR_ERR_INVALID, "lstsq_batched: request %lld has bad arguments", (long long)p);
+    if (r.n > CQR_MAX_B || r.nrhs > 16)
+      return set_err(c, CQR_ERR_UNSUPPORTED, "lstsq_batched: request %lld needs n <= %d and nrhs <= 16",
+                     (long long)p, CQR_MAX_B);
+    bt.m[p] = r.m;
+    bt.n[p] = r.n;
+    bt.aoff[p] = bt.total_a;
+    bt.toff[p] = bt.total_t;
+    bt.total_a += r.m * r.n;
+    bt.total_t += r.n * r.n;
+    LsqWS w;
+    refine = std::max(refine, refine_ws(r.m, r.n, r.nrhs, r.n, &w));
+  }
+  for (int64_t p = 0; p < np; ++p) {
+    Group* g = nullptr;
+    for (Group& x : bt.groups)
+      if (x.b == bt.n[p]) g = &x;
+    if (!g) {
+      bt.groups.emplace_back();
+      g = &bt.groups.back();
+      g->b = (int)bt.n[p];
+    }
+    g->probs.push_back((int)p);
+  }
+  Sizer s;
+  P.oA = s.add(8 * (size_t)bt.total_a);
+  P.oT = s.add(8 * (size_t)bt.total_t);
+  size_t gws = 0;
+  for (Group& g : bt.groups) {
+    CQR_TRY(plan_group(c, &bt, g, false));
+    gws = std::max(gws, g.bytes);
+    P.bp_off.push_back(s.add(sizeof(BatchProb) * g.probs.size()));
+  }
+  P.oG = s.add(gws);
+  P.oR = s.add(refine);
+  P.refine = refine;
+  P.bytes = s.bytes;
+  return CQR_OK;
+}
+
+}  // namespace
+
+size_t cqr_lstsq_batched_workspace_bytes(int64_t nprob, const cqr_request* reqs) {
+  if (nprob < 1 || !reqs) return 0;
+  for (int64_t p = 0; p < nprob; ++p) {
+    const cqr_request& r = reqs[p];
+    if (r.n < 1 || r.n > CQR_MAX_B || r.m < r.n || r.nrhs < 1 || r.nrhs > 16) return 0;
+  }
+  cqr_ctx dummy;   // planning only: set_err needs somewhere to write
+  LsqBatchPlan P;
+  if (plan_lsq_batch(&dummy, nprob, reqs, P) != CQR_OK) return 0;
+  return P.bytes;
+}
+
+int cqr_lstsq_batched(cqr_ctx* c, int64_t nprob, const cqr_request* reqs, int max_iter, double tol, int* status,
+                      int* iters, double* last_update) {
+  CQR_CHECK_CTX(c);
+  if (nprob < 1 || !reqs || !status || max_iter < 0 || !(tol >= 0.0))
+    return set_err(c, CQR_ERR_INVALID, "lstsq_batched: bad arguments");
+  LsqBatchPlan P;
+  CQR_TRY(plan_lsq_batch(c, nprob, reqs, P));
+  CQR_TRY(ensure_ws(c, P.bytes));
+  char* base = static_cast<char*>(c->ws);
+  cqr_batch& bt = P.bt;
+  bt.ws = base + P.oG;
+  bt.ws_bytes = P.bytes - P.oG;
+  double* Ap = reinterpret_cast<double*>(base + P.oA);
+  double* Tp = reinterpret_cast<double*>(base + P.oT);
+  for (size_t gi = 0; gi < bt.groups.size(); ++gi) {
+    Group& g = bt.groups[gi];
+    g.d_bp = reinterpret_cast<BatchProb*>(base + P.bp_off[gi]);
+    // the tables outlive this call's last use of them (it ends in a sync)
+    CQR_HIP(c, hipMemcpyAsync(g.d_bp, g.hp.data(), sizeof(BatchProb) * g.hp.size(), hipMemcpyHostToDevice,
+                              c->stream));
+  }
+  for (int64_t p = 0; p < nprob; ++p)
+    CQR_TRY(launch_copy2d(c, reqs[p].A, reqs[p].lda, Ap + bt.aoff[p], bt.m[p], bt.m[p], bt.n[p]));
+  int rc = CQR_OK;
+  for (Group& g : bt.groups)
+    if ((rc = run_group(c, &bt, g, Ap, Tp)) != CQR_OK) break;
+  for (int64_t p = 0; p < nprob; ++p) {
+    const cqr_request& r = reqs[p];
+    LsqRun run;
+    LsqWS w;
+    refine_ws(r.m, r.n, r.nrhs, r.n, &w);
+    status[p] = rc != CQR_OK ? rc
+                             : lstsq_refine(c, r.A, r.lda, r.m, r.n, r.B, r.ldb, r.nrhs, r.X, r.ldx, r.n,
+                                            Ap + bt.aoff[p], r.m, Tp + bt.toff[p], CQR_PREC_F64, max_iter, tol, false,
+                                            base + P.oR, w, run);
+    if (iters) iters[p] = run.iters;
+    if (last_update) last_update[p] = run.rel;
+  }
+  for (Group& g : bt.groups) g.d_bp = nullptr;   // arena memory, not owned
+  bt.ws = nullptr;
+  CQR_HIP(c, hipStreamSynchronize(c->stream));
+  return rc;
 }
 
 // Leaf-height autotuner: for every group, time each candidate on a copy of

@@ -325,6 +325,22 @@ struct LsqWS {
   size_t Af, Tt, Ri, R, F, G, H, Tmp, Q, N;
 };
 size_t lstsq_layout(int64_t m, int64_t n, int64_t nr, int64_t b, int64_t leaf_rows, Qr2dWS* o, LsqWS* w);
+// The refinement's buffers alone (Af and Tt given by the caller).
+size_t refine_ws(int64_t m, int64_t n, int64_t nr, int64_t b, LsqWS* w);
+struct LsqRun {
+  int iters = 0;
+  double rel = 0.0;
+};
+// The solve and refinement of cqr_lstsq from given factors: [Y\R] of A in
+// Af (ld ldf) and the Tᵀ blocks of its width-b panels in Tt; the buffers of
+// w (refine_ws) at base.  Returns CQR_OK when the last relative update is
+// ≤ tol (or max_iter is 0), CQR_ERR_NOCONVERGE when the refinement ended
+// above tol, CQR_ERR_NONFINITE when the iterate or R holds a NaN or Inf,
+// and −1 (only with `escalate`, for a mixed `prec`) when R's diagonal range
+// already rules the mixed factors out.
+int lstsq_refine(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, const double* B, int64_t ldb,
+                 int64_t nr, double* X, int64_t ldx, int64_t b, const double* Af, int64_t ldf, const double* Tt,
+                 int prec, int max_iter, double tol, bool escalate, char* base, const LsqWS& w, LsqRun& out);
 int hh_recon_core(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t b, double* W,
                   double* T, double* invT_T, double* Tt, int64_t nleaves);
 // Looks up the device address of the fixed scratch array of noalloc.hip

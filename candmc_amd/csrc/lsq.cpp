@@ -130,14 +130,9 @@ int trsm_r_ws(cqr_ctx* c, const double* YR, int64_t ldy, int64_t n, const double
   return CQR_OK;
 }
 
-// Workspace of the least-squares driver: the QR_2D arena followed by the
-// factor copy, the panel factors, R's block inverses and the vectors.
-size_t lstsq_layout(int64_t m, int64_t n, int64_t nr, int64_t b, int64_t leaf_rows, Qr2dWS* o, LsqWS* w) {
-  const int64_t np = (n + b - 1) / b, nb = (n + 127) / 128;
-  Sizer s;
-  s.bytes = qr2d_ws(m, n, b, leaf_rows, o);
-  w->Af = s.add(8 * (size_t)m * n);
-  w->Tt = s.add(8 * np * b * b);
+// R's block inverses and the vectors of the refinement.
+static void refine_layout(int64_t m, int64_t n, int64_t nr, int64_t b, Sizer& s, LsqWS* w) {
+  const int64_t nb = (n + 127) / 128;
   w->Ri = s.add(8 * nb * 128 * 128);
   w->R = s.add(8 * (size_t)m * nr);
   w->F = s.add(8 * (size_t)m * nr);
@@ -146,6 +141,25 @@ size_t lstsq_layout(int64_t m, int64_t n, int64_t nr, int64_t b, int64_t leaf_ro
   w->Tmp = s.add(8 * 128 * nr);
   w->Q = s.add(8 * std::max(apply_q_ws_doubles(m, b, nr), thin_gemm_scratch_doubles(m, n, (int)nr)));
   w->N = s.add(8 * 2 * nr);
+}
+
+// Workspace of the least-squares driver: the QR_2D arena followed by the
+// factor copy, the panel factors and the refinement's buffers.
+size_t lstsq_layout(int64_t m, int64_t n, int64_t nr, int64_t b, int64_t leaf_rows, Qr2dWS* o, LsqWS* w) {
+  const int64_t np = (n + b - 1) / b;
+  Sizer s;
+  s.bytes = qr2d_ws(m, n, b, leaf_rows, o);
+  w->Af = s.add(8 * (size_t)m * n);
+  w->Tt = s.add(8 * np * b * b);
+  refine_layout(m, n, nr, b, s, w);
+  return s.bytes;
+}
+
+// The refinement's buffers alone (factors given by the caller).
+size_t refine_ws(int64_t m, int64_t n, int64_t nr, int64_t b, LsqWS* w) {
+  Sizer s;
+  w->Af = w->Tt = 0;
+  refine_layout(m, n, nr, b, s, w);
   return s.bytes;
 }
 
@@ -205,39 +219,41 @@ double prec_unit(int prec) {
 constexpr double kEscalateKappaU = 1e-2;
 enum { ATTEMPT_ESCALATE = -1 };
 
-struct LsqRun {
-  int iters = 0;
-  double rel = 0.0;
-};
+}  // namespace
 
-// One factorization in `prec` and its refinement.  Returns CQR_OK when the
-// last relative update is ≤ tol (or max_iter is 0), CQR_ERR_NOCONVERGE when
-// the refinement ended above tol, CQR_ERR_NONFINITE when the iterate or R
-// holds a NaN or Inf, and ATTEMPT_ESCALATE (only with `escalate`, for a
-// mixed `prec`) when R's diagonal range already rules the mixed factors out.
-int lstsq_attempt(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, const double* B, int64_t ldb,
+// One factorization in `prec` and its refinement (lstsq_refine).
+static int lstsq_attempt(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, const double* B, int64_t ldb,
                   int64_t nr, double* X, int64_t ldx, int64_t b, int prec, int max_iter, double tol, bool escalate,
                   char* base, const Qr2dWS& o, const LsqWS& w, LsqRun& out) {
-  auto at = [&](size_t off) { return reinterpret_cast<double*>(base + off); };
-  double *Af = at(w.Af), *Tt = at(w.Tt), *Rinv = at(w.Ri), *r = at(w.R), *f = at(w.F), *g = at(w.G),
-         *h = at(w.H), *tmp = at(w.Tmp), *wsq = at(w.Q), *nrm = at(w.N);
+  double* Af = reinterpret_cast<double*>(base + w.Af);
+  double* Tt = reinterpret_cast<double*>(base + w.Tt);
   // factor a copy of A (the caller's A stays the FP64 reference operator)
   CQR_TRY(launch_copy2d(c, A, lda, Af, m, m, n));
   CQR_TRY(qr2d_run(c, Af, m, m, n, b, nullptr, 0, base, o, Tt, prec));
+  return lstsq_refine(c, A, lda, m, n, B, ldb, nr, X, ldx, b, Af, m, Tt, prec, max_iter, tol, escalate, base, w,
+                      out);
+}
+
+int lstsq_refine(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, const double* B, int64_t ldb,
+                 int64_t nr, double* X, int64_t ldx, int64_t b, const double* Af, int64_t ldf, const double* Tt,
+                 int prec, int max_iter, double tol, bool escalate, char* base, const LsqWS& w, LsqRun& out) {
+  auto at = [&](size_t off) { return reinterpret_cast<double*>(base + off); };
+  double *Rinv = at(w.Ri), *r = at(w.R), *f = at(w.F), *g = at(w.G), *h = at(w.H), *tmp = at(w.Tmp),
+         *wsq = at(w.Q), *nrm = at(w.N);
   std::vector<double> hn(std::max<int64_t>(2 * nr, 2));
   {  // κ̂ = max|r_ii| / min|r_ii|, a lower bound on κ(A)
-    CQR_TRY(launch_diag_range(c, Af, m, n, nrm));
+    CQR_TRY(launch_diag_range(c, Af, ldf, n, nrm));
     CQR_HIP(c, hipMemcpyAsync(hn.data(), nrm, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CQR_HIP(c, hipStreamSynchronize(c->stream));
     if (!std::isfinite(hn[0]) || !std::isfinite(hn[1])) return set_err(c, CQR_ERR_NONFINITE, "lstsq: R is not finite");
     if (escalate && prec != CQR_PREC_F64 && !(hn[1] > 0.0 && hn[0] / hn[1] * prec_unit(prec) <= kEscalateKappaU))
       return ATTEMPT_ESCALATE;
   }
-  CQR_TRY(r_block_inverses(c, Af, m, n, Rinv));
+  CQR_TRY(r_block_inverses(c, Af, ldf, n, Rinv));
   // x0 = R⁻¹·(Qᵀb)(0:n)
   CQR_TRY(launch_copy2d(c, B, ldb, f, m, m, nr));
-  CQR_TRY(apply_q_ws(c, Af, m, Tt, m, n, b, f, m, nr, 1, wsq));
-  CQR_TRY(trsm_r_ws(c, Af, m, n, Rinv, f, m, nr, 0, tmp));
+  CQR_TRY(apply_q_ws(c, Af, ldf, Tt, m, n, b, f, m, nr, 1, wsq));
+  CQR_TRY(trsm_r_ws(c, Af, ldf, n, Rinv, f, m, nr, 0, tmp));
   CQR_TRY(launch_copy2d(c, f, m, X, ldx, n, nr));
   if (max_iter == 0) return CQR_OK;
   // r = b − A·x
@@ -272,8 +288,8 @@ int lstsq_attempt(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n
       // factorization, so they are used while each update shrinks by at
       // least 8× (and the first is below half the solution).
       CQR_TRY(thin_gemm(c, 1, m, n, 1.0, A, lda, r, m, (int)nr, 0.0, g, n, wsq));
-      CQR_TRY(trsm_r_ws(c, Af, m, n, Rinv, g, n, nr, 1, tmp));
-      CQR_TRY(trsm_r_ws(c, Af, m, n, Rinv, g, n, nr, 0, tmp));
+      CQR_TRY(trsm_r_ws(c, Af, ldf, n, Rinv, g, n, nr, 1, tmp));
+      CQR_TRY(trsm_r_ws(c, Af, ldf, n, Rinv, g, n, nr, 0, tmp));
       double rel;
       CQR_TRY(rel_update(g, n, &rel));
       if (rel <= 0.5 && (it == 1 || rel <= prev_rel / 8)) {
@@ -299,13 +315,13 @@ int lstsq_attempt(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n
     CQR_TRY(thin_gemm(c, 0, m, n, -1.0, A, lda, X, ldx, (int)nr, 1.0, f, m, nullptr));
     CQR_TRY(thin_gemm(c, 1, m, n, -1.0, A, lda, r, m, (int)nr, 0.0, g, n, wsq));
     // h = R⁻ᵀ·g;  d = Qᵀ·f;  dx = R⁻¹·(d₁ − h);  dr = Q·[h; d₂]
-    CQR_TRY(trsm_r_ws(c, Af, m, n, Rinv, g, n, nr, 1, tmp));               // g := h
-    CQR_TRY(apply_q_ws(c, Af, m, Tt, m, n, b, f, m, nr, 1, wsq));           // f := d
+    CQR_TRY(trsm_r_ws(c, Af, ldf, n, Rinv, g, n, nr, 1, tmp));               // g := h
+    CQR_TRY(apply_q_ws(c, Af, ldf, Tt, m, n, b, f, m, nr, 1, wsq));           // f := d
     CQR_TRY(launch_copy2d(c, f, m, h, n, n, nr));                            // h := d₁
     CQR_TRY(launch_axpby(c, n, nr, -1.0, g, n, 1.0, h, n));                  // h := d₁ − h
-    CQR_TRY(trsm_r_ws(c, Af, m, n, Rinv, h, n, nr, 0, tmp));                // h := dx
+    CQR_TRY(trsm_r_ws(c, Af, ldf, n, Rinv, h, n, nr, 0, tmp));                // h := dx
     CQR_TRY(launch_copy2d(c, g, n, f, m, n, nr));                            // f := [h; d₂]
-    CQR_TRY(apply_q_ws(c, Af, m, Tt, m, n, b, f, m, nr, 0, wsq));           // f := dr
+    CQR_TRY(apply_q_ws(c, Af, ldf, Tt, m, n, b, f, m, nr, 0, wsq));           // f := dr
     CQR_TRY(launch_axpby(c, n, nr, 1.0, h, n, 1.0, X, ldx));
     CQR_TRY(launch_axpby(c, m, nr, 1.0, f, m, 1.0, r, m));
     // stop when every column's update is below tol relative to the solution
@@ -325,11 +341,37 @@ int lstsq_attempt(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n
                  out.rel, tol, out.iters);
 }
 
-}  // namespace
 
 }  // namespace cqr
 
 extern "C" {
+
+size_t cqr_lstsq_factored_workspace_bytes(int64_t m, int64_t n, int64_t nrhs, int64_t b) {
+  if (m < 1 || n < 1 || m < n || nrhs < 1 || b < 1 || b > CQR_MAX_B) return 0;
+  LsqWS w;
+  return refine_ws(m, n, nrhs, b, &w);
+}
+
+int cqr_lstsq_factored(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n, const double* YR,
+                       int64_t ldyr, const double* Tt, int64_t b, const double* B, int64_t ldb, int64_t nrhs,
+                       double* X, int64_t ldx, int max_iter, double tol, int* iters, double* last_update) {
+  CQR_CHECK_CTX(c);
+  if (iters) *iters = 0;
+  if (last_update) *last_update = 0.0;
+  if (!A || !YR || !Tt || !B || !X || m < 1 || n < 1 || m < n || lda < m || ldyr < m || ldb < m || ldx < n ||
+      nrhs < 1 || b < 1 || max_iter < 0 || !(tol >= 0.0))
+    return set_err(c, CQR_ERR_INVALID, "lstsq_factored: bad arguments");
+  if (nrhs > 16) return set_err(c, CQR_ERR_UNSUPPORTED, "lstsq_factored: at most 16 right-hand sides");
+  if (b > CQR_MAX_B) return set_err(c, CQR_ERR_UNSUPPORTED, "lstsq_factored: b > %d", CQR_MAX_B);
+  LsqWS w;
+  CQR_TRY(ensure_ws(c, refine_ws(m, n, nrhs, b, &w)));
+  LsqRun run;
+  const int st = lstsq_refine(c, A, lda, m, n, B, ldb, nrhs, X, ldx, b, YR, ldyr, Tt, CQR_PREC_F64, max_iter, tol,
+                              false, static_cast<char*>(c->ws), w, run);
+  if (iters) *iters = run.iters;
+  if (last_update) *last_update = run.rel;
+  return st;
+}
 
 int cqr_set_lstsq_escalate(cqr_ctx* c, int on) {
   CQR_CHECK_CTX(c);

@@ -12,6 +12,13 @@
 //
 // Two accuracy tiers: FP64 (trailing update in FP64, refined to 1e-12) and
 // mixed (FP32 trailing update, refined with FP64 residuals to 1e-10).
+//
+// Requests of one panel (n ≤ 128, so no trailing update to make mixed) are
+// batched: a slot that takes such a request also takes the queued requests
+// of the same tier behind it that are one panel too (up to max_batch, while
+// the budget allows) and runs them through cqr_lstsq_batched, one batched
+// factorization and then each solve.  A batched request's X depends only on
+// that request, so it gets the same bits in a batch of any size.
 // The serving loop has no counterpart in the reference.
 #include <chrono>
 #include <condition_variable>
@@ -36,9 +43,12 @@ int64_t now_ns() {
 struct Job {
   int64_t id;
   cqr_request req;
-  size_t bytes;
+  size_t bytes;     // footprint: batched (one-panel requests) or single solve
+  bool batch;       // goes through the batched path
   cqr_result res;
 };
+
+bool one_panel(const cqr_request& r) { return r.n <= 128 && r.nrhs <= 16; }
 
 }  // namespace
 
@@ -47,6 +57,7 @@ struct cqr_server {
   size_t budget = 0, in_use = 0;   // in_use: Σ the slots' resident arenas
   std::vector<size_t> resident;    // per slot: its context's arena bytes
   int waiters = 0;                 // threads inside cqr_server_wait
+  int max_batch = 32;              // one-panel requests per batch (1: no batching)
   int64_t next_id = 0;
   bool stop = false;
   std::mutex mu;
@@ -69,24 +80,96 @@ size_t arena_after(const cqr_ctx* c, size_t bytes) {
   return have >= bytes ? have : bytes + bytes / 8;
 }
 
+// Runs a batch of one-panel requests of one tier; fills their results.
+void run_batch(cqr_ctx* c, std::vector<Job>& jobs, int slot) {
+  const int64_t t0 = now_ns();
+  const size_t k = jobs.size();
+  std::vector<cqr_request> rq(k);
+  std::vector<int> st(k, CQR_OK), it(k, 0);
+  std::vector<double> upd(k, 0.0);
+  size_t bytes = 0;
+  for (size_t j = 0; j < k; ++j) {
+    rq[j] = jobs[j].req;
+    bytes += jobs[j].bytes;
+  }
+  const bool mixed = rq[0].tier == CQR_TIER_MIXED;
+  int rc = cqr_reserve(c, bytes);
+  if (rc == CQR_OK)
+    rc = cqr_lstsq_batched(c, (int64_t)k, rq.data(), mixed ? 30 : 4, mixed ? 1e-10 : 1e-12, st.data(), it.data(),
+                           upd.data());
+  if (rc == CQR_OK) rc = cqr_sync(c);
+  const int64_t t1 = now_ns();
+  for (size_t j = 0; j < k; ++j) {
+    cqr_result& r = jobs[j].res;
+    r.t_start_ns = t0;
+    r.t_end_ns = t1;
+    r.status = rc != CQR_OK ? rc : st[j];
+    r.iters = it[j];
+    r.last_update = upd[j];
+    r.prec = rc != CQR_OK ? -1 : CQR_PREC_F64;   // one panel: factored in FP64
+    r.slot = slot;
+    r.batch = (int)k;
+  }
+}
+
+// Runs one request through cqr_lstsq; fills its result.
+void run_single(cqr_ctx* c, Job& job, int slot) {
+  const cqr_request& r = job.req;
+  job.res.t_start_ns = now_ns();
+  const bool mixed = r.tier == CQR_TIER_MIXED;
+  int iters = 0;
+  double upd = 0.0;
+  // the arena grows here, to what admission charged, not inside the solve
+  int st = cqr_reserve(c, job.bytes);
+  if (st == CQR_OK)
+    st = cqr_lstsq(c, r.A, r.lda, r.m, r.n, r.B, r.ldb, r.nrhs, r.X, r.ldx, 128,
+                   mixed ? CQR_PREC_F32 : CQR_PREC_F64, mixed ? 30 : 4, mixed ? 1e-10 : 1e-12, &iters, &upd);
+  if (st == CQR_OK || st == CQR_ERR_NOCONVERGE) {
+    const int ss = cqr_sync(c);
+    if (ss != CQR_OK) st = ss;
+  }
+  job.res.t_end_ns = now_ns();
+  job.res.status = st;
+  job.res.iters = iters;
+  job.res.last_update = upd;
+  job.res.prec = cqr_lstsq_last_prec(c);
+  job.res.slot = slot;
+  job.res.batch = 1;
+}
+
 void serve(cqr_server* s, int slot) {
   cqr_ctx* c = s->ctxs[slot];
   (void)hipSetDevice(s->device);
   for (;;) {
-    Job job;
+    std::vector<Job> jobs;
     {
       std::unique_lock<std::mutex> lk(s->mu);
       for (;;) {
         if (s->stop) return;
         if (!s->queue.empty()) {
           // FIFO: the head is admitted when this slot's arena, grown for it,
-          // keeps the sum of all resident arenas within the budget
-          const size_t after = arena_after(c, s->queue.front().bytes);
+          // keeps the sum of all resident arenas within the budget; a
+          // one-panel head takes the one-panel requests of its tier queued
+          // behind it along, as far as the budget and max_batch allow
+          const Job& head = s->queue.front();
+          const size_t after = arena_after(c, head.bytes);
           if (s->in_use - s->resident[slot] + after <= s->budget) {
-            job = s->queue.front();
-            s->queue.pop_front();
-            s->in_use += after - s->resident[slot];
-            s->resident[slot] = after;
+            size_t sum = head.bytes, charge = after;
+            std::vector<size_t> pick{0};
+            if (head.batch)
+              for (size_t q = 1; q < s->queue.size() && (int)pick.size() < s->max_batch; ++q) {
+                const Job& x = s->queue[q];
+                if (!x.batch || x.req.tier != head.req.tier) continue;
+                const size_t a = arena_after(c, sum + x.bytes);
+                if (s->in_use - s->resident[slot] + a > s->budget) break;
+                sum += x.bytes;
+                charge = a;
+                pick.push_back(q);
+              }
+            for (size_t q : pick) jobs.push_back(s->queue[q]);
+            for (size_t i = pick.size(); i-- > 0;) s->queue.erase(s->queue.begin() + (long)pick[i]);
+            s->in_use += charge - s->resident[slot];
+            s->resident[slot] = charge;
             break;
           }
           if (s->resident[slot] > 0) {
@@ -104,35 +187,19 @@ void serve(cqr_server* s, int slot) {
         s->cv_work.wait(lk);
       }
     }
-    const cqr_request& r = job.req;
-    job.res.t_start_ns = now_ns();
-    const bool mixed = r.tier == CQR_TIER_MIXED;
-    int iters = 0;
-    double upd = 0.0;
-    // the arena grows here, to what admission charged, not inside the solve
-    int st = cqr_reserve(c, job.bytes);
-    if (st == CQR_OK)
-      st = cqr_lstsq(c, r.A, r.lda, r.m, r.n, r.B, r.ldb, r.nrhs, r.X, r.ldx, 128,
-                     mixed ? CQR_PREC_F32 : CQR_PREC_F64, mixed ? 30 : 4, mixed ? 1e-10 : 1e-12, &iters, &upd);
-    if (st == CQR_OK || st == CQR_ERR_NOCONVERGE) {
-      const int ss = cqr_sync(c);
-      if (ss != CQR_OK) st = ss;
-    }
-    job.res.t_end_ns = now_ns();
-    job.res.status = st;
-    job.res.iters = iters;
-    job.res.last_update = upd;
-    job.res.prec = cqr_lstsq_last_prec(c);
-    job.res.slot = slot;
+    if (jobs[0].batch) run_batch(c, jobs, slot);
+    else run_single(c, jobs[0], slot);
     const size_t held = cqr_workspace_bytes(c);
-    job.res.bytes = (int64_t)held;
     {
       std::lock_guard<std::mutex> lk(s->mu);
       // the arena stays resident with the slot (charged until released)
       s->in_use += held - s->resident[slot];
       s->resident[slot] = held;
-      s->done[job.id] = job.res;
-      s->done_order.push_back(job.id);
+      for (Job& job : jobs) {
+        job.res.bytes = (int64_t)held;
+        s->done[job.id] = job.res;
+        s->done_order.push_back(job.id);
+      }
     }
     s->cv_work.notify_all();
     s->cv_done.notify_all();
@@ -190,6 +257,7 @@ int cqr_server_destroy(cqr_server* s) {
       j.res.status = CQR_ERR_CANCELLED;
       j.res.prec = -1;
       j.res.slot = -1;
+      j.res.batch = 0;
       s->done[j.id] = j.res;
       s->done_order.push_back(j.id);
     }
@@ -210,12 +278,19 @@ int64_t cqr_server_submit(cqr_server* s, const cqr_request* r) {
   if (!r->A || !r->B || !r->X || r->m < r->n || r->n < 1 || r->nrhs < 1 || r->nrhs > 16 || r->lda < r->m ||
       r->ldb < r->m || r->ldx < r->n || (r->tier != CQR_TIER_F64 && r->tier != CQR_TIER_MIXED))
     return -CQR_ERR_INVALID;
-  const size_t bytes = cqr_lstsq_workspace_bytes(r->m, r->n, r->nrhs, 128);
+  bool batch;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    batch = s->max_batch > 1 && one_panel(*r);
+  }
+  const size_t bytes =
+      batch ? cqr_lstsq_batched_workspace_bytes(1, r) : cqr_lstsq_workspace_bytes(r->m, r->n, r->nrhs, 128);
   // can never be admitted: its arena (with headroom) exceeds the budget
   if (bytes == 0 || bytes + bytes / 8 > s->budget) return -CQR_ERR_NOMEM;
   Job j;
   j.req = *r;
   j.bytes = bytes;
+  j.batch = batch;
   j.res = cqr_result{};
   j.res.t_submit_ns = now_ns();
   j.res.bytes = (int64_t)bytes;
@@ -271,6 +346,13 @@ int cqr_server_drain(cqr_server* s, cqr_result* out, int max, int* count) {
   }
   if (k) s->cv_done.notify_all();  // waiters on a drained id return
   *count = k;
+  return CQR_OK;
+}
+
+int cqr_server_set_batching(cqr_server* s, int max_batch) {
+  if (!s || max_batch < 1 || max_batch > 4096) return CQR_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->max_batch = max_batch;
   return CQR_OK;
 }
 

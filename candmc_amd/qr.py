@@ -442,7 +442,7 @@ class _Request(ctypes.Structure):
 
 class _Result(ctypes.Structure):
     _fields_ = [("id", ctypes.c_int64), ("status", ctypes.c_int), ("iters", ctypes.c_int), ("slot", ctypes.c_int),
-                ("prec", ctypes.c_int), ("last_update", ctypes.c_double), ("t_submit_ns", ctypes.c_int64), ("t_start_ns", ctypes.c_int64),
+                ("prec", ctypes.c_int), ("batch", ctypes.c_int), ("last_update", ctypes.c_double), ("t_submit_ns", ctypes.c_int64), ("t_start_ns", ctypes.c_int64),
                 ("t_end_ns", ctypes.c_int64), ("bytes", ctypes.c_int64)]
 
 
@@ -452,6 +452,29 @@ def _res(r):
 
 def lstsq_workspace_bytes(m, n, nrhs, b=128):
     return int(load().cqr_lstsq_workspace_bytes(m, n, nrhs, b))
+
+
+def _requests(reqs):
+    arr = (_Request * len(reqs))()
+    for i, (A, lda, m, n, B, ldb, nrhs, X, ldx) in enumerate(reqs):
+        arr[i] = _Request(_pd(A), lda, m, n, _pd(B), ldb, nrhs, _pd(X), ldx, TIER_F64)
+    return arr
+
+
+def lstsq_batched(reqs, max_iter=4, tol=1e-12):
+    """Least squares of many one-panel problems (n ≤ 128): reqs is a list of
+    (A, lda, m, n, B, ldb, nrhs, X, ldx); one batched FP64 factorization,
+    then each solve refined to tol.  Returns [(status, iters, last_update)]."""
+    ctx = context()
+    k = len(reqs)
+    arr = _requests(reqs)
+    st, it, up = (ctypes.c_int * k)(), (ctypes.c_int * k)(), (ctypes.c_double * k)()
+    ctx.check(ctx.lib.cqr_lstsq_batched(ctx.h, k, arr, int(max_iter), float(tol), st, it, up))
+    return [(st[i], it[i], up[i]) for i in range(k)]
+
+
+def lstsq_batched_workspace_bytes(reqs):
+    return int(load().cqr_lstsq_batched_workspace_bytes(len(reqs), _requests(reqs)))
 
 
 class Server:
@@ -493,6 +516,11 @@ class Server:
         for r in out:
             self._keep.pop(r["id"], None)
         return out
+
+    def set_batching(self, max_batch):
+        st = self.lib.cqr_server_set_batching(self.h, int(max_batch))
+        if st:
+            raise CQRError(st, "server_set_batching")
 
     def pending(self):
         q, b = ctypes.c_int64(), ctypes.c_size_t()

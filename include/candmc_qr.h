@@ -292,6 +292,16 @@ int cqr_trsm_r(cqr_ctx* ctx, const double* YR, int64_t ldy, int64_t n, double* B
 int cqr_lstsq(cqr_ctx* ctx, const double* A, int64_t lda, int64_t m, int64_t n, const double* B,
               int64_t ldb, int64_t nrhs, double* X, int64_t ldx, int64_t b, int prec, int max_iter,
               double tol, int* iters, double* last_update);
+/* The solve and FP64 refinement of cqr_lstsq from factors already made:
+ * YR holds [Y\R] of A (m×n, ld ldyr) as cqr_batch_qr or QR_2D leave it,
+ * Tt the Tᵀ blocks of its panels of width b (⌈n/b⌉ blocks of b·b; one
+ * n×n block for a batched problem, b = n).  A is the FP64 operator of the
+ * residuals.  Statuses as cqr_lstsq.  No reference counterpart: it lets a
+ * server factor many requests in one batch and then solve each.          */
+int cqr_lstsq_factored(cqr_ctx* ctx, const double* A, int64_t lda, int64_t m, int64_t n, const double* YR,
+                       int64_t ldyr, const double* Tt, int64_t b, const double* B, int64_t ldb, int64_t nrhs,
+                       double* X, int64_t ldx, int max_iter, double tol, int* iters, double* last_update);
+size_t cqr_lstsq_factored_workspace_bytes(int64_t m, int64_t n, int64_t nrhs, int64_t b);
 /* Escalation of stalled mixed-precision solves to FP64 (on by default).   */
 int cqr_set_lstsq_escalate(cqr_ctx* ctx, int on);
 /* Precision of the factorization that produced the last cqr_lstsq's X
@@ -347,17 +357,35 @@ typedef struct cqr_result {
   int status, iters, slot;
   int prec;          /* factorization precision of X (CQR_PREC_*): a mixed
                         request escalated to FP64 reports CQR_PREC_F64     */
+  int batch;         /* requests factored together with it (1: alone)      */
   double last_update;
   int64_t t_submit_ns, t_start_ns, t_end_ns;   /* steady clock            */
   int64_t bytes;                               /* admitted footprint      */
 } cqr_result;
+/* Least squares for many requests at once (n ≤ 128, nrhs ≤ 16 each): all
+ * of them factored in FP64 by one batched TSQR + Householder reconstruction
+ * (the launches of cqr_batch_qr, per width group), then each solved and
+ * refined in FP64 as cqr_lstsq does, to tol within max_iter steps.  A, B
+ * are read and X written as in cqr_lstsq; the tier field is not used.
+ * status[p] (and iters[p], last_update[p] when non-null) report request p
+ * with cqr_lstsq's statuses.  The context's arena holds everything
+ * (cqr_lstsq_batched_workspace_bytes; 0 for requests it cannot take).
+ * Each request's X depends only on that request.  No reference
+ * counterpart: it is how the server below batches solves.                */
+int cqr_lstsq_batched(cqr_ctx* ctx, int64_t nprob, const cqr_request* reqs, int max_iter, double tol,
+                      int* status, int* iters, double* last_update);
+size_t cqr_lstsq_batched_workspace_bytes(int64_t nprob, const cqr_request* reqs);
 typedef struct cqr_server cqr_server;
 /* Workspace bytes one least-squares solve needs (its admission footprint). */
 size_t cqr_lstsq_workspace_bytes(int64_t m, int64_t n, int64_t nrhs, int64_t b);
 int cqr_server_create(cqr_server** server, int device, int nslots, size_t hbm_budget);
 /* Waits for the requests already running; queued requests that have not
- * started are dropped (their results never appear).                     */
+ * started finish as CQR_ERR_CANCELLED (a thread waiting for one returns). */
 int cqr_server_destroy(cqr_server* server);
+/* One-panel requests (n ≤ 128, nrhs ≤ 16) are factored in batches of up to
+ * max_batch (default 32; 1 turns batching off) through cqr_lstsq_batched.
+ * Applies to requests submitted afterwards.                             */
+int cqr_server_set_batching(cqr_server* server, int max_batch);
 /* Returns the request id (≥ 0) or −status (e.g. −CQR_ERR_NOMEM when the
  * footprint exceeds the whole budget).                                   */
 int64_t cqr_server_submit(cqr_server* server, const cqr_request* request);

@@ -7,7 +7,9 @@
 // exactly one collector; every request of a burst through two slots is
 // served; the slots' resident arenas are what admission charges and never
 // exceed the budget; destroying a server with requests queued cancels them
-// (a waiter returns) and returns once the running one ends.  Built with g++ and run by tests/test_server_logic.py.
+// (a waiter returns) and returns once the running one ends; one-panel
+// requests are batched up to max_batch, within the budget, and wider ones
+// never are.  Built with g++ and run by tests/test_server_logic.py.
 #include <hip/hip_runtime_api.h>
 
 #include <atomic>
@@ -65,6 +67,26 @@ int cqr_lstsq(cqr_ctx*, const double*, int64_t, int64_t, int64_t, const double*,
   return CQR_OK;
 }
 size_t cqr_lstsq_workspace_bytes(int64_t m, int64_t, int64_t, int64_t) { return (size_t)m * 100; }
+static std::atomic<int> batch_max{0}, batch_calls{0};
+static std::atomic<int64_t> batch_wide{0};   // requests with n > 128 seen by a batch
+int cqr_lstsq_batched(cqr_ctx*, int64_t np, const cqr_request* rq, int, double, int* st, int* it, double* up) {
+  std::this_thread::sleep_for(std::chrono::milliseconds(solve_ms.load()));
+  for (int64_t p = 0; p < np; ++p) {
+    st[p] = CQR_OK;
+    it[p] = 1;
+    up[p] = 0.0;
+    if (rq[p].n > 128) ++batch_wide;
+  }
+  ++batch_calls;
+  int cur = batch_max.load();
+  while ((int)np > cur && !batch_max.compare_exchange_weak(cur, (int)np)) {}
+  return CQR_OK;
+}
+size_t cqr_lstsq_batched_workspace_bytes(int64_t np, const cqr_request* rq) {
+  size_t t = 0;
+  for (int64_t p = 0; p < np; ++p) t += (size_t)rq[p].m * 100;
+  return t;
+}
 }
 
 #define CHECK(x)                                         \
@@ -180,6 +202,7 @@ int main() {
   {
     cqr_server* sc = nullptr;
     CHECK(cqr_server_create(&sc, 0, 1, 1 << 20) == CQR_OK);
+    CHECK(cqr_server_set_batching(sc, 1) == CQR_OK);   // one request per run
     solve_ms = 100;
     const int64_t run = cqr_server_submit(sc, &r), queued_id = cqr_server_submit(sc, &r);
     (void)run;
@@ -198,11 +221,58 @@ int main() {
   // destroying with requests queued returns once the running one ends
   cqr_server* s1slot = nullptr;
   CHECK(cqr_server_create(&s1slot, 0, 1, 1 << 20) == CQR_OK);
+  CHECK(cqr_server_set_batching(s1slot, 1) == CQR_OK);
   solve_ms = 50;
   for (int i = 0; i < 20; ++i) CHECK(cqr_server_submit(s1slot, &r) >= 0);
   const auto t0 = std::chrono::steady_clock::now();
   CHECK(cqr_server_destroy(s1slot) == CQR_OK);
   CHECK(std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(500));
+  // batching: one-panel requests queued behind a running one are taken
+  // together, at most max_batch at a time and within the budget; a request
+  // wider than one panel is never part of a batch
+  {
+    cqr_server* sd = nullptr;
+    CHECK(cqr_server_create(&sd, 0, 1, 1 << 20) == CQR_OK);
+    CHECK(cqr_server_set_batching(sd, 8) == CQR_OK);
+    CHECK(cqr_server_set_batching(sd, 0) == CQR_ERR_INVALID);
+    batch_max = 0;
+    batch_calls = 0;
+    batch_wide = 0;
+    arena_peak = 0;
+    solve_ms = 20;
+    cqr_request wide = r;
+    wide.n = 200; wide.m = 400; wide.lda = wide.ldb = 400; wide.ldx = 200;
+    std::vector<int64_t> q;
+    q.push_back(cqr_server_submit(sd, &r));
+    std::this_thread::sleep_for(std::chrono::milliseconds(5));
+    for (int i = 0; i < 30; ++i) q.push_back(cqr_server_submit(sd, i == 10 ? &wide : &r));
+    int most = 0, singles_wide = 0;
+    for (int64_t id : q) {
+      CHECK(cqr_server_wait(sd, id, &res) == CQR_OK && res.status == CQR_OK);
+      CHECK(res.batch >= 1 && res.batch <= 8);
+      most = res.batch > most ? res.batch : most;
+      if (id == q[11]) singles_wide = res.batch;
+    }
+    CHECK(most == 8 && batch_max.load() == 8);
+    CHECK(singles_wide == 1 && batch_wide.load() == 0);
+    CHECK(batch_calls.load() < 30);
+    CHECK(arena_peak.load() <= (1u << 20));
+    CHECK(cqr_server_destroy(sd) == CQR_OK);
+  }
+  // batches stay within the budget: 4000 bytes, one-panel requests of 1000
+  // bytes each, so a batch holds at most three (Σ 3000, arena 3375)
+  {
+    cqr_server* se = nullptr;
+    CHECK(cqr_server_create(&se, 0, 1, 4000) == CQR_OK);
+    batch_max = 0;
+    arena_peak = 0;
+    solve_ms = 5;
+    std::vector<int64_t> q;
+    for (int i = 0; i < 20; ++i) q.push_back(cqr_server_submit(se, &r));
+    for (int64_t id : q) CHECK(cqr_server_wait(se, id, &res) == CQR_OK && res.status == CQR_OK);
+    CHECK(batch_max.load() <= 3 && arena_peak.load() <= 4000);
+    CHECK(cqr_server_destroy(se) == CQR_OK);
+  }
   printf("server logic ok\n");
   return 0;
 }

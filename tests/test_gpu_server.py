@@ -1,7 +1,8 @@
-"""GPU tests of the least-squares request server: each request gives the
-same bits alone and in a concurrent mix, both accuracy tiers meet their
-targets against scipy's FP64 lstsq, admission respects the HBM budget and
-oversized requests are refused."""
+"""GPU tests of the least-squares request server and of the batched solve
+it uses: each request gives the same bits alone and in a concurrent mix (or
+any batch), both accuracy tiers meet their targets against scipy's FP64
+lstsq, admission respects the HBM budget, oversized requests are refused,
+and one-panel requests are batched."""
 import numpy as np
 import pytest
 import scipy.linalg as sl
@@ -122,3 +123,70 @@ def test_oversized_request_refused(Q):
         srv.submit(A, 2000, 2000, 100, B, 2000, 1, X, 100)
     assert e.value.status == 3
     srv.close()
+
+
+def one_panel_problems(Q, seeds, kappa=1e4):
+    """(A, B, m, n, k) of one-panel least-squares problems, sizes from seeds."""
+    out = []
+    for s in seeds:
+        rng = np.random.default_rng(500 + s)
+        n = int(rng.choice([8, 16, 33, 64, 100, 128]))
+        m = int(rng.integers(max(n, 200), 6000))
+        k = int(rng.integers(1, 17))
+        A = O.conditioned_matrix(m, n, kappa, 600 + s)
+        B = np.asfortranarray(A @ rng.standard_normal((n, k)) + 1e-6 * rng.standard_normal((m, k)))
+        out.append((A, B, m, n, k))
+    return out
+
+
+def batched_solve(Q, probs):
+    dev = [(Q.to_gpu(A), Q.to_gpu(B), torch.zeros(n * k, dtype=torch.float64, device="cuda"))
+           for A, B, m, n, k in probs]
+    reqs = [(dA, m, m, n, dB, m, k, dX, n) for (A, B, m, n, k), (dA, dB, dX) in zip(probs, dev)]
+    res = Q.lstsq_batched(reqs, max_iter=4, tol=1e-12)
+    return [Q.from_gpu(dX, n, k) for (A, B, m, n, k), (_, _, dX) in zip(probs, dev)], res
+
+
+def test_lstsq_batched_against_scipy_and_batch_independent(Q):
+    """cqr_lstsq_batched: every problem within FP64 accuracy of scipy's
+    lstsq, and each X bit-identical whether solved in a batch of 24, in a
+    reordered sub-batch or alone."""
+    probs = one_panel_problems(Q, range(24))
+    xs, res = batched_solve(Q, probs)
+    for (A, B, m, n, k), x, (st, it, up) in zip(probs, xs, res):
+        want = sl.lstsq(A, B)[0]
+        err = np.abs(x - want).max() / np.abs(want).max()
+        assert st in (0, 6) and err < max(1e-12, 50 * 1e4 * EPS), (m, n, k, st, err)
+    sub = [5, 17, 2, 11, 0]
+    xs2, _ = batched_solve(Q, [probs[i] for i in sub])
+    for i, x in zip(sub, xs2):
+        assert np.array_equal(x, xs[i])
+    x1, _ = batched_solve(Q, [probs[7]])
+    assert np.array_equal(x1[0], xs[7])
+
+
+def test_server_batches_one_panel_requests(Q):
+    """A burst of one-panel requests through one slot is factored in batches
+    (res.batch > 1), results bit-identical to the same requests solved one
+    at a time (batches of one), within FP64 accuracy; a wider request in the
+    burst runs alone."""
+    probs = one_panel_problems(Q, range(100, 140))
+    reqs = [(A, B, m, n, k, 1e4, Q.TIER_F64) for A, B, m, n, k in probs]
+    wide = make_requests(Q)[0]                         # 3000×200: two panels
+    reqs.insert(10, wide[:6] + (Q.TIER_F64,))
+    srv = Q.Server(nslots=1)
+    srv.set_batching(16)
+    xb, rb = run(Q, srv, reqs, together=True)
+    srv.close()
+    one = Q.Server(nslots=1)
+    xo, ro = run(Q, one, reqs, together=False)
+    one.close()
+    assert max(r["batch"] for r in rb) > 1 and max(r["batch"] for r in rb) <= 16
+    assert rb[10]["batch"] == 1
+    assert all(r["batch"] == 1 for r in ro)
+    for q, a, b, r in zip(reqs, xb, xo, rb):
+        A, B = q[0], q[1]
+        assert r["status"] == 0, r
+        assert np.array_equal(a, b)
+        want = sl.lstsq(A, B)[0]
+        assert np.abs(a - want).max() / np.abs(want).max() < 1e-11
