@@ -287,13 +287,17 @@ def batch_leg(qr, torch, nprob=2048, seed=13, loop=True):
     return res
 
 
-def serve_leg(qr, torch, nreq=160, rates=(100.0, 400.0, 1600.0), nslots=4, seed=17):
-    """Poisson stream of least-squares requests: m lognormal (median 4096,
+def serve_leg(qr, torch, nreq=1000, rates=(500.0, 2000.0, 8000.0), nslots=4, seed=17, n_fixed=None):
+    """Poisson streams of least-squares requests: m lognormal (median 4096,
     σ = 1, capped at 65536), n ~ U[8, 256], 1-16 right-hand sides, column
-    scaling to κ ≤ 1e6; both tiers; p50/p99 latency and solves/s per rate."""
+    scaling to κ ≤ 1e6.  For both tiers, with the server's batching of
+    one-panel requests (n ≤ 128) off and on: p50/p99 latency and solves/s
+    at each offered rate, then a burst of all requests at once (the
+    capacity).  Each request's X is compared with the FP64 tier's X from
+    the unbatched server (refined to 1e-12)."""
     rng = np.random.default_rng(seed)
     ms = np.minimum(65536, np.maximum(1, rng.lognormal(np.log(4096), 1.0, nreq))).astype(np.int64)
-    ns = rng.integers(8, 257, nreq)
+    ns = rng.integers(8, 257, nreq) if n_fixed is None else np.full(nreq, n_fixed)
     ms = np.maximum(ms, ns)
     ks = rng.integers(1, 17, nreq)
     kap = 10.0 ** rng.uniform(0, 6, nreq)
@@ -306,37 +310,56 @@ def serve_leg(qr, torch, nreq=160, rates=(100.0, 400.0, 1600.0), nslots=4, seed=
         X = torch.zeros(int(n) * int(k), dtype=torch.float64, device="cuda")
         reqs.append((A, B, X, int(m), int(n), int(k)))
     torch.cuda.synchronize()
-    out = {"nreq": nreq, "nslots": nslots, "seed": seed,
-           "m": "lognormal(median 4096, sigma 1) capped at 65536", "n": "U[8,256]", "nrhs": "U{1..16}",
+    out = {"nreq": nreq, "nslots": nslots, "seed": seed, "one_panel_requests": int(np.sum(ns <= 128)),
+           "m": "lognormal(median 4096, sigma 1) capped at 65536",
+           "n": "U[8,256]" if n_fixed is None else str(n_fixed), "nrhs": "U{1..16}",
            "kappa": "column scaling 1..1e-6 (log-uniform)", "tiers": {}}
-    srv = qr.Server(nslots=nslots, hbm_budget=64 << 30)
-    for tier, tname in ((qr.TIER_MIXED, "mixed_1e-10"), (qr.TIER_F64, "fp64_1e-12")):
-        # warm every slot's arena with the largest request
-        big = int(np.argmax(ms * ns))
-        ids = [srv.submit(reqs[big][0], reqs[big][3], reqs[big][3], reqs[big][4], reqs[big][1], reqs[big][3],
-                          reqs[big][5], reqs[big][2], reqs[big][4], tier) for _ in range(nslots)]
-        for i in ids:
-            srv.wait(i)
-        rows = []
-        for rate in rates:
-            gaps = rng.exponential(1.0 / rate, nreq)
-            t0 = time.perf_counter()
-            due = t0 + np.cumsum(gaps)
-            ids = []
-            for (A, B, X, m, n, k), d in zip(reqs, due):
-                while time.perf_counter() < d:
-                    pass
-                ids.append(srv.submit(A, m, m, n, B, m, k, X, n, tier))
-            res = [srv.wait(i) for i in ids]
-            lat = np.array([(r["t_end_ns"] - r["t_submit_ns"]) * 1e-6 for r in res])
-            span = (max(r["t_end_ns"] for r in res) - min(r["t_submit_ns"] for r in res)) * 1e-9
-            rows.append({"offered_qps": rate, "solves_per_s": round(nreq / span, 1),
-                         "p50_ms": round(float(np.percentile(lat, 50)), 2),
-                         "p99_ms": round(float(np.percentile(lat, 99)), 2),
-                         "mean_iters": round(float(np.mean([r["iters"] for r in res])), 2),
-                         "failed": int(sum(r["status"] != 0 for r in res))})
-        out["tiers"][tname] = rows
-    srv.close()
+
+    def stream(srv, tier, rate):
+        """Submits all requests (Poisson arrivals at `rate`, or all at once
+        for rate None) and collects them."""
+        t0 = time.perf_counter()
+        due = t0 + (np.cumsum(rng.exponential(1.0 / rate, nreq)) if rate else np.zeros(nreq))
+        ids = []
+        for (A, B, X, m, n, k), d in zip(reqs, due):
+            while time.perf_counter() < d:
+                pass
+            ids.append(srv.submit(A, m, m, n, B, m, k, X, n, tier))
+        res = [srv.wait(i) for i in ids]
+        lat = np.array([(r["t_end_ns"] - r["t_submit_ns"]) * 1e-6 for r in res])
+        span = (max(r["t_end_ns"] for r in res) - min(r["t_submit_ns"] for r in res)) * 1e-9
+        return res, {"offered_qps": rate if rate else "burst", "solves_per_s": round(nreq / span, 1),
+                     "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                     "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                     "mean_iters": round(float(np.mean([r["iters"] for r in res])), 2),
+                     "mean_batch": round(float(np.mean([r["batch"] for r in res])), 2),
+                     "noconverge": int(sum(r["status"] == 6 for r in res)),
+                     "failed": int(sum(r["status"] not in (0, 6) for r in res))}
+
+    ref = None
+    for tier, tname in ((qr.TIER_F64, "fp64_1e-12"), (qr.TIER_MIXED, "mixed_1e-10")):
+        out["tiers"][tname] = {}
+        for batching in (1, 32):
+            srv = qr.Server(nslots=nslots, hbm_budget=64 << 30)
+            srv.set_batching(batching)
+            # warm every slot's arena with the largest request
+            big = int(np.argmax(ms * ns))
+            A, B, X, m, n, k = reqs[big]
+            for i in [srv.submit(A, m, m, n, B, m, k, X, n, tier) for _ in range(nslots)]:
+                srv.wait(i)
+            rows = [stream(srv, tier, rate)[1] for rate in rates]
+            res, burst = stream(srv, tier, None)
+            srv.close()
+            xs = [X.clone() for _, _, X, _, _, _ in reqs]
+            if ref is None:
+                ref = xs      # the FP64 tier without batching: the reference X
+            errs = np.array([float((x - r).abs().max() / r.abs().max().clamp_min(1e-300)) for x, r in zip(xs, ref)])
+            burst["err_vs_fp64_max"] = float(f"{errs.max():.3g}")
+            burst["err_vs_fp64_p99"] = float(f"{np.percentile(errs, 99):.3g}")
+            rows.append(burst)
+            out["tiers"][tname]["batching_off" if batching == 1 else "batching_32"] = rows
+    out["capacity_solves_per_s"] = {f"{t}/{b}": rows[-1]["solves_per_s"]
+                                    for t, d in out["tiers"].items() for b, rows in d.items()}
     return out
 
 
@@ -486,6 +509,9 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_serve:
         out["serving"] = serve_leg(qr, torch)
+        # the same stream with every request 64 columns wide: one shape, so
+        # the server's batches are one width group each
+        out["serving_n64"] = serve_leg(qr, torch, n_fixed=64)
         torch.cuda.empty_cache()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A_host, b, args.cpu_seconds)

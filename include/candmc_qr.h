@@ -349,8 +349,9 @@ typedef struct cqr_request {
   int64_t ldb, nrhs;
   double* X;
   int64_t ldx;
-  int tier;          /* CQR_TIER_F64: FP64 to 1e-12; CQR_TIER_MIXED: FP32
-                        trailing update, FP64 refinement to 1e-10          */
+  int tier;          /* CQR_TIER_F64: FP64 to 1e-12; CQR_TIER_MIXED: FP64
+                        refinement to 1e-10, with the FP32 trailing update
+                        when n ≥ 1024 (narrower: FP64 factors)            */
 } cqr_request;
 typedef struct cqr_result {
   int64_t id;

@@ -94,10 +94,11 @@ def test_admission_respects_budget(Q):
 
 
 def test_mixed_request_at_kappa_1e8_is_not_silent(Q):
-    """A mixed-tier request the FP32 factors cannot refine (κ = 1e8, κ·u₃₂
-    ≫ 1) is refactored in FP64 (prec reports it) and ends CQR_OK within
-    1e-10 or CQR_ERR_NOCONVERGE with FP64-quality X."""
-    m, n, k, kappa = 4096, 200, 2, 1e8
+    """A mixed-tier request wide enough for the FP32 trailing update
+    (n ≥ 1024) that the FP32 factors cannot refine (κ = 1e8, κ·u₃₂ ≫ 1) is
+    refactored in FP64 (prec reports it) and ends CQR_OK within 1e-10 or
+    CQR_ERR_NOCONVERGE with FP64-quality X."""
+    m, n, k, kappa = 4096, 1024, 2, 1e8
     A = O.conditioned_matrix(m, n, kappa, 300)
     rng = np.random.default_rng(301)
     B = np.asfortranarray(A @ rng.standard_normal((n, k)) + 1e-6 * rng.standard_normal((m, k)))
