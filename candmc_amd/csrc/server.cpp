@@ -16,8 +16,8 @@
 //
 // Requests of one panel (n ≤ 128, so no trailing update to make mixed) are
 // batched: a slot that takes such a request also takes the queued requests
-// of the same tier behind it that are one panel too (up to max_batch, while
-// the budget allows) and runs them through cqr_lstsq_batched, one batched
+// of the same tier and width behind it (up to max_batch, while the budget
+// allows) and runs them through cqr_lstsq_batched, one batched
 // factorization and then each solve.  A batched request's X depends only on
 // that request, so it gets the same bits in a batch of any size.
 // The serving loop has no counterpart in the reference.
@@ -169,7 +169,10 @@ void serve(cqr_server* s, int slot) {
             if (head.batch)
               for (size_t q = 1; q < s->queue.size() && (int)pick.size() < s->max_batch; ++q) {
                 const Job& x = s->queue[q];
-                if (!x.batch || x.req.tier != head.req.tier) continue;
+                // one width: a batch is then one launch sequence (requests of
+                // other widths stay queued for the other slots, which would
+                // otherwise idle while this one ran the widths in turn)
+                if (!x.batch || x.req.tier != head.req.tier || x.req.n != head.req.n) continue;
                 const size_t a = arena_after(c, sum + x.bytes);
                 if (s->in_use - s->resident[slot] + a > s->budget) break;
                 sum += x.bytes;

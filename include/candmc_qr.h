@@ -383,9 +383,10 @@ int cqr_server_create(cqr_server** server, int device, int nslots, size_t hbm_bu
 /* Waits for the requests already running; queued requests that have not
  * started finish as CQR_ERR_CANCELLED (a thread waiting for one returns). */
 int cqr_server_destroy(cqr_server* server);
-/* One-panel requests (n ≤ 128, nrhs ≤ 16) are factored in batches of up to
- * max_batch (default 32; 1 turns batching off) through cqr_lstsq_batched.
- * Applies to requests submitted afterwards.                             */
+/* One-panel requests (n ≤ 128, nrhs ≤ 16) of one tier and one width are
+ * factored in batches of up to max_batch (default 32; 1 turns batching
+ * off) through cqr_lstsq_batched.  Applies to requests submitted
+ * afterwards.                                                           */
 int cqr_server_set_batching(cqr_server* server, int max_batch);
 /* Returns the request id (≥ 0) or −status (e.g. −CQR_ERR_NOMEM when the
  * footprint exceeds the whole budget).                                   */

@@ -8,8 +8,8 @@
 // served; the slots' resident arenas are what admission charges and never
 // exceed the budget; destroying a server with requests queued cancels them
 // (a waiter returns) and returns once the running one ends; one-panel
-// requests are batched up to max_batch, within the budget, and wider ones
-// never are.  Built with g++ and run by tests/test_server_logic.py.
+// requests of one width are batched up to max_batch, within the budget, and
+// wider ones never are.  Built with g++ and run by tests/test_server_logic.py.
 #include <hip/hip_runtime_api.h>
 
 #include <atomic>
@@ -69,6 +69,7 @@ int cqr_lstsq(cqr_ctx*, const double*, int64_t, int64_t, int64_t, const double*,
 size_t cqr_lstsq_workspace_bytes(int64_t m, int64_t, int64_t, int64_t) { return (size_t)m * 100; }
 static std::atomic<int> batch_max{0}, batch_calls{0};
 static std::atomic<int64_t> batch_wide{0};   // requests with n > 128 seen by a batch
+static std::atomic<int64_t> batch_mixed{0};  // batches holding two widths
 int cqr_lstsq_batched(cqr_ctx*, int64_t np, const cqr_request* rq, int, double, int* st, int* it, double* up) {
   std::this_thread::sleep_for(std::chrono::milliseconds(solve_ms.load()));
   for (int64_t p = 0; p < np; ++p) {
@@ -76,6 +77,7 @@ int cqr_lstsq_batched(cqr_ctx*, int64_t np, const cqr_request* rq, int, double, 
     it[p] = 1;
     up[p] = 0.0;
     if (rq[p].n > 128) ++batch_wide;
+    if (rq[p].n != rq[0].n) ++batch_mixed;
   }
   ++batch_calls;
   int cur = batch_max.load();
@@ -245,7 +247,10 @@ int main() {
     std::vector<int64_t> q;
     q.push_back(cqr_server_submit(sd, &r));
     std::this_thread::sleep_for(std::chrono::milliseconds(5));
+    cqr_request other = r;            // one panel too, another width
+    other.n = 6; other.ldx = 6;
     for (int i = 0; i < 30; ++i) q.push_back(cqr_server_submit(sd, i == 10 ? &wide : &r));
+    for (int i = 0; i < 6; ++i) q.push_back(cqr_server_submit(sd, &other));
     int most = 0, singles_wide = 0;
     for (int64_t id : q) {
       CHECK(cqr_server_wait(sd, id, &res) == CQR_OK && res.status == CQR_OK);
@@ -255,6 +260,7 @@ int main() {
     }
     CHECK(most == 8 && batch_max.load() == 8);
     CHECK(singles_wide == 1 && batch_wide.load() == 0);
+    CHECK(batch_mixed.load() == 0);   // a batch holds one width
     CHECK(batch_calls.load() < 30);
     CHECK(arena_peak.load() <= (1u << 20));
     CHECK(cqr_server_destroy(sd) == CQR_OK);
