@@ -29,6 +29,7 @@ SIGNATURES = {
     "cqr_set_check_finite": (_I, [_P, _I]),
     "cqr_set_leaf_rows": (_I, [_P, _I64]),
     "cqr_set_lookahead": (_I, [_P, _I]),
+    "cqr_set_wait_limit": (_I, [_P, _I]),
     "cqr_status_string": (_c.c_char_p, [_I]),
     "cqr_profile_enable": (_I, [_P, _I]),
     "cqr_profile_read": (_I, [_P, _I, _c.POINTER(_D), _c.POINTER(_D), _c.POINTER(_D), _c.POINTER(_I64)]),

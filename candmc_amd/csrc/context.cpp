@@ -160,10 +160,10 @@ int cqr_create(cqr_ctx** ctx, int device) {
   c->device = device;
   if (hipMalloc(&c->d_flag, sizeof(int)) != hipSuccess ||
       hipMalloc(&c->d_cnt, sizeof(uint64_t) * CQR_MAX_TREE_NODES * 2) != hipSuccess ||
-      hipMalloc(&c->d_terr, sizeof(int)) != hipSuccess ||
+      hipMalloc(&c->d_terr, 2 * sizeof(int)) != hipSuccess ||   // [0] timeout flag, [1] poll bound
       hipMalloc(&c->d_qflags, sizeof(uint64_t) * CQR_MAX_TREE_NODES * 8) != hipSuccess ||
       hipMemset(c->d_qflags, 0, sizeof(uint64_t) * CQR_MAX_TREE_NODES * 8) != hipSuccess ||
-      hipMemset(c->d_terr, 0, sizeof(int)) != hipSuccess ||
+      hipMemset(c->d_terr, 0, 2 * sizeof(int)) != hipSuccess ||
       hipMemset(c->d_cnt, 0, sizeof(uint64_t) * CQR_MAX_TREE_NODES * 2) != hipSuccess ||
       hipMemset(c->d_flag, 0, sizeof(int)) != hipSuccess ||
       hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
@@ -231,6 +231,14 @@ int cqr_sync(cqr_ctx* c) {
     CQR_HIP(c, hipMemset(c->d_terr, 0, sizeof(int)));
     return set_err(c, CQR_ERR_DEVICE, "TSQR tree node timed out waiting for a child");
   }
+  return CQR_OK;
+}
+
+int cqr_set_wait_limit(cqr_ctx* c, int polls) {
+  CQR_CHECK_CTX(c);
+  if (polls < 0) return set_err(c, CQR_ERR_INVALID, "set_wait_limit: polls must be >= 0");
+  CQR_HIP(c, hipStreamSynchronize(c->stream));
+  CQR_HIP(c, hipMemcpy(c->d_terr + 1, &polls, sizeof(int), hipMemcpyHostToDevice));
   return CQR_OK;
 }
 

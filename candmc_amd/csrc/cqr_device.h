@@ -128,14 +128,21 @@ __device__ inline void flag_store(uint64_t* f, uint64_t v) {
 }
 // All of this wave's outstanding memory operations have completed.
 __device__ inline void wait_memory() { __builtin_amdgcn_s_waitcnt(0); }
-// Bounded wait (by one thread) for *f ≥ want; false on timeout.
-__device__ inline bool flag_wait(const uint64_t* f, uint64_t want) {
+// Bounded wait (by one thread) for *f ≥ want, at most `limit` polls; false
+// on timeout.
+__device__ inline bool flag_wait(const uint64_t* f, uint64_t want, long limit) {
   long spins = 0;
   while (flag_load(f) < want) {
     __builtin_amdgcn_s_sleep(2);
-    if (++spins > (1L << 22)) return false;
+    if (++spins > limit) return false;
   }
   return true;
+}
+// The poll bound of a launch: err[1] when the host set one (a diagnostic
+// override that lets a test drive the timeout path), else 2²² polls.
+__device__ inline long wait_limit(const int* err) {
+  const int v = err ? err[1] : 0;
+  return v > 0 ? (long)v : (1L << 22);
 }
 
 // ---------------------------------------------------------------------------

@@ -87,8 +87,8 @@ ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, doubl
   // Thread 0 waits until both inputs have published column i (bounded).
   auto wait_inputs = [&](int i) {
     const uint64_t want = epoch * 256 + (uint64_t)(i + 1);
-    if (ptop >= 0 && !flag_wait(cnt + ptop, want)) s_err = 1;
-    if (pbot >= 0 && !flag_wait(cnt + pbot, want)) s_err = 1;
+    if (ptop >= 0 && !flag_wait(cnt + ptop, want, wait_limit(err))) s_err = 1;
+    if (pbot >= 0 && !flag_wait(cnt + pbot, want, wait_limit(err))) s_err = 1;
   };
   // Column i of both inputs (rows ≤ i), device-coherent loads.
   double ctop[RPL], cbot[RPL], ntop[RPL], nbot[RPL];
@@ -428,13 +428,13 @@ __device__ __forceinline__ void btree_body(const double* __restrict__ leafR, dou
     if (tid == 0) {
       const uint64_t want = epoch * 256 + (uint64_t)(c0 + NB);
       if (ptop >= 0) {
-        if (!flag_wait(cnt + ptop, want)) s_err = 1;
-      } else if (lcnt && !flag_wait(lcnt + z, want)) {
+        if (!flag_wait(cnt + ptop, want, wait_limit(err))) s_err = 1;
+      } else if (lcnt && !flag_wait(lcnt + z, want, wait_limit(err))) {
         s_err = 1;
       }
       if (pbot >= 0) {
-        if (!flag_wait(cnt + pbot, want)) s_err = 1;
-      } else if (lcnt && !flag_wait(lcnt + snd, want)) {
+        if (!flag_wait(cnt + pbot, want, wait_limit(err))) s_err = 1;
+      } else if (lcnt && !flag_wait(lcnt + snd, want, wait_limit(err))) {
         s_err = 1;
       }
     }

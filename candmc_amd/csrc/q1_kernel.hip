@@ -105,7 +105,7 @@ q1_kernel(const double* __restrict__ nodeV2, const double* __restrict__ nodeT, c
   // ---- input block: C_z[:, j0:j0+16] (identity at the root)
   if (tid == 0) s_ok = 1;
   if (parent >= 0 && tid == 0) {
-    if (!flag_wait(flags + (int64_t)parent * ncb + cb, epoch)) s_ok = 0;
+    if (!flag_wait(flags + (int64_t)parent * ncb + cb, epoch, wait_limit(err))) s_ok = 0;
   }
   __syncthreads();
   if (!s_ok) {

@@ -71,6 +71,10 @@ class Context:
     def set_lookahead(self, on: bool):
         self.check(self.lib.cqr_set_lookahead(self.h, 1 if on else 0))
 
+    def set_wait_limit(self, polls: int):
+        """Diagnostic bound on in-launch waits (0: default)."""
+        self.check(self.lib.cqr_set_wait_limit(self.h, int(polls)))
+
     def set_check_finite(self, on: bool):
         self.check(self.lib.cqr_set_check_finite(self.h, 1 if on else 0))
 

@@ -70,6 +70,13 @@ int cqr_set_leaf_rows(cqr_ctx* ctx, int64_t rows);    /* 0 = automatic        */
  * alg/QR/qr_2d/qr_2d.cxx:378-635, QR_2D_pipe).  Results differ from the
  * non-lookahead order only in the summation order of Yᵀ·A.               */
 int cqr_set_lookahead(cqr_ctx* ctx, int on);
+/* Diagnostic: bounds every wait between workgroups of one launch (TSQR tree,
+ * construct_Q1) to `polls` polls; 0 restores the default (2²², far beyond
+ * any real wait).  A wait that gives up makes its launch finish with what
+ * it has, and the next cqr_sync returns CQR_ERR_DEVICE (and clears the
+ * flag); a test uses a tiny bound to exercise that path.  No reference
+ * counterpart.                                                          */
+int cqr_set_wait_limit(cqr_ctx* ctx, int polls);
 /* Kernel timing: while enabled, the library records a HIP event pair on
  * its stream around every kernel it launches; cqr_profile_read waits for
  * the stream and returns, for one kernel class, the summed event time,
