@@ -109,34 +109,56 @@ def cpu_dgeqrf(A):
             "kind": "lapack", "sample": f"dgeqrf of the same {m}x{n} matrix", "seconds": round(dt, 3)}
 
 
-def cpu_baseline(A, b, max_seconds):
-    """The oracle's restatement of QR_2D (numpy + LAPACK), timed on host cores."""
+def _panels_flops(m, n, b, panels):
+    p = min(panels * b, n)
+    return qr_flops(m, n) - (qr_flops(m - p, n - p) if p < n else 0.0)
+
+
+def cpu_threaded(A, b, max_seconds):
+    """The oracle's QR_2D (numpy + LAPACK) in one process on threaded BLAS."""
     from threadpoolctl import threadpool_info
 
     from oracle import candmc_oracle as O
     m, n = A.shape
     threads = max([d.get("num_threads", 1) for d in threadpool_info() if d.get("user_api") == "blas"] or [1])
-    # estimate: the full factorization when it fits the time budget, else the
-    # first panels (flops = F(m,n) − F(m−pb, n−pb))
+    # the full factorization when it fits the time budget, else the first
+    # panels (flops = F(m,n) − F(m−pb, n−pb))
     W = np.asfortranarray(A.copy())
     t = time.perf_counter()
     O.QR_2D(W, b, max_panels=2, copy=False)
     t2 = time.perf_counter() - t
-    f2 = qr_flops(m, n) - qr_flops(m - 2 * b, n - 2 * b)
-    est_full = t2 * qr_flops(m, n) / f2 * 0.8
+    est_full = t2 * qr_flops(m, n) / _panels_flops(m, n, b, 2) * 0.8
     panels = n // b if est_full <= max_seconds else max(2, int(max_seconds / t2 * 2))
     panels = min(panels, n // b)
     W = np.asfortranarray(A.copy())
     t = time.perf_counter()
     O.QR_2D(W, b, max_panels=panels, copy=False)
     dt = time.perf_counter() - t
-    p = min(panels * b, n)
-    flops = qr_flops(m, n) - (qr_flops(m - p, n - p) if p < n else 0.0)
-    sample = (f"QR_2D of the same {m}x{n} matrix, b={b}, "
+    sample = (f"QR_2D of the same {m}x{n} matrix, b={b}, one process, threaded BLAS, "
               + ("full factorization" if panels >= n // b else f"first {panels} panels"))
-    return {"value": round(flops / dt / 1e9, 2), "unit": "GFLOP/s", "cores": int(threads),
+    return {"value": round(_panels_flops(m, n, b, panels) / dt / 1e9, 2), "unit": "GFLOP/s",
+            "cores": int(threads), "kind": "port", "sample": sample, "seconds": round(dt, 3)}
+
+
+def cpu_baseline(A, b, max_seconds):
+    """The reference's CPU path as it runs: QR_2D on a P×1 process grid, one
+    single-threaded LAPACK rank per host core (P = the cores this process may
+    use), restated in oracle/candmc_ranks.py; plus the same restatement in
+    one process on threaded BLAS and a multithreaded dgeqrf line."""
+    from oracle import candmc_ranks as CR
+    m, n = A.shape
+    P = len(os.sched_getaffinity(0))
+    _, t2 = CR.qr2d_ranks(A, b, P, max_panels=2, gather=False)
+    est_full = t2 * qr_flops(m, n) / _panels_flops(m, n, b, 2) * 0.8
+    panels = n // b if est_full <= max_seconds else max(2, int(max_seconds / t2 * 2))
+    panels = min(panels, n // b)
+    _, dt = CR.qr2d_ranks(A, b, P, max_panels=panels, gather=False)
+    sample = (f"QR_2D of the same {m}x{n} matrix, b={b}, on {P} single-threaded rank processes "
+              f"(P x 1 grid, b-row blocks cyclic), "
+              + ("full factorization" if panels >= n // b else f"first {panels} panels"))
+    return {"value": round(_panels_flops(m, n, b, panels) / dt / 1e9, 2), "unit": "GFLOP/s", "cores": int(P),
             "kind": "port", "sample": sample, "seconds": round(dt, 3), "host": host_info(),
-            "dgeqrf": cpu_dgeqrf(A)}
+            "threaded_blas": cpu_threaded(A, b, max_seconds), "dgeqrf": cpu_dgeqrf(A)}
 
 
 # --------------------------------------------------------------------------
@@ -387,12 +409,17 @@ def main():
               "factors each matrix on a single MI355X and has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
+    m, n, b = args.m, args.n, args.b
+    A_host = gaussian(m, n, args.seed)
+    # the CPU baseline starts its rank processes before this process touches
+    # the GPU (a child forked from a process with the GPU initialised does
+    # not start other programs cleanly everywhere)
+    cpu = None if args.no_cpu else cpu_baseline(A_host, b, args.cpu_seconds)
+
     import torch
 
     from candmc_amd import qr
 
-    m, n, b = args.m, args.n, args.b
-    A_host = gaussian(m, n, args.seed)
     dev = torch.device("cuda:0")
     A0 = qr.to_gpu(A_host)
     A = torch.empty_like(A0)
@@ -513,8 +540,8 @@ def main():
         # the server's batches are one width group each
         out["serving_n64"] = serve_leg(qr, torch, n_fixed=64)
         torch.cuda.empty_cache()
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(A_host, b, args.cpu_seconds)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
 

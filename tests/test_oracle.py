@@ -508,3 +508,17 @@ def test_tree_data_packed_layout():
                                   np.triu(V2))
             # the blocked T's diagonal is the nodes' τ: the reflectors reproduce [R_r; R_s] → [R; 0]
             assert np.all(np.abs(np.diag(T[np.arange(b) % nb, np.arange(b)])) <= 2.0)
+
+
+@pytest.mark.parametrize("m,n,b,P", [(1024, 256, 64, 4), (768, 192, 64, 3), (512, 256, 64, 8)])
+def test_qr2d_on_rank_processes_matches_lapack(m, n, b, P):
+    """The CPU baseline's restatement of QR_2D on a P×1 grid of rank
+    processes (b-row blocks cyclic, tree over the working ranks, HR on the
+    panel's owner, Σ Yᵀ·A reduced across ranks) gives dgeqrf's [Y\\R]; with
+    P = 8 and 8 blocks the last panels run on fewer working ranks."""
+    from oracle import candmc_ranks as CR
+    A = O.gaussian(m, n, 21 + P)
+    YR, sec = CR.qr2d_ranks(A, b, P)
+    q, tau, work, info = sl.lapack.dgeqrf(np.asfortranarray(A))
+    assert info == 0 and sec > 0
+    assert np.abs(YR - q).max() / np.abs(q).max() < 1e-12
