@@ -804,8 +804,8 @@ int leaf_qr_batched(cqr_ctx* c, double* Abase, int64_t total_m, int L, int64_t r
                                   : leaf_launch<8, 4, 16>(c, Abase, 0, total_m, L, b, R, T, tau, bp, nprob);
   if (b <= 64) return rows <= 256 ? leaf_launch<8, 8, 4>(c, Abase, 0, total_m, L, b, R, T, tau, bp, nprob)
                                   : leaf_launch<8, 8, 8>(c, Abase, 0, total_m, L, b, R, T, tau, bp, nprob);
-  return rows <= 128 ? leaf_launch<8, 16, 2>(c, Abase, 0, total_m, L, b, R, T, tau, bp, nprob)
-                     : leaf_launch<8, 16, 4>(c, Abase, 0, total_m, L, b, R, T, tau, bp, nprob);
+  return rows <= 128 ? leaf_launch<16, 8, 2>(c, Abase, 0, total_m, L, b, R, T, tau, bp, nprob)
+                     : leaf_launch<16, 8, 4>(c, Abase, 0, total_m, L, b, R, T, tau, bp, nprob);
 }
 
 int leaf_qr(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int b, double* R, double* T,
@@ -817,8 +817,10 @@ int leaf_qr(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int b, double*
                                   : leaf_launch<8, 4, 16>(c, A, lda, m, L, b, R, T, tau);
   if (b <= 64) return rows <= 256 ? leaf_launch<8, 8, 4>(c, A, lda, m, L, b, R, T, tau)
                                   : leaf_launch<8, 8, 8>(c, A, lda, m, L, b, R, T, tau);
-  return rows <= 128 ? leaf_launch<8, 16, 2>(c, A, lda, m, L, b, R, T, tau)
-                     : leaf_launch<8, 16, 4>(c, A, lda, m, L, b, R, T, tau);
+  // b > 64: sixteen waves of eight columns (109 VGPRs, four waves per SIMD
+  // hide more of the column step's latency than eight waves of sixteen)
+  return rows <= 128 ? leaf_launch<16, 8, 2>(c, A, lda, m, L, b, R, T, tau)
+                     : leaf_launch<16, 8, 4>(c, A, lda, m, L, b, R, T, tau);
 }
 
 int64_t leaf_capacity_rows(int64_t b) {
