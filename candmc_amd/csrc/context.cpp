@@ -158,6 +158,9 @@ int cqr_create(cqr_ctx** ctx, int device) {
   if (hipSetDevice(device) != hipSuccess) return CQR_ERR_DEVICE;
   cqr_ctx* c = new cqr_ctx();
   c->device = device;
+  if (hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      c->num_cu < 1)
+    c->num_cu = 256;
   if (hipMalloc(&c->d_flag, sizeof(int)) != hipSuccess ||
       hipMalloc(&c->d_cnt, sizeof(uint64_t) * CQR_MAX_TREE_NODES * 2) != hipSuccess ||
       hipMalloc(&c->d_terr, 2 * sizeof(int)) != hipSuccess ||   // [0] timeout flag, [1] poll bound

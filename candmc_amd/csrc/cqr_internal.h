@@ -28,6 +28,7 @@ struct cqr_ctx {
   double acc_ms[PK_NKINDS] = {}, acc_flops[PK_NKINDS] = {}, acc_bytes[PK_NKINDS] = {};
   int64_t acc_n[PK_NKINDS] = {};
   int device = 0;
+  int num_cu = 256;             // compute units of the device
   hipStream_t stream = nullptr;
   void* ws = nullptr;           // device workspace arena
   size_t ws_bytes = 0;

@@ -768,8 +768,11 @@ static int leaf_tree_launch(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L
 int leaf_tree(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L, int b, double* leafR, double* leafT,
               double* leafTau, double* nodeR, double* V2, double* T, double* tau, int nnodes) {
   const int64_t rows = (m + L - 1) / L;
+  // fused only while every leaf and node workgroup is resident at once (one
+  // per CU, for the LDS): beyond that the nodes start only as leaves retire
+  // and the two launches in order are faster (65536×2048: 32.6 → 30.4 ms)
   const bool fuse = nnodes > 0 && b > 64 && b <= 128 && b % 8 == 0 && rows <= 256 && L <= CQR_MAX_TREE_NODES &&
-                    nnodes <= CQR_MAX_TREE_NODES;
+                    nnodes <= CQR_MAX_TREE_NODES && L + nnodes <= c->num_cu;
   if (!fuse) {
     CQR_TRY(leaf_qr(c, A, lda, m, L, b, leafR, leafT, leafTau));
     return launch_ptree(c, leafR, nodeR, V2, T, tau, L, nnodes, b);
