@@ -9,6 +9,7 @@
 // problem in a device table.  The whole sequence can be captured in a HIP
 // graph and replayed.  The leaf height of every group is a tunable that
 // cqr_batch_tune picks by timing the candidates on the caller's data.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -23,9 +24,13 @@ struct Group {
   std::vector<int> probs;        // problem indices, in packing order
   int64_t leaf_rows = 0;         // 0 = automatic
   std::vector<BatchProb> hp;
+  std::vector<BatchSeg> hseg;    // tree order (nseg_tree), then construct_Q1 order (nseg_q1)
+  std::vector<char> htab;        // device table image: hp, then hseg
+  int nseg_tree = 0, nseg_q1 = 0;
   int total_leaves = 0, total_nodes = 0, total_items = 0;
   int64_t total_rows = 0, max_leaf_rows = 0, max_m = 0;
-  BatchProb* d_bp = nullptr;
+  BatchProb* d_bp = nullptr;     // the device table (d_seg follows it)
+  BatchSeg* d_seg = nullptr;
   // byte offsets into the batch arena
   size_t leafR, leafT, leafTau, nodeR, nodeV2, nodeT, nodeTau, C, Y, W, Uinv, Winv, LinvT, cnt, qflags, bytes;
 };
@@ -57,8 +62,69 @@ struct cqr_batch {
 
 namespace {
 
+// Tree levels of an L-leaf problem: level k has pairs[k] nodes from node0[k].
+void tree_levels(int L, std::vector<int>& node0, std::vector<int>& pairs) {
+  node0.clear();
+  pairs.clear();
+  for (int n = L, first = 0; n > 1; n = n / 2 + n % 2) {
+    node0.push_back(first);
+    pairs.push_back(n / 2);
+    first += n / 2;
+  }
+}
+
+// Dispatch order of the group's tree nodes and construct_Q1 items: level by
+// level across all problems (problems with the deepest trees first inside a
+// level, as their chains are the longest), then construct_Q1's leaf items.
+void build_order(Group& g) {
+  const int np = (int)g.hp.size(), ncb = (g.b + 15) / 16;
+  std::vector<int> order(np);
+  for (int i = 0; i < np; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return g.hp[x].L > g.hp[y].L; });
+  std::vector<std::vector<int>> n0(np), pr(np);
+  int depth = 0;
+  for (int i = 0; i < np; ++i) {
+    tree_levels(g.hp[i].L, n0[i], pr[i]);
+    depth = std::max(depth, (int)n0[i].size());
+  }
+  g.hseg.clear();
+  int next = 0;
+  for (int k = 0; k < depth; ++k)           // tree: children before parents
+    for (int i : order)
+      if (k < (int)n0[i].size()) {
+        g.hseg.push_back(BatchSeg{next, i, n0[i][k], pr[i][k]});
+        next += pr[i][k];
+      }
+  g.nseg_tree = (int)g.hseg.size();
+  next = 0;
+  for (int d = 0; d < depth; ++d)           // construct_Q1: the root level first
+    for (int i : order) {
+      const int D = (int)n0[i].size();
+      if (d >= D) continue;
+      const int k = D - 1 - d, nn = g.hp[i].L - 1;
+      g.hseg.push_back(BatchSeg{next, i, (nn - n0[i][k] - pr[i][k]) * ncb, pr[i][k] * ncb});
+      next += pr[i][k] * ncb;
+    }
+  for (int i : order) {                     // then every leaf
+    const int L = g.hp[i].L;
+    g.hseg.push_back(BatchSeg{next, i, (L - 1) * ncb, L * ncb});
+    next += L * ncb;
+  }
+  g.nseg_q1 = (int)g.hseg.size() - g.nseg_tree;
+  const size_t pb = sizeof(BatchProb) * g.hp.size(), sb = sizeof(BatchSeg) * g.hseg.size();
+  g.htab.resize(pb + sb);
+  std::memcpy(g.htab.data(), g.hp.data(), pb);
+  std::memcpy(g.htab.data() + pb, g.hseg.data(), sb);
+}
+
+// Places the group's device table at d (g.htab.size() bytes).
+void set_table(Group& g, void* d) {
+  g.d_bp = static_cast<BatchProb*>(d);
+  g.d_seg = reinterpret_cast<BatchSeg*>(static_cast<char*>(d) + sizeof(BatchProb) * g.hp.size());
+}
+
 // Plans a group's problems; with `alloc` its device table is allocated and
-// filled here, without it the caller places and fills g.d_bp.
+// filled here, without it the caller places (set_table) and fills it.
 int plan_group(cqr_ctx* c, cqr_batch* bt, Group& g, bool alloc = true) {
   const int b = g.b;
   g.hp.clear();
@@ -103,11 +169,15 @@ int plan_group(cqr_ctx* c, cqr_batch* bt, Group& g, bool alloc = true) {
   g.cnt = s.add(8 * nn);
   g.qflags = s.add(8 * nn * (size_t)((b + 15) / 16));
   g.bytes = s.bytes;
+  build_order(g);
   if (!alloc) return CQR_OK;
   if (g.d_bp) (void)hipFree(g.d_bp);
   g.d_bp = nullptr;
-  CQR_HIP(c, hipMalloc(&g.d_bp, sizeof(BatchProb) * np));
-  CQR_HIP(c, hipMemcpy(g.d_bp, g.hp.data(), sizeof(BatchProb) * np, hipMemcpyHostToDevice));
+  g.d_seg = nullptr;
+  void* d = nullptr;
+  CQR_HIP(c, hipMalloc(&d, g.htab.size()));
+  set_table(g, d);
+  CQR_HIP(c, hipMemcpy(d, g.htab.data(), g.htab.size(), hipMemcpyHostToDevice));
   return CQR_OK;
 }
 
@@ -140,9 +210,9 @@ int run_group(cqr_ctx* c, cqr_batch* bt, const Group& g, double* A, double* Tt) 
   CQR_TRY(leaf_qr_batched(c, A, g.total_rows, g.total_leaves, g.max_leaf_rows, b, at(g.leafR), at(g.leafT),
                           at(g.leafTau), g.d_bp, np));
   CQR_TRY(launch_ptree_batched(c, at(g.leafR), at(g.nodeR), at(g.nodeV2), at(g.nodeT), at(g.nodeTau), cnt,
-                               g.total_nodes, b, g.d_bp, np));
+                               g.total_nodes, b, g.d_bp, g.d_seg, g.nseg_tree));
   CQR_TRY(launch_q1_batched(c, at(g.nodeV2), at(g.nodeT), at(g.leafT), A, at(g.C), at(g.Y), qf, g.total_items, b,
-                            g.d_bp, np));
+                            g.d_bp, g.d_seg + g.nseg_tree, g.nseg_q1));
   CQR_TRY(hr_core_batched(c, g.d_bp, np, at(g.Y), at(g.leafR), at(g.nodeR), A, at(g.W), at(g.Uinv), at(g.Winv),
                           at(g.LinvT), b));
   // A_p(b:, :) = Q1_p(b:, :)·U_p⁻¹   (hh_recon.cxx:294-301)
@@ -401,7 +471,7 @@ int plan_lsq_batch(cqr_ctx* c, int64_t np, const cqr_request* rq, LsqBatchPlan& 
   for (Group& g : bt.groups) {
     CQR_TRY(plan_group(c, &bt, g, false));
     gws = std::max(gws, g.bytes);
-    P.bp_off.push_back(s.add(sizeof(BatchProb) * g.probs.size()));
+    P.bp_off.push_back(s.add(g.htab.size()));
   }
   P.oG = s.add(gws);
   P.oR = s.add(refine);
@@ -440,9 +510,9 @@ int cqr_lstsq_batched(cqr_ctx* c, int64_t nprob, const cqr_request* reqs, int ma
   double* Tp = reinterpret_cast<double*>(base + P.oT);
   for (size_t gi = 0; gi < bt.groups.size(); ++gi) {
     Group& g = bt.groups[gi];
-    g.d_bp = reinterpret_cast<BatchProb*>(base + P.bp_off[gi]);
+    set_table(g, base + P.bp_off[gi]);
     // the tables outlive this call's last use of them (it ends in a sync)
-    CQR_HIP(c, hipMemcpyAsync(g.d_bp, g.hp.data(), sizeof(BatchProb) * g.hp.size(), hipMemcpyHostToDevice,
+    CQR_HIP(c, hipMemcpyAsync(g.d_bp, g.htab.data(), g.htab.size(), hipMemcpyHostToDevice,
                               c->stream));
   }
   for (int64_t p = 0; p < nprob; ++p)

@@ -162,7 +162,8 @@ inline BMat bmat(const double* p, int64_t ld, int64_t zs = 0, int mode = BM_STRI
 // lda = m), its leaves are global leaves leaf0.., its tree nodes global
 // nodes node0.., its construct_Q1 items item0.., its Q1 block starts at
 // row offset yoff (doubles) of the Y scratch (ld m), its Tᵀ at toff.
-// Kernels find their problem by binary search on leaf0 / node0 / item0.
+// The leaf kernels find their problem by binary search on leaf0; the tree
+// and construct_Q1 launches map their blocks through a BatchSeg table.
 // ---------------------------------------------------------------------------
 struct BatchProb {
   int64_t aoff, m, yoff, toff;
@@ -178,6 +179,28 @@ __device__ inline int bp_find(const BatchProb* bp, int nprob, int idx, int which
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (bp_first(bp[mid], which) <= idx) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Dispatch order of a batched launch with dependencies between its blocks
+// (the tree nodes, construct_Q1's items): global block ids g0.. of a
+// segment are the problem's local nodes (items) local0.., one level of one
+// problem per segment.  Segments run level by level across all problems
+// (children before parents for the tree, the root first for construct_Q1),
+// so a block waits only on blocks of earlier levels, which every problem
+// dispatched before it, instead of on a deep chain inside one problem while
+// the other problems' ready work sits undispatched.
+struct BatchSeg {
+  int g0, prob, local0, n;
+};
+// segment holding global block idx (g0 increasing)
+__device__ inline int seg_find(const BatchSeg* s, int nseg, int idx) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (s[mid].g0 <= idx) lo = mid;
     else hi = mid - 1;
   }
   return lo;
@@ -201,9 +224,11 @@ int launch_q1(cqr_ctx* c, const double* nodeV2, const double* nodeT, const doubl
 int leaf_qr_batched(cqr_ctx* c, double* Abase, int64_t total_m, int L, int64_t rows, int b, double* R, double* T,
                     double* tau, const BatchProb* bp, int nprob);
 int launch_ptree_batched(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
-                         uint64_t* cnt, int total_nodes, int b, const BatchProb* bp, int nprob);
+                         uint64_t* cnt, int total_nodes, int b, const BatchProb* bp, const BatchSeg* seg,
+                         int nseg);
 int launch_q1_batched(cqr_ctx* c, const double* nodeV2, const double* nodeT, const double* leafT, const double* Vbase,
-                      double* C, double* Y, uint64_t* flags, int total_items, int b, const BatchProb* bp, int nprob);
+                      double* C, double* Y, uint64_t* flags, int total_items, int b, const BatchProb* bp,
+                      const BatchSeg* seg, int nseg);
 int q1_items_per_problem(int L, int b);
 int hr_core_batched(cqr_ctx* c, const BatchProb* bp, int nprob, const double* Y, const double* leafR,
                     const double* nodeR, double* Abase, double* W, double* Uinv, double* Winv, double* LinvT, int b);

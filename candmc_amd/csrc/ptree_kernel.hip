@@ -45,14 +45,15 @@ template <int CPW, int RPL>
 __global__ void __launch_bounds__(512)
 ptree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, double* __restrict__ V2out,
              double* __restrict__ Tout, double* __restrict__ tauout, uint64_t* __restrict__ cnt,
-             uint64_t epoch, int L, int b, int* __restrict__ err, const BatchProb* __restrict__ bp, int nprob) {
+             uint64_t epoch, int L, int b, int* __restrict__ err, const BatchProb* __restrict__ bp, const BatchSeg* __restrict__ seg, int nseg) {
   constexpr int WAVES = 8;
   int nid = blockIdx.x;
   const int lane = threadIdx.x & 63, w = wave_id();
   const int64_t bb = (int64_t)b * b;
   if (bp) {  // batched: this problem's leaves and nodes are contiguous slices
-    const BatchProb P = bp[bp_find(bp, nprob, nid, BP_NODE)];
-    nid -= P.node0;
+    const BatchSeg S = seg[seg_find(seg, nseg, nid)];
+    const BatchProb P = bp[S.prob];
+    nid = S.local0 + (nid - S.g0);
     L = P.L;
     leafR += (int64_t)P.leaf0 * bb;
     nodeR += (int64_t)P.node0 * bb;
@@ -351,7 +352,7 @@ template <int BMAX>
 __device__ __forceinline__ void btree_body(const double* __restrict__ leafR, double* __restrict__ nodeR,
                                   double* __restrict__ V2out, double* __restrict__ Tout,
                                   double* __restrict__ tauout, uint64_t* __restrict__ cnt, uint64_t epoch, int L,
-                                  int b, int* __restrict__ err, const BatchProb* __restrict__ bp, int nprob, int nid,
+                                  int b, int* __restrict__ err, const BatchProb* __restrict__ bp, const BatchSeg* __restrict__ seg, int nseg, int nid,
                                   double* smem, uint64_t* __restrict__ lcnt) {
   constexpr int NT = 4 * BMAX, NB = 8, RPL = (BMAX + 63) / 64;
   constexpr int PACK = BMAX * (BMAX + 1) / 2;
@@ -367,8 +368,9 @@ __device__ __forceinline__ void btree_body(const double* __restrict__ leafR, dou
   const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
   const int64_t bb = (int64_t)b * b;
   if (bp) {
-    const BatchProb P = bp[bp_find(bp, nprob, nid, BP_NODE)];
-    nid -= P.node0;
+    const BatchSeg S = seg[seg_find(seg, nseg, nid)];
+    const BatchProb P = bp[S.prob];
+    nid = S.local0 + (nid - S.g0);
     L = P.L;
     leafR += (int64_t)P.leaf0 * bb;
     nodeR += (int64_t)P.node0 * bb;
@@ -684,9 +686,9 @@ template <int BMAX>
 __global__ void __launch_bounds__(4 * BMAX)
 btree_kernel(const double* __restrict__ leafR, double* __restrict__ nodeR, double* __restrict__ V2out,
              double* __restrict__ Tout, double* __restrict__ tauout, uint64_t* __restrict__ cnt,
-             uint64_t epoch, int L, int b, int* __restrict__ err, const BatchProb* __restrict__ bp, int nprob) {
+             uint64_t epoch, int L, int b, int* __restrict__ err, const BatchProb* __restrict__ bp, const BatchSeg* __restrict__ seg, int nseg) {
   extern __shared__ double smem[];
-  btree_body<BMAX>(leafR, nodeR, V2out, Tout, tauout, cnt, epoch, L, b, err, bp, nprob, blockIdx.x, smem, nullptr);
+  btree_body<BMAX>(leafR, nodeR, V2out, Tout, tauout, cnt, epoch, L, b, err, bp, seg, nseg, blockIdx.x, smem, nullptr);
 }
 
 // Leaves and tree in one launch (b in (64, 128], leaves of at most 256
@@ -705,44 +707,44 @@ leaf_tree_kernel(double* __restrict__ A, int64_t lda, int64_t m, int L, int b, d
   if ((int)blockIdx.x < L)
     leaf_qr_body<8, 16, RPL>(A, lda, m, L, b, leafR, leafT, leafTau, nullptr, 0, blockIdx.x, smem, lcnt, epoch);
   else
-    btree_body<128>(leafR, nodeR, V2out, Tout, tauout, cnt, epoch, L, b, err, nullptr, 0, blockIdx.x - L, smem,
+    btree_body<128>(leafR, nodeR, V2out, Tout, tauout, cnt, epoch, L, b, err, nullptr, nullptr, 0, blockIdx.x - L, smem,
                     lcnt);
 }
 
 template <int BMAX>
 static int btree_launch(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
-                        uint64_t* cnt, int L, int nnodes, int b, uint64_t epoch, const BatchProb* bp, int nprob) {
+                        uint64_t* cnt, int L, int nnodes, int b, uint64_t epoch, const BatchProb* bp, const BatchSeg* seg, int nseg) {
   const size_t lds = sizeof(double) * btree_smem_doubles<BMAX>();
   auto k = btree_kernel<BMAX>;
   CQR_TRY(set_max_lds(c, (const void*)k, (int)lds));
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(k, dim3(nnodes), dim3(4 * BMAX), lds, c->stream, leafR, nodeR, V2, T, tau, cnt, epoch, L, b,
-                     c->d_terr, bp, nprob);
+                     c->d_terr, bp, seg, nseg);
   prof_end(c, ev, PK_TREE, nnodes * (2.0 / 3.0) * b * (double)b * b, nnodes * 40.0 * b * b);
   return check_launch(c, "btree");
 }
 
 template <int CPW, int RPL>
 static int ptree_launch(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
-                        uint64_t* cnt, int L, int nnodes, int b, uint64_t epoch, const BatchProb* bp, int nprob) {
+                        uint64_t* cnt, int L, int nnodes, int b, uint64_t epoch, const BatchProb* bp, const BatchSeg* seg, int nseg) {
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL((ptree_kernel<CPW, RPL>), dim3(nnodes), dim3(512), 0, c->stream, leafR, nodeR, V2, T, tau,
-                     cnt, epoch, L, b, c->d_terr, bp, nprob);
+                     cnt, epoch, L, b, c->d_terr, bp, seg, nseg);
   prof_end(c, ev, PK_TREE, nnodes * (2.0 / 3.0) * b * (double)b * b, nnodes * 40.0 * b * b);
   return check_launch(c, "ptree");
 }
 
 static int ptree_dispatch(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
-                          uint64_t* cnt, int L, int nnodes, int b, uint64_t epoch, const BatchProb* bp, int nprob) {
+                          uint64_t* cnt, int L, int nnodes, int b, uint64_t epoch, const BatchProb* bp, const BatchSeg* seg, int nseg) {
   if (b % 8 == 0) {
-    if (b <= 32) return btree_launch<32>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
-    if (b <= 64) return btree_launch<64>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
-    if (b <= 128) return btree_launch<128>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
+    if (b <= 32) return btree_launch<32>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, seg, nseg);
+    if (b <= 64) return btree_launch<64>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, seg, nseg);
+    if (b <= 128) return btree_launch<128>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, seg, nseg);
   }
-  if (b <= 16) return ptree_launch<2, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
-  if (b <= 32) return ptree_launch<4, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
-  if (b <= 64) return ptree_launch<8, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
-  return ptree_launch<16, 2>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, nprob);
+  if (b <= 16) return ptree_launch<2, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, seg, nseg);
+  if (b <= 32) return ptree_launch<4, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, seg, nseg);
+  if (b <= 64) return ptree_launch<8, 1>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, seg, nseg);
+  return ptree_launch<16, 2>(c, leafR, nodeR, V2, T, tau, cnt, L, nnodes, b, epoch, bp, seg, nseg);
 }
 
 template <int RPL>
@@ -785,16 +787,16 @@ int launch_ptree(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, dou
                  int L, int nnodes, int b) {
   if (nnodes <= 0) return CQR_OK;
   if (nnodes > CQR_MAX_TREE_NODES) return set_err(c, CQR_ERR_UNSUPPORTED, "tree of %d nodes", nnodes);
-  return ptree_dispatch(c, leafR, nodeR, V2, T, tau, c->d_cnt, L, nnodes, b, ++c->tree_epoch, nullptr, 0);
+  return ptree_dispatch(c, leafR, nodeR, V2, T, tau, c->d_cnt, L, nnodes, b, ++c->tree_epoch, nullptr, nullptr, 0);
 }
 
 // Batched: all trees of a group in one launch; cnt (one word per node) must
 // hold values below 256 (the batch driver zeroes it with a kernel before
 // every run, so the launch is graph-replayable with epoch 1).
 int launch_ptree_batched(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
-                         uint64_t* cnt, int total_nodes, int b, const BatchProb* bp, int nprob) {
+                         uint64_t* cnt, int total_nodes, int b, const BatchProb* bp, const BatchSeg* seg, int nseg) {
   if (total_nodes <= 0) return CQR_OK;
-  return ptree_dispatch(c, leafR, nodeR, V2, T, tau, cnt, 0, total_nodes, b, 1, bp, nprob);
+  return ptree_dispatch(c, leafR, nodeR, V2, T, tau, cnt, 0, total_nodes, b, 1, bp, seg, nseg);
 }
 
 }  // namespace cqr

@@ -54,7 +54,8 @@ __global__ void __launch_bounds__(NT)
 q1_kernel(const double* __restrict__ nodeV2, const double* __restrict__ nodeT, const double* __restrict__ leafT,
           const double* __restrict__ V, int64_t ldv, int64_t m, double* __restrict__ C,
           double* __restrict__ Q, int64_t ldq, uint64_t* __restrict__ flags, uint64_t epoch, int L,
-          int nnodes, int b, int* __restrict__ err, const BatchProb* __restrict__ bp, int nprob) {
+          int nnodes, int b, int* __restrict__ err, const BatchProb* __restrict__ bp,
+          const BatchSeg* __restrict__ seg, int nseg) {
   __shared__ double S0[128][LDS_LD];
   __shared__ double S1[128][LDS_LD];
   __shared__ int s_ok;
@@ -63,8 +64,9 @@ q1_kernel(const double* __restrict__ nodeV2, const double* __restrict__ nodeT, c
   int item = blockIdx.x;
   if (bp) {  // batched: V is the group's base; the problem's slices of every buffer
     const int64_t bb0 = (int64_t)b * b;
-    const BatchProb P = bp[bp_find(bp, nprob, item, BP_ITEM)];
-    item -= P.item0;
+    const BatchSeg S = seg[seg_find(seg, nseg, item)];
+    const BatchProb P = bp[S.prob];
+    item = S.local0 + (item - S.g0);
     L = P.L;
     nnodes = P.L - 1;
     nodeV2 += (int64_t)P.node0 * bb0;
@@ -247,20 +249,23 @@ int launch_q1(cqr_ctx* c, const double* nodeV2, const double* nodeT, const doubl
   const uint64_t epoch = ++c->q1_epoch;
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(q1_kernel, dim3((unsigned)((nnodes + L) * ncb)), dim3(NT), 0, c->stream, nodeV2, nodeT,
-                     leafT, V, ldv, m, C, Q, ldq, c->d_qflags, epoch, L, nnodes, b, c->d_terr, nullptr, 0);
+                     leafT, V, ldv, m, C, Q, ldq, c->d_qflags, epoch, L, nnodes, b, c->d_terr, nullptr, nullptr,
+                     0);
   prof_end(c, ev, PK_BGEMM, 4.0 * nnodes * (double)b * b * b + 4.0 * (double)m * b * b,
            8.0 * (3.0 * nnodes * b * b + 2.0 * m * b + 2.0 * L * b * b));
   return check_launch(c, "q1_kernel");
 }
 
-// Batched: every problem of a group (items numbered problem by problem);
+// Batched: every problem of a group, items in the order of the segment
+// table (node levels from the top across all problems, then the leaves);
 // flags must be zero before the launch (epoch 1, graph-replayable).
 int launch_q1_batched(cqr_ctx* c, const double* nodeV2, const double* nodeT, const double* leafT, const double* Vbase,
-                      double* C, double* Y, uint64_t* flags, int total_items, int b, const BatchProb* bp, int nprob) {
+                      double* C, double* Y, uint64_t* flags, int total_items, int b, const BatchProb* bp,
+                      const BatchSeg* seg, int nseg) {
   if (total_items <= 0) return CQR_OK;
   hipEvent_t ev = prof_begin(c);
   hipLaunchKernelGGL(q1_kernel, dim3((unsigned)total_items), dim3(NT), 0, c->stream, nodeV2, nodeT, leafT, Vbase,
-                     (int64_t)0, (int64_t)0, C, Y, (int64_t)0, flags, (uint64_t)1, 0, 0, b, c->d_terr, bp, nprob);
+                     (int64_t)0, (int64_t)0, C, Y, (int64_t)0, flags, (uint64_t)1, 0, 0, b, c->d_terr, bp, seg, nseg);
   prof_end(c, ev, PK_BGEMM, 0.0, 0.0);
   return check_launch(c, "q1_kernel_batched");
 }
