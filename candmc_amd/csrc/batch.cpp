@@ -24,9 +24,11 @@ struct Group {
   std::vector<int> probs;        // problem indices, in packing order
   int64_t leaf_rows = 0;         // 0 = automatic
   std::vector<BatchProb> hp;
-  std::vector<BatchSeg> hseg;    // tree order (nseg_tree), then construct_Q1 order (nseg_q1)
+  // block orders: tree (nseg_tree segments), construct_Q1 (nseg_q1), Y
+  // product tiles (nseg_y, ytiles blocks)
+  std::vector<BatchSeg> hseg;
   std::vector<char> htab;        // device table image: hp, then hseg
-  int nseg_tree = 0, nseg_q1 = 0;
+  int nseg_tree = 0, nseg_q1 = 0, nseg_y = 0, ytiles = 0;
   int total_leaves = 0, total_nodes = 0, total_items = 0;
   int64_t total_rows = 0, max_leaf_rows = 0, max_m = 0;
   BatchProb* d_bp = nullptr;     // the device table (d_seg follows it)
@@ -111,6 +113,17 @@ void build_order(Group& g) {
     next += L * ncb;
   }
   g.nseg_q1 = (int)g.hseg.size() - g.nseg_tree;
+  next = 0;
+  const int64_t bm = batched_y_tile_rows();
+  for (int i : order) {                     // Y product: the row tiles below the top b rows
+    const int64_t rows = g.hp[i].m - g.b;
+    if (rows <= 0) continue;
+    const int t = (int)((rows + bm - 1) / bm);
+    g.hseg.push_back(BatchSeg{next, i, 0, t});
+    next += t;
+  }
+  g.nseg_y = (int)g.hseg.size() - g.nseg_tree - g.nseg_q1;
+  g.ytiles = next;
   const size_t pb = sizeof(BatchProb) * g.hp.size(), sb = sizeof(BatchSeg) * g.hseg.size();
   g.htab.resize(pb + sb);
   std::memcpy(g.htab.data(), g.hp.data(), pb);
@@ -216,10 +229,8 @@ int run_group(cqr_ctx* c, cqr_batch* bt, const Group& g, double* A, double* Tt) 
   CQR_TRY(hr_core_batched(c, g.d_bp, np, at(g.Y), at(g.leafR), at(g.nodeR), A, at(g.W), at(g.Uinv), at(g.Winv),
                           at(g.LinvT), b));
   // A_p(b:, :) = Q1_p(b:, :)·U_p⁻¹   (hh_recon.cxx:294-301)
-  if (g.max_m > b)
-    CQR_TRY(gemm_f64_batched_prob(c, 0, 0, 0, b, b, 1.0, bmat(at(g.Y), 0, 0, BM_PROB_Y), SM_NONE,
-                                  bmat(at(g.Uinv), b, bb), SM_UPPER, 0.0, bmat(A, 0, 0, BM_PROB_A), SM_NONE, np,
-                                  g.d_bp, b, 1, (int)(g.max_m - b)));
+  CQR_TRY(update_y_batched(c, g.d_bp, g.d_seg + g.nseg_tree + g.nseg_q1, g.nseg_y, g.ytiles, b, at(g.Y),
+                           at(g.Uinv), A));
   // Tᵀ_p = −L_p⁻¹·W_pᵀ   (W = −T·Y1ᵀ)
   return gemm_f64_batched_prob(c, 1, 1, b, b, b, -1.0, bmat(at(g.LinvT), b, bb), SM_UNIT_UPPER, bmat(at(g.W), b, bb),
                                SM_UPPER, 0.0, bmat(Tt, b, 0, BM_PROB_T), SM_NONE, np, g.d_bp, 0, 0, b);

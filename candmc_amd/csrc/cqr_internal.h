@@ -230,6 +230,11 @@ int launch_q1_batched(cqr_ctx* c, const double* nodeV2, const double* nodeT, con
                       double* C, double* Y, uint64_t* flags, int total_items, int b, const BatchProb* bp,
                       const BatchSeg* seg, int nseg);
 int q1_items_per_problem(int L, int b);
+// Batched Y(b:m) = Q1(b:m)·U⁻¹ of every problem (gemm_update.hip): one
+// block per batched_y_tile_rows() rows of a problem, mapped by `seg`.
+int batched_y_tile_rows();
+int update_y_batched(cqr_ctx* c, const BatchProb* bp, const BatchSeg* seg, int nseg, int total_tiles, int b,
+                     const double* Y, const double* Uinv, double* A);
 int hr_core_batched(cqr_ctx* c, const BatchProb* bp, int nprob, const double* Y, const double* leafR,
                     const double* nodeR, double* Abase, double* W, double* Uinv, double* Winv, double* LinvT, int b);
 int gemm_f64_batched_prob(cqr_ctx* c, int ta, int tb, int m, int n, int k, double alpha, const BMat& A, int smA,

@@ -251,4 +251,69 @@ int update_gemm_split(cqr_ctx* c, int64_t M, int64_t N, int64_t n1, int64_t K, c
   return launch_update(c, g, kind, 0);
 }
 
+// ---------------------------------------------------------------------------
+// Batched Y = Q1·U⁻¹ below the top b rows of every problem of a group
+// (hh_recon.cxx:294-301 per problem): the same tile with per-block problem
+// lookup.  A block's tile is a 128-row block of one problem times all of
+// its b ≤ 128 columns (one tile column: 128 wide for b > 64, else 64), with
+// K = b; the blocks of a problem are consecutive, so they share its U⁻¹
+// in one XCD's L2.  Block u → (problem, row tile) through the segment
+// table (one segment per problem with m > b).
+// ---------------------------------------------------------------------------
+namespace {
+struct YBatchArgs {
+  const BatchProb* bp;
+  const BatchSeg* seg;
+  int nseg, b, total, per_xcd;
+  const double* Y;      // the group's Q1 rows (problem p at Y + yoff, ld m)
+  const double* Uinv;   // b×b per problem, strictly lower part zero
+  double* A;            // the group's A (problem p at A + aoff, ld m)
+};
+}  // namespace
+
+template <int WN>
+__global__ void __launch_bounds__(128 * WN, WN == 2 ? 2 : 1) upd_y_batched_f64(YBatchArgs a) {
+  constexpr int BM = 128, BN = 64 * WN;
+  __shared__ __align__(16) double Ps[2 * UBK * (BM + 16)];
+  __shared__ __align__(16) double Zs[2 * BN * (UBK + 2)];
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int u = xcd * a.per_xcd + slot;
+  if (slot >= a.per_xcd || u >= a.total) return;
+  const BatchSeg S = a.seg[seg_find(a.seg, a.nseg, u)];
+  const BatchProb P = a.bp[S.prob];
+  const int b = a.b;
+  UpdArgs g{};
+  g.M = P.m - b;
+  g.N = b;
+  g.K = b;
+  g.P = a.Y + P.yoff + b;
+  g.ldp = P.m;
+  g.Z = a.Uinv + (int64_t)S.prob * b * b;
+  g.ldz = b;
+  g.C = a.A + P.aoff + b;
+  g.ldc = P.m;
+  g.alpha = 1.0;
+  g.n1 = b;
+  // 16-byte loads: even leading dimensions and offsets on 16-byte bases
+  g.vec = (P.m % 2 == 0) && (b % 2 == 0) && ((P.yoff + b) % 2 == 0);
+  const int64_t m0 = (int64_t)(S.local0 + (u - S.g0)) * BM;
+  if (g.vec && m0 + BM <= g.M && BN <= b && b % UBK == 0) upd_tile<2, WN, true>(g, m0, 0, Ps, Zs);
+  else upd_tile<2, WN, false>(g, m0, 0, Ps, Zs);
+}
+
+int batched_y_tile_rows() { return 128; }
+
+int update_y_batched(cqr_ctx* c, const BatchProb* bp, const BatchSeg* seg, int nseg, int total_tiles, int b,
+                     const double* Y, const double* Uinv, double* A) {
+  if (total_tiles <= 0) return CQR_OK;
+  if (b < 1 || b > 128) return set_err(c, CQR_ERR_UNSUPPORTED, "update_y_batched: b = %d", b);
+  YBatchArgs a{bp, seg, nseg, b, total_tiles, (total_tiles + 7) / 8, Y, Uinv, A};
+  const dim3 grid((unsigned)(8 * a.per_xcd));
+  hipEvent_t ev = prof_begin(c);
+  if (b > 64) hipLaunchKernelGGL((upd_y_batched_f64<2>), grid, dim3(256), 0, c->stream, a);
+  else hipLaunchKernelGGL((upd_y_batched_f64<1>), grid, dim3(128), 0, c->stream, a);
+  prof_end(c, ev, PK_GEMM_SMALL, 0.0, 0.0);
+  return check_launch(c, "upd_y_batched_f64");
+}
+
 }  // namespace cqr

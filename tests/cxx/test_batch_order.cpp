@@ -7,7 +7,8 @@
 // exactly one block, that the segments tile the block range, and that every
 // dependency comes earlier: a tree node after the nodes producing its two
 // inputs, a construct_Q1 item after its parent node's item of the same
-// column block.  The dependencies are derived here from the binary tree's
+// column block; and that the Y product's row tiles cover every problem's
+// rows below its top b once.  The dependencies are derived here from the binary tree's
 // pairing (receiver z, sender z + ceil(n/2) on a level of n live ranks,
 // alg/QR/tsqr/bitree_tsqr.cxx:172-208).  Built by tests/test_batch_order.py.
 #include "../../candmc_amd/csrc/batch.cpp"
@@ -76,6 +77,7 @@ void check_group(const std::vector<int>& Ls, int b) {
   for (int L : Ls) {
     BatchProb P{};
     P.L = L;
+    P.m = 100 * L + b - 50;   // some problems of one panel (m ≤ b), most taller
     P.leaf0 = leaf0;
     P.node0 = node0;
     P.item0 = item0;
@@ -125,6 +127,26 @@ void check_group(const std::vector<int>& Ls, int b) {
     next = S.g0 + S.n;
   }
   CHECK(next == item0, "q1 blocks %d, items %d", next, item0);
+  // Y product: every row tile below the top b rows of every problem, once
+  std::map<std::pair<int, int>, int> yblock;
+  next = 0;
+  for (int s = g.nseg_tree + g.nseg_q1; s < (int)g.hseg.size(); ++s) {
+    const BatchSeg& S = g.hseg[s];
+    CHECK(S.g0 == next, "Y segment %d starts at %d, expected %d", s, S.g0, next);
+    for (int j = 0; j < S.n; ++j)
+      CHECK(yblock.emplace(std::make_pair(S.prob, S.local0 + j), S.g0 + j).second, "Y tile dispatched twice");
+    next = S.g0 + S.n;
+  }
+  CHECK(g.nseg_tree + g.nseg_q1 + g.nseg_y == (int)g.hseg.size(), "segment counts do not add up");
+  CHECK(next == g.ytiles, "Y blocks %d, tiles %d", next, g.ytiles);
+  int64_t want = 0;
+  for (const BatchProb& P : g.hp) {
+    const int64_t rows = P.m - b, t = rows > 0 ? (rows + batched_y_tile_rows() - 1) / batched_y_tile_rows() : 0;
+    want += t;
+    for (int j = 0; j < t; ++j)
+      CHECK(yblock.count(std::make_pair((int)(&P - g.hp.data()), j)) == 1, "Y tile missing");
+  }
+  CHECK(want == g.ytiles, "Y tiles %d, expected %lld", g.ytiles, (long long)want);
   for (int p = 0; p < np; ++p) {
     const Tree t(Ls[p]);
     const int nn = Ls[p] - 1;
