@@ -80,6 +80,17 @@ q1_kernel(const double* __restrict__ nodeV2, const double* __restrict__ nodeT, c
     ldq = P.m;
     flags += (int64_t)P.node0 * ncb;
   }
+  if (!bp) {
+    // The leaf items of eight consecutive leaves are dealt so that leaf z's
+    // items run on XCD z mod 8 (blocks go to XCD blockIdx mod 8), the XCD
+    // whose L2 holds its V and T from the leaf factorization (leaf z is
+    // block z there).  A partial last group keeps the linear order.
+    const int s0 = nnodes * ncb, rel = item - s0;
+    if (rel >= 0) {
+      const int grp = rel / (8 * ncb);
+      if (8 * (grp + 1) <= L) item = s0 + (8 * grp + (item & 7)) * ncb + (rel - grp * 8 * ncb) / 8;
+    }
+  }
   const int cb = item % ncb;
   const int j0 = cb * CW, nj = min(CW, b - j0);
   const int R = min(b, j0 + CW);
