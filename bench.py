@@ -402,6 +402,8 @@ def main():
     ap.add_argument("--no-tree", action="store_true", help="skip the implicit-Q (apply_tsqr_QT) leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-loop", action="store_true", help="skip the per-problem loop baseline of the batch")
+    ap.add_argument("--no-qr", action="store_true",
+                    help="skip the headline QR_2D (to profile the other legs on their own)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -420,111 +422,115 @@ def main():
 
     from candmc_amd import qr
 
-    dev = torch.device("cuda:0")
-    A0 = qr.to_gpu(A_host)
-    A = torch.empty_like(A0)
-    ctx = qr.context()
-    stream = torch.cuda.current_stream()
+    if args.no_qr:   # profiling the other legs on their own
+        out = {"metric": METRIC, "value": None, "unit": "GFLOP/s", "n_gpus": 1,
+               "note": "headline QR_2D skipped (--no-qr)"}
+    else:
+        dev = torch.device("cuda:0")
+        A0 = qr.to_gpu(A_host)
+        A = torch.empty_like(A0)
+        ctx = qr.context()
+        stream = torch.cuda.current_stream()
 
-    for _ in range(args.warmup):
-        A.copy_(A0)
-        qr.QR_2D(A, m, m, n, b, sync=False)
-    ctx.sync()
-
-    def run_steps(profiled):
-        """K factorizations, each bracketed by events on the launch stream;
-        with `profiled`, the library also brackets every kernel with events
-        (its per-kernel durations feed the roofline and the breakdown)."""
-        if profiled:
-            ctx.profile(True)
-            ctx.profile_reset()
-        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in range(args.steps)]
-        torch.cuda.synchronize()
-        w0 = time.perf_counter()
-        for s in range(args.steps):
+        for _ in range(args.warmup):
             A.copy_(A0)
-            ev[s][0].record(stream)
             qr.QR_2D(A, m, m, n, b, sync=False)
-            ev[s][1].record(stream)
-        torch.cuda.synchronize()
-        w = time.perf_counter() - w0
-        ctx.sync()   # raises if any tree wait of these steps gave up
-        pr = None
-        if profiled:
-            pr = ctx.profile_read()
-            ctx.profile(False)
-        return [a.elapsed_time(z) for a, z in ev], w, pr
+        ctx.sync()
 
-    # the headline steps run without per-kernel events (they cost time of
-    # their own); the same K steps are then repeated with them
-    step_ms, wall, _ = run_steps(False)
-    step_ms_prof, _, prof = run_steps(True)
-    ms = float(np.mean(step_ms))
-    F = qr_flops(m, n)
-    gflops = F / (ms * 1e-3) / 1e9
+        def run_steps(profiled):
+            """K factorizations, each bracketed by events on the launch stream;
+            with `profiled`, the library also brackets every kernel with events
+            (its per-kernel durations feed the roofline and the breakdown)."""
+            if profiled:
+                ctx.profile(True)
+                ctx.profile_reset()
+            ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                  for _ in range(args.steps)]
+            torch.cuda.synchronize()
+            w0 = time.perf_counter()
+            for s in range(args.steps):
+                A.copy_(A0)
+                ev[s][0].record(stream)
+                qr.QR_2D(A, m, m, n, b, sync=False)
+                ev[s][1].record(stream)
+            torch.cuda.synchronize()
+            w = time.perf_counter() - w0
+            ctx.sync()   # raises if any tree wait of these steps gave up
+            pr = None
+            if profiled:
+                pr = ctx.profile_read()
+                ctx.profile(False)
+            return [a.elapsed_time(z) for a, z in ev], w, pr
 
-    # dominant kernel class by time, its algorithmic flops/bytes per launch
-    kinds = {k: v for k, v in prof.items() if v[3] > 0}
-    dom = max(kinds, key=lambda k: kinds[k][0])
-    dms, dfl, dby, dn = kinds[dom]
-    is_mfma = dom.startswith("gemm")
-    achieved = (dfl / (dms * 1e-3) / 1e12) if is_mfma else (dby / (dms * 1e-3) / 1e9)
-    peak = PEAK_FP64_TFLOPS if is_mfma else PEAK_HBM_GBS
-    # HBM bytes per launch from the committed counter runs (FETCH_SIZE and
-    # WRITE_SIZE, calibrated on a 1 GiB copy) when they cover this kernel
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", f"qr2d_32768x4096_b128_{dom}_traffic.json")
-    if os.path.exists(tf) and (m, n, b) == (32768, 4096, 128):
-        with open(tf) as fh:
-            tj = json.load(fh)
-        if tj.get("kernel_class") == dom:
-            traffic = tj["traffic_bytes_per_launch"]
-    roofline = {
-        "bound": "mfma" if is_mfma else "hbm",
-        "kernel": dom,
-        "achieved": round(achieved, 3),
-        "peak": peak,
-        "unit": "TFLOP/s" if is_mfma else "GB/s",
-        "frac": round(achieved / peak, 4),
-        "traffic": traffic,
-        "algorithmic_bytes_per_launch": round(dby / dn),
-        "avg_launch_ms": round(dms / dn, 5),
-        "launches_per_step": dn / args.steps,
-    }
-    breakdown = {k: {"ms_per_step": round(v[0] / args.steps, 3), "launches_per_step": v[3] / args.steps,
-                     "tflops": round(v[1] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else None}
-                 for k, v in kinds.items()}
+        # the headline steps run without per-kernel events (they cost time of
+        # their own); the same K steps are then repeated with them
+        step_ms, wall, _ = run_steps(False)
+        step_ms_prof, _, prof = run_steps(True)
+        ms = float(np.mean(step_ms))
+        F = qr_flops(m, n)
+        gflops = F / (ms * 1e-3) / 1e9
 
-    out = {
-        "metric": METRIC,
-        "value": round(gflops, 2),
-        "unit": "GFLOP/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "FP64 QR_2D (TSQR+HR panels, WY update) of a 32768x4096 N(0,1) matrix",
-                   "m": m, "n": n, "b": b, "seed": args.seed},
-        "qr_roofline_frac": round(gflops / 1e3 / PEAK_FP64_TFLOPS, 4),
-        "roofline": roofline,
-        "breakdown": breakdown,
-        "wall_ms_per_step_incl_restore": round(wall / args.steps * 1e3, 3),
-        "ms_per_step_with_kernel_events": round(float(np.mean(step_ms_prof)), 3),
-    }
+        # dominant kernel class by time, its algorithmic flops/bytes per launch
+        kinds = {k: v for k, v in prof.items() if v[3] > 0}
+        dom = max(kinds, key=lambda k: kinds[k][0])
+        dms, dfl, dby, dn = kinds[dom]
+        is_mfma = dom.startswith("gemm")
+        achieved = (dfl / (dms * 1e-3) / 1e12) if is_mfma else (dby / (dms * 1e-3) / 1e9)
+        peak = PEAK_FP64_TFLOPS if is_mfma else PEAK_HBM_GBS
+        # HBM bytes per launch from the committed counter runs (FETCH_SIZE and
+        # WRITE_SIZE, calibrated on a 1 GiB copy) when they cover this kernel
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", f"qr2d_32768x4096_b128_{dom}_traffic.json")
+        if os.path.exists(tf) and (m, n, b) == (32768, 4096, 128):
+            with open(tf) as fh:
+                tj = json.load(fh)
+            if tj.get("kernel_class") == dom:
+                traffic = tj["traffic_bytes_per_launch"]
+        roofline = {
+            "bound": "mfma" if is_mfma else "hbm",
+            "kernel": dom,
+            "achieved": round(achieved, 3),
+            "peak": peak,
+            "unit": "TFLOP/s" if is_mfma else "GB/s",
+            "frac": round(achieved / peak, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": round(dby / dn),
+            "avg_launch_ms": round(dms / dn, 5),
+            "launches_per_step": dn / args.steps,
+        }
+        breakdown = {k: {"ms_per_step": round(v[0] / args.steps, 3), "launches_per_step": v[3] / args.steps,
+                         "tflops": round(v[1] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else None}
+                     for k, v in kinds.items()}
 
-    if not args.no_accuracy:
-        YR = qr.from_gpu(A, m, n)
-        res, orth = accuracy(A_host, YR)
-        out["accuracy"] = {"resid_over_n_eps": float(f"{res:.4g}"), "orth": float(f"{orth:.4g}"),
-                           "probe": "randomized ‖Ax−QRx‖/(‖A‖‖x‖nε), ‖QᵀQv−v‖/‖v‖"}
-    del A, A0
-    torch.cuda.empty_cache()
+        out = {
+            "metric": METRIC,
+            "value": round(gflops, 2),
+            "unit": "GFLOP/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": round(ms, 3),
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "FP64 QR_2D (TSQR+HR panels, WY update) of a 32768x4096 N(0,1) matrix",
+                       "m": m, "n": n, "b": b, "seed": args.seed},
+            "qr_roofline_frac": round(gflops / 1e3 / PEAK_FP64_TFLOPS, 4),
+            "roofline": roofline,
+            "breakdown": breakdown,
+            "wall_ms_per_step_incl_restore": round(wall / args.steps * 1e3, 3),
+            "ms_per_step_with_kernel_events": round(float(np.mean(step_ms_prof)), 3),
+        }
+
+        if not args.no_accuracy:
+            YR = qr.from_gpu(A, m, n)
+            res, orth = accuracy(A_host, YR)
+            out["accuracy"] = {"resid_over_n_eps": float(f"{res:.4g}"), "orth": float(f"{orth:.4g}"),
+                               "probe": "randomized ‖Ax−QRx‖/(‖A‖‖x‖nε), ‖QᵀQv−v‖/‖v‖"}
+        del A, A0
+        torch.cuda.empty_cache()
     if not args.no_lsq:
         out["lsq_mixed"] = lsq_leg(qr, torch)
         torch.cuda.empty_cache()
