@@ -110,6 +110,30 @@ def test_batch_autotune_keeps_results(Q):
     check_against_lapack(mats, bt.unpack(A), bt.unpack_t(Tt))
 
 
+def test_batch_at_the_bench_distribution(Q):
+    """96 problems drawn like the bench's batch (m ~ U[256, 16384], n in
+    {16, 32, 64, 128}): trees of many depths share the level-by-level
+    launches, and the row tiles of the Y product reach m = 16384; every
+    problem against LAPACK, eager and graph-replayed."""
+    rng = np.random.default_rng(13)
+    m = [int(x) for x in rng.integers(256, 16385, 96)]
+    n = [int(x) for x in rng.choice([16, 32, 64, 128], 96)]
+    m[0], n[0] = 16384, 128                       # the largest shape of the distribution
+    mats = [O.gaussian(mm, nn, 100 + i) for i, (mm, nn) in enumerate(zip(m, n))]
+    bt = Q.Batch(m, n)
+    A0 = bt.pack(mats)
+    A, Tt = A0.clone(), bt.new_t(A0)
+    bt.qr(A, Tt)
+    check_against_lapack(mats, bt.unpack(A), bt.unpack_t(Tt))
+    A2, T2 = A0.clone(), bt.new_t(A0)
+    for _ in range(2):
+        A2.copy_(A0)
+        torch.cuda.synchronize()
+        bt.qr(A2, T2, use_graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(A2, A) and torch.equal(T2, Tt)
+
+
 def test_batch_rejects_wide_problem(Q):
     from candmc_amd._lib import CQRError
     with pytest.raises(CQRError):
