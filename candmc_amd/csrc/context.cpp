@@ -90,12 +90,27 @@ static int prof_drain(cqr_ctx* c) {
   return CQR_OK;
 }
 
+// The lookahead's second stream, created on first use: a context that never
+// factors with lookahead (a served solve of a few panels) holds only its own
+// stream.  HIP deals the streams of a process round-robin onto a few
+// hardware queues (four by default), so every extra stream is one more that
+// can land on the queue of another context's stream and serialise with it.
+int ensure_side(cqr_ctx* c) {
+  if (c->side) return CQR_OK;
+  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) {
+    c->side = nullptr;
+    (void)hipGetLastError();
+    return set_err(c, CQR_ERR_DEVICE, "cannot create the lookahead stream");
+  }
+  return CQR_OK;
+}
+
 int ensure_ws(cqr_ctx* c, size_t bytes) {
   if (bytes <= c->ws_bytes) return CQR_OK;
   // Growing replaces the arena: wait for work that may still use it.
   if (c->ws) {
     CQR_HIP(c, hipStreamSynchronize(c->stream));
-    CQR_HIP(c, hipStreamSynchronize(c->side));
+    if (c->side) CQR_HIP(c, hipStreamSynchronize(c->side));
     CQR_HIP(c, hipFree(c->ws));
     c->ws = nullptr;
     c->ws_bytes = 0;
@@ -169,7 +184,6 @@ int cqr_create(cqr_ctx** ctx, int device) {
       hipMemset(c->d_terr, 0, 2 * sizeof(int)) != hipSuccess ||
       hipMemset(c->d_cnt, 0, sizeof(uint64_t) * CQR_MAX_TREE_NODES * 2) != hipSuccess ||
       hipMemset(c->d_flag, 0, sizeof(int)) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_gate, hipEventDisableTiming) != hipSuccess) {
@@ -262,7 +276,7 @@ int cqr_trim(cqr_ctx* c) {
   (void)hipSetDevice(c->device);
   if (!c->ws) return CQR_OK;
   CQR_HIP(c, hipStreamSynchronize(c->stream));
-  CQR_HIP(c, hipStreamSynchronize(c->side));
+  if (c->side) CQR_HIP(c, hipStreamSynchronize(c->side));
   CQR_HIP(c, hipFree(c->ws));
   c->ws = nullptr;
   c->ws_bytes = 0;

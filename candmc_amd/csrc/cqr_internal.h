@@ -99,6 +99,8 @@ struct Sizer {
     return off;
   }
 };
+// the lookahead stream (c->side), created on first use
+int ensure_side(cqr_ctx* c);
 int ensure_ws(cqr_ctx* c, size_t bytes);
 template <class T>
 inline T* ws_at(cqr_ctx* c, size_t off) {

@@ -460,6 +460,7 @@ int qr2d_run(cqr_ctx* c, double* A, int64_t lda, int64_t m, int64_t k, int64_t b
   double* Ye2[2] = {reinterpret_cast<double*>(base + o.Ye),
                     reinterpret_cast<double*>(base + o.Ye) + 2 * m * std::min(k, b)};
   if (la) {  // the side stream starts after everything already queued
+    CQR_TRY(ensure_side(c));
     CQR_HIP(c, hipEventRecord(c->ev_fork, c->stream));
     CQR_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
   }
