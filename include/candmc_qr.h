@@ -386,6 +386,10 @@ size_t cqr_lstsq_batched_workspace_bytes(int64_t nprob, const cqr_request* reqs)
 typedef struct cqr_server cqr_server;
 /* Workspace bytes one least-squares solve needs (its admission footprint). */
 size_t cqr_lstsq_workspace_bytes(int64_t m, int64_t n, int64_t nrhs, int64_t b);
+/* Each slot drives its own stream.  HIP maps a process's streams onto a
+ * few hardware queues (GPU_MAX_HW_QUEUES, four by default, read when HIP
+ * starts): give the process at least nslots + 4 so that the slots'
+ * streams do not share queues and serialise.                             */
 int cqr_server_create(cqr_server** server, int device, int nslots, size_t hbm_budget);
 /* Waits for the requests already running; queued requests that have not
  * started finish as CQR_ERR_CANCELLED (a thread waiting for one returns). */
