@@ -28,7 +28,6 @@ namespace cqr {
 namespace {
 
 constexpr int CW = 16;     // columns per work item
-constexpr int NT = 256;    // threads per item
 constexpr int LDS_LD = CW + 1;
 
 // The node consuming rank r's block above level `lv` (lv = −1 for a leaf):
@@ -50,7 +49,8 @@ __device__ inline int parent_node(int r, int lv, int L) {
 
 }  // namespace
 
-__global__ void __launch_bounds__(NT)
+template <int NWV>
+__global__ void __launch_bounds__(64 * NWV)
 q1_kernel(const double* __restrict__ nodeV2, const double* __restrict__ nodeT, const double* __restrict__ leafT,
           const double* __restrict__ V, int64_t ldv, int64_t m, double* __restrict__ C,
           double* __restrict__ Q, int64_t ldq, uint64_t* __restrict__ flags, uint64_t epoch, int L,
@@ -94,7 +94,8 @@ q1_kernel(const double* __restrict__ nodeV2, const double* __restrict__ nodeT, c
   const int cb = item % ncb;
   const int j0 = cb * CW, nj = min(CW, b - j0);
   const int R = min(b, j0 + CW);
-  const int tid = threadIdx.x, i = tid & 127, jh = tid >> 7;
+  constexpr int NT = 64 * NWV, CPT = 16 * 128 / NT;   // threads; C columns loaded per thread
+  const int tid = threadIdx.x, i = tid & 127, jq = tid / 128;
   const int64_t bb = (int64_t)b * b;
 
   const bool is_node = item < nnodes * ncb;
@@ -127,23 +128,23 @@ q1_kernel(const double* __restrict__ nodeV2, const double* __restrict__ nodeT, c
   }
   double* Cz = C + (int64_t)z * bb;
   {
-    // thread (k = tid & 127, half jh) loads 8 columns of row k, all in flight
-    double cv[8];
+    // thread (k = tid & 127, group jq) loads CPT columns of row k, all in flight
+    double cv[CPT];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int jj = jh * 8 + j;
+    for (int j = 0; j < CPT; ++j) {
+      const int jj = jq * CPT + j;
       double v = 0.0;
       if (i < R && jj < nj)
         v = parent >= 0 ? load_coherent(Cz + i + (int64_t)(j0 + jj) * b) : (i == j0 + jj ? 1.0 : 0.0);
       cv[j] = v;
     }
 #pragma unroll
-    for (int j = 0; j < 8; ++j) S0[i][jh * 8 + j] = cv[j];
+    for (int j = 0; j < CPT; ++j) S0[i][jq * CPT + j] = cv[j];
   }
   __syncthreads();
 
   // Products on FP64 MFMA (16×16 tiles, a wave per tile row): the 16-column
-  // block is one tile wide; wave w takes tile rows w, w+4, ...  Fragment
+  // block is one tile wide; wave w takes tile rows w, w+NWV, ...  Fragment
   // layout as in mfma_tile16: lane l returns row (l & 15), columns
   // (l >> 4) + 4v.
   const int lane = tid & 63, w = tid >> 6;
@@ -156,20 +157,20 @@ q1_kernel(const double* __restrict__ nodeV2, const double* __restrict__ nodeT, c
     const double* Tn = nodeT + (int64_t)nid * bb;
     const double* V2 = nodeV2 + (int64_t)nid * bb;
     // X = T·C_z (T upper: row tile t needs k ≥ 16t)
-    for (int t = w; t < nrt; t += 4) {
+    for (int t = w; t < nrt; t += NWV) {
       const int m0 = 16 * t;
       auto xT = [&](int r, int k) { return (m0 + r < b && k < R) ? Tn[(m0 + r) + (int64_t)k * b] : 0.0; };
       const dbl4 acc = mfma_tile16_k128(xT, fromS(S0, R), m0, R, lane);
 #pragma unroll
       for (int v = 0; v < 4; ++v) S1[m0 + tr][tc + 4 * v] = m0 + tr < R ? acc[v] : 0.0;
     }
-    for (int t = nrt + w; t < 8; t += 4)   // rows ≥ R of X are zero
+    for (int t = nrt + w; t < 8; t += NWV)   // rows ≥ R of X are zero
 #pragma unroll
       for (int v = 0; v < 4; ++v) S1[16 * t + tr][tc + 4 * v] = 0.0;
     __syncthreads();
     // C_s = −V2·X, C_z −= X
     double* Cs = C + (int64_t)snd * bb;
-    for (int t = w; t * 16 < b; t += 4) {
+    for (int t = w; t * 16 < b; t += NWV) {
       const int m0 = 16 * t;
       dbl4 acc = {0.0, 0.0, 0.0, 0.0};
       if (t < nrt) {
@@ -197,7 +198,7 @@ q1_kernel(const double* __restrict__ nodeV2, const double* __restrict__ nodeT, c
   const double* Vl = V + r0;
   const int64_t ml = r1 - r0;
   // Z = V_topᵀ·C_l: Z[i] = Σ_{i≤k<R} V_top[k][i]·C[k], R rows
-  for (int t = w; t < nrt; t += 4) {
+  for (int t = w; t < nrt; t += NWV) {
     const int m0 = 16 * t;
     auto xVt = [&](int r, int k) {
       const int col = m0 + r;
@@ -208,13 +209,14 @@ q1_kernel(const double* __restrict__ nodeV2, const double* __restrict__ nodeT, c
     for (int v = 0; v < 4; ++v) S1[m0 + tr][tc + 4 * v] = m0 + tr < R ? acc[v] : 0.0;
   }
   __syncthreads();
-  // M = T_l·Z, overwriting Z in S1 once every tile is computed (a wave
-  // holds at most two of the eight tile rows)
+  // M = T_l·Z, overwriting Z in S1 once every tile is computed (wave w
+  // holds tile rows w, w+NWV, ... of the eight)
   const double* Tl = leafT + (int64_t)z * bb;
-  dbl4 macc[2];
+  constexpr int TPW = 8 / NWV;
+  dbl4 macc[TPW];
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int t = w + 4 * q, m0 = 16 * t;
+  for (int q = 0; q < TPW; ++q) {
+    const int t = w + NWV * q, m0 = 16 * t;
     macc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
     if (t < nrt) {
       auto xT = [&](int r, int k) { return (m0 + r < b && k < R) ? Tl[(m0 + r) + (int64_t)k * b] : 0.0; };
@@ -223,15 +225,15 @@ q1_kernel(const double* __restrict__ nodeV2, const double* __restrict__ nodeT, c
   }
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int t = w + 4 * q, m0 = 16 * t;
+  for (int q = 0; q < TPW; ++q) {
+    const int t = w + NWV * q, m0 = 16 * t;
     if (t < nrt)
 #pragma unroll
       for (int v = 0; v < 4; ++v) S1[m0 + tr][tc + 4 * v] = m0 + tr < R ? macc[q][v] : 0.0;
   }
   __syncthreads();
   // Q1_l = [C_l; 0] − V_l·M over the leaf rows (V_l unit lower on top)
-  for (int64_t t = w; t * 16 < ml; t += 4) {
+  for (int64_t t = w; t * 16 < ml; t += NWV) {
     const int64_t m0 = 16 * t;
     auto xV = [&](int r, int k) {
       const int64_t row = m0 + r;
@@ -259,7 +261,10 @@ int launch_q1(cqr_ctx* c, const double* nodeV2, const double* nodeT, const doubl
     return set_err(c, CQR_ERR_UNSUPPORTED, "construct_Q1: tree of %d nodes", nnodes);
   const uint64_t epoch = ++c->q1_epoch;
   hipEvent_t ev = prof_begin(c);
-  hipLaunchKernelGGL(q1_kernel, dim3((unsigned)((nnodes + L) * ncb)), dim3(NT), 0, c->stream, nodeV2, nodeT,
+  // eight waves an item here (each wave one tile row of a product: the
+  // tree's critical chain is latency-bound), four in the batched launch
+  // (twice the items per CU for throughput)
+  hipLaunchKernelGGL(q1_kernel<8>, dim3((unsigned)((nnodes + L) * ncb)), dim3(512), 0, c->stream, nodeV2, nodeT,
                      leafT, V, ldv, m, C, Q, ldq, c->d_qflags, epoch, L, nnodes, b, c->d_terr, nullptr, nullptr,
                      0);
   prof_end(c, ev, PK_BGEMM, 4.0 * nnodes * (double)b * b * b + 4.0 * (double)m * b * b,
@@ -275,7 +280,7 @@ int launch_q1_batched(cqr_ctx* c, const double* nodeV2, const double* nodeT, con
                       const BatchSeg* seg, int nseg) {
   if (total_items <= 0) return CQR_OK;
   hipEvent_t ev = prof_begin(c);
-  hipLaunchKernelGGL(q1_kernel, dim3((unsigned)total_items), dim3(NT), 0, c->stream, nodeV2, nodeT, leafT, Vbase,
+  hipLaunchKernelGGL(q1_kernel<4>, dim3((unsigned)total_items), dim3(256), 0, c->stream, nodeV2, nodeT, leafT, Vbase,
                      (int64_t)0, (int64_t)0, C, Y, (int64_t)0, flags, (uint64_t)1, 0, 0, b, c->d_terr, bp, seg, nseg);
   prof_end(c, ev, PK_BGEMM, 0.0, 0.0);
   return check_launch(c, "q1_kernel_batched");
