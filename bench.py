@@ -416,8 +416,14 @@ def main():
     # streams of this run (torch's, the library context's two, one per
     # server slot): streams then share queues and the slots' work
     # serialises (served capacity at n = 64: 3085 solves/s on four queues,
-    # 5211 on eight).  The headline QR is the same either way.
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+    # 5211 on eight).  The headline QR is the same either way.  (The
+    # environment may already name four, HIP's default: raise it.)
+    try:
+        queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+    except ValueError:
+        queues = 4
+    if queues < 8:
+        os.environ["GPU_MAX_HW_QUEUES"] = "8"
     m, n, b = args.m, args.n, args.b
     A_host = gaussian(m, n, args.seed)
     # the CPU baseline starts its rank processes before this process touches
