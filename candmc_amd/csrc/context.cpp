@@ -1,6 +1,9 @@
 // Context, errors, workspace arena and TSQR planning (host only).
+#include <algorithm>
 #include <cstdarg>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <mutex>
 #include <set>
 #include <tuple>
@@ -25,6 +28,119 @@ int check_launch(cqr_ctx* c, const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(c, CQR_ERR_DEVICE, "%s launch: %s", what, hipGetErrorString(e));
   return CQR_OK;
+}
+
+// Launches whose workgroups wait on other workgroups of the same launch
+// (the TSQR tree, construct_Q1) make progress alone: each XCD dispatches a
+// launch's blocks in order, and a block only waits on blocks of lower
+// index.  Two such launches at once on different streams have no common
+// order: each can fill an XCD's CUs with blocks that wait on producers the
+// other keeps from being dispatched, until their poll bound runs out (seen
+// with 16 server slots: seconds per request).  So such launches run
+// together only while all of their blocks fit on the device at once: each
+// launch's need is its blocks per XCD (they are dealt round-robin to the
+// eight XCDs) over the blocks a CU holds (the occupancy API), against the
+// CUs of one XCD.  A launch that would overflow it first waits for the
+// oldest running ones to finish (a stream wait on their completion
+// events; finished ones are dropped on the host); one that needs the whole
+// device runs alone.  Other launches (GEMMs, the leaf QR) are not counted:
+// their blocks always finish.  A stream being captured into a graph is not
+// gated (a capture cannot wait on an event recorded outside it).
+namespace {
+constexpr int kGateDevices = 64, kGateEvents = 64;
+struct SpinGate {
+  std::mutex mu;
+  struct Running {
+    hipEvent_t ev;
+    int need;
+  };
+  std::deque<Running> running;   // launch order
+  std::vector<hipEvent_t> pool;  // free events (all created up front)
+  int used = 0, cap = 0;
+  bool ready = false;
+};
+SpinGate& spin_gate(int device) {
+  static SpinGate gates[kGateDevices];
+  return gates[device];
+}
+// blocks of `fn` one CU holds (cached)
+int blocks_per_cu(const void* fn, int threads, size_t lds) {
+  static std::mutex mu;
+  static std::map<std::tuple<const void*, int, size_t>, int> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  const auto key = std::make_tuple(fn, threads, lds);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, threads, lds) != hipSuccess || n < 1) {
+    (void)hipGetLastError();
+    n = 1;
+  }
+  cache[key] = n;
+  return n;
+}
+}  // namespace
+
+int spin_gate_init(cqr_ctx* c) {
+  if (c->device < 0 || c->device >= kGateDevices) return CQR_OK;
+  SpinGate& g = spin_gate(c->device);
+  std::lock_guard<std::mutex> lk(g.mu);
+  if (g.ready) return CQR_OK;
+  for (int k = 0; k < kGateEvents; ++k) {
+    hipEvent_t e = nullptr;
+    CQR_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    g.pool.push_back(e);
+  }
+  g.cap = std::max(1, c->num_cu / 8);
+  g.ready = true;
+  return CQR_OK;
+}
+
+SpinScope::SpinScope(cqr_ctx* ctx, const void* fn, int64_t blocks, int threads, size_t lds) : c(ctx) {
+  if (c->device < 0 || c->device >= kGateDevices) return;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+    (void)hipGetLastError();
+    return;
+  }
+  SpinGate& g = spin_gate(c->device);
+  g.mu.lock();
+  if (!g.ready) {
+    g.mu.unlock();
+    return;
+  }
+  held = true;
+  const int64_t per_xcd = (blocks + 7) / 8, per_cu = blocks_per_cu(fn, threads, lds);
+  need = (int)std::min<int64_t>(g.cap, (per_xcd + per_cu - 1) / per_cu);
+  // finished launches give their room back
+  while (!g.running.empty() && hipEventQuery(g.running.front().ev) == hipSuccess) {
+    g.used -= g.running.front().need;
+    g.pool.push_back(g.running.front().ev);
+    g.running.pop_front();
+  }
+  (void)hipGetLastError();   // hipEventQuery's hipErrorNotReady
+  // then wait for the oldest running ones until this launch fits
+  while (!g.running.empty() && (g.used + need > g.cap || g.pool.empty())) {
+    if (hipStreamWaitEvent(c->stream, g.running.front().ev, 0) != hipSuccess)
+      status_ = set_err(c, CQR_ERR_DEVICE, "cannot order a waiting launch after a running one");
+    g.used -= g.running.front().need;
+    g.pool.push_back(g.running.front().ev);
+    g.running.pop_front();
+  }
+}
+
+SpinScope::~SpinScope() {
+  if (!held) return;
+  SpinGate& g = spin_gate(c->device);
+  hipEvent_t e = g.pool.back();
+  if (hipEventRecord(e, c->stream) == hipSuccess) {
+    g.pool.pop_back();
+    g.running.push_back(SpinGate::Running{e, need});
+    g.used += need;
+  } else {
+    (void)hipGetLastError();
+  }
+  g.mu.unlock();
 }
 
 int set_max_lds(cqr_ctx* c, const void* fn, int bytes) {
@@ -199,7 +315,7 @@ int cqr_create(cqr_ctx** ctx, int device) {
     delete c;
     return CQR_ERR_NOMEM;
   }
-  if (cqr::noalloc_setup(c) != CQR_OK) {
+  if (cqr::spin_gate_init(c) != CQR_OK || cqr::noalloc_setup(c) != CQR_OK) {
     cqr_destroy(c);
     return CQR_ERR_DEVICE;
   }

@@ -99,6 +99,28 @@ struct Sizer {
     return off;
   }
 };
+// Admits a launch whose workgroups wait on each other (context.cpp): it
+// may run beside other such launches of the device only while all of
+// their blocks fit on it at once, else its stream first waits for the
+// oldest running ones.  Construct one right before the launch, on the
+// launching stream, with the launch's kernel, blocks, threads per block and
+// dynamic LDS; it records the launch's completion when it goes out of
+// scope.  status() != CQR_OK: an ordering wait could not be queued.
+int spin_gate_init(cqr_ctx* c);
+class SpinScope {
+ public:
+  SpinScope(cqr_ctx* c, const void* fn, int64_t blocks, int threads, size_t lds);
+  ~SpinScope();
+  SpinScope(const SpinScope&) = delete;
+  SpinScope& operator=(const SpinScope&) = delete;
+  int status() const { return status_; }
+
+ private:
+  cqr_ctx* c;
+  bool held = false;
+  int need = 0;
+  int status_ = CQR_OK;
+};
 // the lookahead stream (c->side), created on first use
 int ensure_side(cqr_ctx* c);
 int ensure_ws(cqr_ctx* c, size_t bytes);

@@ -718,8 +718,12 @@ static int btree_launch(cqr_ctx* c, const double* leafR, double* nodeR, double* 
   auto k = btree_kernel<BMAX>;
   CQR_TRY(set_max_lds(c, (const void*)k, (int)lds));
   hipEvent_t ev = prof_begin(c);
-  hipLaunchKernelGGL(k, dim3(nnodes), dim3(4 * BMAX), lds, c->stream, leafR, nodeR, V2, T, tau, cnt, epoch, L, b,
-                     c->d_terr, bp, seg, nseg);
+  {
+    SpinScope gate(c, (const void*)k, nnodes, 4 * BMAX, lds);
+    CQR_TRY(gate.status());
+    hipLaunchKernelGGL(k, dim3(nnodes), dim3(4 * BMAX), lds, c->stream, leafR, nodeR, V2, T, tau, cnt, epoch, L, b,
+                       c->d_terr, bp, seg, nseg);
+  }
   prof_end(c, ev, PK_TREE, nnodes * (2.0 / 3.0) * b * (double)b * b, nnodes * 40.0 * b * b);
   return check_launch(c, "btree");
 }
@@ -728,8 +732,12 @@ template <int CPW, int RPL>
 static int ptree_launch(cqr_ctx* c, const double* leafR, double* nodeR, double* V2, double* T, double* tau,
                         uint64_t* cnt, int L, int nnodes, int b, uint64_t epoch, const BatchProb* bp, const BatchSeg* seg, int nseg) {
   hipEvent_t ev = prof_begin(c);
-  hipLaunchKernelGGL((ptree_kernel<CPW, RPL>), dim3(nnodes), dim3(512), 0, c->stream, leafR, nodeR, V2, T, tau,
-                     cnt, epoch, L, b, c->d_terr, bp, seg, nseg);
+  {
+    SpinScope gate(c, (const void*)ptree_kernel<CPW, RPL>, nnodes, 512, 0);
+    CQR_TRY(gate.status());
+    hipLaunchKernelGGL((ptree_kernel<CPW, RPL>), dim3(nnodes), dim3(512), 0, c->stream, leafR, nodeR, V2, T, tau,
+                       cnt, epoch, L, b, c->d_terr, bp, seg, nseg);
+  }
   prof_end(c, ev, PK_TREE, nnodes * (2.0 / 3.0) * b * (double)b * b, nnodes * 40.0 * b * b);
   return check_launch(c, "ptree");
 }
@@ -757,8 +765,12 @@ static int leaf_tree_launch(cqr_ctx* c, double* A, int64_t lda, int64_t m, int L
   CQR_TRY(set_max_lds(c, (const void*)k, (int)lds));
   const uint64_t epoch = ++c->tree_epoch;
   hipEvent_t ev = prof_begin(c);
-  hipLaunchKernelGGL(k, dim3(L + nnodes), dim3(512), lds, c->stream, A, lda, m, L, b, leafR, leafT, leafTau, nodeR,
-                     V2, T, tau, c->d_cnt, c->d_cnt + CQR_MAX_TREE_NODES, epoch, c->d_terr);
+  {
+    SpinScope gate(c, (const void*)k, L + nnodes, 512, lds);
+    CQR_TRY(gate.status());
+    hipLaunchKernelGGL(k, dim3(L + nnodes), dim3(512), lds, c->stream, A, lda, m, L, b, leafR, leafT, leafTau, nodeR,
+                       V2, T, tau, c->d_cnt, c->d_cnt + CQR_MAX_TREE_NODES, epoch, c->d_terr);
+  }
   prof_end(c, ev, PK_TREE,
            2.0 * m * b * b - (2.0 / 3.0) * L * (double)b * b * b + nnodes * (2.0 / 3.0) * b * (double)b * b,
            16.0 * m * b + nnodes * 40.0 * b * b);

@@ -264,9 +264,13 @@ int launch_q1(cqr_ctx* c, const double* nodeV2, const double* nodeT, const doubl
   // eight waves an item here (each wave one tile row of a product: the
   // tree's critical chain is latency-bound), four in the batched launch
   // (twice the items per CU for throughput)
-  hipLaunchKernelGGL(q1_kernel<8>, dim3((unsigned)((nnodes + L) * ncb)), dim3(512), 0, c->stream, nodeV2, nodeT,
-                     leafT, V, ldv, m, C, Q, ldq, c->d_qflags, epoch, L, nnodes, b, c->d_terr, nullptr, nullptr,
-                     0);
+  {
+    SpinScope gate(c, (const void*)q1_kernel<8>, (int64_t)(nnodes + L) * ncb, 512, 0);
+    CQR_TRY(gate.status());
+    hipLaunchKernelGGL(q1_kernel<8>, dim3((unsigned)((nnodes + L) * ncb)), dim3(512), 0, c->stream, nodeV2, nodeT,
+                       leafT, V, ldv, m, C, Q, ldq, c->d_qflags, epoch, L, nnodes, b, c->d_terr, nullptr, nullptr,
+                       0);
+  }
   prof_end(c, ev, PK_BGEMM, 4.0 * nnodes * (double)b * b * b + 4.0 * (double)m * b * b,
            8.0 * (3.0 * nnodes * b * b + 2.0 * m * b + 2.0 * L * b * b));
   return check_launch(c, "q1_kernel");
@@ -280,8 +284,12 @@ int launch_q1_batched(cqr_ctx* c, const double* nodeV2, const double* nodeT, con
                       const BatchSeg* seg, int nseg) {
   if (total_items <= 0) return CQR_OK;
   hipEvent_t ev = prof_begin(c);
-  hipLaunchKernelGGL(q1_kernel<4>, dim3((unsigned)total_items), dim3(256), 0, c->stream, nodeV2, nodeT, leafT, Vbase,
-                     (int64_t)0, (int64_t)0, C, Y, (int64_t)0, flags, (uint64_t)1, 0, 0, b, c->d_terr, bp, seg, nseg);
+  {
+    SpinScope gate(c, (const void*)q1_kernel<4>, total_items, 256, 0);
+    CQR_TRY(gate.status());
+    hipLaunchKernelGGL(q1_kernel<4>, dim3((unsigned)total_items), dim3(256), 0, c->stream, nodeV2, nodeT, leafT, Vbase,
+                       (int64_t)0, (int64_t)0, C, Y, (int64_t)0, flags, (uint64_t)1, 0, 0, b, c->d_terr, bp, seg, nseg);
+  }
   prof_end(c, ev, PK_BGEMM, 0.0, 0.0);
   return check_launch(c, "q1_kernel_batched");
 }
