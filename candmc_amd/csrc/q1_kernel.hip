@@ -261,15 +261,21 @@ int launch_q1(cqr_ctx* c, const double* nodeV2, const double* nodeT, const doubl
     return set_err(c, CQR_ERR_UNSUPPORTED, "construct_Q1: tree of %d nodes", nnodes);
   const uint64_t epoch = ++c->q1_epoch;
   hipEvent_t ev = prof_begin(c);
-  // eight waves an item here (each wave one tile row of a product: the
-  // tree's critical chain is latency-bound), four in the batched launch
-  // (twice the items per CU for throughput)
+  // Deep trees take eight waves an item (each wave one tile row of a
+  // product: the chain of tree levels is latency-bound); shallow ones four,
+  // two items a CU, so that the launch needs half the device's room beside
+  // other such launches (a served solve's).  The batched launch takes four.
+  const int64_t items = (int64_t)(nnodes + L) * ncb;
+  const bool deep = L >= 64;
   {
-    SpinScope gate(c, (const void*)q1_kernel<8>, (int64_t)(nnodes + L) * ncb, 512, 0);
+    SpinScope gate(c, deep ? (const void*)q1_kernel<8> : (const void*)q1_kernel<4>, items, deep ? 512 : 256, 0);
     CQR_TRY(gate.status());
-    hipLaunchKernelGGL(q1_kernel<8>, dim3((unsigned)((nnodes + L) * ncb)), dim3(512), 0, c->stream, nodeV2, nodeT,
-                       leafT, V, ldv, m, C, Q, ldq, c->d_qflags, epoch, L, nnodes, b, c->d_terr, nullptr, nullptr,
-                       0);
+    if (deep)
+      hipLaunchKernelGGL(q1_kernel<8>, dim3((unsigned)items), dim3(512), 0, c->stream, nodeV2, nodeT, leafT, V, ldv, m,
+                         C, Q, ldq, c->d_qflags, epoch, L, nnodes, b, c->d_terr, nullptr, nullptr, 0);
+    else
+      hipLaunchKernelGGL(q1_kernel<4>, dim3((unsigned)items), dim3(256), 0, c->stream, nodeV2, nodeT, leafT, V, ldv, m,
+                         C, Q, ldq, c->d_qflags, epoch, L, nnodes, b, c->d_terr, nullptr, nullptr, 0);
   }
   prof_end(c, ev, PK_BGEMM, 4.0 * nnodes * (double)b * b * b + 4.0 * (double)m * b * b,
            8.0 * (3.0 * nnodes * b * b + 2.0 * m * b + 2.0 * L * b * b));
