@@ -1,8 +1,9 @@
 """GPU tests of the least-squares request server and of the batched solve
 it uses: each request gives the same bits alone and in a concurrent mix (or
-any batch), both accuracy tiers meet their targets against scipy's FP64
-lstsq, admission respects the HBM budget, oversized requests are refused,
-and one-panel requests are batched."""
+any batch), also tall requests on many slots at once, both accuracy tiers
+meet their targets against scipy's FP64 lstsq, admission respects the HBM
+budget, oversized requests are refused, and one-panel requests are
+batched."""
 import numpy as np
 import pytest
 import scipy.linalg as sl
@@ -79,6 +80,31 @@ def test_same_result_alone_and_in_mix(Q):
         assert err < max(target, 50 * kappa * EPS), (m, n, k, kappa, tier, err)
         if r["status"] == 6:
             assert r["last_update"] > target
+
+
+def test_many_slots_tall_requests(Q):
+    """Twelve slots factoring tall requests at once (32768×256 and 65536×130:
+    leaf + tree and construct_Q1 launches of a workgroup per CU, hundreds of
+    them): such launches from different slots are admitted together only
+    while they all fit on the device, so none waits on workgroups another
+    keeps out.  Every request succeeds and gives the same bits as alone."""
+    reqs = []
+    for i in range(12):
+        m, n = (32768, 256) if i % 2 == 0 else (65536, 130)
+        A = O.gaussian(m, n, 300 + i)
+        rng = np.random.default_rng(400 + i)
+        B = np.asfortranarray(A @ rng.standard_normal((n, 1)) + 1e-6 * rng.standard_normal((m, 1)))
+        reqs.append((A, B, m, n, 1, 10.0, Q.TIER_F64))
+    one = Q.Server(nslots=1)
+    xa, ra = run(Q, one, reqs, together=False)
+    one.close()
+    many = Q.Server(nslots=12)
+    xm, rm = run(Q, many, reqs, together=True)
+    many.close()
+    assert all(r["status"] == 0 for r in ra + rm), [(r["slot"], r["status"]) for r in rm]
+    assert len({r["slot"] for r in rm}) > 1
+    for a, b in zip(xa, xm):
+        assert np.array_equal(a, b)
 
 
 def test_admission_respects_budget(Q):
