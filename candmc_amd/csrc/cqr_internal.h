@@ -38,6 +38,7 @@ struct cqr_ctx {
   uint64_t* d_cnt = nullptr;    // column counters: tree nodes, then (second half) leaves
   int* d_terr = nullptr;        // set by a tree node whose wait for a child timed out
   uint64_t* d_qflags = nullptr; // per (node, column block) flags of construct_Q1
+  uint64_t* d_sflags = nullptr; // per 128-row block flags of the one-launch triangular solve
   uint64_t q1_epoch = 0;
   uint64_t tree_epoch = 0;      // launch epoch folded into the counters
   hipStream_t side = nullptr;   // second stream of the QR_2D lookahead
@@ -231,6 +232,7 @@ __device__ inline int seg_find(const BatchSeg* s, int nseg, int idx) {
 }
 
 #define CQR_MAX_TREE_NODES 8192
+#define CQR_TRSM_MAX_BLOCKS 4096   // 128-row blocks of the one-launch triangular solve
 
 // Column-pipelined tree over all levels (ptree_kernel.hip): node outputs R
 // in nodeR (nnodes×b×b), V2/T/tau per node in forward level order.
@@ -282,6 +284,9 @@ int launch_signed_nlu(cqr_ctx* c, double* A, int64_t lda, double* R, int64_t lda
                       int64_t* signs, int b);
 int launch_tri_inverse(cqr_ctx* c, const double* X, int64_t ldx, double* Xinv,
                        int64_t ldi, int b, int upper, int unit);
+// every blk-wide diagonal block of X's upper triangle (n×n) in one launch;
+// block p's inverse at Xinv + p·blk² with ld bj (blk ≤ 128)
+int launch_tri_inverse_diag(cqr_ctx* c, const double* X, int64_t ldx, double* Xinv, int64_t n, int blk);
 int launch_tau_recon(cqr_ctx* c, char s, int64_t ncol, int64_t nrow, int64_t lda,
                      const double* Y, double* tau);
 int launch_pack(cqr_ctx* c, int op, const double* src, double* dst, int64_t b,
@@ -345,6 +350,10 @@ int thin_gemm(cqr_ctx* c, int ta, int64_t m, int64_t k, double alpha, const doub
               int unit_top = 0);
 // One step of the blocked R solve (lsq_kernels.hip): blocks [q0, q0+nq)
 // take block p's solution into account; block `solve` is then finished.
+// B(0:n, :) := R⁻¹·B or R⁻ᵀ·B (nr ≤ 16 columns) in one launch, with the
+// 128-wide diagonal-block inverses Rinv
+int launch_trsm_chain(cqr_ctx* c, const double* R, int64_t ldr, int64_t n, const double* Rinv, double* B,
+                      int64_t ldb, int nr, int trans);
 int launch_trsm_step(cqr_ctx* c, const double* R, int64_t ldr, int64_t n, const double* Rinv, double* B, int64_t ldb,
                      int nr, int trans, int p, int q0, int nq, int solve);
 int launch_axpby(cqr_ctx* c, int64_t m, int64_t n, double alpha, const double* X, int64_t ldx, double beta,

@@ -72,12 +72,7 @@ size_t apply_q_ws_doubles(int64_t m, int64_t b, int64_t nr) {
 
 // Inverses of the diagonal blocks of R (upper triangle of YR), 128 wide.
 int r_block_inverses(cqr_ctx* c, const double* YR, int64_t ldy, int64_t n, double* Rinv) {
-  const int64_t nb = (n + 127) / 128;
-  for (int64_t p = 0; p < nb; ++p) {
-    const int64_t j = p * 128, bj = std::min<int64_t>(128, n - j);
-    CQR_TRY(launch_tri_inverse(c, YR + j + j * ldy, ldy, Rinv + p * 128 * 128, bj, (int)bj, 1, 0));
-  }
-  return CQR_OK;
+  return launch_tri_inverse_diag(c, YR, ldy, Rinv, n, 128);
 }
 
 // B(0:n, :) := R⁻¹·B (trans = 0) or R⁻ᵀ·B (trans = 1), blocked with the
@@ -85,6 +80,7 @@ int r_block_inverses(cqr_ctx* c, const double* YR, int64_t ldy, int64_t n, doubl
 int trsm_r_ws(cqr_ctx* c, const double* YR, int64_t ldy, int64_t n, const double* Rinv, double* B, int64_t ldb,
               int64_t nr, int trans, double* tmp) {
   const int64_t nb = (n + 127) / 128;
+  if (nr <= 16 && nb > 1 && nb <= CQR_TRSM_MAX_BLOCKS) return launch_trsm_chain(c, YR, ldy, n, Rinv, B, ldb, (int)nr, trans);
   if (nr <= 16) {
     // first diagonal block alone, then one launch per block step: the
     // remaining blocks take the new block into account and the next one

@@ -262,6 +262,138 @@ int launch_trsm_step(cqr_ctx* c, const double* R, int64_t ldr, int64_t n, const 
   return check_launch(c, "trsm_step");
 }
 
+// The whole blocked triangular solve B(0:n, :) := R⁻¹·B (trans = 0) or
+// R⁻ᵀ·B (trans = 1) in one launch, one workgroup per 128-row block q.  A
+// block waits for the blocks it depends on (q + 1.. for R⁻¹, ..q − 1 for
+// R⁻ᵀ), one at a time in the order they finish, subtracts R(q, p)·x_p (or
+// R(p, q)ᵀ·x_p) as each x_p is published, then forms x_q = op(R_qq⁻¹)·y_q
+// and publishes it (device-coherent stores, then flags[q] = 1).  Workgroup
+// ids follow the finishing order, so every block waits only on smaller ids
+// and dispatch order alone guarantees progress.  R's block for the next
+// dependency and R_qq⁻¹ are loaded into registers before each wait.  The
+// flags must be zero at launch; a wait that times out sets err[0] and the
+// block finishes (and publishes) with what it has.
+template <int NR>
+__global__ void __launch_bounds__(256)
+trsm_chain_kernel(const double* __restrict__ R, int64_t ldr, int64_t n, const double* __restrict__ Rinv,
+                  double* __restrict__ B, int64_t ldb, int nr, int trans, uint64_t* __restrict__ flags,
+                  int* __restrict__ err) {
+  __shared__ double xs[128][NR + 1];
+  __shared__ double ys[128][NR + 1];
+  __shared__ double part[2][128][NR + 1];
+  __shared__ int s_err;
+  const int nb = (int)((n + 127) / 128);
+  const int q = trans ? (int)blockIdx.x : nb - 1 - (int)blockIdx.x;
+  const int t = threadIdx.x, r = t & 127, h = t >> 7, kb = h * 64;   // row of the block, half of the k range
+  const int64_t jq = (int64_t)q * 128;
+  const int bq = (int)min<int64_t>(128, n - jq);
+  for (int e = t; e < 128 * NR; e += 256) {
+    const int rr = e & 127, j = e >> 7;
+    ys[rr][j] = (rr < bq && j < nr) ? B[jq + rr + (int64_t)j * ldb] : 0.0;
+  }
+  if (t == 0) s_err = 0;
+  // op(R_qq⁻¹)(r, k) for this thread's half of k: upper (k ≥ r), or the
+  // transpose (k ≤ r)
+  const double* Ri = Rinv + (int64_t)q * 128 * 128;
+  double ri[64];
+#pragma unroll
+  for (int u = 0; u < 64; ++u) {
+    const int k = kb + u;
+    const bool ok = r < bq && k < bq && (trans ? k <= r : k >= r);
+    ri[u] = Ri[ok ? (trans ? k + (int64_t)r * bq : r + (int64_t)k * bq) : 0];
+  }
+  const int ndep = trans ? q : nb - 1 - q;
+  double a[64];
+  auto load_block = [&](int i) {   // R's block against the i-th dependency
+    const int p = trans ? i : nb - 1 - i;
+    const int64_t jp = (int64_t)p * 128;
+    const int bp = (int)min<int64_t>(128, n - jp);
+#pragma unroll
+    for (int u = 0; u < 64; ++u) {
+      const int k = kb + u;
+      const bool ok = r < bq && k < bp;
+      const int64_t idx = trans ? (jp + k) + (jq + r) * ldr : (jq + r) + (jp + k) * ldr;
+      a[u] = R[ok ? idx : 0];
+    }
+  };
+  if (ndep > 0) load_block(0);
+  for (int i = 0; i < ndep; ++i) {
+    const int p = trans ? i : nb - 1 - i;
+    const int64_t jp = (int64_t)p * 128;
+    const int bp = (int)min<int64_t>(128, n - jp);
+    if (t == 0 && !s_err && !flag_wait(flags + p, 1, wait_limit(err))) s_err = 1;
+    __syncthreads();
+    for (int e = t; e < 128 * NR; e += 256) {
+      const int rr = e & 127, j = e >> 7;
+      xs[rr][j] = (rr < bp && j < nr) ? load_coherent(B + jp + rr + (int64_t)j * ldb) : 0.0;
+    }
+    __syncthreads();
+    double acc[NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) acc[j] = 0.0;
+#pragma unroll
+    for (int u = 0; u < 64; ++u) {
+      if (kb + u < bp) {
+#pragma unroll
+        for (int j = 0; j < NR; ++j) acc[j] += a[u] * xs[kb + u][j];
+      }
+    }
+    if (i + 1 < ndep) load_block(i + 1);
+#pragma unroll
+    for (int j = 0; j < NR; ++j) part[h][r][j] = acc[j];
+    __syncthreads();
+    if (h == 0 && r < bq) {
+#pragma unroll
+      for (int j = 0; j < NR; ++j) ys[r][j] -= part[0][r][j] + part[1][r][j];
+    }
+  }
+  __syncthreads();
+  double acc[NR];
+#pragma unroll
+  for (int j = 0; j < NR; ++j) acc[j] = 0.0;
+#pragma unroll
+  for (int u = 0; u < 64; ++u) {
+    const int k = kb + u;
+    if (k < bq && r < bq && (trans ? k <= r : k >= r)) {
+#pragma unroll
+      for (int j = 0; j < NR; ++j) acc[j] += ri[u] * ys[k][j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NR; ++j) part[h][r][j] = acc[j];
+  __syncthreads();
+  if (h == 0 && r < bq) {
+#pragma unroll
+    for (int j = 0; j < NR; ++j)
+      if (j < nr) store_coherent(B + jq + r + (int64_t)j * ldb, part[0][r][j] + part[1][r][j]);
+  }
+  wait_memory();   // this wave's x_q stores have reached memory
+  __syncthreads();
+  if (t == 0) {
+    if (s_err) err[0] = 1;
+    flag_store(flags + q, 1);
+  }
+}
+
+int launch_trsm_chain(cqr_ctx* c, const double* R, int64_t ldr, int64_t n, const double* Rinv, double* B,
+                      int64_t ldb, int nr, int trans) {
+  const int64_t nb = (n + 127) / 128;
+  if (nr < 1 || nr > 16 || nb < 1 || nb > CQR_TRSM_MAX_BLOCKS)
+    return set_err(c, CQR_ERR_UNSUPPORTED, "trsm chain: %d columns, %lld blocks", nr, (long long)nb);
+  CQR_TRY(launch_fill2d(c, reinterpret_cast<double*>(c->d_sflags), nb, nb, 1, 0.0));   // all-zero bits
+  auto run = [&](auto k) -> int {
+    SpinScope gate(c, (const void*)k, nb, 256, 0);
+    CQR_TRY(gate.status());
+    hipLaunchKernelGGL(k, dim3((unsigned)nb), dim3(256), 0, c->stream, R, ldr, n, Rinv, B, ldb, nr, trans,
+                       c->d_sflags, c->d_terr);
+    return CQR_OK;
+  };
+  if (nr == 1) CQR_TRY(run(trsm_chain_kernel<1>));
+  else if (nr <= 4) CQR_TRY(run(trsm_chain_kernel<4>));
+  else CQR_TRY(run(trsm_chain_kernel<16>));
+  return check_launch(c, "trsm_chain");
+}
+
 // Y = alpha·X + beta·Y on an m×n block.
 __global__ void axpby_kernel(int64_t m, int64_t n, double alpha, const double* __restrict__ X, int64_t ldx,
                              double beta, double* __restrict__ Y, int64_t ldy) {

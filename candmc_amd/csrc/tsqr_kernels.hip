@@ -651,11 +651,9 @@ signed_nlu_kernel(double* __restrict__ A, int64_t lda, double* __restrict__ R, i
 
 // Inverse of a triangular b×b matrix (one workgroup).  upper: invert the
 // upper triangle of X; else the lower triangle (through its transpose).
-__global__ void __launch_bounds__(512)
-tri_inverse_kernel(const double* __restrict__ X, int64_t ldx, double* __restrict__ Xi, int64_t ldi,
-                   int b, int upper, int unit) {
+__device__ __forceinline__ void tri_inverse_body(const double* __restrict__ X, int64_t ldx, double* __restrict__ Xi,
+                                                 int64_t ldi, int b, int upper, int unit, double* smem) {
   constexpr int NT = 512;
-  extern __shared__ double smem[];
   double* M = smem;
   double* d = M + b * b;
   const int tid = threadIdx.x;
@@ -668,6 +666,24 @@ tri_inverse_kernel(const double* __restrict__ X, int64_t ldx, double* __restrict
     if (upper) Xi[rr + cc * ldi] = v;
     else Xi[cc + rr * ldi] = v;
   }
+}
+
+__global__ void __launch_bounds__(512)
+tri_inverse_kernel(const double* __restrict__ X, int64_t ldx, double* __restrict__ Xi, int64_t ldi,
+                   int b, int upper, int unit) {
+  extern __shared__ double smem[];
+  tri_inverse_body(X, ldx, Xi, ldi, b, upper, unit, smem);
+}
+
+// Inverses of the diagonal blocks of the upper triangle of an n×n matrix,
+// blk wide (the last one narrower), one workgroup per block: block p's
+// inverse is stored bj×bj (ld bj) at Xi + p·blk².
+__global__ void __launch_bounds__(512)
+tri_inverse_diag_kernel(const double* __restrict__ X, int64_t ldx, double* __restrict__ Xi, int64_t n, int blk) {
+  extern __shared__ double smem[];
+  const int64_t j = (int64_t)blockIdx.x * blk;
+  const int bj = (int)(n - j < blk ? n - j : blk);
+  tri_inverse_body(X + j + j * ldx, ldx, Xi + (int64_t)blockIdx.x * blk * blk, bj, bj, 1, 0, smem);
 }
 
 // tau_recon (alg/shared/util.cxx:437-465), one wave per column.
@@ -902,6 +918,15 @@ int launch_tri_inverse(cqr_ctx* c, const double* X, int64_t ldx, double* Xi, int
   const size_t lds = sizeof(double) * ((size_t)b * b + b);
   hipLaunchKernelGGL(tri_inverse_kernel, dim3(1), dim3(512), lds, c->stream, X, ldx, Xi, ldi, b, upper, unit);
   return check_launch(c, "tri_inverse");
+}
+
+int launch_tri_inverse_diag(cqr_ctx* c, const double* X, int64_t ldx, double* Xi, int64_t n, int blk) {
+  if (n < 1 || blk < 1 || blk > 128) return set_err(c, CQR_ERR_INVALID, "tri_inverse_diag: n %lld, blk %d", (long long)n, blk);
+  CQR_TRY(big_lds_attr(c, (const void*)tri_inverse_diag_kernel));
+  const size_t lds = sizeof(double) * ((size_t)blk * blk + blk);
+  const int64_t nb = (n + blk - 1) / blk;
+  hipLaunchKernelGGL(tri_inverse_diag_kernel, dim3((unsigned)nb), dim3(512), lds, c->stream, X, ldx, Xi, n, blk);
+  return check_launch(c, "tri_inverse_diag");
 }
 
 
