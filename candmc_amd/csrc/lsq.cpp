@@ -44,11 +44,20 @@ int apply_q_ws(cqr_ctx* c, const double* YR, int64_t ldy, const double* Tt, int6
       const double* Yp = YR + j + j * ldy;
       double* Bp = B + j;
       const int bt = (int)std::min(bj, mj);
-      // Z = Yᵀ·B over all the panel's rows (its top block unit lower)
-      CQR_TRY(thin_gemm(c, 1, mj, bj, 1.0, Yp, ldy, Bp, ldb, (int)nr, 0.0, Z, bj, scr, bt));
-      // Z' = Tᵀ·Z (Qᵀ) or T·Z (Q); Tt holds Tᵀ (lower, ld bj)
-      CQR_TRY(gemm_f64_ws(c, trans ? 0 : 1, 0, bj, nr, bj, 1.0, Tt + p * b * b, bj, SM_LOWER, Z, bj, SM_NONE, 0.0,
-                          Zp, bj, nullptr));
+      if (nr <= 4 && bj <= 128) {
+        // Z = Yᵀ·B as slab partials, then their sum and Z' = Tᵀ·Z (Qᵀ) or
+        // T·Z (Q) in one workgroup; Tt holds Tᵀ (lower, ld bj)
+        CQR_TRY(thin_gemm(c, 1, mj, bj, 1.0, Yp, ldy, Bp, ldb, (int)nr, 0.0, nullptr, bj, scr, bt));
+        const int64_t slab = thin_t_slab_rows(mj, bj);
+        CQR_TRY(launch_slab_reduce_t(c, scr, (int)((mj + slab - 1) / slab), (int)bj, (int)nr, Tt + p * b * b, bj,
+                                     trans ? 0 : 1, Zp, bj));
+      } else {
+        // Z = Yᵀ·B over all the panel's rows (its top block unit lower)
+        CQR_TRY(thin_gemm(c, 1, mj, bj, 1.0, Yp, ldy, Bp, ldb, (int)nr, 0.0, Z, bj, scr, bt));
+        // Z' = Tᵀ·Z (Qᵀ) or T·Z (Q); Tt holds Tᵀ (lower, ld bj)
+        CQR_TRY(gemm_f64_ws(c, trans ? 0 : 1, 0, bj, nr, bj, 1.0, Tt + p * b * b, bj, SM_LOWER, Z, bj, SM_NONE, 0.0,
+                            Zp, bj, nullptr));
+      }
       // B −= Y·Z'
       CQR_TRY(thin_gemm(c, 0, mj, bj, -1.0, Yp, ldy, Zp, bj, (int)nr, 1.0, Bp, ldb, nullptr, bt));
     }

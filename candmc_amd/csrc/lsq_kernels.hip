@@ -149,6 +149,62 @@ __global__ void __launch_bounds__(1024) slab_reduce_kernel(const double* __restr
   }
 }
 
+// Z' = op(T)·Σ_z P_z in one workgroup, for the few-column apply of Q or Qᵀ
+// (k ≤ 128 rows, nr ≤ 4 columns): the slab partials of Z = Yᵀ·B are summed
+// in a fixed order (G groups of slabs, each in slab order, then the group
+// sums in order), then Z'(i, j) = Σ_l op(Tt)(i, l)·Z(l, j) with Tt's lower
+// triangle (op the transpose when ta) staged in LDS, summed in l order over
+// two halves.
+__global__ void __launch_bounds__(1024) slab_reduce_t_kernel(const double* __restrict__ P, int nslab, int k, int nr,
+                                                             const double* __restrict__ Tt, int64_t ldt, int ta,
+                                                             double* __restrict__ Zp, int64_t ldz) {
+  extern __shared__ double sm[];
+  double* ts = sm;             // op(Tt), k×k, ld k
+  double* gs = ts + k * k;     // group sums, G·kn ≤ 1024
+  double* zs = gs + 1024;      // Z, kn ≤ 512
+  double* part = zs + 512;     // two halves of the l range, 2 × 512
+  const int t = threadIdx.x, kn = k * nr, G = 1024 / kn;
+  for (int e = t; e < k * k; e += 1024) {   // Tt in storage order (coalesced)
+    const int row = e % k, col = e / k;
+    const double v = row >= col ? Tt[row + (int64_t)col * ldt] : 0.0;
+    if (ta) ts[col + row * k] = v;
+    else ts[row + col * k] = v;
+  }
+  if (t < G * kn) {
+    const int g = t / kn, e = t - g * kn;
+    double sum = 0.0;
+    for (int z0 = g; z0 < nslab; z0 += 16 * G) {
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int z = z0 + u * G;
+        v[u] = P[(int64_t)(z < nslab ? z : 0) * kn + e];
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (z0 + u * G < nslab) sum += v[u];
+    }
+    gs[g * kn + e] = sum;
+  }
+  __syncthreads();
+  if (t < kn) {
+    double tot = 0.0;
+    for (int g = 0; g < G; ++g) tot += gs[g * kn + t];
+    zs[t] = tot;
+  }
+  __syncthreads();
+  const int h = t >> 9, o = t & 511;
+  double acc = 0.0;
+  if (o < kn) {
+    const int i = o % k, j = o / k, l1 = min(k, h * 64 + 64);
+    for (int l = h * 64; l < l1; ++l)
+      if (ta ? l >= i : l <= i) acc += ts[i + l * k] * zs[l + j * k];
+  }
+  part[h * 512 + o] = acc;
+  __syncthreads();
+  if (t < kn) Zp[(t % k) + (int64_t)(t / k) * ldz] = part[t] + part[512 + t];
+}
+
 // One step of the blocked solve with R (upper triangle of a [Y\R] block, ld
 // ldr) on nr ≤ NR columns of B, in 128-row blocks.  Workgroup q of [q0,
 // q0 + grid) updates its block with block p of the solution,
@@ -338,7 +394,6 @@ trsm_chain_kernel(const double* __restrict__ R, int64_t ldr, int64_t n, const do
         for (int j = 0; j < NR; ++j) acc[j] += a[u] * xs[kb + u][j];
       }
     }
-    if (i + 1 < ndep) load_block(i + 1);
 #pragma unroll
     for (int j = 0; j < NR; ++j) part[h][r][j] = acc[j];
     __syncthreads();
@@ -346,6 +401,7 @@ trsm_chain_kernel(const double* __restrict__ R, int64_t ldr, int64_t n, const do
 #pragma unroll
       for (int j = 0; j < NR; ++j) ys[r][j] -= part[0][r][j] + part[1][r][j];
     }
+    if (i + 1 < ndep) load_block(i + 1);   // in flight during the next wait
   }
   __syncthreads();
   double acc[NR];
@@ -461,9 +517,20 @@ int thin_gemm(cqr_ctx* c, int ta, int64_t m, int64_t k, double alpha, const doub
   else if (nr <= 4) hipLaunchKernelGGL(thin_gemm_t_kernel<4>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, slab, scratch, unit_top);
   else hipLaunchKernelGGL(thin_gemm_t_kernel<16>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, slab, scratch, unit_top);
   CQR_TRY(check_launch(c, "thin_gemm_t"));
+  if (!C) return CQR_OK;   // the caller sums the slab partials itself
   hipLaunchKernelGGL(slab_reduce_kernel, dim3((unsigned)((k * nr + 63) / 64)), dim3(1024), 0, c->stream, scratch, nslab, k,
                      nr, alpha, beta, C, ldc);
   return check_launch(c, "slab_reduce");
+}
+
+int launch_slab_reduce_t(cqr_ctx* c, const double* P, int nslab, int k, int nr, const double* Tt, int64_t ldt,
+                         int ta, double* Zp, int64_t ldz) {
+  if (k < 1 || k > 128 || nr < 1 || nr > 4 || nslab < 1)
+    return set_err(c, CQR_ERR_UNSUPPORTED, "slab_reduce_t: k %d, %d columns", k, nr);
+  const size_t lds = sizeof(double) * ((size_t)k * k + 1024 + 512 + 1024);
+  CQR_TRY(set_max_lds(c, (const void*)slab_reduce_t_kernel, (int)(sizeof(double) * (128 * 128 + 2560))));
+  hipLaunchKernelGGL(slab_reduce_t_kernel, dim3(1), dim3(1024), lds, c->stream, P, nslab, k, nr, Tt, ldt, ta, Zp, ldz);
+  return check_launch(c, "slab_reduce_t");
 }
 
 // Rows per slab of the Aᵀ·X product: about 512 workgroups in total.
