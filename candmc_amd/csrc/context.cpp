@@ -298,6 +298,8 @@ int cqr_create(cqr_ctx** ctx, int device) {
       hipMalloc(&c->d_qflags, sizeof(uint64_t) * CQR_MAX_TREE_NODES * 8) != hipSuccess ||
       hipMemset(c->d_qflags, 0, sizeof(uint64_t) * CQR_MAX_TREE_NODES * 8) != hipSuccess ||
       hipMalloc(&c->d_sflags, sizeof(uint64_t) * CQR_TRSM_MAX_BLOCKS) != hipSuccess ||
+      hipHostMalloc(&c->h_pin, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_host, hipEventDisableTiming) != hipSuccess ||
       hipMemset(c->d_terr, 0, 2 * sizeof(int)) != hipSuccess ||
       hipMemset(c->d_cnt, 0, sizeof(uint64_t) * CQR_MAX_TREE_NODES * 2) != hipSuccess ||
       hipMemset(c->d_flag, 0, sizeof(int)) != hipSuccess ||
@@ -310,6 +312,8 @@ int cqr_create(cqr_ctx** ctx, int device) {
     if (c->d_terr) (void)hipFree(c->d_terr);
     if (c->d_qflags) (void)hipFree(c->d_qflags);
     if (c->d_sflags) (void)hipFree(c->d_sflags);
+    if (c->h_pin) (void)hipHostFree(c->h_pin);
+    if (c->ev_host) (void)hipEventDestroy(c->ev_host);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
@@ -341,6 +345,8 @@ int cqr_destroy(cqr_ctx* c) {
   if (c->d_terr) (void)hipFree(c->d_terr);
   if (c->d_qflags) (void)hipFree(c->d_qflags);
   if (c->d_sflags) (void)hipFree(c->d_sflags);
+  if (c->h_pin) (void)hipHostFree(c->h_pin);
+  if (c->ev_host) (void)hipEventDestroy(c->ev_host);
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);

@@ -39,6 +39,8 @@ struct cqr_ctx {
   int* d_terr = nullptr;        // set by a tree node whose wait for a child timed out
   uint64_t* d_qflags = nullptr; // per (node, column block) flags of construct_Q1
   uint64_t* d_sflags = nullptr; // per 128-row block flags of the one-launch triangular solve
+  double* h_pin = nullptr;      // 64 doubles of pinned host memory for small device-to-host reads
+  hipEvent_t ev_host = nullptr; // recorded after a copy into h_pin
   uint64_t q1_epoch = 0;
   uint64_t tree_epoch = 0;      // launch epoch folded into the counters
   hipStream_t side = nullptr;   // second stream of the QR_2D lookahead

@@ -44,9 +44,11 @@ int apply_q_ws(cqr_ctx* c, const double* YR, int64_t ldy, const double* Tt, int6
       const double* Yp = YR + j + j * ldy;
       double* Bp = B + j;
       const int bt = (int)std::min(bj, mj);
-      if (nr <= 4 && bj <= 128) {
+      if (nr <= 4 && bj <= 128 && mj >= 16384) {
         // Z = Yᵀ·B as slab partials, then their sum and Z' = Tᵀ·Z (Qᵀ) or
-        // T·Z (Q) in one workgroup; Tt holds Tᵀ (lower, ld bj)
+        // T·Z (Q) in one workgroup (it takes a whole CU's LDS: only for tall
+        // panels, whose many slab partials pay for it); Tt holds Tᵀ (lower,
+        // ld bj)
         CQR_TRY(thin_gemm(c, 1, mj, bj, 1.0, Yp, ldy, Bp, ldb, (int)nr, 0.0, nullptr, bj, scr, bt));
         const int64_t slab = thin_t_slab_rows(mj, bj);
         CQR_TRY(launch_slab_reduce_t(c, scr, (int)((mj + slab - 1) / slab), (int)bj, (int)nr, Tt + p * b * b, bj,
@@ -89,7 +91,10 @@ int r_block_inverses(cqr_ctx* c, const double* YR, int64_t ldy, int64_t n, doubl
 int trsm_r_ws(cqr_ctx* c, const double* YR, int64_t ldy, int64_t n, const double* Rinv, double* B, int64_t ldb,
               int64_t nr, int trans, double* tmp) {
   const int64_t nb = (n + 127) / 128;
-  if (nr <= 16 && nb > 1 && nb <= CQR_TRSM_MAX_BLOCKS) return launch_trsm_chain(c, YR, ldy, n, Rinv, B, ldb, (int)nr, trans);
+  // the one-launch solve from four blocks on: its workgroups each need a
+  // whole CU, which beside other work (a server's other slots) costs more
+  // than the two or three launches of a short solve
+  if (nr <= 16 && nb >= 4 && nb <= CQR_TRSM_MAX_BLOCKS) return launch_trsm_chain(c, YR, ldy, n, Rinv, B, ldb, (int)nr, trans);
   if (nr <= 16) {
     // first diagonal block alone, then one launch per block step: the
     // remaining blocks take the new block into account and the next one
@@ -245,20 +250,21 @@ int lstsq_refine(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n,
   auto at = [&](size_t off) { return reinterpret_cast<double*>(base + off); };
   double *Rinv = at(w.Ri), *r = at(w.R), *f = at(w.F), *g = at(w.G), *h = at(w.H), *tmp = at(w.Tmp),
          *wsq = at(w.Q), *nrm = at(w.N);
-  std::vector<double> hn(std::max<int64_t>(2 * nr, 2));
-  {  // κ̂ = max|r_ii| / min|r_ii|, a lower bound on κ(A)
-    CQR_TRY(launch_diag_range(c, Af, ldf, n, nrm));
-    CQR_HIP(c, hipMemcpyAsync(hn.data(), nrm, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    CQR_HIP(c, hipStreamSynchronize(c->stream));
-    if (!std::isfinite(hn[0]) || !std::isfinite(hn[1])) return set_err(c, CQR_ERR_NONFINITE, "lstsq: R is not finite");
-    if (escalate && prec != CQR_PREC_F64 && !(hn[1] > 0.0 && hn[0] / hn[1] * prec_unit(prec) <= kEscalateKappaU))
-      return ATTEMPT_ESCALATE;
-  }
+  double* hn = c->h_pin;   // pinned: 2·nr ≤ 32 doubles
+  // κ̂ = max|r_ii| / min|r_ii|, a lower bound on κ(A): copied out now and
+  // read once x0 is formed in the workspace, before X is first written
+  CQR_TRY(launch_diag_range(c, Af, ldf, n, nrm));
+  CQR_HIP(c, hipMemcpyAsync(hn, nrm, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CQR_HIP(c, hipEventRecord(c->ev_host, c->stream));
   CQR_TRY(r_block_inverses(c, Af, ldf, n, Rinv));
   // x0 = R⁻¹·(Qᵀb)(0:n)
   CQR_TRY(launch_copy2d(c, B, ldb, f, m, m, nr));
   CQR_TRY(apply_q_ws(c, Af, ldf, Tt, m, n, b, f, m, nr, 1, wsq));
   CQR_TRY(trsm_r_ws(c, Af, ldf, n, Rinv, f, m, nr, 0, tmp));
+  CQR_HIP(c, hipEventSynchronize(c->ev_host));
+  if (!std::isfinite(hn[0]) || !std::isfinite(hn[1])) return set_err(c, CQR_ERR_NONFINITE, "lstsq: R is not finite");
+  if (escalate && prec != CQR_PREC_F64 && !(hn[1] > 0.0 && hn[0] / hn[1] * prec_unit(prec) <= kEscalateKappaU))
+    return ATTEMPT_ESCALATE;
   CQR_TRY(launch_copy2d(c, f, m, X, ldx, n, nr));
   if (max_iter == 0) return CQR_OK;
   // r = b − A·x
@@ -269,7 +275,7 @@ int lstsq_refine(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n,
   auto rel_update = [&](const double* d, int64_t ldd, double* res) -> int {
     CQR_TRY(launch_colnorm(c, d, ldd, n, (int)nr, nrm));
     CQR_TRY(launch_colnorm(c, X, ldx, n, (int)nr, nrm + nr));
-    CQR_HIP(c, hipMemcpyAsync(hn.data(), nrm, sizeof(double) * 2 * nr, hipMemcpyDeviceToHost, c->stream));
+    CQR_HIP(c, hipMemcpyAsync(hn, nrm, sizeof(double) * 2 * nr, hipMemcpyDeviceToHost, c->stream));
     CQR_HIP(c, hipStreamSynchronize(c->stream));
     double rel = 0.0;
     for (int64_t j = 0; j < nr; ++j) {

@@ -64,18 +64,37 @@ def test_apply_qt_and_q(Q, nrhs):
     assert rel(Q.from_gpu(dB, m, nrhs), B) < 1e-13
 
 
-@pytest.mark.parametrize("n,trans", [(300, False), (300, True), (128, False), (517, True)])
-def test_trsm_r(Q, n, trans):
+# n ≥ 384 (four 128-row blocks) takes the one-launch solve, shorter ones the
+# per-block launches; 1030 leaves a 6-row last block
+# panels of at least 16384 rows with at most four columns sum their slab
+# partials and apply T in one workgroup
+@pytest.mark.parametrize("m,k,b,nrhs", [(20000, 260, 64, 1), (17000, 300, 128, 4)])
+def test_apply_qt_and_q_tall(Q, m, k, b, nrhs):
+    A = O.gaussian(m, k, 5)
+    B = O.gaussian(m, nrhs, 6)
+    dA = Q.to_gpu(A)
+    Tt = Q.geqrf2d(dA, m, m, k, b)
+    YR = Q.from_gpu(dA, m, k)
+    dB = Q.to_gpu(B)
+    Q.apply_qt(dA, m, Tt, m, k, b, dB, m, nrhs, trans=True)
+    assert rel(Q.from_gpu(dB, m, nrhs), O.apply_QT(YR, B)) < 1e-12
+    Q.apply_qt(dA, m, Tt, m, k, b, dB, m, nrhs, trans=False)    # Q·(Qᵀ·B) = B
+    assert rel(Q.from_gpu(dB, m, nrhs), B) < 1e-13
+
+
+@pytest.mark.parametrize("n,trans,nrhs", [(300, False, 3), (300, True, 3), (128, False, 3), (517, True, 3),
+                                          (517, False, 1), (640, True, 16), (1030, False, 16), (1030, True, 1)])
+def test_trsm_r(Q, n, trans, nrhs):
     A = O.gaussian(n + 50, n, 9)
     dA = Q.to_gpu(A)
     Q.QR_2D(dA, n + 50, n + 50, n, 64)
     YR = Q.from_gpu(dA, n + 50, n)
     R = np.triu(YR[:n, :n])
-    B = O.gaussian(n, 3, 10)
+    B = O.gaussian(n, nrhs, 10)
     dB = Q.to_gpu(B)
-    Q.trsm_r(dA, n + 50, n, dB, n, 3, trans=trans)
+    Q.trsm_r(dA, n + 50, n, dB, n, nrhs, trans=trans)
     want = sl.solve_triangular(R, B, trans=1 if trans else 0, lower=False)
-    assert rel(Q.from_gpu(dB, n, 3), want) < 1e-11
+    assert rel(Q.from_gpu(dB, n, nrhs), want) < 1e-11
 
 
 @pytest.mark.parametrize("m,n,nrhs,b", [(2000, 300, 1, 128), (1500, 256, 4, 64), (3000, 400, 16, 128)])
