@@ -344,6 +344,8 @@ int wy_explicit_apply(cqr_ctx* c, const double* YT, int64_t m, int64_t n, int b,
 // X with nr ≤ 16 columns; the transposed form needs
 // thin_gemm_scratch_doubles(m, k, nr) of scratch.
 int64_t thin_t_slab_rows(int64_t m, int64_t k);
+int thin_n_splits(int64_t m, int64_t k);                      // column chunks of a plain A·X
+size_t thin_n_scratch_doubles(int64_t m, int64_t k, int nr);  // its partial sums
 size_t thin_gemm_scratch_doubles(int64_t m, int64_t k, int nr);
 // unit_top > 0: A's first unit_top rows are unit lower (A(r, c) = 0 for
 // c > r and 1 at c = r, whatever is stored there).

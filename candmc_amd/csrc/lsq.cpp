@@ -17,6 +17,7 @@
 // Cheaper corrected semi-normal-equation steps (dx = R⁻¹R⁻ᵀAᵀr, no Q;
 // Björck 1987) come first and are kept while they contract fast, which
 // they do when κ(A)²·u of the factorization is small.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -149,7 +150,8 @@ static void refine_layout(int64_t m, int64_t n, int64_t nr, int64_t b, Sizer& s,
   w->G = s.add(8 * (size_t)n * nr);
   w->H = s.add(8 * (size_t)n * nr);
   w->Tmp = s.add(8 * 128 * nr);
-  w->Q = s.add(8 * std::max(apply_q_ws_doubles(m, b, nr), thin_gemm_scratch_doubles(m, n, (int)nr)));
+  w->Q = s.add(8 * std::max({apply_q_ws_doubles(m, b, nr), thin_gemm_scratch_doubles(m, n, (int)nr),
+                              thin_n_scratch_doubles(m, n, (int)nr)}));
   w->N = s.add(8 * 2 * nr);
 }
 
@@ -269,7 +271,7 @@ int lstsq_refine(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n,
   if (max_iter == 0) return CQR_OK;
   // r = b − A·x
   CQR_TRY(launch_copy2d(c, B, ldb, r, m, m, nr));
-  CQR_TRY(thin_gemm(c, 0, m, n, -1.0, A, lda, X, ldx, (int)nr, 1.0, r, m, nullptr));
+  CQR_TRY(thin_gemm(c, 0, m, n, -1.0, A, lda, X, ldx, (int)nr, 1.0, r, m, wsq));
   // max over the columns of ‖d_j‖ / ‖x_j‖ (one host synchronisation); NaN
   // if any column's update or solution norm is NaN
   auto rel_update = [&](const double* d, int64_t ldd, double* res) -> int {
@@ -306,7 +308,7 @@ int lstsq_refine(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n,
       if (rel <= 0.5 && (it == 1 || rel <= prev_rel / 8)) {
         CQR_TRY(launch_axpby(c, n, nr, 1.0, g, n, 1.0, X, ldx));
         CQR_TRY(launch_copy2d(c, B, ldb, r, m, m, nr));
-        CQR_TRY(thin_gemm(c, 0, m, n, -1.0, A, lda, X, ldx, (int)nr, 1.0, r, m, nullptr));
+        CQR_TRY(thin_gemm(c, 0, m, n, -1.0, A, lda, X, ldx, (int)nr, 1.0, r, m, wsq));
         out.iters += 1;
         out.rel = rel;
         if (rel <= tol) return CQR_OK;
@@ -323,7 +325,7 @@ int lstsq_refine(cqr_ctx* c, const double* A, int64_t lda, int64_t m, int64_t n,
     // f = b − r − A·x,  g = −Aᵀ·r   (FP64)
     CQR_TRY(launch_copy2d(c, B, ldb, f, m, m, nr));
     CQR_TRY(launch_axpby(c, m, nr, -1.0, r, m, 1.0, f, m));
-    CQR_TRY(thin_gemm(c, 0, m, n, -1.0, A, lda, X, ldx, (int)nr, 1.0, f, m, nullptr));
+    CQR_TRY(thin_gemm(c, 0, m, n, -1.0, A, lda, X, ldx, (int)nr, 1.0, f, m, wsq));
     CQR_TRY(thin_gemm(c, 1, m, n, -1.0, A, lda, r, m, (int)nr, 0.0, g, n, wsq));
     // h = R⁻ᵀ·g;  d = Qᵀ·f;  dx = R⁻¹·(d₁ − h);  dr = Q·[h; d₂]
     CQR_TRY(trsm_r_ws(c, Af, ldf, n, Rinv, g, n, nr, 1, tmp));               // g := h

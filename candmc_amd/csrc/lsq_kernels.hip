@@ -12,25 +12,28 @@ constexpr int TT_ROWS = 32;   // row slab of A and X staged in LDS (Aᵀ·X)
 }  // namespace
 
 // C (m×nr) = alpha·A·X + beta·C, A m×k, X k×nr (nr ≤ NR); one row per thread.
+// With P, workgroup row y covers columns [y·kchunk, (y+1)·kchunk) and stores
+// its raw sums at P + (y·nr + j)·m (thin_n_reduce_kernel finishes C).
 template <int NR>
 __global__ void __launch_bounds__(256)
 thin_gemm_n_kernel(const double* __restrict__ A, int64_t lda, int64_t m, int64_t k,
                    const double* __restrict__ X, int64_t ldx, int nr, double alpha, double beta,
-                   double* __restrict__ C, int64_t ldc, int unit_top) {
+                   double* __restrict__ C, int64_t ldc, int unit_top, int64_t kchunk, double* __restrict__ P) {
   __shared__ double Xs[TN_ROWS][NR];
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t kb = (int64_t)blockIdx.y * kchunk, ke = min<int64_t>(k, kb + kchunk);
   double acc[NR];
 #pragma unroll
   for (int j = 0; j < NR; ++j) acc[j] = 0.0;
-  for (int64_t k0 = 0; k0 < k; k0 += TN_ROWS) {
+  for (int64_t k0 = kb; k0 < ke; k0 += TN_ROWS) {
     __syncthreads();
     for (int e = threadIdx.x; e < TN_ROWS * NR; e += 256) {
       const int kk = e % TN_ROWS, j = e / TN_ROWS;
-      Xs[kk][j] = (k0 + kk < k && j < nr) ? X[(k0 + kk) + (int64_t)j * ldx] : 0.0;
+      Xs[kk][j] = (k0 + kk < ke && j < nr) ? X[(k0 + kk) + (int64_t)j * ldx] : 0.0;
     }
     __syncthreads();
     if (r < unit_top) {        // unit-lower top rows: A(r, c) = 0 for c > r, 1 at c = r
-      const int kend = (int)min<int64_t>(TN_ROWS, k - k0);
+      const int kend = (int)min<int64_t>(TN_ROWS, ke - k0);
       for (int kk = 0; kk < kend; ++kk) {
         const int64_t cidx = k0 + kk;
         const double a = cidx < r ? A[r + cidx * lda] : (cidx == r ? 1.0 : 0.0);
@@ -38,7 +41,7 @@ thin_gemm_n_kernel(const double* __restrict__ A, int64_t lda, int64_t m, int64_t
         for (int j = 0; j < NR; ++j) acc[j] += a * Xs[kk][j];
       }
     } else if (r < m) {
-      const int kend = (int)min<int64_t>(TN_ROWS, k - k0);
+      const int kend = (int)min<int64_t>(TN_ROWS, ke - k0);
       const double* ap = A + r + k0 * lda;
       int kk = 0;
       for (; kk + 4 <= kend; kk += 4) {
@@ -59,10 +62,27 @@ thin_gemm_n_kernel(const double* __restrict__ A, int64_t lda, int64_t m, int64_t
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
       if (j >= nr) continue;
-      double* cp = C + r + (int64_t)j * ldc;
-      *cp = alpha * acc[j] + (beta != 0.0 ? beta * *cp : 0.0);
+      if (P) {
+        P[((int64_t)blockIdx.y * nr + j) * m + r] = acc[j];
+      } else {
+        double* cp = C + r + (int64_t)j * ldc;
+        *cp = alpha * acc[j] + (beta != 0.0 ? beta * *cp : 0.0);
+      }
     }
   }
+}
+
+// C = alpha·Σ_y P_y + beta·C for the column-split A·X (sum in y order).
+__global__ void __launch_bounds__(256) thin_n_reduce_kernel(const double* __restrict__ P, int ny, int64_t m, int nr,
+                                                            double alpha, double beta, double* __restrict__ C,
+                                                            int64_t ldc) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= m * nr) return;
+  const int64_t r = e % m, j = e / m;
+  double sum = 0.0;
+  for (int y = 0; y < ny; ++y) sum += P[((int64_t)y * nr + j) * m + r];
+  double* cp = C + r + j * ldc;
+  *cp = alpha * sum + (beta != 0.0 ? beta * *cp : 0.0);
 }
 
 // Partial P_z (k×nr, ld k) = A(rows of slab z, :)ᵀ·X(rows of slab z, :) for
@@ -503,11 +523,20 @@ int thin_gemm(cqr_ctx* c, int ta, int64_t m, int64_t k, double alpha, const doub
   if (nr < 1 || nr > 16) return set_err(c, CQR_ERR_UNSUPPORTED, "thin product with %d columns", nr);
   if (m <= 0 || k <= 0) return CQR_OK;
   if (!ta) {
-    const dim3 grid((unsigned)((m + 255) / 256));
-    if (nr == 1) hipLaunchKernelGGL(thin_gemm_n_kernel<1>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, alpha, beta, C, ldc, unit_top);
-    else if (nr <= 4) hipLaunchKernelGGL(thin_gemm_n_kernel<4>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, alpha, beta, C, ldc, unit_top);
-    else hipLaunchKernelGGL(thin_gemm_n_kernel<16>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, alpha, beta, C, ldc, unit_top);
-    return check_launch(c, "thin_gemm_n");
+    // one row a thread; a tall and wide product with scratch also splits its
+    // columns over thin_n_splits() workgroup rows, for more waves in flight
+    const int ny = (scratch && unit_top == 0) ? thin_n_splits(m, k) : 1;
+    const int64_t kchunk = ny > 1 ? ((k + ny - 1) / ny + TN_ROWS - 1) / TN_ROWS * TN_ROWS : k;
+    double* P = ny > 1 ? scratch : nullptr;
+    const dim3 grid((unsigned)((m + 255) / 256), (unsigned)ny);
+    if (nr == 1) hipLaunchKernelGGL(thin_gemm_n_kernel<1>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, alpha, beta, C, ldc, unit_top, kchunk, P);
+    else if (nr <= 4) hipLaunchKernelGGL(thin_gemm_n_kernel<4>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, alpha, beta, C, ldc, unit_top, kchunk, P);
+    else hipLaunchKernelGGL(thin_gemm_n_kernel<16>, grid, dim3(256), 0, c->stream, A, lda, m, k, X, ldx, nr, alpha, beta, C, ldc, unit_top, kchunk, P);
+    CQR_TRY(check_launch(c, "thin_gemm_n"));
+    if (ny == 1) return CQR_OK;
+    hipLaunchKernelGGL(thin_n_reduce_kernel, dim3((unsigned)((m * nr + 255) / 256)), dim3(256), 0, c->stream, P, ny, m, nr,
+                       alpha, beta, C, ldc);
+    return check_launch(c, "thin_n_reduce");
   }
   // C (k×nr) = alpha·Aᵀ·X + beta·C, A m×k; slabs of rows, then a fixed-order sum
   const int64_t slab = thin_t_slab_rows(m, k);
@@ -540,6 +569,15 @@ int64_t thin_t_slab_rows(int64_t m, int64_t k) {
   int64_t slab = (m + nslab - 1) / nslab;
   slab = std::max<int64_t>(TT_ROWS, (slab + TT_ROWS - 1) / TT_ROWS * TT_ROWS);
   return slab;
+}
+
+// Column chunks of a plain A·X (at least 256 columns each): 8 from
+// 8192×1024 on, else 1.
+int thin_n_splits(int64_t m, int64_t k) { return (m >= 8192 && k >= 1024) ? 8 : 1; }
+
+size_t thin_n_scratch_doubles(int64_t m, int64_t k, int nr) {
+  const int ny = thin_n_splits(m, k);
+  return ny > 1 ? (size_t)ny * m * nr : 0;
 }
 
 size_t thin_gemm_scratch_doubles(int64_t m, int64_t k, int nr) {

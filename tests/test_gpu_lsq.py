@@ -109,6 +109,20 @@ def test_lstsq_fp64(Q, m, n, nrhs, b):
     assert 1 <= iters <= 3
 
 
+# from 8192×1024 the residual products A·x split their columns over eight
+# workgroup rows; 16500 rows also reach the one-workgroup T product of the Q
+# application, 1100 columns a ragged last block of the one-launch solve
+@pytest.mark.parametrize("m,n,nrhs", [(8192, 1024, 1), (16500, 1100, 5)])
+def test_lstsq_fp64_tall_wide(Q, m, n, nrhs):
+    A = O.gaussian(m, n, 21)
+    B = O.gaussian(m, nrhs, 22)
+    X, iters, upd = Q.lstsq(Q.to_gpu(A), m, m, n, Q.to_gpu(B), m, nrhs, b=128, max_iter=3, tol=1e-12)
+    x = Q.from_gpu(X, n, nrhs)
+    want = sl.lstsq(A, B)[0]
+    assert rel(x, want) < 1e-12 and upd <= 1e-12
+    assert 1 <= iters <= 3
+
+
 def _conditioned_problem(m, n, nrhs, kappa, seed):
     A = O.conditioned_matrix(m, n, kappa, seed)
     rng = np.random.default_rng(seed + 1)
