@@ -352,16 +352,16 @@ size_t thin_gemm_scratch_doubles(int64_t m, int64_t k, int nr);
 int thin_gemm(cqr_ctx* c, int ta, int64_t m, int64_t k, double alpha, const double* A, int64_t lda,
               const double* X, int64_t ldx, int nr, double beta, double* C, int64_t ldc, double* scratch,
               int unit_top = 0);
-// One step of the blocked R solve (lsq_kernels.hip): blocks [q0, q0+nq)
-// take block p's solution into account; block `solve` is then finished.
-// B(0:n, :) := R⁻¹·B or R⁻ᵀ·B (nr ≤ 16 columns) in one launch, with the
 // Z' = op(Tt)·Σ slab partials of thin_gemm(ta = 1, C = nullptr) (k ≤ 128,
 // nr ≤ 4), one workgroup
 int launch_slab_reduce_t(cqr_ctx* c, const double* P, int nslab, int k, int nr, const double* Tt, int64_t ldt,
                          int ta, double* Zp, int64_t ldz);
+// B(0:n, :) := R⁻¹·B or R⁻ᵀ·B (nr ≤ 16 columns) in one launch, with the
 // 128-wide diagonal-block inverses Rinv
 int launch_trsm_chain(cqr_ctx* c, const double* R, int64_t ldr, int64_t n, const double* Rinv, double* B,
                       int64_t ldb, int nr, int trans);
+// One step of the blocked R solve (lsq_kernels.hip): blocks [q0, q0+nq)
+// take block p's solution into account; block `solve` is then finished.
 int launch_trsm_step(cqr_ctx* c, const double* R, int64_t ldr, int64_t n, const double* Rinv, double* B, int64_t ldb,
                      int nr, int trans, int p, int q0, int nq, int solve);
 int launch_axpby(cqr_ctx* c, int64_t m, int64_t n, double alpha, const double* X, int64_t ldx, double beta,
