@@ -44,3 +44,30 @@ def test_forced_wait_timeout_is_reported_then_cleared(Q):
     assert np.abs(R - s[:, None] * Rl).max() / np.abs(Rl).max() < 1e-12
     with pytest.raises(CQRError):
         ctx.set_wait_limit(-1)
+
+
+def test_forced_wait_timeout_in_the_one_launch_solve(Q):
+    """The one-launch triangular solve's blocks wait for the blocks they
+    depend on with the same bounded wait: forced down, the deep blocks give
+    up, every workgroup finishes, the sync reports it, and the next solve is
+    correct."""
+    from candmc_amd._lib import CQRError
+    n = 1030                                # nine 128-row blocks, the last ragged
+    A = O.gaussian(n + 50, n, 9)
+    dA = Q.to_gpu(A)
+    Q.QR_2D(dA, n + 50, n + 50, n, 64)
+    R = np.triu(Q.from_gpu(dA, n + 50, n)[:n, :n])
+    B = O.gaussian(n, 2, 10)
+    ctx = Q.context()
+    print("forced timeout: trsm_r", n, flush=True)
+    try:
+        ctx.set_wait_limit(1)
+        with pytest.raises(CQRError) as e:
+            Q.trsm_r(dA, n + 50, n, Q.to_gpu(B), n, 2)
+        assert e.value.status == 4          # CQR_ERR_DEVICE
+    finally:
+        ctx.set_wait_limit(0)
+    dB = Q.to_gpu(B)
+    Q.trsm_r(dA, n + 50, n, dB, n, 2)
+    want = sl.solve_triangular(R, B, lower=False)
+    assert np.abs(Q.from_gpu(dB, n, 2) - want).max() / np.abs(want).max() < 1e-11
