@@ -342,7 +342,11 @@ int wy_explicit_apply(cqr_ctx* c, const double* YT, int64_t m, int64_t n, int b,
 
 // Least-squares kernels (lsq_kernels.hip): C = alpha·op(A)·X + beta·C for
 // X with nr ≤ 16 columns; the transposed form needs
-// thin_gemm_scratch_doubles(m, k, nr) of scratch.
+// thin_gemm_scratch_doubles(m, k, nr) of scratch and, with C = nullptr,
+// only leaves its slab partials there (for launch_slab_reduce_t).  The
+// plain form given scratch (and unit_top = 0) splits the columns of a large
+// product over thin_n_splits(m, k) workgroup rows; the scratch then holds
+// thin_n_scratch_doubles(m, k, nr).
 int64_t thin_t_slab_rows(int64_t m, int64_t k);
 int thin_n_splits(int64_t m, int64_t k);                      // column chunks of a plain A·X
 size_t thin_n_scratch_doubles(int64_t m, int64_t k, int nr);  // its partial sums
